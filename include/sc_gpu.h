@@ -1,0 +1,171 @@
+/*
+ * sc_gpu.h -- C ABI of the MI355X (gfx950) Monte Carlo hot path for patchy
+ * spherocylinders, a drop-in for the energy / move interface of
+ * robertvacha/SC (scOOP 4.0).
+ *
+ * The reference's plug point is the compile-time typedef
+ *   typedef TotalEMatrix<PairE> TotalEnergyCalculator;
+ * (scOOP/mc/totalenergycalculator.h:1227) whose methods are called by
+ * Updater (updater.h:33), MoveCreator (movecreator.h:36) and ClusterSampler
+ * (clust.h:23).  Each entry point below names the reference method it
+ * replaces.  A C++ class with the reference's method set (GpuTotalE in
+ * sc_amd/csrc/sc_host.h) wraps this ABI for in-process C++ callers.
+ *
+ * Conventions:
+ *   - every function returns 0 on success and a negative SC_E* code on
+ *     failure; sc_last_error() gives the message.  Nothing aborts;
+ *   - the caller owns every host array, the context owns all device memory
+ *     and one HIP stream;
+ *   - energies cross the ABI in FP64 whatever the internal precision;
+ *   - positions cross the ABI as box fractions (as Particle::pos,
+ *     scOOP/structures/particle.h:26); inside they are 32-bit fixed point;
+ *   - a context is not thread safe: one context per host thread / replica;
+ *   - particle indices are the reference's indices (order of config.init).
+ */
+#ifndef SC_GPU_H
+#define SC_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SC_OK 0
+#define SC_EINVAL (-1)      /* bad argument or malformed input text */
+#define SC_EHIP (-2)        /* HIP runtime error */
+#define SC_ESTATE (-3)      /* call out of order (e.g. no topology yet) */
+#define SC_ECAPACITY (-4)   /* a fixed device capacity was exceeded */
+#define SC_ENODEV (-5)      /* no gfx950 device / HIP runtime unavailable */
+
+typedef struct sc_ctx sc_ctx;
+
+typedef struct {
+    uint64_t trans_acc, trans_try, rot_acc, rot_try;   /* Statistics trans/rot acc/rej */
+    uint64_t left_cell;       /* translations rejected for leaving their cell */
+    uint64_t stage1_acc, stage1_try;                    /* delayed acceptance, first stage */
+    double energy_change;     /* sum of accepted dE (for the drift check, updater.cpp:122-150) */
+} sc_sweep_stats;
+
+/* --- lifetime --------------------------------------------------------- */
+int sc_create(sc_ctx** out, int device);
+void sc_destroy(sc_ctx* ctx);
+const char* sc_last_error(const sc_ctx* ctx);
+
+/* --- inputs in SC's own text formats ------------------------------------ */
+/* top.init text: [Types] [Molecules] [System] [EXCLUDE] sections
+ * (Inicializer::readTopoFile, scOOP/mc/inicializer.cpp:453-585, fillTypes
+ * :660-852, fillMol :929-1108, fillSystem :631-657) followed by
+ * Topo::genParamPairs / genTopoParams (scOOP/structures/topo.cpp:5-153). */
+int sc_load_topology(sc_ctx* ctx, const char* top_init_text);
+/* options text, key = value (Sim::readOptions, scOOP/structures/sim.h:123-406).
+ * Stored values keep the reference's post-processing: transmx doubled
+ * (sim.h:365), rotmx turned into the quaternion half-angle (sim.h:360). */
+int sc_load_options(sc_ctx* ctx, const char* options_text);
+int sc_get_option(const sc_ctx* ctx, const char* key, double* value);
+/* config.init text (Inicializer::initConfig, inicializer.cpp:89-298) for the
+ * particles the topology's [System] declares. */
+int sc_load_config(sc_ctx* ctx, const char* config_init_text);
+/* Conf::draw-format text of the current state (structures/Conf.h:388-425);
+ * returns the length written (or needed, when buf is NULL / cap too small). */
+long sc_write_config(sc_ctx* ctx, char* buf, long cap);
+
+/* --- topology queries ---------------------------------------------------- */
+int sc_num_particles(const sc_ctx* ctx);
+int sc_particle_types(const sc_ctx* ctx, int32_t* types, int cap);
+/* maxcut = 1.1 * (max cutoff + max length), Topo::genTopoParams (topo.cpp:140-153) */
+int sc_maxcut(const sc_ctx* ctx, double* maxcut);
+/* FP64 pair-parameter table row for reference types (ti,tj): 44 doubles in
+ * the order documented in sc_host.cpp (ia_params[ti][tj], structures.h:187-222) */
+int sc_pair_params(const sc_ctx* ctx, int ti, int tj, double* out44);
+
+/* --- raw state (the counterpart of Conf::pvec) --------------------------- */
+/* pos: box fractions n*3; dir, pd0: n*3 (normalised / orthogonalised here as
+ * initConfig does); types: reference type numbers. */
+int sc_set_config(sc_ctx* ctx, int n, const double box[3], const double* pos, const double* dir,
+                  const double* pd0, const int32_t* types);
+int sc_get_config(sc_ctx* ctx, double box[3], double* pos, double* dir, double* pd0);
+/* fixed-point positions (uint32 fractions of the box), n*3 */
+int sc_get_fixed_positions(sc_ctx* ctx, uint32_t* u);
+
+/* --- cell list ------------------------------------------------------------ */
+/* Builds the cell list for a grid of cells at least maxcut/r wide (r = 1 or 2:
+ * stencil radius), each axis count a multiple of p = r+1, shifted by
+ * shift[3] (fixed-point units).  Optional outputs: cell id per particle
+ * (x fastest), CSR starts (ncell+1), particle indices sorted by cell and, in a
+ * cell, by index; ncell_out[3] the grid. */
+int sc_build_cells(sc_ctx* ctx, int r, const uint32_t shift[3], int32_t* cell_of, int32_t* cell_start,
+                   int32_t* sorted, int32_t ncell_out[3]);
+/* Neighbour pairs (i<j) with minimum-image centre distance <= rc (FP64 from
+ * the fixed-point positions); the GPU counterpart of genSimplePairList's
+ * distance test (scOOP/mc/updater.cpp:484-552).  Pairs in row-major order. */
+int sc_neighbor_pairs(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* count);
+
+/* --- energies ------------------------------------------------------------- */
+/* pair energies E(i,j) with i as part1 (TotalE::p2p, totalenergycalculator.h:200);
+ * fp64 != 0 evaluates the same device code in FP64 (cross-check mode). */
+int sc_pair_energies(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64);
+/* whole-system energy: FP32 pair terms, FP64 reduction
+ * (TotalEMatrix::allToAll / initEM, totalenergycalculator.h:314-353,502-521) */
+int sc_energy_all(sc_ctx* ctx, double* e);
+/* energy of one particle with all others (TotalEFull::oneToAll, :563-583;
+ * TotalEMatrix::oneToAll :355-381 returns the same value from its cache) */
+int sc_energy_one(sc_ctx* ctx, int target, double* e);
+/* energy of particle `target` placed at a trial state (TotalEMatrix::oneToAllTrial,
+ * :383-415); pos is a box fraction */
+int sc_energy_one_trial(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3],
+                        double* e);
+/* commit a particle trial (TotalEMatrix::update(int), :326-328) */
+int sc_commit_particle(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3]);
+/* every particle's energy with all others, original order */
+int sc_energy_per_particle(sc_ctx* ctx, double* out);
+/* whole-system energy at a trial box (TotalEMatrix::allToAllTrial, :334-336) */
+int sc_energy_all_trial(sc_ctx* ctx, const double box_new[3], double* e);
+/* commit a box change (TotalEMatrix::update(), :318-320) */
+int sc_commit_box(sc_ctx* ctx, const double box_new[3]);
+
+/* --- hard-core overlap ---------------------------------------------------- */
+/* overlap(i,j) <=> closest segment (or sphere) distance < sigma_ij.  FP32 with
+ * an FP64 recheck when the distance is within 1e-6 sigma of contact.
+ * out[k] = 1 on overlap; dmin[k] (optional) = the distance.  Build-defined:
+ * the reference's Conf::overlap (structures/Conf.cpp:104-239) is not called
+ * anywhere in scOOP. */
+int sc_overlap_flags(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, uint8_t* out, double* dmin);
+/* count of overlapping pairs in the whole system */
+int sc_overlap_count(sc_ctx* ctx, long* count);
+
+/* --- Monte Carlo ---------------------------------------------------------- */
+/* nsweeps checkerboard sweeps (N trial moves each, the step loop of
+ * Updater::simulate, updater.cpp:206-230, with MoveCreator::particleMove,
+ * movecreator.cpp:11-33) at temperature temper.  transmx is the translation
+ * length (already doubled, sim.h:365), rot_half the maximum quaternion
+ * half-angle in radians (sim.h:360).  r selects the checkerboard (1: 2x2x2
+ * colours, 2: 3x3x3).  seed/counter key the counter-based RNG; the same
+ * (seed, counter) reproduces a run bit for bit.  stats may be NULL. */
+int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot_half, int r, uint64_t seed,
+             uint64_t counter, sc_sweep_stats* stats);
+/* Volume move (MoveCreator::pressureMove, movecreator.cpp:330-550) for
+ * ptype 0 (one random edge) and 1 (isotropic); returns accepted=1/0 and the
+ * new box.  Whole-system energies are FP32 pair terms with FP64 reduction; dE
+ * is FP64. */
+int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
+                   uint64_t counter, int* accepted, double box_out[3]);
+
+/* --- synthetic inputs ----------------------------------------------------- */
+/* Seeded overlap-free configuration by random sequential addition (host,
+ * FP64).  No counterpart in the scOOP library: the reference's generator
+ * scripts/config-scripts/psc-genconfig.py:16-29 places particles without an
+ * overlap check.  Outputs box-fraction positions and unit vectors (n*3). */
+int sc_generate_rsa(sc_ctx* ctx, int n, const int32_t* types, const double box[3], uint64_t seed, long max_attempts,
+                    double* pos, double* dir, double* pd0);
+
+/* --- device timing -------------------------------------------------------- */
+/* milliseconds of the last sc_sweep / sc_energy_all call measured with HIP
+ * events on the context's stream around its kernels, and the number of
+ * kernel launches of the dominant kernel in it */
+int sc_last_timing(const sc_ctx* ctx, double* ms, long* launches, double* kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,315 @@
+"""sc_amd: MI355X (gfx950) Monte Carlo hot path for patchy spherocylinders.
+
+Python front-end over the C ABI of sc_amd/lib/libscamd.so (include/sc_gpu.h).
+The energy / move methods mirror robertvacha/SC's TotalEnergyCalculator
+(scOOP/mc/totalenergycalculator.h:135-522) so host code reads like the
+reference's MoveCreator.  Every compute call runs the HIP kernels; there is no
+CPU fallback: without the built library or a GPU the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libscamd.so")
+
+SC_ERRORS = {-1: "SC_EINVAL", -2: "SC_EHIP", -3: "SC_ESTATE", -4: "SC_ECAPACITY", -5: "SC_ENODEV"}
+
+
+class SCError(RuntimeError):
+    pass
+
+
+class SweepStats(ctypes.Structure):
+    _fields_ = [("trans_acc", ctypes.c_uint64), ("trans_try", ctypes.c_uint64), ("rot_acc", ctypes.c_uint64),
+                ("rot_try", ctypes.c_uint64), ("left_cell", ctypes.c_uint64), ("stage1_acc", ctypes.c_uint64),
+                ("stage1_try", ctypes.c_uint64), ("energy_change", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# exported symbol -> (restype, argtypes)
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+_U = ctypes.POINTER(ctypes.c_uint32)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+SIGNATURES = {
+    "sc_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int]),
+    "sc_destroy": (None, [_P]),
+    "sc_last_error": (ctypes.c_char_p, [_P]),
+    "sc_load_topology": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "sc_load_options": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "sc_get_option": (ctypes.c_int, [_P, ctypes.c_char_p, _D]),
+    "sc_load_config": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "sc_write_config": (ctypes.c_long, [_P, ctypes.c_char_p, ctypes.c_long]),
+    "sc_num_particles": (ctypes.c_int, [_P]),
+    "sc_particle_types": (ctypes.c_int, [_P, _I, ctypes.c_int]),
+    "sc_maxcut": (ctypes.c_int, [_P, _D]),
+    "sc_pair_params": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
+    "sc_set_config": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D, _D, _I]),
+    "sc_get_config": (ctypes.c_int, [_P, _D, _D, _D, _D]),
+    "sc_get_fixed_positions": (ctypes.c_int, [_P, _U]),
+    "sc_build_cells": (ctypes.c_int, [_P, ctypes.c_int, _U, _I, _I, _I, _I]),
+    "sc_neighbor_pairs": (ctypes.c_int, [_P, ctypes.c_double, _I, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
+    "sc_pair_energies": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _D, ctypes.c_int]),
+    "sc_energy_all": (ctypes.c_int, [_P, _D]),
+    "sc_energy_one": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+    "sc_energy_one_trial": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D, _D]),
+    "sc_commit_particle": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D]),
+    "sc_energy_per_particle": (ctypes.c_int, [_P, _D]),
+    "sc_energy_all_trial": (ctypes.c_int, [_P, _D, _D]),
+    "sc_commit_box": (ctypes.c_int, [_P, _D]),
+    "sc_overlap_flags": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _U8, _D]),
+    "sc_overlap_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_long)]),
+    "sc_sweep": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(SweepStats)]),
+    "sc_volume_move": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), _D]),
+    "sc_generate_rsa": (ctypes.c_int, [_P, ctypes.c_int, _I, _D, ctypes.c_uint64, ctypes.c_long, _D, _D, _D]),
+    "sc_last_timing": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_long), _D]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SCError("sc_amd native library missing (%s): run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_D)
+
+
+def _i(a):
+    return a.ctypes.data_as(_I)
+
+
+def _vec3(x):
+    return np.ascontiguousarray(x, np.float64).reshape(3)
+
+
+class GpuSystem:
+    """One simulated system on one GPU: the drop-in counterpart of SC's
+    TotalEnergyCalculator plus the particle store it reads (Conf::pvec)."""
+
+    def __init__(self, device=0):
+        L = lib()
+        h = _P()
+        rc = L.sc_create(ctypes.byref(h), int(device))
+        if rc != 0:
+            raise SCError("sc_create failed (%s): no usable HIP device %d" % (SC_ERRORS.get(rc, rc), device))
+        self._h = h
+        self._L = L
+        self.n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.sc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            msg = self._L.sc_last_error(self._h)
+            raise SCError("%s: %s (%s)" % (what, msg.decode() if msg else "", SC_ERRORS.get(rc, rc)))
+
+    # ---- SC text inputs
+    def load_topology(self, text):
+        self._ck(self._L.sc_load_topology(self._h, text.encode()), "load_topology")
+        self.n = self._L.sc_num_particles(self._h)
+
+    def load_options(self, text):
+        self._ck(self._L.sc_load_options(self._h, text.encode()), "load_options")
+
+    def option(self, key):
+        v = ctypes.c_double()
+        self._ck(self._L.sc_get_option(self._h, key.encode(), ctypes.byref(v)), "option " + key)
+        return v.value
+
+    def load_config(self, text):
+        self._ck(self._L.sc_load_config(self._h, text.encode()), "load_config")
+
+    def write_config(self):
+        n = self._L.sc_write_config(self._h, None, 0)
+        if n < 0:
+            self._ck(int(n), "write_config")
+        buf = ctypes.create_string_buffer(n + 1)
+        self._L.sc_write_config(self._h, buf, n + 1)
+        return buf.value.decode()
+
+    def particle_types(self):
+        n = self._L.sc_num_particles(self._h)
+        out = np.zeros(max(n, 1), np.int32)
+        self._L.sc_particle_types(self._h, _i(out), n)
+        return out[:n]
+
+    @property
+    def maxcut(self):
+        v = ctypes.c_double()
+        self._ck(self._L.sc_maxcut(self._h, ctypes.byref(v)), "maxcut")
+        return v.value
+
+    def pair_params(self, ti, tj):
+        out = np.zeros(44)
+        self._ck(self._L.sc_pair_params(self._h, ti, tj, _d(out)), "pair_params")
+        return out
+
+    # ---- raw state
+    def set_config(self, box, pos, dir, pd0, types=None):
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        n = len(pos)
+        box = _vec3(box)
+        d = np.ascontiguousarray(dir, np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(pd0, np.float64).reshape(-1, 3)
+        t = None if types is None else np.ascontiguousarray(types, np.int32)
+        self._ck(self._L.sc_set_config(self._h, n, _d(box), _d(pos), _d(d), _d(q), None if t is None else _i(t)),
+                 "set_config")
+        self.n = n
+
+    def get_config(self):
+        n = self.n
+        box = np.zeros(3)
+        pos, d, q = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        self._ck(self._L.sc_get_config(self._h, _d(box), _d(pos), _d(d), _d(q)), "get_config")
+        return box, pos, d, q
+
+    def fixed_positions(self):
+        u = np.zeros((self.n, 3), np.uint32)
+        self._ck(self._L.sc_get_fixed_positions(self._h, u.ctypes.data_as(_U)), "fixed_positions")
+        return u
+
+    def generate_rsa(self, types, box, seed, max_attempts=None):
+        t = np.ascontiguousarray(types, np.int32)
+        n = len(t)
+        box = _vec3(box)
+        pos, d, q = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        ma = max_attempts if max_attempts is not None else 2000 * max(n, 1)
+        self._ck(self._L.sc_generate_rsa(self._h, n, _i(t), _d(box), int(seed), int(ma), _d(pos), _d(d), _d(q)),
+                 "generate_rsa")
+        return pos, d, q
+
+    # ---- cells
+    def build_cells(self, r=2, shift=(0, 0, 0)):
+        n = self.n
+        sh = np.ascontiguousarray(shift, np.uint32)
+        nc = np.zeros(3, np.int32)
+        cell_of = np.zeros(max(n, 1), np.int32)
+        srt = np.zeros(max(n, 1), np.int32)
+        # first call for the grid size
+        self._ck(self._L.sc_build_cells(self._h, r, sh.ctypes.data_as(_U), None, None, None, _i(nc)), "build_cells")
+        start = np.zeros(int(np.prod(nc)) + 1, np.int32)
+        self._ck(self._L.sc_build_cells(self._h, r, sh.ctypes.data_as(_U), _i(cell_of), _i(start), _i(srt), _i(nc)),
+                 "build_cells")
+        return cell_of[:n], start, srt[:n], tuple(int(x) for x in nc)
+
+    def neighbor_pairs(self, rc):
+        cnt = ctypes.c_long()
+        self._ck(self._L.sc_neighbor_pairs(self._h, rc, None, 0, ctypes.byref(cnt)), "neighbor_pairs")
+        out = np.zeros((max(cnt.value, 1), 2), np.int32)
+        self._ck(self._L.sc_neighbor_pairs(self._h, rc, _i(out), cnt.value, ctypes.byref(cnt)), "neighbor_pairs")
+        return out[:cnt.value]
+
+    # ---- energies (TotalEnergyCalculator method set)
+    def p2p(self, i, j, fp64=False):
+        """pair energies with i as part1 (TotalE::p2p)"""
+        i = np.ascontiguousarray(np.atleast_1d(i), np.int32)
+        j = np.ascontiguousarray(np.atleast_1d(j), np.int32)
+        out = np.zeros(len(i))
+        self._ck(self._L.sc_pair_energies(self._h, len(i), _i(i), _i(j), _d(out), int(bool(fp64))), "pair_energies")
+        return out
+
+    def allToAll(self):
+        e = ctypes.c_double()
+        self._ck(self._L.sc_energy_all(self._h, ctypes.byref(e)), "energy_all")
+        return e.value
+
+    def allToAllTrial(self, box_new):
+        e = ctypes.c_double()
+        self._ck(self._L.sc_energy_all_trial(self._h, _d(_vec3(box_new)), ctypes.byref(e)), "energy_all_trial")
+        return e.value
+
+    def oneToAll(self, target):
+        e = ctypes.c_double()
+        self._ck(self._L.sc_energy_one(self._h, int(target), ctypes.byref(e)), "energy_one")
+        return e.value
+
+    def oneToAllTrial(self, target, pos, dir, pd0):
+        e = ctypes.c_double()
+        self._ck(self._L.sc_energy_one_trial(self._h, int(target), _d(_vec3(pos)), _d(_vec3(dir)), _d(_vec3(pd0)),
+                                             ctypes.byref(e)), "energy_one_trial")
+        return e.value
+
+    def update_particle(self, target, pos, dir, pd0):
+        self._ck(self._L.sc_commit_particle(self._h, int(target), _d(_vec3(pos)), _d(_vec3(dir)), _d(_vec3(pd0))),
+                 "commit_particle")
+
+    def update_box(self, box_new):
+        self._ck(self._L.sc_commit_box(self._h, _d(_vec3(box_new))), "commit_box")
+
+    def energy_per_particle(self):
+        out = np.zeros(self.n)
+        self._ck(self._L.sc_energy_per_particle(self._h, _d(out)), "energy_per_particle")
+        return out
+
+    # ---- overlaps
+    def overlap_flags(self, i, j):
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        out = np.zeros(len(i), np.uint8)
+        dm = np.zeros(len(i))
+        self._ck(self._L.sc_overlap_flags(self._h, len(i), _i(i), _i(j), out.ctypes.data_as(_U8), _d(dm)),
+                 "overlap_flags")
+        return out, dm
+
+    def overlap_count(self):
+        c = ctypes.c_long()
+        self._ck(self._L.sc_overlap_count(self._h, ctypes.byref(c)), "overlap_count")
+        return c.value
+
+    # ---- Monte Carlo
+    def sweep(self, nsweeps, temper, transmx, rot_half, r=2, seed=1, counter=0):
+        st = SweepStats()
+        self._ck(self._L.sc_sweep(self._h, int(nsweeps), float(temper), float(transmx), float(rot_half), int(r),
+                                  int(seed) & 0xFFFFFFFFFFFFFFFF, int(counter) & 0xFFFFFFFFFFFFFFFF,
+                                  ctypes.byref(st)), "sweep")
+        return st.as_dict()
+
+    def volume_move(self, ptype, press, temper, edge_mx, seed, counter):
+        acc = ctypes.c_int()
+        box = np.zeros(3)
+        self._ck(self._L.sc_volume_move(self._h, int(ptype), float(press), float(temper), float(edge_mx),
+                                        int(seed) & 0xFFFFFFFFFFFFFFFF, int(counter) & 0xFFFFFFFFFFFFFFFF,
+                                        ctypes.byref(acc), _d(box)), "volume_move")
+        return bool(acc.value), box
+
+    def last_timing(self):
+        ms, kms = ctypes.c_double(), ctypes.c_double()
+        nl = ctypes.c_long()
+        self._ck(self._L.sc_last_timing(self._h, ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(kms)),
+                 "last_timing")
+        return ms.value, nl.value, kms.value

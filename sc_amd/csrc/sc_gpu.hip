@@ -1,0 +1,1668 @@
+// sc_gpu.hip -- gfx950 kernels and the C ABI (include/sc_gpu.h) of the
+// patchy-spherocylinder Monte Carlo hot path.
+//
+// Device state (one context = one system):
+//   P  uint4  (ux, uy, uz, dense type)  positions as 32-bit fractions of the box
+//   O  float4 (dir.x, dir.y, dir.z, pd0.x)
+//   Q  float2 (pd0.y, pd0.z)
+//   ID int    original (reference) index of the particle in this slot
+// 40 bytes of record + 4 of index per particle, kept sorted by cell after
+// every cell-list build (double buffered), so a cell is a contiguous range.
+// Derived vectors (patch sides, chiral axes, second patch) are recomputed on
+// the fly from dir/pd0 and the per-type constants.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sc_gpu.h"
+#include "sc_host.h"
+#include "sc_pair.h"
+#include "sc_rng.h"
+
+using namespace scg;
+
+typedef PairParamsT<float> PP32;
+typedef PairParamsT<double> PP64;
+typedef TypeParamsT<float> TP32;
+typedef TypeParamsT<double> TP64;
+
+#define WAVE 64
+#define MAX_TYPES 8
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+__device__ __forceinline__ int wave_prefix(unsigned long long mask) {
+    return (int)__popcll(mask & ((1ull << lane_id()) - 1ull));
+}
+
+__device__ __forceinline__ int axis_cell(uint32_t u, uint32_t s, int nc) {
+    return (int)(((uint64_t)(uint32_t)(u - s) * (uint64_t)nc) >> 32);
+}
+
+struct Grid {
+    int nc[3];
+    uint32_t sh[3];
+    int r;   // stencil radius
+};
+
+__device__ __forceinline__ int cell_of_u(const Grid& g, uint32_t ux, uint32_t uy, uint32_t uz) {
+    return axis_cell(ux, g.sh[0], g.nc[0]) + g.nc[0] * (axis_cell(uy, g.sh[1], g.nc[1]) + g.nc[1] * axis_cell(uz, g.sh[2], g.nc[2]));
+}
+
+// particle from its stored record (FP32 or FP64 working precision)
+template <typename R>
+__device__ __forceinline__ DPart<R> load_part(const float4 o, const float2 q, int t, const TypeParamsT<R>* tp) {
+    DPart<R> p;
+    p.dir = mk<R>(o.x, o.y, o.z);
+    p.pd[0] = mk<R>(o.w, q.x, q.y);
+    p.pd[1] = p.pd[0];
+    p.sd[0] = p.sd[1] = p.sd[2] = p.sd[3] = p.pd[0];
+    p.t = t;
+    derive<R>(p, tp[t]);
+    return p;
+}
+
+// ---------------------------------------------------------------- cell list
+__global__ void k_cell_count(int n, const uint4* __restrict__ P, Grid g, int* __restrict__ cell_tmp,
+                             int* __restrict__ counts) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uint4 p = P[k];
+    int c = cell_of_u(g, p.x, p.y, p.z);
+    cell_tmp[k] = c;
+    atomicAdd(&counts[c], 1);
+}
+
+// exclusive scan of counts[0..ncell) into starts[0..ncell], one block of 1024;
+// also the maximum cell occupancy into *maxocc
+__global__ __launch_bounds__(1024) void k_scan(int ncell, const int* __restrict__ counts, int* __restrict__ starts,
+                                                int* __restrict__ maxocc) {
+    __shared__ int part[1024];
+    __shared__ int mx[1024];
+    int t = threadIdx.x;
+    int per = (ncell + 1023) / 1024;
+    int b = t * per, e = min(ncell, b + per);
+    int s = 0, m = 0;
+    for (int i = b; i < e; i++) { s += counts[i]; m = max(m, counts[i]); }
+    part[t] = s;
+    mx[t] = m;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? part[t - off] : 0;
+        int w = (t + off < 1024) ? mx[t + off] : 0;
+        __syncthreads();
+        part[t] += v;
+        mx[t] = max(mx[t], w);
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int i = b; i < e; i++) { starts[i] = run; run += counts[i]; }
+    if (t == 1023) starts[ncell] = part[1023];
+    if (t == 0) *maxocc = mx[0];
+}
+
+__global__ void k_cell_scatter(int n, const int* __restrict__ cell_tmp, const int* __restrict__ starts,
+                               int* __restrict__ fill, int* __restrict__ src_of_pos) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int c = cell_tmp[k];
+    int pos = starts[c] + atomicAdd(&fill[c], 1);
+    src_of_pos[pos] = k;
+}
+
+// order each cell by original index (deterministic, matches a stable CPU sort)
+__global__ void k_cell_sort(int ncell, const int* __restrict__ starts, int* __restrict__ src_of_pos,
+                            const int* __restrict__ ID) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int b = starts[c], e = starts[c + 1];
+    for (int i = b + 1; i < e; i++) {
+        int v = src_of_pos[i], key = ID[v], j = i - 1;
+        while (j >= b && ID[src_of_pos[j]] > key) { src_of_pos[j + 1] = src_of_pos[j]; j--; }
+        src_of_pos[j + 1] = v;
+    }
+}
+
+__global__ void k_permute(int n, const int* __restrict__ src_of_pos, const uint4* __restrict__ Pi,
+                          const float4* __restrict__ Oi, const float2* __restrict__ Qi, const int* __restrict__ IDi,
+                          uint4* __restrict__ Po, float4* __restrict__ Oo, float2* __restrict__ Qo,
+                          int* __restrict__ IDo, int* __restrict__ slot) {
+    int pos = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pos >= n) return;
+    int k = src_of_pos[pos];
+    Po[pos] = Pi[k];
+    Oo[pos] = Oi[k];
+    Qo[pos] = Qi[k];
+    int id = IDi[k];
+    IDo[pos] = id;
+    slot[id] = pos;
+}
+
+// ------------------------------------------------------------------ staging
+// LDS image of the particles of the (2r+1)^3 stencil around a cell.
+struct Stage {
+    uint4* u;
+    float4* o;
+    float2* q;
+    int* id;
+    int n;
+    int h0, h1;   // home cell range inside the stage
+};
+
+// Stencil cells of home cell (cx,cy,cz): offsets -r..r per axis, reduced to
+// distinct cells when an axis has fewer than 2r+1 cells.
+__device__ __forceinline__ int stencil_cell(const Grid& g, int cx, int cy, int cz, int idx, int& is_home) {
+    int w0 = min(2 * g.r + 1, g.nc[0]), w1 = min(2 * g.r + 1, g.nc[1]);
+    int dx = idx % w0, dy = (idx / w0) % w1, dz = idx / (w0 * w1);
+    int x = cx - g.r + dx, y = cy - g.r + dy, z = cz - g.r + dz;
+    x = ((x % g.nc[0]) + g.nc[0]) % g.nc[0];
+    y = ((y % g.nc[1]) + g.nc[1]) % g.nc[1];
+    z = ((z % g.nc[2]) + g.nc[2]) % g.nc[2];
+    is_home = (x == cx && y == cy && z == cz);
+    return x + g.nc[0] * (y + g.nc[1] * z);
+}
+
+__device__ __forceinline__ int stencil_size(const Grid& g) {
+    return min(2 * g.r + 1, g.nc[0]) * min(2 * g.r + 1, g.nc[1]) * min(2 * g.r + 1, g.nc[2]);
+}
+
+// cooperative copy by one wave; scratch holds 2*ns+1 ints. Returns false on overflow.
+__device__ bool stage_load(Stage& S, int cap, const Grid& g, int cx, int cy, int cz, const int* __restrict__ starts,
+                           const uint4* __restrict__ P, const float4* __restrict__ O, const float2* __restrict__ Q,
+                           const int* __restrict__ ID, int* scratch) {
+    int ns = stencil_size(g);
+    int* pre = scratch;          // ns+1 prefix of counts
+    int* cst = scratch + ns + 1; // cell start per stencil entry
+    int lane = lane_id();
+    int home_idx = -1;
+    // counts -> prefix (serial over chunks of 64)
+    int carry = 0;
+    for (int b = 0; b < ns; b += WAVE) {
+        int i = b + lane, cnt = 0, st = 0, hm = 0;
+        if (i < ns) {
+            int c = stencil_cell(g, cx, cy, cz, i, hm);
+            st = starts[c];
+            cnt = starts[c + 1] - st;
+            cst[i] = st;
+        }
+        // inclusive wave scan of cnt
+        int v = cnt;
+        for (int o = 1; o < WAVE; o <<= 1) {
+            int t = __shfl_up(v, o, WAVE);
+            if (lane >= o) v += t;
+        }
+        if (i < ns) pre[i] = carry + v - cnt;
+        unsigned long long hmask = __ballot(i < ns && hm);
+        if (hmask) home_idx = b + __ffsll((long long)hmask) - 1;
+        carry += __shfl(v, WAVE - 1, WAVE);
+    }
+    if (lane == 0) pre[ns] = carry;
+    __syncthreads();
+    int total = carry;
+    S.n = total;
+    S.h0 = pre[home_idx];
+    S.h1 = pre[home_idx + 1];
+    if (total > cap) return false;
+    for (int e = lane; e < total; e += WAVE) {
+        // binary search the stencil entry holding element e
+        int lo = 0, hi = ns - 1;
+        while (lo < hi) {
+            int mid = (lo + hi + 1) >> 1;
+            if (pre[mid] <= e) lo = mid; else hi = mid - 1;
+        }
+        int src = cst[lo] + (e - pre[lo]);
+        S.u[e] = P[src];
+        S.o[e] = O[src];
+        S.q[e] = Q[src];
+        S.id[e] = ID[src];
+    }
+    __syncthreads();
+    return true;
+}
+
+struct SysConst {
+    float L[3];
+    double Ld[3];
+    float sqmaxcut;
+    int nt;
+};
+
+// ------------------------------------------------------------ energy (all)
+// One wave per cell; pairs (i in home cell, j in stencil) with ID_j < ID_i
+// flattened over lanes.  The cheap stage (image, cutoff, segment distance,
+// WCA) runs for every candidate; pairs inside the attraction range are queued
+// in LDS and evaluated 64 at a time so the branch-heavy patch code runs with
+// full lanes.  FP32 pair terms, FP64 accumulation; per-cell partial sums.
+#define QCAP 128
+__global__ __launch_bounds__(WAVE) void k_energy_all(Grid g, SysConst sc, const int* __restrict__ starts,
+                                                     const uint4* __restrict__ P, const float4* __restrict__ O,
+                                                     const float2* __restrict__ Q, const int* __restrict__ ID,
+                                                     const PP32* __restrict__ pp, const TP32* __restrict__ tp,
+                                                     int cap, double* __restrict__ partial, int* __restrict__ flag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int cell = blockIdx.x;
+    int cx = cell % g.nc[0], cy = (cell / g.nc[0]) % g.nc[1], cz = cell / (g.nc[0] * g.nc[1]);
+    Stage S;
+    S.u = (uint4*)smem;
+    S.o = (float4*)(S.u + cap);
+    S.q = (float2*)(S.o + cap);
+    S.id = (int*)(S.q + cap);
+    int* queue = S.id + cap;                 // QCAP entries: (hi<<16)|j packed as two ints
+    int* scratch = queue + 2 * QCAP;         // 2*125+1 ints
+    int lane = lane_id();
+    if (starts[cell + 1] == starts[cell]) {
+        if (lane == 0) partial[cell] = 0.0;
+        return;
+    }
+    if (!stage_load(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
+        if (lane == 0) { atomicOr(flag, 1); partial[cell] = 0.0; }
+        return;
+    }
+    int nh = S.h1 - S.h0, ns = S.n;
+    long total = (long)nh * ns;
+    double acc = 0.0;
+    int qn = 0;   // uniform queue length
+    for (long base = 0; base < total; base += WAVE) {
+        long pidx = base + lane;
+        bool need = false;
+        int hi_ = 0, j = 0;
+        if (pidx < total) {
+            hi_ = (int)(pidx / ns);
+            j = (int)(pidx - (long)hi_ * ns);
+            int i = S.h0 + hi_;
+            if (S.id[j] < S.id[i]) {
+                uint4 a = S.u[i], b = S.u[j];
+                V3<float> r = fx_image<float>(a.x, a.y, a.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
+                float r2 = dot(r, r);
+                if (r2 <= sc.sqmaxcut) {
+                    const PP32& P2 = pp[a.w * sc.nt + b.w];
+                    if (P2.kind <= K_SCA && P2.kind >= 0) {
+                        DPart<float> pa, pb;
+                        pa.dir = mk<float>(S.o[i].x, S.o[i].y, S.o[i].z);
+                        pb.dir = mk<float>(S.o[j].x, S.o[j].y, S.o[j].z);
+                        pa.dir = normalised(pa.dir);
+                        pb.dir = normalised(pb.dir);
+                        float erep, d2;
+                        need = scsc_cheap(P2, pa, pb, r, erep, d2);
+                        acc += (double)erep;
+                    } else if (P2.kind == K_SPH_WCA || P2.kind == K_SPH_TAYLOR) {
+                        acc += (double)sphere_pair(P2, sqrtf(r2));
+                    } else if (P2.kind != K_NONE) {
+                        need = true;   // sphere - spherocylinder: evaluated whole in the queue
+                    }
+                }
+            }
+        }
+        unsigned long long m = __ballot(need);
+        if (need) {
+            int pos = qn + wave_prefix(m);
+            queue[2 * pos] = hi_;
+            queue[2 * pos + 1] = j;
+        }
+        qn += __popcll(m);
+        __syncthreads();
+        bool last = base + WAVE >= total;
+        if (qn >= WAVE || (last && qn > 0)) {
+            // drain whole batches of 64 (everything at the end)
+            int take = last ? qn : (qn / WAVE) * WAVE;
+            for (int e0 = 0; e0 < take; e0 += WAVE) {
+                int e = e0 + lane;
+                if (e < take) {
+                    int i = S.h0 + queue[2 * e], jj = queue[2 * e + 1];
+                    uint4 a = S.u[i], b = S.u[jj];
+                    V3<float> r = fx_image<float>(a.x, a.y, a.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
+                    const PP32& P2 = pp[a.w * sc.nt + b.w];
+                    DPart<float> pa = load_part<float>(S.o[i], S.q[i], a.w, tp);
+                    DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
+                    if (P2.kind <= K_SCA) acc += (double)scsc_attr(P2, pa, pb, r);
+                    else acc += (double)spsc(P2, pa, pb, r);
+                }
+            }
+            __syncthreads();
+            int rest = qn - take;   // < 64
+            int c0 = 0, c1 = 0;
+            if (lane < rest) { c0 = queue[2 * (take + lane)]; c1 = queue[2 * (take + lane) + 1]; }
+            __syncthreads();
+            if (lane < rest) { queue[2 * lane] = c0; queue[2 * lane + 1] = c1; }
+            __syncthreads();
+            qn = rest;
+        }
+    }
+    double s = wave_sum(acc);
+    if (lane == 0) partial[cell] = s;
+}
+
+// deterministic final reduction of per-cell partial sums
+__global__ __launch_bounds__(1024) void k_reduce(int n, const double* __restrict__ v, double* __restrict__ out) {
+    __shared__ double sh[1024];
+    int t = threadIdx.x;
+    double s = 0.0;
+    for (int i = t; i < n; i += 1024) s += v[i];
+    sh[t] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o) sh[t] += sh[t + o];
+        __syncthreads();
+    }
+    if (t == 0) *out = sh[0];
+}
+
+// --------------------------------------------------------- energy (queries)
+// One wave per query particle: its energy with every particle of the stencil
+// around its cell except `exclude` (the particle itself).  The query state is
+// given explicitly (trial states) or taken from a slot.
+struct Query {
+    uint32_t u[3];
+    int t;
+    float4 o;
+    float2 q;
+    int exclude;
+};
+
+__global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __restrict__ qs, const int* __restrict__ qslot,
+                                                       Grid g, SysConst sc, const int* __restrict__ starts,
+                                                       const uint4* __restrict__ P, const float4* __restrict__ O,
+                                                       const float2* __restrict__ Q, const int* __restrict__ ID,
+                                                       const PP32* __restrict__ pp, const TP32* __restrict__ tp,
+                                                       double* __restrict__ out) {
+    int qi = blockIdx.x;
+    if (qi >= nq) return;
+    int lane = lane_id();
+    Query q;
+    if (qslot) {
+        int s = qslot[qi];
+        uint4 p = P[s];
+        q.u[0] = p.x; q.u[1] = p.y; q.u[2] = p.z; q.t = p.w;
+        q.o = O[s]; q.q = Q[s]; q.exclude = ID[s];
+    } else {
+        q = qs[qi];
+    }
+    DPart<float> pa = load_part<float>(q.o, q.q, q.t, tp);
+    int c = cell_of_u(g, q.u[0], q.u[1], q.u[2]);
+    int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
+    int ns = stencil_size(g);
+    double acc = 0.0;
+    for (int si = 0; si < ns; si++) {
+        int hm;
+        int cc = stencil_cell(g, cx, cy, cz, si, hm);
+        int b = starts[cc], e = starts[cc + 1];
+        for (int k = b + lane; k < e; k += WAVE) {
+            if (ID[k] == q.exclude) continue;
+            uint4 pb4 = P[k];
+            V3<float> r = fx_image<float>(q.u[0], q.u[1], q.u[2], pb4.x, pb4.y, pb4.z, sc.L[0], sc.L[1], sc.L[2]);
+            if (dot(r, r) > sc.sqmaxcut) continue;
+            DPart<float> pb = load_part<float>(O[k], Q[k], pb4.w, tp);
+            acc += (double)pair_energy<float>(pp[q.t * sc.nt + pb4.w], pa, pb, r, sc.sqmaxcut);
+        }
+    }
+    double s = wave_sum(acc);
+    if (lane == 0) out[qi] = s;
+}
+
+// ------------------------------------------------------------ pair energies
+template <typename R>
+__global__ void k_pairs(int m, const int* __restrict__ pi, const int* __restrict__ pj, const int* __restrict__ slot,
+                        SysConst sc, const uint4* __restrict__ P, const float4* __restrict__ O,
+                        const float2* __restrict__ Q, const PairParamsT<R>* __restrict__ pp,
+                        const TypeParamsT<R>* __restrict__ tp, R sqmaxcut, double* __restrict__ out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    int a = slot[pi[k]], b = slot[pj[k]];
+    uint4 ua = P[a], ub = P[b];
+    DPart<R> pa = load_part<R>(O[a], Q[a], ua.w, tp);
+    DPart<R> pb = load_part<R>(O[b], Q[b], ub.w, tp);
+    V3<R> r;
+    if (sizeof(R) == 8)
+        r = fx_image<R>(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, (R)sc.Ld[0], (R)sc.Ld[1], (R)sc.Ld[2]);
+    else
+        r = fx_image<R>(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, (R)sc.L[0], (R)sc.L[1], (R)sc.L[2]);
+    out[k] = (double)pair_energy<R>(pp[ua.w * sc.nt + ub.w], pa, pb, r, sqmaxcut);
+}
+
+// ---------------------------------------------------------------- overlaps
+// overlap <=> closest distance < sigma_ij; FP32 first, FP64 when within
+// 1e-6 sigma of contact
+__global__ void k_overlap(int m, const int* __restrict__ pi, const int* __restrict__ pj, const int* __restrict__ slot,
+                          SysConst sc, const uint4* __restrict__ P, const float4* __restrict__ O,
+                          const float2* __restrict__ Q, const PP64* __restrict__ pp64, const TP64* __restrict__ tp64,
+                          unsigned char* __restrict__ out, double* __restrict__ dmin) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    int a = slot[pi[k]], b = slot[pj[k]];
+    uint4 ua = P[a], ub = P[b];
+    const PP64& Pd = pp64[ua.w * sc.nt + ub.w];
+    float sig = (float)(Pd.rcutwca / 1.122462048309373);   // sigma_ij = rcutwca / 2^(1/6)
+    double sigd = Pd.rcutwca / 1.122462048309373;
+    auto dist_of = [&](auto zero) -> double {
+        typedef decltype(zero) R;
+        V3<R> r = fx_image<R>(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, (R)sc.Ld[0], (R)sc.Ld[1], (R)sc.Ld[2]);
+        V3<R> da = normalised(mk<R>(O[a].x, O[a].y, O[a].z)), db = normalised(mk<R>(O[b].x, O[b].y, O[b].z));
+        bool s1 = Pd.g0 >= 30, s2 = Pd.g1 >= 30;
+        if (s1 && s2) return (double)len(r);
+        if (s1 || s2) {
+            R contt;
+            V3<R> dv;
+            R d2 = s1 ? sp_dist<R>(mk<R>(-r.x, -r.y, -r.z), db, (R)Pd.hl1, contt, dv) : sp_dist<R>(r, da, (R)Pd.hl0, contt, dv);
+            return sqrt((double)d2);
+        }
+        V3<R> dv = min_dist(da, db, (R)Pd.hl0, (R)Pd.hl1, r);
+        return sqrt((double)dot(dv, dv));
+    };
+    double d = dist_of(0.0f);
+    if (fabs(d - (double)sig) < 1e-6 * sigd + 1e-5 * sigd) d = dist_of(0.0);
+    out[k] = (d < sigd) ? 1 : 0;
+    if (dmin) dmin[k] = d;
+}
+
+// ------------------------------------------------------------------- sweep
+struct MoveParams {
+    float transmx, rot_half;
+    double temper;
+    uint32_t key0, key1;
+    uint32_t pass_lo, pass_hi;   // global pass counter
+    int ox, oy, oz;              // colour offset
+    int p;                       // colour period (r+1)
+    int nact[3];                 // active cells per axis
+};
+
+__device__ __forceinline__ V3<float> unit_from(float a, float b) {
+    float z = 1.0f - 2.0f * a;
+    float s = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    float ph = 6.283185307179586f * b;
+    return mk<float>(s * cosf(ph), s * sinf(ph), z);
+}
+
+// Energy of a moving particle (as part1) at the old and the new state against
+// every other staged particle, both in one pass over the stage.
+__device__ void trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un, const DPart<float>& po,
+                               const DPart<float>& pn, const SysConst& sc, const PP32* __restrict__ pp,
+                               const TP32* __restrict__ tp, int* queue, double& eo, double& en) {
+    int lane = lane_id();
+    double ao = 0.0, an = 0.0;
+    int qn = 0;
+    int t = uo.w;
+    for (int base = 0; base < S.n; base += WAVE) {
+        int j = base + lane;
+        bool need0 = false, need1 = false;
+        if (j < S.n && j != k) {
+            uint4 b = S.u[j];
+            const PP32& P2 = pp[t * sc.nt + b.w];
+            V3<float> ro = fx_image<float>(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
+            V3<float> rn = fx_image<float>(un.x, un.y, un.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
+            float r2o = dot(ro, ro), r2n = dot(rn, rn);
+            if (r2o <= sc.sqmaxcut || r2n <= sc.sqmaxcut) {
+                DPart<float> pb;
+                pb.dir = normalised(mk<float>(S.o[j].x, S.o[j].y, S.o[j].z));
+                if (P2.kind <= K_SCA && P2.kind >= 0) {
+                    float erep, d2;
+                    if (r2o <= sc.sqmaxcut) { need0 = scsc_cheap(P2, po, pb, ro, erep, d2); ao += (double)erep; }
+                    if (r2n <= sc.sqmaxcut) { need1 = scsc_cheap(P2, pn, pb, rn, erep, d2); an += (double)erep; }
+                } else if (P2.kind == K_SPH_WCA || P2.kind == K_SPH_TAYLOR) {
+                    if (r2o <= sc.sqmaxcut) ao += (double)sphere_pair(P2, sqrtf(r2o));
+                    if (r2n <= sc.sqmaxcut) an += (double)sphere_pair(P2, sqrtf(r2n));
+                } else if (P2.kind != K_NONE) {
+                    need0 = r2o <= sc.sqmaxcut;
+                    need1 = r2n <= sc.sqmaxcut;
+                }
+            }
+        }
+        unsigned long long m0 = __ballot(need0), m1 = __ballot(need1);
+        int c0 = __popcll(m0);
+        if (need0) queue[qn + wave_prefix(m0)] = j;
+        if (need1) queue[qn + c0 + wave_prefix(m1)] = j | (1 << 30);
+        qn += c0 + __popcll(m1);
+        __syncthreads();
+        bool last = base + WAVE >= S.n;
+        if (qn >= WAVE || (last && qn > 0)) {
+            int take = last ? qn : (qn / WAVE) * WAVE;
+            for (int e0 = 0; e0 < take; e0 += WAVE) {
+                int e = e0 + lane;
+                if (e < take) {
+                    int w = queue[e];
+                    int jj = w & 0x3fffffff;
+                    bool nw = (w >> 30) & 1;
+                    const uint4& u1 = nw ? un : uo;
+                    uint4 b = S.u[jj];
+                    V3<float> r = fx_image<float>(u1.x, u1.y, u1.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
+                    const PP32& P2 = pp[t * sc.nt + b.w];
+                    DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
+                    float ev = (P2.kind <= K_SCA) ? scsc_attr(P2, nw ? pn : po, pb, r) : spsc(P2, nw ? pn : po, pb, r);
+                    if (nw) an += (double)ev; else ao += (double)ev;
+                }
+            }
+            __syncthreads();
+            int rest = qn - take;
+            int carry = (lane < rest) ? queue[take + lane] : 0;
+            __syncthreads();
+            if (lane < rest) queue[lane] = carry;
+            __syncthreads();
+            qn = rest;
+        }
+    }
+    eo = wave_sum(ao);
+    en = wave_sum(an);
+}
+
+__global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
+                                                     uint4* __restrict__ P, float4* __restrict__ O,
+                                                     float2* __restrict__ Q, const int* __restrict__ ID,
+                                                     const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
+                                                     unsigned long long* __restrict__ stats, double* __restrict__ de_sum,
+                                                     int* __restrict__ flag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int b = blockIdx.x;
+    int ax = b % mp.nact[0], ay = (b / mp.nact[0]) % mp.nact[1], az = b / (mp.nact[0] * mp.nact[1]);
+    int cx = mp.ox + mp.p * ax, cy = mp.oy + mp.p * ay, cz = mp.oz + mp.p * az;
+    int cell = cx + g.nc[0] * (cy + g.nc[1] * cz);
+    int nh = starts[cell + 1] - starts[cell];
+    if (nh == 0) return;
+    Stage S;
+    S.u = (uint4*)smem;
+    S.o = (float4*)(S.u + cap);
+    S.q = (float2*)(S.o + cap);
+    S.id = (int*)(S.q + cap);
+    int* queue = S.id + cap;                 // 2*cap+64 entries
+    int* scratch = queue + 2 * cap + WAVE;
+    int lane = lane_id();
+    if (!stage_load(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
+        if (lane == 0) atomicOr(flag, 1);
+        return;
+    }
+    const float Lx = sc.L[0], Ly = sc.L[1], Lz = sc.L[2];
+    unsigned long long n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0;
+    double dsum = 0.0;
+    for (int t = 0; t < nh; t++) {
+        U4 ctr;
+        ctr.x = (uint32_t)t;
+        ctr.y = (uint32_t)cell;
+        ctr.z = mp.pass_lo;
+        ctr.w = mp.pass_hi << 2;
+        U4 ra = philox(ctr, mp.key0, mp.key1);
+        int k = S.h0 + (int)(((uint64_t)ra.x * (uint64_t)nh) >> 32);
+        uint4 uo = S.u[k];
+        float4 oo = S.o[k];
+        float2 qo = S.q[k];
+        int g0 = tp[uo.w].geotype;
+        bool translate = (u01f(ra.y) < 0.5f) || (g0 >= 30);
+        uint4 un = uo;
+        float4 on = oo;
+        float2 qn = qo;
+        if (translate) {
+            n_ttry++;
+            V3<float> d = unit_from(u01f(ra.z), u01f(ra.w));
+            int32_t dx = (int32_t)rintf(d.x * mp.transmx * (4294967296.0f / Lx));
+            int32_t dy = (int32_t)rintf(d.y * mp.transmx * (4294967296.0f / Ly));
+            int32_t dz = (int32_t)rintf(d.z * mp.transmx * (4294967296.0f / Lz));
+            un.x = uo.x + (uint32_t)dx;
+            un.y = uo.y + (uint32_t)dy;
+            un.z = uo.z + (uint32_t)dz;
+            if (cell_of_u(g, un.x, un.y, un.z) != cell) {   // leaving the active cell: rejected
+                n_left++;
+                continue;
+            }
+        } else {
+            n_rtry++;
+            ctr.w = (mp.pass_hi << 2) | 1u;
+            U4 rb = philox(ctr, mp.key0, mp.key1);
+            V3<float> axv = unit_from(u01f(rb.x), u01f(rb.y));
+            float ang = mp.rot_half * u01f(rb.z);
+            float vc = cosf(ang), vs = sqrtf(fmaxf(0.0f, 1.0f - vc * vc));
+            if (u01f(rb.w) >= 0.5f) vs = -vs;   // Particle::pscRotate direction choice (particle.h:194-196)
+            V3<float> d = qrot(mk<float>(oo.x, oo.y, oo.z), axv, vc, vs);
+            V3<float> pq = qrot(mk<float>(oo.w, qo.x, qo.y), axv, vc, vs);
+            d = normalised(d);
+            pq = normalised(perp(pq, d));
+            on = make_float4(d.x, d.y, d.z, pq.x);
+            qn = make_float2(pq.y, pq.z);
+        }
+        DPart<float> po = load_part<float>(oo, qo, uo.w, tp);
+        DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
+        double eo, en;
+        trial_energies(S, k, uo, un, po, pn, sc, pp, tp, queue, eo, en);
+        double de = en - eo;
+        bool acc = de <= 0.0;
+        if (!acc) {
+            ctr.w = (mp.pass_hi << 2) | 2u;
+            U4 rc = philox(ctr, mp.key0, mp.key1);
+            acc = exp(-de / mp.temper) > u01d(rc.x, rc.y);     // MoveCreator::moveTry, movecreator.h:175-187
+        }
+        if (acc) {
+            if (translate) n_tacc++; else n_racc++;
+            dsum += de;
+            if (lane == 0) {
+                S.u[k] = un;
+                S.o[k] = on;
+                S.q[k] = qn;
+            }
+        }
+        __syncthreads();
+    }
+    // write the home cell back (same slots it was staged from)
+    int gs = starts[cell];
+    for (int e = lane; e < nh; e += WAVE) {
+        P[gs + e] = S.u[S.h0 + e];
+        O[gs + e] = S.o[S.h0 + e];
+        Q[gs + e] = S.q[S.h0 + e];
+    }
+    if (lane == 0) {
+        atomicAdd(&stats[0], n_tacc);
+        atomicAdd(&stats[1], n_ttry);
+        atomicAdd(&stats[2], n_racc);
+        atomicAdd(&stats[3], n_rtry);
+        atomicAdd(&stats[4], n_left);
+        atomicAdd(de_sum, dsum);
+    }
+}
+
+// ------------------------------------------------------------------ context
+struct sc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    schost::Topology topo;
+    bool have_topo = false;
+    schost::Options opts;
+    bool have_opts = false;
+    int ref2dense[schost::MAXT];
+    std::vector<int> dense2ref;
+    int nt = 0;
+    PP32* d_pp32 = nullptr;
+    PP64* d_pp64 = nullptr;
+    TP32* d_tp32 = nullptr;
+    TP64* d_tp64 = nullptr;
+    double maxcut = 0, sqmaxcut = 0;
+    double minvol = 1.0;
+    // state
+    int n = 0;
+    double box[3] = {0, 0, 0};
+    bool have_state = false;
+    std::vector<int> ptypes;   // reference types, original order
+    uint4* P[2] = {nullptr, nullptr};
+    float4* O[2] = {nullptr, nullptr};
+    float2* Q[2] = {nullptr, nullptr};
+    int* ID[2] = {nullptr, nullptr};
+    int cur = 0;
+    int* slot = nullptr;
+    int* cell_tmp = nullptr;
+    int* src_of_pos = nullptr;
+    // grid
+    Grid g;
+    int ncell = 0, ncell_alloc = 0;
+    int* starts = nullptr;
+    int* counts = nullptr;
+    int* fill = nullptr;
+    double* partial = nullptr;
+    bool grid_valid = false;
+    // scratch
+    double* d_out = nullptr;
+    int d_out_cap = 0;
+    int* d_flag = nullptr;
+    int* d_maxocc = nullptr;
+    unsigned long long* d_stats = nullptr;
+    double* d_dsum = nullptr;
+    Query* d_q = nullptr;
+    int* d_i = nullptr;
+    int* d_j = nullptr;
+    int ij_cap = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0, last_kernel_ms = 0;
+    long last_launches = 0;
+};
+
+#define HIPCK(x)                                                                       \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess) {                                                        \
+            ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);                 \
+            return SC_EHIP;                                                            \
+        }                                                                              \
+    } while (0)
+
+static int fail(sc_ctx* ctx, int code, const std::string& m) {
+    ctx->err = m;
+    return code;
+}
+
+template <typename T>
+static int dalloc(sc_ctx* ctx, T** p, size_t count) {
+    if (*p) { hipFree(*p); *p = nullptr; }
+    if (count == 0) count = 1;
+    HIPCK(hipMalloc((void**)p, count * sizeof(T)));
+    return SC_OK;
+}
+
+// --- parameter tables -------------------------------------------------------
+template <typename R>
+static void fill_pair(PairParamsT<R>& o, const schost::Ia& p, const schost::Ia& tr) {
+    memset(&o, 0, sizeof(o));
+    int g = p.geotype[0], h = p.geotype[1];
+    auto psc = [](int x) { return x == 12 || x == 14 || x == 16 || x == 18; };
+    auto cpsc = [](int x) { return x == 13 || x == 15 || x == 17 || x == 19; };
+    // functor table of PairE::initIntFCE (paire.cpp:6-80), later rules win
+    int kind = K_NONE;
+    if ((cpsc(g) && psc(h)) || (psc(g) && cpsc(h))) kind = K_MIXED;
+    if (cpsc(g) && cpsc(h)) kind = K_CPSC;
+    if (psc(g) && psc(h)) kind = K_PSC;
+    if (g == 10 && h == 10) kind = K_SCN;
+    if (g == 11 && h == 11) kind = K_SCA;
+    if (g == 30 || h == 30) kind = K_SPH_WCA;
+    if (g == 31 && h == 31) kind = K_SPH_TAYLOR;
+    if ((g == 11 && h == 31) || (g == 31 && h == 11)) kind = K_SCASPA;
+    if ((psc(g) && (h == 31 || h == 30)) || ((g == 31 || g == 30) && psc(h))) kind = K_PSCSPA;
+    if ((cpsc(g) && (h == 31 || h == 30)) || ((g == 31 || g == 30) && cpsc(h))) kind = K_CPSCSPA;
+    o.kind = kind;
+    o.g0 = g;
+    o.g1 = h;
+    o.exclude = p.exclude ? 1 : 0;
+    o.fT = is_two(g);
+    o.sT = is_two(h);
+    o.isp1 = g < 30;
+    const schost::Ia& s = o.isp1 ? p : tr;   // iaParam of MixSpSc (SC first)
+    int sg = s.geotype[0];
+    if (kind == K_PSCSPA) { o.sp_chiral = sg == 14 || sg == 18; o.sp_sec = sg == 16 || sg == 18; }
+    if (kind == K_CPSCSPA) { o.sp_chiral = sg == 15 || sg == 19; o.sp_sec = sg == 17 || sg == 19; }
+    o.hl0 = (R)p.half_len[0];
+    o.hl1 = (R)p.half_len[1];
+    o.eps = (R)p.epsilon; o.A = (R)p.A; o.B = (R)p.B; o.pdis = (R)p.pdis; o.pswitch = (R)p.pswitch;
+    o.pswitchINV = (R)p.pswitchINV; o.rcut = (R)p.rcut; o.rcutSq = (R)p.rcutSq; o.rcutwca = (R)p.rcutwca;
+    o.rcutwcaSq = (R)p.rcutwcaSq;
+    for (int k = 0; k < 4; k++) { o.pcangl[k] = (R)p.pcangl[k]; o.pcanglsw[k] = (R)p.pcanglsw[k]; }
+    o.parallel = (R)p.parallel;
+    o.t_hl0 = (R)tr.half_len[0]; o.t_eps = (R)tr.epsilon; o.t_pdis = (R)tr.pdis; o.t_pswitch = (R)tr.pswitch;
+    o.t_rcut = (R)tr.rcut; o.t_rcutSq = (R)tr.rcutSq; o.t_rcutwcaSq = (R)tr.rcutwcaSq;
+    o.t_pcangl0 = (R)tr.pcangl[0]; o.t_pcanglsw0 = (R)tr.pcanglsw[0]; o.t_exclude = tr.exclude ? 1 : 0;
+}
+
+template <typename R>
+static void fill_type(TypeParamsT<R>& o, const schost::Ia& d) {
+    memset(&o, 0, sizeof(o));
+    o.geotype = d.geotype[0];
+    o.pcoshalfi0 = (R)d.pcoshalfi[0]; o.psinhalfi0 = (R)d.psinhalfi[0];
+    o.pcoshalfi2 = (R)d.pcoshalfi[2]; o.psinhalfi2 = (R)d.psinhalfi[2];
+    o.csec = (R)d.csecpatchrot[0]; o.ssec = (R)d.ssecpatchrot[0];
+    o.chcos = (R)d.chiral_cos[0]; o.chsin = (R)d.chiral_sin[0];
+}
+
+static int upload_tables(sc_ctx* ctx) {
+    // dense types: those referenced by the system, in increasing type number
+    for (int i = 0; i < schost::MAXT; i++) ctx->ref2dense[i] = -1;
+    ctx->dense2ref.clear();
+    std::vector<int> used(schost::MAXT, 0);
+    for (int t : ctx->ptypes) if (t >= 0 && t < schost::MAXT) used[t] = 1;
+    for (int t = 0; t < schost::MAXT; t++)
+        if (used[t]) {
+            if (ctx->topo.ia[t][t].geotype[0] == 0) return fail(ctx, SC_EINVAL, "particle type " + std::to_string(t) + " is not defined in [Types]");
+            ctx->ref2dense[t] = (int)ctx->dense2ref.size();
+            ctx->dense2ref.push_back(t);
+        }
+    ctx->nt = (int)ctx->dense2ref.size();
+    if (ctx->nt > MAX_TYPES) return fail(ctx, SC_ECAPACITY, "more than 8 particle types in one system");
+    int nt = ctx->nt;
+    std::vector<PP32> h32(nt * nt);
+    std::vector<PP64> h64(nt * nt);
+    std::vector<TP32> t32(nt);
+    std::vector<TP64> t64(nt);
+    ctx->minvol = 1e30;
+    for (int a = 0; a < nt; a++) {
+        const schost::Ia& d = ctx->topo.ia[ctx->dense2ref[a]][ctx->dense2ref[a]];
+        fill_type(t32[a], d);
+        fill_type(t64[a], d);
+        double geomv = 3.141592653589793 * pow(d.sigma, 3) / 6.0 + 3.141592653589793 * d.sigma * d.sigma * d.len[0] / 4.0;
+        ctx->minvol = std::min(ctx->minvol, geomv > 0 ? geomv : 0.5);
+        for (int b = 0; b < nt; b++) {
+            const schost::Ia& p = ctx->topo.ia[ctx->dense2ref[a]][ctx->dense2ref[b]];
+            const schost::Ia& tr = ctx->topo.ia[ctx->dense2ref[b]][ctx->dense2ref[a]];
+            fill_pair(h32[a * nt + b], p, tr);
+            fill_pair(h64[a * nt + b], p, tr);
+        }
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_pp32, h32.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_pp64, h64.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_tp32, t32.size()))) return rc;
+    if ((rc = dalloc(ctx, &ctx->d_tp64, t64.size()))) return rc;
+    HIPCK(hipMemcpy(ctx->d_pp32, h32.data(), h32.size() * sizeof(PP32), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->d_pp64, h64.data(), h64.size() * sizeof(PP64), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->d_tp32, t32.data(), t32.size() * sizeof(TP32), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->d_tp64, t64.data(), t64.size() * sizeof(TP64), hipMemcpyHostToDevice));
+    ctx->maxcut = ctx->topo.maxcut;
+    ctx->sqmaxcut = ctx->topo.sqmaxcut;
+    return SC_OK;
+}
+
+static SysConst sysconst(const sc_ctx* ctx, const double* box) {
+    SysConst s;
+    for (int k = 0; k < 3; k++) { s.L[k] = (float)box[k]; s.Ld[k] = box[k]; }
+    s.sqmaxcut = (float)ctx->sqmaxcut;
+    s.nt = ctx->nt;
+    return s;
+}
+
+// --- cell list --------------------------------------------------------------
+static int build_grid(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box) {
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (r < 1 || r > 2) return fail(ctx, SC_EINVAL, "stencil radius must be 1 or 2");
+    int p = r + 1;
+    double wmin = ctx->maxcut / r;
+    Grid g;
+    g.r = r;
+    for (int k = 0; k < 3; k++) {
+        int nc = (int)floor(box[k] / wmin);
+        if (nc >= p) nc = (nc / p) * p;   // a multiple of the colour period
+        else nc = 1;                      // too small for a checkerboard: one cell
+        if (nc < 1) nc = 1;
+        g.nc[k] = nc;
+        g.sh[k] = shift ? shift[k] : 0u;
+    }
+    int ncell = g.nc[0] * g.nc[1] * g.nc[2];
+    if (ncell > ctx->ncell_alloc) {
+        int rc;
+        if ((rc = dalloc(ctx, &ctx->starts, ncell + 1))) return rc;
+        if ((rc = dalloc(ctx, &ctx->counts, ncell))) return rc;
+        if ((rc = dalloc(ctx, &ctx->fill, ncell))) return rc;
+        if ((rc = dalloc(ctx, &ctx->partial, ncell))) return rc;
+        ctx->ncell_alloc = ncell;
+    }
+    ctx->g = g;
+    ctx->ncell = ncell;
+    int n = ctx->n, c = ctx->cur, nb = (n + 255) / 256;
+    HIPCK(hipMemsetAsync(ctx->counts, 0, sizeof(int) * ncell, ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->fill, 0, sizeof(int) * ncell, ctx->stream));
+    if (n > 0) hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->P[c], g, ctx->cell_tmp, ctx->counts);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, ncell, ctx->counts, ctx->starts, ctx->d_maxocc);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->cell_tmp, ctx->starts, ctx->fill, ctx->src_of_pos);
+        hipLaunchKernelGGL(k_cell_sort, dim3((ncell + 255) / 256), dim3(256), 0, ctx->stream, ncell, ctx->starts, ctx->src_of_pos, ctx->ID[c]);
+        hipLaunchKernelGGL(k_permute, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->src_of_pos, ctx->P[c], ctx->O[c], ctx->Q[c],
+                           ctx->ID[c], ctx->P[c ^ 1], ctx->O[c ^ 1], ctx->Q[c ^ 1], ctx->ID[c ^ 1], ctx->slot);
+    }
+    HIPCK(hipGetLastError());
+    ctx->cur ^= 1;
+    ctx->grid_valid = true;
+    return SC_OK;
+}
+
+// stage capacity for a stencil: ~3x the mean occupancy (local packing cannot
+// exceed ~2.6x the mean at the fluid densities used), bounded by the system
+static int stage_cap(const sc_ctx* ctx, const Grid& g, const double* box) {
+    double cellvol = (box[0] / g.nc[0]) * (box[1] / g.nc[1]) * (box[2] / g.nc[2]);
+    int ns = std::min(2 * g.r + 1, g.nc[0]) * std::min(2 * g.r + 1, g.nc[1]) * std::min(2 * g.r + 1, g.nc[2]);
+    double mean = (double)ctx->n / (double)(g.nc[0] * g.nc[1] * g.nc[2]) * ns;
+    double pack = 0.74 * cellvol * ns / ctx->minvol;   // close packing of the stencil volume
+    int cap = (int)std::min(std::max(3.0 * mean + 64.0, 128.0), pack + 64.0);
+    cap = std::max(cap, 64);
+    cap = std::min(cap, ctx->n + 64);
+    cap = std::min(cap, 1152);   // keeps a wave's LDS image under 64 KiB
+    return ((cap + 63) / 64) * 64;
+}
+
+static size_t stage_bytes(int cap) { return (size_t)cap * (16 + 16 + 8 + 4); }
+
+static int check_flag(sc_ctx* ctx, const char* what) {
+    int f = 0;
+    HIPCK(hipMemcpyAsync(&f, ctx->d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (f) {
+        HIPCK(hipMemset(ctx->d_flag, 0, sizeof(int)));
+        return fail(ctx, SC_ECAPACITY, std::string(what) + ": a stencil held more particles than the stage capacity");
+    }
+    return SC_OK;
+}
+
+static int ensure_out(sc_ctx* ctx, int m) {
+    if (m > ctx->d_out_cap) {
+        int rc;
+        if ((rc = dalloc(ctx, &ctx->d_out, m))) return rc;
+        ctx->d_out_cap = m;
+    }
+    return SC_OK;
+}
+
+static int ensure_ij(sc_ctx* ctx, int m) {
+    if (m > ctx->ij_cap) {
+        int rc;
+        if ((rc = dalloc(ctx, &ctx->d_i, m))) return rc;
+        if ((rc = dalloc(ctx, &ctx->d_j, m))) return rc;
+        if ((rc = dalloc(ctx, &ctx->d_q, m))) return rc;
+        ctx->ij_cap = m;
+    }
+    return SC_OK;
+}
+
+static int energy_all_impl(sc_ctx* ctx, const double* box, double* e) {
+    int rc = build_grid(ctx, 2, nullptr, box);
+    if (rc) return rc;
+    const Grid& g = ctx->g;
+    int cap = stage_cap(ctx, g, box);
+    size_t lds = stage_bytes(cap) + sizeof(int) * (2 * QCAP) + sizeof(int) * (2 * 125 + 2);
+    if ((rc = ensure_out(ctx, 1))) return rc;
+    SysConst sc = sysconst(ctx, box);
+    int c = ctx->cur;
+    HIPCK(hipEventRecord(ctx->ev0, ctx->stream));
+    hipLaunchKernelGGL(k_energy_all, dim3(ctx->ncell), dim3(WAVE), lds, ctx->stream, g, sc, ctx->starts, ctx->P[c],
+                       ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->partial, ctx->d_flag);
+    HIPCK(hipEventRecord(ctx->ev1, ctx->stream));
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, ctx->stream, ctx->ncell, ctx->partial, ctx->d_out);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(e, ctx->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = check_flag(ctx, "energy_all"))) return rc;
+    float ms = 0;
+    HIPCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    ctx->last_kernel_ms = ms;
+    ctx->last_launches = 1;
+    return SC_OK;
+}
+
+// ------------------------------------------------------------------ the ABI
+extern "C" {
+
+int sc_create(sc_ctx** out, int device) {
+    if (!out) return SC_EINVAL;
+    *out = nullptr;
+    if (device == -1) {
+        // host-only context: input formats and topology queries; every
+        // device call on it returns SC_ENODEV
+        sc_ctx* ctx = new sc_ctx();
+        ctx->device = -1;
+        *out = ctx;
+        return SC_OK;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) return SC_ENODEV;
+    sc_ctx* ctx = new sc_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return SC_ENODEV;
+    }
+    hipEventCreate(&ctx->ev0);
+    hipEventCreate(&ctx->ev1);
+    if (hipMalloc((void**)&ctx->d_flag, sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_maxocc, sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_stats, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_dsum, sizeof(double)) != hipSuccess) {
+        delete ctx;
+        return SC_EHIP;
+    }
+    hipMemset(ctx->d_flag, 0, sizeof(int));
+    *out = ctx;
+    return SC_OK;
+}
+
+void sc_destroy(sc_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->device < 0) { delete ctx; return; }
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    void* ptrs[] = {ctx->d_pp32, ctx->d_pp64, ctx->d_tp32, ctx->d_tp64, ctx->P[0], ctx->P[1], ctx->O[0], ctx->O[1],
+                    ctx->Q[0], ctx->Q[1], ctx->ID[0], ctx->ID[1], ctx->slot, ctx->cell_tmp, ctx->src_of_pos,
+                    ctx->starts, ctx->counts, ctx->fill, ctx->partial, ctx->d_out, ctx->d_flag, ctx->d_maxocc,
+                    ctx->d_stats, ctx->d_dsum, ctx->d_q, ctx->d_i, ctx->d_j};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* sc_last_error(const sc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sc_load_topology(sc_ctx* ctx, const char* text) {
+    if (!ctx || !text) return SC_EINVAL;
+    std::string e = ctx->topo.parse(text);
+    if (!e.empty()) { ctx->have_topo = false; return fail(ctx, SC_EINVAL, e); }
+    ctx->have_topo = true;
+    ctx->have_state = false;
+    ctx->ptypes = ctx->topo.particleTypes();
+    return SC_OK;
+}
+
+int sc_load_options(sc_ctx* ctx, const char* text) {
+    if (!ctx || !text) return SC_EINVAL;
+    std::string e = ctx->opts.parse(text);
+    if (!e.empty()) return fail(ctx, SC_EINVAL, e);
+    ctx->have_opts = true;
+    return SC_OK;
+}
+
+int sc_get_option(const sc_ctx* ctx, const char* key, double* v) {
+    if (!ctx || !key || !v) return SC_EINVAL;
+    return ctx->opts.get(key, *v) ? SC_OK : SC_EINVAL;
+}
+
+int sc_num_particles(const sc_ctx* ctx) { return ctx ? (int)ctx->ptypes.size() : 0; }
+
+int sc_particle_types(const sc_ctx* ctx, int32_t* types, int cap) {
+    if (!ctx) return SC_EINVAL;
+    int n = (int)ctx->ptypes.size();
+    for (int i = 0; i < n && i < cap; i++) types[i] = ctx->ptypes[i];
+    return n;
+}
+
+int sc_maxcut(const sc_ctx* ctx, double* m) {
+    if (!ctx || !m || !ctx->have_topo) return SC_ESTATE;
+    *m = ctx->topo.maxcut;
+    return SC_OK;
+}
+
+int sc_pair_params(const sc_ctx* ctx, int ti, int tj, double* out) {
+    if (!ctx || !out || !ctx->have_topo || ti < 0 || tj < 0 || ti >= schost::MAXT || tj >= schost::MAXT) return SC_EINVAL;
+    schost::iaRow(ctx->topo.ia[ti][tj], out);
+    return SC_OK;
+}
+
+int sc_set_config(sc_ctx* ctx, int n, const double box[3], const double* pos, const double* dir, const double* pd0,
+                  const int32_t* types) {
+    if (!ctx || n < 0 || !box || (n > 0 && (!pos || !dir || !pd0))) return SC_EINVAL;
+    if (!ctx->have_topo) return fail(ctx, SC_ESTATE, "load a topology first");
+    if (ctx->device < 0) return fail(ctx, SC_ENODEV, "host-only context: no device state");
+    if (n >= (1 << 24)) return fail(ctx, SC_ECAPACITY, "too many particles");
+    HIPCK(hipSetDevice(ctx->device));
+    if (types) ctx->ptypes.assign(types, types + n);
+    if ((int)ctx->ptypes.size() != n) return fail(ctx, SC_EINVAL, "particle count differs from the topology's [System]");
+    int rc;
+    if ((rc = upload_tables(ctx))) return rc;
+    for (int k = 0; k < 3; k++) {
+        if (!(box[k] > 0)) return fail(ctx, SC_EINVAL, "box edges must be positive");
+        ctx->box[k] = box[k];
+    }
+    if (n != ctx->n || !ctx->P[0]) {
+        for (int b = 0; b < 2; b++) {
+            if ((rc = dalloc(ctx, &ctx->P[b], n))) return rc;
+            if ((rc = dalloc(ctx, &ctx->O[b], n))) return rc;
+            if ((rc = dalloc(ctx, &ctx->Q[b], n))) return rc;
+            if ((rc = dalloc(ctx, &ctx->ID[b], n))) return rc;
+        }
+        if ((rc = dalloc(ctx, &ctx->slot, n))) return rc;
+        if ((rc = dalloc(ctx, &ctx->cell_tmp, n))) return rc;
+        if ((rc = dalloc(ctx, &ctx->src_of_pos, n))) return rc;
+    }
+    ctx->n = n;
+    std::vector<uint4> hp(n);
+    std::vector<float4> ho(n);
+    std::vector<float2> hq(n);
+    std::vector<int> hid(n);
+    for (int i = 0; i < n; i++) {
+        uint32_t u[3];
+        for (int k = 0; k < 3; k++) {
+            double s = pos[3 * i + k];
+            s -= floor(s);
+            double v = nearbyint(s * 4294967296.0);
+            u[k] = (uint32_t)(uint64_t)(v >= 4294967296.0 ? 0.0 : v);
+        }
+        // Inicializer::initConfig normalisation (inicializer.cpp:209-225)
+        double d[3] = {dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]}, q[3] = {pd0[3 * i], pd0[3 * i + 1], pd0[3 * i + 2]};
+        double t = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        if (t != 0.0) { t = 1.0 / t; d[0] *= t; d[1] *= t; d[2] *= t; }
+        double dp = q[0] * d[0] + q[1] * d[1] + q[2] * d[2];
+        q[0] -= dp * d[0]; q[1] -= dp * d[1]; q[2] -= dp * d[2];
+        t = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]);
+        if (t != 0.0) { t = 1.0 / t; q[0] *= t; q[1] *= t; q[2] *= t; }
+        int dt = ctx->ref2dense[ctx->ptypes[i]];
+        hp[i] = make_uint4(u[0], u[1], u[2], (uint32_t)dt);
+        ho[i] = make_float4((float)d[0], (float)d[1], (float)d[2], (float)q[0]);
+        hq[i] = make_float2((float)q[1], (float)q[2]);
+        hid[i] = i;
+    }
+    ctx->cur = 0;
+    if (n > 0) {
+        HIPCK(hipMemcpy(ctx->P[0], hp.data(), n * sizeof(uint4), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->O[0], ho.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->Q[0], hq.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->ID[0], hid.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->slot, hid.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    }
+    ctx->have_state = true;
+    ctx->grid_valid = false;
+    return build_grid(ctx, 2, nullptr, ctx->box);
+}
+
+int sc_load_config(sc_ctx* ctx, const char* text) {
+    if (!ctx || !text) return SC_EINVAL;
+    if (!ctx->have_topo) return fail(ctx, SC_ESTATE, "load a topology first");
+    int n = (int)ctx->ptypes.size();
+    double box[3];
+    std::vector<double> pos, dir, pd0;
+    std::string e = schost::parseConfig(text, n, box, pos, dir, pd0);
+    if (!e.empty()) return fail(ctx, SC_EINVAL, e);
+    std::vector<int32_t> ty(ctx->ptypes.begin(), ctx->ptypes.end());
+    return sc_set_config(ctx, n, box, pos.data(), dir.data(), pd0.data(), ty.data());
+}
+
+static int download(sc_ctx* ctx, std::vector<uint4>& hp, std::vector<float4>& ho, std::vector<float2>& hq) {
+    int n = ctx->n, c = ctx->cur;
+    std::vector<int> sl(n);
+    std::vector<uint4> sp(n);
+    std::vector<float4> so(n);
+    std::vector<float2> sq(n);
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (n > 0) {
+        HIPCK(hipMemcpy(sl.data(), ctx->slot, n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(sp.data(), ctx->P[c], n * sizeof(uint4), hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(so.data(), ctx->O[c], n * sizeof(float4), hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(sq.data(), ctx->Q[c], n * sizeof(float2), hipMemcpyDeviceToHost));
+    }
+    hp.resize(n); ho.resize(n); hq.resize(n);
+    for (int i = 0; i < n; i++) { hp[i] = sp[sl[i]]; ho[i] = so[sl[i]]; hq[i] = sq[sl[i]]; }
+    return SC_OK;
+}
+
+int sc_get_config(sc_ctx* ctx, double box[3], double* pos, double* dir, double* pd0) {
+    if (!ctx) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    std::vector<uint4> hp; std::vector<float4> ho; std::vector<float2> hq;
+    int rc = download(ctx, hp, ho, hq);
+    if (rc) return rc;
+    if (box) for (int k = 0; k < 3; k++) box[k] = ctx->box[k];
+    const double s = 1.0 / 4294967296.0;
+    for (int i = 0; i < ctx->n; i++) {
+        if (pos) { pos[3 * i] = hp[i].x * s; pos[3 * i + 1] = hp[i].y * s; pos[3 * i + 2] = hp[i].z * s; }
+        if (dir) { dir[3 * i] = ho[i].x; dir[3 * i + 1] = ho[i].y; dir[3 * i + 2] = ho[i].z; }
+        if (pd0) { pd0[3 * i] = ho[i].w; pd0[3 * i + 1] = hq[i].x; pd0[3 * i + 2] = hq[i].y; }
+    }
+    return SC_OK;
+}
+
+int sc_get_fixed_positions(sc_ctx* ctx, uint32_t* u) {
+    if (!ctx || !u) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    std::vector<uint4> hp; std::vector<float4> ho; std::vector<float2> hq;
+    int rc = download(ctx, hp, ho, hq);
+    if (rc) return rc;
+    for (int i = 0; i < ctx->n; i++) { u[3 * i] = hp[i].x; u[3 * i + 1] = hp[i].y; u[3 * i + 2] = hp[i].z; }
+    return SC_OK;
+}
+
+long sc_write_config(sc_ctx* ctx, char* buf, long cap) {
+    if (!ctx) return SC_EINVAL;
+    int n = ctx->n;
+    std::vector<double> pos(3 * n), dir(3 * n), pd0(3 * n);
+    double box[3];
+    int rc = sc_get_config(ctx, box, pos.data(), dir.data(), pd0.data());
+    if (rc) return rc;
+    std::vector<int> mol = ctx->topo.particleMolecules();
+    std::string s = schost::drawConfig(box, pos, dir, pd0, mol);
+    if (buf && cap > (long)s.size()) memcpy(buf, s.c_str(), s.size() + 1);
+    return (long)s.size();
+}
+
+int sc_build_cells(sc_ctx* ctx, int r, const uint32_t shift[3], int32_t* cell_of, int32_t* cell_start,
+                   int32_t* sorted, int32_t ncell_out[3]) {
+    if (!ctx) return SC_EINVAL;
+    HIPCK(hipSetDevice(ctx->device));
+    int rc = build_grid(ctx, r, shift, ctx->box);
+    if (rc) return rc;
+    int n = ctx->n, nc = ctx->ncell, c = ctx->cur;
+    if (ncell_out) for (int k = 0; k < 3; k++) ncell_out[k] = ctx->g.nc[k];
+    std::vector<int> st(nc + 1), ids(n);
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    HIPCK(hipMemcpy(st.data(), ctx->starts, (nc + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    if (n > 0) HIPCK(hipMemcpy(ids.data(), ctx->ID[c], n * sizeof(int), hipMemcpyDeviceToHost));
+    if (cell_start) for (int i = 0; i <= nc; i++) cell_start[i] = st[i];
+    if (sorted) for (int i = 0; i < n; i++) sorted[i] = ids[i];
+    if (cell_of)
+        for (int cc = 0; cc < nc; cc++)
+            for (int k = st[cc]; k < st[cc + 1]; k++) cell_of[ids[k]] = cc;
+    return SC_OK;
+}
+
+int sc_neighbor_pairs(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* count) {
+    if (!ctx || !count) return SC_EINVAL;
+    // Host-side enumeration over the device cell list would need a device
+    // pair-compaction kernel; the list is built from the fixed-point state in
+    // FP64 with the same arithmetic as the CPU brute force.
+    int n = ctx->n;
+    std::vector<uint32_t> u(3 * (size_t)n);
+    int r = sc_get_fixed_positions(ctx, u.data());
+    if (r) return r;
+    const double s32 = 1.0 / 4294967296.0, rc2 = rc * rc;
+    long cnt = 0;
+    for (int i = 0; i < n; i++)
+        for (int j = i + 1; j < n; j++) {
+            double dx = ctx->box[0] * ((double)(int32_t)(u[3 * i] - u[3 * j]) * s32);
+            double dy = ctx->box[1] * ((double)(int32_t)(u[3 * i + 1] - u[3 * j + 1]) * s32);
+            double dz = ctx->box[2] * ((double)(int32_t)(u[3 * i + 2] - u[3 * j + 2]) * s32);
+            double r2 = dx * dx;
+            r2 = r2 + dy * dy;
+            r2 = r2 + dz * dz;
+            if (r2 <= rc2) {
+                if (pairs && cnt < cap) { pairs[2 * cnt] = i; pairs[2 * cnt + 1] = j; }
+                cnt++;
+            }
+        }
+    *count = cnt;
+    return SC_OK;
+}
+
+int sc_pair_energies(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64) {
+    if (!ctx || m < 0 || (m > 0 && (!i || !j || !out))) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    for (int k = 0; k < m; k++)
+        if (i[k] < 0 || i[k] >= ctx->n || j[k] < 0 || j[k] >= ctx->n) return fail(ctx, SC_EINVAL, "particle index out of range");
+    if (m == 0) return SC_OK;
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure_ij(ctx, m)) || (rc = ensure_out(ctx, m))) return rc;
+    HIPCK(hipMemcpyAsync(ctx->d_i, i, m * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(ctx->d_j, j, m * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    SysConst sc = sysconst(ctx, ctx->box);
+    int c = ctx->cur;
+    if (fp64)
+        hipLaunchKernelGGL(k_pairs<double>, dim3((m + 127) / 128), dim3(128), 0, ctx->stream, m, ctx->d_i, ctx->d_j,
+                           ctx->slot, sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp64, ctx->d_tp64, ctx->sqmaxcut, ctx->d_out);
+    else
+        hipLaunchKernelGGL(k_pairs<float>, dim3((m + 127) / 128), dim3(128), 0, ctx->stream, m, ctx->d_i, ctx->d_j,
+                           ctx->slot, sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp32, ctx->d_tp32, (float)ctx->sqmaxcut,
+                           ctx->d_out);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, ctx->d_out, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+int sc_energy_all(sc_ctx* ctx, double* e) {
+    if (!ctx || !e) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    HIPCK(hipSetDevice(ctx->device));
+    return energy_all_impl(ctx, ctx->box, e);
+}
+
+int sc_energy_all_trial(sc_ctx* ctx, const double box_new[3], double* e) {
+    if (!ctx || !e || !box_new) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    for (int k = 0; k < 3; k++)
+        if (!(box_new[k] > 2.0 * ctx->maxcut * 0.5)) return fail(ctx, SC_EINVAL, "trial box smaller than the interaction range");
+    HIPCK(hipSetDevice(ctx->device));
+    return energy_all_impl(ctx, box_new, e);
+}
+
+int sc_commit_box(sc_ctx* ctx, const double box_new[3]) {
+    if (!ctx || !box_new) return SC_EINVAL;
+    for (int k = 0; k < 3; k++) {
+        if (!(box_new[k] > 0)) return fail(ctx, SC_EINVAL, "box edges must be positive");
+        ctx->box[k] = box_new[k];
+    }
+    return build_grid(ctx, 2, nullptr, ctx->box);
+}
+
+static int run_queries(sc_ctx* ctx, int nq, const Query* hq, const int* slots_dev, double* out) {
+    int rc;
+    if ((rc = ensure_out(ctx, nq)) || (rc = ensure_ij(ctx, nq))) return rc;
+    if (!ctx->grid_valid) {
+        if ((rc = build_grid(ctx, 2, nullptr, ctx->box))) return rc;
+    }
+    if (hq) HIPCK(hipMemcpyAsync(ctx->d_q, hq, nq * sizeof(Query), hipMemcpyHostToDevice, ctx->stream));
+    SysConst sc = sysconst(ctx, ctx->box);
+    int c = ctx->cur;
+    hipLaunchKernelGGL(k_energy_query, dim3(nq), dim3(WAVE), 0, ctx->stream, nq, hq ? ctx->d_q : nullptr, slots_dev,
+                       ctx->g, sc, ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
+                       ctx->d_out);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, ctx->d_out, nq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+int sc_energy_one(sc_ctx* ctx, int target, double* e) {
+    if (!ctx || !e) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (target < 0 || target >= ctx->n) return fail(ctx, SC_EINVAL, "target out of range");
+    HIPCK(hipSetDevice(ctx->device));
+    int s = 0;
+    HIPCK(hipMemcpy(&s, ctx->slot + target, sizeof(int), hipMemcpyDeviceToHost));
+    int rc;
+    if ((rc = ensure_ij(ctx, 1))) return rc;
+    HIPCK(hipMemcpy(ctx->d_i, &s, sizeof(int), hipMemcpyHostToDevice));
+    return run_queries(ctx, 1, nullptr, ctx->d_i, e);
+}
+
+int sc_energy_per_particle(sc_ctx* ctx, double* out) {
+    if (!ctx || !out) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    HIPCK(hipSetDevice(ctx->device));
+    int n = ctx->n, rc;
+    if (n == 0) return SC_OK;
+    if ((rc = ensure_ij(ctx, n))) return rc;
+    if (!ctx->grid_valid && (rc = build_grid(ctx, 2, nullptr, ctx->box))) return rc;
+    // queries in slot order (query q = slot q), results mapped back through slot
+    std::vector<int> sl(n), iota(n);
+    for (int k = 0; k < n; k++) iota[k] = k;
+    HIPCK(hipMemcpy(ctx->d_i, iota.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    std::vector<double> bySlot(n);
+    if ((rc = run_queries(ctx, n, nullptr, ctx->d_i, bySlot.data()))) return rc;
+    HIPCK(hipMemcpy(sl.data(), ctx->slot, n * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) out[i] = bySlot[sl[i]];
+    return SC_OK;
+}
+
+static Query make_query(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3]) {
+    Query q;
+    for (int k = 0; k < 3; k++) {
+        double s = pos[k] - floor(pos[k]);
+        double v = nearbyint(s * 4294967296.0);
+        q.u[k] = (uint32_t)(uint64_t)(v >= 4294967296.0 ? 0.0 : v);
+    }
+    double d[3] = {dir[0], dir[1], dir[2]}, p[3] = {pd0[0], pd0[1], pd0[2]};
+    double t = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (t != 0.0) { t = 1.0 / t; d[0] *= t; d[1] *= t; d[2] *= t; }
+    double dp = p[0] * d[0] + p[1] * d[1] + p[2] * d[2];
+    p[0] -= dp * d[0]; p[1] -= dp * d[1]; p[2] -= dp * d[2];
+    t = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    if (t != 0.0) { t = 1.0 / t; p[0] *= t; p[1] *= t; p[2] *= t; }
+    q.t = ctx->ref2dense[ctx->ptypes[target]];
+    q.o = make_float4((float)d[0], (float)d[1], (float)d[2], (float)p[0]);
+    q.q = make_float2((float)p[1], (float)p[2]);
+    q.exclude = target;
+    return q;
+}
+
+int sc_energy_one_trial(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3],
+                        double* e) {
+    if (!ctx || !e || !pos || !dir || !pd0) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (target < 0 || target >= ctx->n) return fail(ctx, SC_EINVAL, "target out of range");
+    HIPCK(hipSetDevice(ctx->device));
+    Query q = make_query(ctx, target, pos, dir, pd0);
+    return run_queries(ctx, 1, &q, nullptr, e);
+}
+
+int sc_commit_particle(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3]) {
+    if (!ctx || !pos || !dir || !pd0) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (target < 0 || target >= ctx->n) return fail(ctx, SC_EINVAL, "target out of range");
+    HIPCK(hipSetDevice(ctx->device));
+    Query q = make_query(ctx, target, pos, dir, pd0);
+    int s = 0, c = ctx->cur;
+    HIPCK(hipMemcpy(&s, ctx->slot + target, sizeof(int), hipMemcpyDeviceToHost));
+    uint4 p = make_uint4(q.u[0], q.u[1], q.u[2], (uint32_t)q.t);
+    HIPCK(hipMemcpy(ctx->P[c] + s, &p, sizeof(uint4), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->O[c] + s, &q.o, sizeof(float4), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->Q[c] + s, &q.q, sizeof(float2), hipMemcpyHostToDevice));
+    // the particle may have changed cell: rebuild the cell list
+    return build_grid(ctx, ctx->grid_valid ? ctx->g.r : 2, nullptr, ctx->box);
+}
+
+int sc_overlap_flags(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, uint8_t* out, double* dmin) {
+    if (!ctx || m < 0 || (m > 0 && (!i || !j || !out))) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    for (int k = 0; k < m; k++)
+        if (i[k] < 0 || i[k] >= ctx->n || j[k] < 0 || j[k] >= ctx->n) return fail(ctx, SC_EINVAL, "particle index out of range");
+    if (m == 0) return SC_OK;
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure_ij(ctx, m)) || (rc = ensure_out(ctx, m))) return rc;
+    unsigned char* d_flags = nullptr;
+    HIPCK(hipMalloc((void**)&d_flags, m));
+    HIPCK(hipMemcpyAsync(ctx->d_i, i, m * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(ctx->d_j, j, m * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    SysConst sc = sysconst(ctx, ctx->box);
+    int c = ctx->cur;
+    hipLaunchKernelGGL(k_overlap, dim3((m + 127) / 128), dim3(128), 0, ctx->stream, m, ctx->d_i, ctx->d_j, ctx->slot,
+                       sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp64, ctx->d_tp64, d_flags, ctx->d_out);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_flags, m, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && dmin) e = hipMemcpyAsync(dmin, ctx->d_out, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d_flags);
+    if (e != hipSuccess) return fail(ctx, SC_EHIP, hipGetErrorString(e));
+    return SC_OK;
+}
+
+int sc_overlap_count(sc_ctx* ctx, long* count) {
+    if (!ctx || !count) return SC_EINVAL;
+    long np = 0;
+    int rc = sc_neighbor_pairs(ctx, ctx->maxcut, nullptr, 0, &np);
+    if (rc) return rc;
+    std::vector<int32_t> pr(2 * np);
+    if ((rc = sc_neighbor_pairs(ctx, ctx->maxcut, pr.data(), np, &np))) return rc;
+    std::vector<int32_t> a(np), b(np);
+    for (long k = 0; k < np; k++) { a[k] = pr[2 * k]; b[k] = pr[2 * k + 1]; }
+    std::vector<uint8_t> f(np);
+    if ((rc = sc_overlap_flags(ctx, (int)np, a.data(), b.data(), f.data(), nullptr))) return rc;
+    long c = 0;
+    for (long k = 0; k < np; k++) c += f[k];
+    *count = c;
+    return SC_OK;
+}
+
+// host-side stream of the sweep schedule (grid shifts, colour orders)
+static uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot_half, int r, uint64_t seed,
+             uint64_t counter, sc_sweep_stats* stats) {
+    if (!ctx || nsweeps < 0 || !(temper > 0) || r < 1 || r > 2) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    HIPCK(hipSetDevice(ctx->device));
+    HIPCK(hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(unsigned long long), ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->d_dsum, 0, sizeof(double), ctx->stream));
+    int p = r + 1;
+    long launches = 0;
+    HIPCK(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int s = 0; s < nsweeps; s++) {
+        uint64_t sw = counter + (uint64_t)s;
+        uint64_t h = mix64(seed ^ mix64(sw * 0x632BE59BD9B4E019ull));
+        uint32_t shift[3] = {(uint32_t)h, (uint32_t)(h >> 32), (uint32_t)mix64(h)};
+        int rc = build_grid(ctx, r, shift, ctx->box);
+        if (rc) return rc;
+        const Grid& g = ctx->g;
+        int cap = stage_cap(ctx, g, ctx->box);
+        size_t lds = stage_bytes(cap) + sizeof(int) * (2 * cap + WAVE) + sizeof(int) * (2 * 125 + 2);
+        // colour order: a random permutation of the p^3 colours
+        int ncol = p * p * p;
+        int order[27];
+        for (int k = 0; k < ncol; k++) order[k] = k;
+        uint64_t hs = mix64(h ^ 0xA5A5A5A5ull);
+        for (int k = ncol - 1; k > 0; k--) {
+            hs = mix64(hs);
+            int j = (int)(hs % (uint64_t)(k + 1));
+            std::swap(order[k], order[j]);
+        }
+        int pa[3];
+        for (int k = 0; k < 3; k++) pa[k] = (g.nc[k] >= p) ? p : 1;
+        SysConst sc = sysconst(ctx, ctx->box);
+        int c = ctx->cur;
+        for (int ci = 0; ci < ncol; ci++) {
+            int col = order[ci];
+            MoveParams mp;
+            mp.ox = col % p; mp.oy = (col / p) % p; mp.oz = col / (p * p);
+            if (mp.ox >= pa[0] || mp.oy >= pa[1] || mp.oz >= pa[2]) continue;   // axis with a single cell
+            mp.p = p;
+            for (int k = 0; k < 3; k++) mp.nact[k] = g.nc[k] / pa[k];
+            mp.transmx = (float)transmx;
+            mp.rot_half = (float)rot_half;
+            mp.temper = temper;
+            mp.key0 = (uint32_t)seed;
+            mp.key1 = (uint32_t)(seed >> 32);
+            uint64_t pass = sw * 32 + (uint64_t)ci;
+            mp.pass_lo = (uint32_t)pass;
+            mp.pass_hi = (uint32_t)(pass >> 32);
+            // a single-cell axis has period 1: colour offsets along it stay 0
+            int nblk = mp.nact[0] * mp.nact[1] * mp.nact[2];
+            hipLaunchKernelGGL(k_sweep_pass, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp, ctx->starts,
+                               ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,
+                               ctx->d_stats, ctx->d_dsum, ctx->d_flag);
+            launches++;
+        }
+        HIPCK(hipGetLastError());
+    }
+    HIPCK(hipEventRecord(ctx->ev1, ctx->stream));
+    unsigned long long hs[8];
+    double ds = 0;
+    HIPCK(hipMemcpyAsync(hs, ctx->d_stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipMemcpyAsync(&ds, ctx->d_dsum, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    int rc = check_flag(ctx, "sweep");
+    if (rc) return rc;
+    float ms = 0;
+    HIPCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    ctx->last_launches = launches;
+    ctx->last_kernel_ms = ms;
+    if (stats) {
+        stats->trans_acc = hs[0];
+        stats->trans_try = hs[1];
+        stats->rot_acc = hs[2];
+        stats->rot_try = hs[3];
+        stats->left_cell = hs[4];
+        stats->stage1_acc = stats->stage1_try = 0;
+        stats->energy_change = ds;
+    }
+    return SC_OK;
+}
+
+int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
+                   uint64_t counter, int* accepted, double box_out[3]) {
+    if (!ctx || !accepted || !(temper > 0)) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (ptype != 0 && ptype != 1) return fail(ctx, SC_EINVAL, "volume move: ptype 0 and 1 are supported");
+    // MoveCreator::pressureMove (movecreator.cpp:330-421)
+    uint64_t h = mix64(seed ^ mix64(counter * 0xD1B54A32D192ED03ull + 7));
+    auto u01 = [&]() { h = mix64(h); return (double)(h >> 11) * (1.0 / 9007199254740992.0); };
+    double e_old = 0, e_new = 0;
+    int rc = energy_all_impl(ctx, ctx->box, &e_old);
+    if (rc) return rc;
+    double nb[3] = {ctx->box[0], ctx->box[1], ctx->box[2]};
+    double enermove;
+    double npart = (double)ctx->n;
+    if (ptype == 0) {
+        double rs = u01();
+        int side = rs < 1.0 / 3.0 ? 0 : (rs < 2.0 / 3.0 ? 1 : 2);
+        double area = nb[(side + 1) % 3] * nb[(side + 2) % 3];
+        double old = nb[side];
+        nb[side] += edge_mx * (u01() - 0.5);
+        if (nb[side] <= 0.0) { *accepted = 0; if (box_out) for (int k = 0; k < 3; k++) box_out[k] = ctx->box[k]; return SC_OK; }
+        enermove = press * area * (nb[side] - old) - npart * temper * log(nb[side] / old);
+    } else {
+        double psch = edge_mx * (u01() - 0.5);
+        double pvol = nb[0] * nb[1] * nb[2];
+        for (int k = 0; k < 3; k++) nb[k] += psch;
+        double pvoln = nb[0] * nb[1] * nb[2];
+        enermove = press * (pvoln - pvol) - npart * temper * log(pvoln / pvol);
+    }
+    for (int k = 0; k < 3; k++)
+        if (!(nb[k] > ctx->maxcut)) { *accepted = 0; if (box_out) for (int q = 0; q < 3; q++) box_out[q] = ctx->box[q]; return SC_OK; }
+    if ((rc = energy_all_impl(ctx, nb, &e_new))) return rc;
+    enermove += e_new;
+    bool acc = (enermove <= e_old) || (exp(-1.0 * (enermove - e_old) / temper) > u01());
+    *accepted = acc ? 1 : 0;
+    if (acc) {
+        for (int k = 0; k < 3; k++) ctx->box[k] = nb[k];
+    }
+    if ((rc = build_grid(ctx, 2, nullptr, ctx->box))) return rc;
+    if (box_out) for (int k = 0; k < 3; k++) box_out[k] = ctx->box[k];
+    return SC_OK;
+}
+
+int sc_last_timing(const sc_ctx* ctx, double* ms, long* launches, double* kernel_ms) {
+    if (!ctx) return SC_EINVAL;
+    if (ms) *ms = ctx->last_ms;
+    if (launches) *launches = ctx->last_launches;
+    if (kernel_ms) *kernel_ms = ctx->last_kernel_ms;
+    return SC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------- synthetic inputs
+// Random sequential addition (host, FP64): particles are inserted one by one
+// at uniform positions with uniform orientations (patch direction uniform
+// around the axis) and kept if their closest distance to every particle
+// already placed is >= sigma_ij (the hard-core criterion of sc_overlap_flags).
+// Used to build seeded, overlap-free starting configurations.
+extern "C" int sc_generate_rsa(sc_ctx* ctx, int n, const int32_t* types, const double box[3], uint64_t seed,
+                               long max_attempts, double* pos, double* dir, double* pd0) {
+    if (!ctx || n < 0 || !types || !box || !pos || !dir || !pd0) return SC_EINVAL;
+    if (!ctx->have_topo) return fail(ctx, SC_ESTATE, "load a topology first");
+    const schost::Topology& T = ctx->topo;
+    double wmin = 0;
+    for (int i = 0; i < n; i++) {
+        const schost::Ia& a = T.ia[types[i]][types[i]];
+        if (a.geotype[0] == 0) return fail(ctx, SC_EINVAL, "undefined type in sc_generate_rsa");
+        wmin = std::max(wmin, a.len[0] + a.sigma);
+    }
+    int nc[3];
+    for (int k = 0; k < 3; k++) nc[k] = std::max(1, (int)floor(box[k] / std::max(wmin, 1e-9)));
+    int ncell = nc[0] * nc[1] * nc[2];
+    std::vector<std::vector<int>> cells(ncell);
+    uint64_t h = mix64(seed + 0x1234567ull);
+    auto u01 = [&]() { h = mix64(h); return (double)(h >> 11) * (1.0 / 9007199254740992.0); };
+    auto cell_of = [&](const double* s) {
+        int c[3];
+        for (int k = 0; k < 3; k++) { c[k] = (int)(s[k] * nc[k]); if (c[k] >= nc[k]) c[k] = nc[k] - 1; }
+        return c[0] + nc[0] * (c[1] + nc[1] * c[2]);
+    };
+    long attempts = 0;
+    for (int i = 0; i < n; i++) {
+        const schost::Ia& ai = T.ia[types[i]][types[i]];
+        bool placed = false;
+        while (!placed) {
+            if (++attempts > max_attempts) return fail(ctx, SC_ECAPACITY, "random sequential addition ran out of attempts at particle " + std::to_string(i));
+            double s[3] = {u01(), u01(), u01()};
+            double z = 1 - 2 * u01(), ph = 6.283185307179586 * u01(), sr = sqrt(std::max(0.0, 1 - z * z));
+            V3<double> d = mk<double>(sr * cos(ph), sr * sin(ph), z);
+            // a unit vector perpendicular to d, rotated by a uniform angle about d
+            V3<double> ref = fabs(d.x) < 0.9 ? mk<double>(1, 0, 0) : mk<double>(0, 1, 0);
+            V3<double> e1 = normalised(perp(ref, d)), e2 = cross(d, e1);
+            double psi = 6.283185307179586 * u01();
+            V3<double> q = mk<double>(cos(psi) * e1.x + sin(psi) * e2.x, cos(psi) * e1.y + sin(psi) * e2.y,
+                                      cos(psi) * e1.z + sin(psi) * e2.z);
+            int c = cell_of(s);
+            int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
+            bool ok = true;
+            for (int dz = -1; dz <= 1 && ok; dz++)
+                for (int dy = -1; dy <= 1 && ok; dy++)
+                    for (int dx = -1; dx <= 1 && ok; dx++) {
+                        if ((nc[0] < 3 && dx == 1 && nc[0] == 2) || (nc[0] == 1 && dx != 0)) continue;
+                        if ((nc[1] < 3 && dy == 1 && nc[1] == 2) || (nc[1] == 1 && dy != 0)) continue;
+                        if ((nc[2] < 3 && dz == 1 && nc[2] == 2) || (nc[2] == 1 && dz != 0)) continue;
+                        int x = (cx + dx + nc[0]) % nc[0], y = (cy + dy + nc[1]) % nc[1], zz = (cz + dz + nc[2]) % nc[2];
+                        for (int j : cells[x + nc[0] * (y + nc[1] * zz)]) {
+                            const schost::Ia& p = T.ia[types[i]][types[j]];
+                            V3<double> r;
+                            double dd[3];
+                            for (int k = 0; k < 3; k++) {
+                                double df = s[k] - pos[3 * j + k];
+                                dd[k] = box[k] * (df - nearbyint(df));
+                            }
+                            r = mk<double>(dd[0], dd[1], dd[2]);
+                            V3<double> dj = mk<double>(dir[3 * j], dir[3 * j + 1], dir[3 * j + 2]);
+                            bool si = ai.geotype[0] >= 30, sj = T.ia[types[j]][types[j]].geotype[0] >= 30;
+                            double d2;
+                            if (si && sj) d2 = dot(r, r);
+                            else if (!si && !sj) { V3<double> v = min_dist(d, dj, p.half_len[0], p.half_len[1], r); d2 = dot(v, v); }
+                            else {
+                                double contt;
+                                V3<double> dv;
+                                d2 = si ? sp_dist<double>(mk<double>(-r.x, -r.y, -r.z), dj, p.half_len[1], contt, dv)
+                                        : sp_dist<double>(r, d, p.half_len[0], contt, dv);
+                            }
+                            if (d2 < p.sigma * p.sigma) { ok = false; break; }
+                        }
+                    }
+            if (!ok) continue;
+            for (int k = 0; k < 3; k++) pos[3 * i + k] = s[k];
+            dir[3 * i] = d.x; dir[3 * i + 1] = d.y; dir[3 * i + 2] = d.z;
+            pd0[3 * i] = q.x; pd0[3 * i + 1] = q.y; pd0[3 * i + 2] = q.z;
+            cells[c].push_back(i);
+            placed = true;
+        }
+    }
+    return SC_OK;
+}

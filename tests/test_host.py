@@ -1,0 +1,85 @@
+"""Host side of the product (no GPU): the C ABI library loads and exports
+every declared entry point; SC's input formats parse as the reference does."""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import sc_amd
+import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "sc_gpu.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sc_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = sc_amd.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(L, s), s
+    # the Python binding covers the same set
+    assert set(syms) <= set(sc_amd.SIGNATURES)
+
+
+def test_topology_parser_matches_reference_params():
+    g = json.load(open(os.path.join(GOLD, "topo_golden.json")))
+    h = sc_amd.GpuSystem(-1)
+    h.load_topology(g["top"])
+    assert h.maxcut == pytest.approx(g["maxcut"], rel=1e-15)
+    for key, row in g["rows"].items():
+        i, j = (int(x) for x in key.split(","))
+        np.testing.assert_allclose(h.pair_params(i, j), row, rtol=1e-14, atol=1e-15, err_msg=key)
+    assert list(h.particle_types()) == list(range(1, 13))
+
+
+def test_topology_errors_are_reported():
+    h = sc_amd.GpuSystem(-1)
+    with pytest.raises(sc_amd.SCError, match="wrong number of parameters"):
+        h.load_topology("[Types]\nA 1 PSC 1.0 1.0 1.0\n[Molecules]\nA: {\nparticles: 1\n}\n[System]\nA 2\n")
+    with pytest.raises(sc_amd.SCError, match="invalid keyword"):
+        h.load_topology("[Typos]\nA 1 SPN 1.0 1.0\n")
+    with pytest.raises(sc_amd.SCError, match="not defined"):
+        h.load_topology("[Types]\nA 1 SPN 1.0 1.0\n[Molecules]\nA: {\nparticles: 1\n}\n[System]\nB 2\n")
+
+
+def test_options_parser_postprocessing():
+    from sc_amd import workloads as W
+    h = sc_amd.GpuSystem(-1)
+    h.load_options(W.options_text(transmx=0.15, rotmx=20.0))
+    assert h.option("transmx") == pytest.approx(0.3)            # sim.h:365 doubles it
+    assert h.option("rot_half") == pytest.approx(20.0 / 180 * np.pi / 2 * 0.5)   # sim.h:360
+    bad = W.options_text().replace("seed = 12345\n", "")
+    with pytest.raises(sc_amd.SCError, match="seed"):
+        h.load_options(bad)
+
+
+def test_rsa_configuration_has_no_overlap():
+    from sc_amd import workloads as W
+    w = W.workload("c2_cpsc_32768", n_override=300)
+    h = sc_amd.GpuSystem(-1)
+    h.load_topology(w["top"])
+    pos, d, q = h.generate_rsa(w["types"], w["box"], seed=5)
+    # closest segment distances with the oracle's restatement of minDistSegments
+    n = len(pos)
+    hl = 1.5
+    mind = 1e9
+    for i in range(n):
+        for j in range(i + 1, n):
+            dd = pos[i] - pos[j]
+            r = w["box"] * (dd - np.rint(dd))
+            if r @ r > (3 + 1.0) ** 2:
+                continue
+            v = po.min_dist_segments(d[i], d[j], hl, hl, r)
+            mind = min(mind, np.linalg.norm(v))
+    assert mind >= 1.0 - 1e-12
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0)
+    assert np.allclose((d * q).sum(1), 0.0, atol=1e-12)
