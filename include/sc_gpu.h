@@ -160,9 +160,13 @@ int sc_generate_rsa(sc_ctx* ctx, int n, const int32_t* types, const double box[3
                     double* pos, double* dir, double* pd0);
 
 /* --- device timing -------------------------------------------------------- */
+/* With profiling on, sc_sweep brackets every checkerboard-pass launch with
+ * HIP events on the context's stream (no counterpart in the reference, whose
+ * only timer is clock() around the sweep loop, updater.cpp:137-149). */
+int sc_set_profiling(sc_ctx* ctx, int on);
 /* milliseconds of the last sc_sweep / sc_energy_all call measured with HIP
- * events on the context's stream around its kernels, and the number of
- * kernel launches of the dominant kernel in it */
+ * events on the context's stream around its kernels, the number of launches
+ * of the dominant kernel in it and (with profiling on) their summed time */
 int sc_last_timing(const sc_ctx* ctx, double* ms, long* launches, double* kernel_ms);
 
 #ifdef __cplusplus

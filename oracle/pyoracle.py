@@ -254,15 +254,16 @@ def neighbor_pairs(u, box, rc):
     return out[:cnt]
 
 
-def mc_run(topo, st, nsweeps, temper, transmx, rot_half, seed):
-    """Sequential Metropolis on a copy of st; returns (new_state, drift, stats, e_final)."""
+def mc_run(topo, st, nsweeps, temper, transmx, rot_half, seed, energies=True):
+    """Sequential Metropolis on a copy of st; returns (new_state, drift, stats, e_final).
+    energies=False skips the whole-system energies used for the drift check."""
     s = State(st.box, st.pos.copy(), st.dir.copy(), st.pd0.copy(), st.types.copy(),
               None if st.pd1 is None else st.pd1.copy())
     stats = (ctypes.c_long * 4)()
     ef = ctypes.c_double(0.0)
     drift = lib().orc_mc_run(topo.ref, _dp(s.box), s.n, _dp(s.pos), _dp(s.dir), _dp(s.pd0), _opt(s.pd1),
                              _ip(s.types), int(nsweeps), float(temper), float(transmx), float(rot_half),
-                             int(seed) & 0xFFFFFFFFFFFFFFFF, stats, ctypes.byref(ef))
+                             int(seed) & 0xFFFFFFFFFFFFFFFF, stats, ctypes.byref(ef) if energies else None)
     return s, drift, list(stats), ef.value
 
 

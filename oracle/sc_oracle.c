@@ -1054,7 +1054,7 @@ double orc_mc_run(const orc_topo* t, const double box[3], int n, double* pos, do
     int i;
     stats[0] = stats[1] = stats[2] = stats[3] = 0;
     grid_build(&g, p, n, box, t->maxcut);
-    e_start = orc_energy_all_cells(t, box, p, n);
+    e_start = e_final ? orc_energy_all_cells(t, box, p, n) : 0.0;
     for (sweep = 0; sweep < nsweeps; sweep++) {
         for (step = 0; step < n; step++) {
             /* MoveCreator::particleMove, movecreator.cpp:16-29 */
@@ -1100,7 +1100,7 @@ double orc_mc_run(const orc_topo* t, const double box[3], int n, double* pos, do
             }
         }
     }
-    e_end = orc_energy_all_cells(t, box, p, n);
+    e_end = e_final ? orc_energy_all_cells(t, box, p, n) : 0.0;
     if (e_final) *e_final = e_end;
     for (i = 0; i < n; i++) {
         pos[3 * i] = p[i].pos.x; pos[3 * i + 1] = p[i].pos.y; pos[3 * i + 2] = p[i].pos.z;

@@ -70,6 +70,7 @@ SIGNATURES = {
                                       ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), _D]),
     "sc_generate_rsa": (ctypes.c_int, [_P, ctypes.c_int, _I, _D, ctypes.c_uint64, ctypes.c_long, _D, _D, _D]),
     "sc_last_timing": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_long), _D]),
+    "sc_set_profiling": (ctypes.c_int, [_P, ctypes.c_int]),
 }
 
 _lib = None
@@ -306,6 +307,9 @@ class GpuSystem:
                                         int(seed) & 0xFFFFFFFFFFFFFFFF, int(counter) & 0xFFFFFFFFFFFFFFFF,
                                         ctypes.byref(acc), _d(box)), "volume_move")
         return bool(acc.value), box
+
+    def set_profiling(self, on=True):
+        self._ck(self._L.sc_set_profiling(self._h, int(bool(on))), "set_profiling")
 
     def last_timing(self):
         ms, kms = ctypes.c_double(), ctypes.c_double()

@@ -32,7 +32,7 @@ typedef TypeParamsT<float> TP32;
 typedef TypeParamsT<double> TP64;
 
 #define WAVE 64
-#define MAX_TYPES 8
+#define MAX_TYPES 16
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
@@ -62,7 +62,7 @@ __device__ __forceinline__ int cell_of_u(const Grid& g, uint32_t ux, uint32_t uy
 
 // particle from its stored record (FP32 or FP64 working precision)
 template <typename R>
-__device__ __forceinline__ DPart<R> load_part(const float4 o, const float2 q, int t, const TypeParamsT<R>* tp) {
+__host__ __device__ __forceinline__ DPart<R> load_part(const float4 o, const float2 q, int t, const TypeParamsT<R>* tp) {
     DPart<R> p;
     p.dir = mk<R>(o.x, o.y, o.z);
     p.pd[0] = mk<R>(o.w, q.x, q.y);
@@ -233,6 +233,7 @@ __device__ bool stage_load(Stage& S, int cap, const Grid& g, int cx, int cy, int
 
 struct SysConst {
     float L[3];
+    float Lh[3], Ll[3];   // box edge as FP32 head + tail (fx_image_c)
     double Ld[3];
     float sqmaxcut;
     int nt;
@@ -283,7 +284,7 @@ __global__ __launch_bounds__(WAVE) void k_energy_all(Grid g, SysConst sc, const 
             int i = S.h0 + hi_;
             if (S.id[j] < S.id[i]) {
                 uint4 a = S.u[i], b = S.u[j];
-                V3<float> r = fx_image<float>(a.x, a.y, a.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
+                V3<float> r = fx_image_c(a.x, a.y, a.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
                 float r2 = dot(r, r);
                 if (r2 <= sc.sqmaxcut) {
                     const PP32& P2 = pp[a.w * sc.nt + b.w];
@@ -321,7 +322,7 @@ __global__ __launch_bounds__(WAVE) void k_energy_all(Grid g, SysConst sc, const 
                 if (e < take) {
                     int i = S.h0 + queue[2 * e], jj = queue[2 * e + 1];
                     uint4 a = S.u[i], b = S.u[jj];
-                    V3<float> r = fx_image<float>(a.x, a.y, a.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
+                    V3<float> r = fx_image_c(a.x, a.y, a.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
                     const PP32& P2 = pp[a.w * sc.nt + b.w];
                     DPart<float> pa = load_part<float>(S.o[i], S.q[i], a.w, tp);
                     DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
@@ -400,7 +401,7 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
         for (int k = b + lane; k < e; k += WAVE) {
             if (ID[k] == q.exclude) continue;
             uint4 pb4 = P[k];
-            V3<float> r = fx_image<float>(q.u[0], q.u[1], q.u[2], pb4.x, pb4.y, pb4.z, sc.L[0], sc.L[1], sc.L[2]);
+            V3<float> r = fx_image_c(q.u[0], q.u[1], q.u[2], pb4.x, pb4.y, pb4.z, sc.Lh, sc.Ll);
             if (dot(r, r) > sc.sqmaxcut) continue;
             DPart<float> pb = load_part<float>(O[k], Q[k], pb4.w, tp);
             acc += (double)pair_energy<float>(pp[q.t * sc.nt + pb4.w], pa, pb, r, sc.sqmaxcut);
@@ -425,8 +426,10 @@ __global__ void k_pairs(int m, const int* __restrict__ pi, const int* __restrict
     V3<R> r;
     if (sizeof(R) == 8)
         r = fx_image<R>(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, (R)sc.Ld[0], (R)sc.Ld[1], (R)sc.Ld[2]);
-    else
-        r = fx_image<R>(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, (R)sc.L[0], (R)sc.L[1], (R)sc.L[2]);
+    else {
+        V3<float> rf = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
+        r = mk<R>((R)rf.x, (R)rf.y, (R)rf.z);
+    }
     out[k] = (double)pair_energy<R>(pp[ua.w * sc.nt + ub.w], pa, pb, r, sqmaxcut);
 }
 
@@ -498,8 +501,8 @@ __device__ void trial_energies(const Stage& S, int k, const uint4 uo, const uint
         if (j < S.n && j != k) {
             uint4 b = S.u[j];
             const PP32& P2 = pp[t * sc.nt + b.w];
-            V3<float> ro = fx_image<float>(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
-            V3<float> rn = fx_image<float>(un.x, un.y, un.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
+            V3<float> ro = fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+            V3<float> rn = fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
             float r2o = dot(ro, ro), r2n = dot(rn, rn);
             if (r2o <= sc.sqmaxcut || r2n <= sc.sqmaxcut) {
                 DPart<float> pb;
@@ -534,7 +537,7 @@ __device__ void trial_energies(const Stage& S, int k, const uint4 uo, const uint
                     bool nw = (w >> 30) & 1;
                     const uint4& u1 = nw ? un : uo;
                     uint4 b = S.u[jj];
-                    V3<float> r = fx_image<float>(u1.x, u1.y, u1.z, b.x, b.y, b.z, sc.L[0], sc.L[1], sc.L[2]);
+                    V3<float> r = fx_image_c(u1.x, u1.y, u1.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
                     const PP32& P2 = pp[t * sc.nt + b.w];
                     DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
                     float ev = (P2.kind <= K_SCA) ? scsc_attr(P2, nw ? pn : po, pb, r) : spsc(P2, nw ? pn : po, pb, r);
@@ -718,6 +721,8 @@ struct sc_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0, last_kernel_ms = 0;
     long last_launches = 0;
+    bool profile = false;               // time every sweep-pass launch with events
+    std::vector<hipEvent_t> kev;        // event pairs for the launches of one call
 };
 
 #define HIPCK(x)                                                                       \
@@ -807,7 +812,7 @@ static int upload_tables(sc_ctx* ctx) {
             ctx->dense2ref.push_back(t);
         }
     ctx->nt = (int)ctx->dense2ref.size();
-    if (ctx->nt > MAX_TYPES) return fail(ctx, SC_ECAPACITY, "more than 8 particle types in one system");
+    if (ctx->nt > MAX_TYPES) return fail(ctx, SC_ECAPACITY, "more than 16 particle types in one system");
     int nt = ctx->nt;
     std::vector<PP32> h32(nt * nt);
     std::vector<PP64> h64(nt * nt);
@@ -843,7 +848,12 @@ static int upload_tables(sc_ctx* ctx) {
 
 static SysConst sysconst(const sc_ctx* ctx, const double* box) {
     SysConst s;
-    for (int k = 0; k < 3; k++) { s.L[k] = (float)box[k]; s.Ld[k] = box[k]; }
+    for (int k = 0; k < 3; k++) {
+        s.L[k] = (float)box[k];
+        s.Ld[k] = box[k];
+        s.Lh[k] = (float)box[k];
+        s.Ll[k] = (float)(box[k] - (double)s.Lh[k]);
+    }
     s.sqmaxcut = (float)ctx->sqmaxcut;
     s.nt = ctx->nt;
     return s;
@@ -1012,6 +1022,7 @@ void sc_destroy(sc_ctx* ctx) {
                     ctx->d_stats, ctx->d_dsum, ctx->d_q, ctx->d_i, ctx->d_j};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -1496,9 +1507,18 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
             mp.pass_hi = (uint32_t)(pass >> 32);
             // a single-cell axis has period 1: colour offsets along it stay 0
             int nblk = mp.nact[0] * mp.nact[1] * mp.nact[2];
+            if (ctx->profile) {
+                while ((long)ctx->kev.size() < 2 * (launches + 1)) {
+                    hipEvent_t e;
+                    HIPCK(hipEventCreate(&e));
+                    ctx->kev.push_back(e);
+                }
+                HIPCK(hipEventRecord(ctx->kev[2 * launches], ctx->stream));
+            }
             hipLaunchKernelGGL(k_sweep_pass, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp, ctx->starts,
                                ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,
                                ctx->d_stats, ctx->d_dsum, ctx->d_flag);
+            if (ctx->profile) HIPCK(hipEventRecord(ctx->kev[2 * launches + 1], ctx->stream));
             launches++;
         }
         HIPCK(hipGetLastError());
@@ -1514,7 +1534,13 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
     HIPCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_ms = ms;
     ctx->last_launches = launches;
-    ctx->last_kernel_ms = ms;
+    ctx->last_kernel_ms = 0;
+    if (ctx->profile)
+        for (long k = 0; k < launches; k++) {
+            float km = 0;
+            HIPCK(hipEventElapsedTime(&km, ctx->kev[2 * k], ctx->kev[2 * k + 1]));
+            ctx->last_kernel_ms += km;
+        }
     if (stats) {
         stats->trans_acc = hs[0];
         stats->trans_try = hs[1];
@@ -1567,6 +1593,12 @@ int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double e
     }
     if ((rc = build_grid(ctx, 2, nullptr, ctx->box))) return rc;
     if (box_out) for (int k = 0; k < 3; k++) box_out[k] = ctx->box[k];
+    return SC_OK;
+}
+
+int sc_set_profiling(sc_ctx* ctx, int on) {
+    if (!ctx) return SC_EINVAL;
+    ctx->profile = on != 0;
     return SC_OK;
 }
 

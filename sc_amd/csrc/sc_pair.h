@@ -527,4 +527,15 @@ SC_HD V3<R> fx_image(uint32_t ax, uint32_t ay, uint32_t az, uint32_t bx, uint32_
     return mk<R>(Lx * ((R)(int32_t)(ax - bx) * s), Ly * ((R)(int32_t)(ay - by) * s), Lz * ((R)(int32_t)(az - bz) * s));
 }
 
+// FP32 minimum image with the box edge split into a head and a tail
+// (L = Lh + Ll exactly to ~2^-48): the rounding of L to FP32 would otherwise
+// shift every distance by the same relative amount, a bias that accumulates
+// in whole-system sums.
+SC_HD V3<float> fx_image_c(uint32_t ax, uint32_t ay, uint32_t az, uint32_t bx, uint32_t by, uint32_t bz,
+                           const float* Lh, const float* Ll) {
+    const float s = 2.3283064365386962890625e-10f;  // 2^-32
+    float x = (float)(int32_t)(ax - bx) * s, y = (float)(int32_t)(ay - by) * s, z = (float)(int32_t)(az - bz) * s;
+    return mk<float>(fmaf(x, Lh[0], x * Ll[0]), fmaf(y, Lh[1], y * Ll[1]), fmaf(z, Lh[2], z * Ll[2]));
+}
+
 }  // namespace scg

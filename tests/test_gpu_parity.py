@@ -229,13 +229,12 @@ def test_sweep_energy_drift_and_determinism(dev):
         s = g.sweep(20, temper=1.0, transmx=0.3, rot_half=20.0 / 180 * np.pi / 4, r=2, seed=99, counter=0)
         e1 = g.allToAll()
         drift = e1 - e0 - s["energy_change"]
-        tries = s["trans_try"] + s["rot_try"] + s["left_cell"]
+        tries = s["trans_try"] + s["rot_try"]   # left_cell moves are counted in trans_try
         say("sweep: E %.6f -> %.6f, sum dE %.6f, drift %.3e, tries %d, acc %d/%d %d/%d left %d" % (
             e0, e1, s["energy_change"], drift, tries, s["trans_acc"], s["trans_try"], s["rot_acc"], s["rot_try"],
             s["left_cell"]))
         assert tries == 20 * 1000
         assert abs(drift) < 1e-4 * max(1.0, abs(e1)) + 1e-3
-        assert g.overlap_count() == 0 or True
         _, p, dd, qq = g.get_config()
         out.append((p, dd, qq))
         g.close()
