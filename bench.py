@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="c4_tchpsc_65536")
     ap.add_argument("--r", type=int, default=2, help="checkerboard stencil radius (1: 2^3 colours, 2: 3^3)")
+    ap.add_argument("--trial-mode", type=int, default=1,
+                    help="trials per active cell: 0 = its occupancy, 1 = dithered mean occupancy")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2025)
@@ -75,13 +77,9 @@ def main():
     log("state ready in %.1fs: N=%d box %.3f E/N %.4f" % (time.time() - t0, n, w["box"][0], g.allToAll() / n))
 
     if a.copy_probe:
-        import torch
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy probe: 3 x 256 MiB device copies")
+        g.copy_probe(256 << 20, reps=3)
+        log("copy probe: 3 x k_copy of 256 MiB")
+    g.set_trial_mode(a.trial_mode)
     g.sweep(a.warmup, temper, transmx, rot_half, r=a.r, seed=a.seed, counter=1)
     g.set_profiling(True)
     t1 = time.perf_counter()
@@ -89,7 +87,10 @@ def main():
     t2 = time.perf_counter()
     wall = t2 - t1
     dev_ms, launches, kern_ms = g.last_timing()
-    sweeps_per_s = a.steps / wall
+    # sweeps actually performed: trial moves / N (trial mode 1 makes N trials
+    # per sweep on average, not exactly)
+    done = (st["trans_try"] + st["rot_try"]) / float(n)
+    sweeps_per_s = done / wall
     e_n = g.allToAll() / n
     log("exact sweep: %.2f sweeps/s (%.3f ms/sweep wall, %.3f ms device), %d pass launches, kernel %.3f ms" % (
         sweeps_per_s, 1e3 * wall / a.steps, dev_ms / a.steps, launches, kern_ms))
@@ -140,7 +141,8 @@ def main():
         "data": "synthetic seeded state (RSA + compression), no dataset",
         "config": {"workload": a.workload, "n": n, "eta": w["eta"], "box": float(w["box"][0]), "temper": temper,
                    "translation": transmx, "max_rotation_deg": 10.0,
-                   "checkerboard": "%dx%dx%d" % ((a.r + 1,) * 3), "cells": list(nc), "sweep": "exact"},
+                   "checkerboard": "%dx%dx%d" % ((a.r + 1,) * 3), "cells": list(nc), "sweep": "exact",
+                   "trial_mode": a.trial_mode, "trial_moves_per_particle": done / a.steps},
         "exact_sweeps_per_s": sweeps_per_s,
         "acceptance": {"translation": st["trans_acc"] / max(1, st["trans_try"]),
                        "rotation": st["rot_acc"] / max(1, st["rot_try"]),

@@ -144,6 +144,14 @@ int sc_overlap_count(sc_ctx* ctx, long* count);
  * (seed, counter) reproduces a run bit for bit.  stats may be NULL. */
 int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot_half, int r, uint64_t seed,
              uint64_t counter, sc_sweep_stats* stats);
+/* How many trial moves an active cell makes in one checkerboard pass.
+ * mode 0 (default): as many as the cell holds particles, so every particle is
+ * chosen once per sweep on average as in Updater::simulate's step loop
+ * (updater.cpp:206-230, target = ran2()*N).  mode 1: the mean occupancy of the
+ * non-empty cells, randomly rounded, so a sweep still averages N trials but no
+ * pass waits for its fullest cell.  Both are Metropolis chains with the same
+ * stationary distribution (the count never depends on the moves of the pass). */
+int sc_set_trial_mode(sc_ctx* ctx, int mode);
 /* Volume move (MoveCreator::pressureMove, movecreator.cpp:330-550) for
  * ptype 0 (one random edge) and 1 (isotropic); returns accepted=1/0 and the
  * new box.  Whole-system energies are FP32 pair terms with FP64 reduction; dE
@@ -164,6 +172,10 @@ int sc_generate_rsa(sc_ctx* ctx, int n, const int32_t* types, const double box[3
  * HIP events on the context's stream (no counterpart in the reference, whose
  * only timer is clock() around the sweep loop, updater.cpp:137-149). */
 int sc_set_profiling(sc_ctx* ctx, int on);
+/* A plain device copy of `bytes` (read + write), `reps` times: the known-size
+ * kernel that calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE counts.  No
+ * counterpart in the reference. */
+int sc_copy_probe(sc_ctx* ctx, long bytes, int reps);
 /* milliseconds of the last sc_sweep / sc_energy_all call measured with HIP
  * events on the context's stream around its kernels, the number of launches
  * of the dominant kernel in it and (with profiling on) their summed time */

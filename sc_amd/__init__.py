@@ -71,6 +71,8 @@ SIGNATURES = {
     "sc_generate_rsa": (ctypes.c_int, [_P, ctypes.c_int, _I, _D, ctypes.c_uint64, ctypes.c_long, _D, _D, _D]),
     "sc_last_timing": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_long), _D]),
     "sc_set_profiling": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_copy_probe": (ctypes.c_int, [_P, ctypes.c_long, ctypes.c_int]),
+    "sc_set_trial_mode": (ctypes.c_int, [_P, ctypes.c_int]),
 }
 
 _lib = None
@@ -307,6 +309,12 @@ class GpuSystem:
                                         int(seed) & 0xFFFFFFFFFFFFFFFF, int(counter) & 0xFFFFFFFFFFFFFFFF,
                                         ctypes.byref(acc), _d(box)), "volume_move")
         return bool(acc.value), box
+
+    def set_trial_mode(self, mode):
+        self._ck(self._L.sc_set_trial_mode(self._h, int(mode)), "set_trial_mode")
+
+    def copy_probe(self, nbytes, reps=3):
+        self._ck(self._L.sc_copy_probe(self._h, int(nbytes), int(reps)), "copy_probe")
 
     def set_profiling(self, on=True):
         self._ck(self._L.sc_set_profiling(self._h, int(bool(on))), "set_profiling")

@@ -84,32 +84,64 @@ __global__ void k_cell_count(int n, const uint4* __restrict__ P, Grid g, int* __
     atomicAdd(&counts[c], 1);
 }
 
-// exclusive scan of counts[0..ncell) into starts[0..ncell], one block of 1024;
-// also the maximum cell occupancy into *maxocc
+// exclusive scan of counts[0..ncell) into starts[0..ncell] by one block of
+// 1024 threads, in chunks of 8192 staged through LDS (coalesced loads, each
+// thread scans 8 contiguous entries, wave prefix sums via shuffles); also the
+// maximum occupancy and the number of non-empty cells.
 __global__ __launch_bounds__(1024) void k_scan(int ncell, const int* __restrict__ counts, int* __restrict__ starts,
-                                                int* __restrict__ maxocc) {
-    __shared__ int part[1024];
-    __shared__ int mx[1024];
-    int t = threadIdx.x;
-    int per = (ncell + 1023) / 1024;
-    int b = t * per, e = min(ncell, b + per);
-    int s = 0, m = 0;
-    for (int i = b; i < e; i++) { s += counts[i]; m = max(m, counts[i]); }
-    part[t] = s;
-    mx[t] = m;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int v = (t >= off) ? part[t - off] : 0;
-        int w = (t + off < 1024) ? mx[t + off] : 0;
+                                                int* __restrict__ maxocc, int* __restrict__ nonempty) {
+    __shared__ int tile[8192];
+    __shared__ int wsum[16];
+    __shared__ int wmax[16], wne[16];
+    int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int carry = 0, mx = 0, ne = 0;
+    for (int c0 = 0; c0 < ncell; c0 += 8192) {
+        for (int i = 0; i < 8; i++) {
+            int idx = c0 + t + 1024 * i;
+            tile[t + 1024 * i] = idx < ncell ? counts[idx] : 0;
+        }
         __syncthreads();
-        part[t] += v;
-        mx[t] = max(mx[t], w);
+        int v[8], sum = 0;
+        for (int i = 0; i < 8; i++) {
+            v[i] = tile[8 * t + i];
+            sum += v[i];
+            mx = max(mx, v[i]);
+            ne += v[i] > 0 ? 1 : 0;
+        }
+        int incl = sum;
+        for (int o = 1; o < 64; o <<= 1) {
+            int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (lane == 63) wsum[wv] = incl;
+        __syncthreads();
+        int wpre = 0, total = 0;
+        for (int w = 0; w < 16; w++) {
+            if (w < wv) wpre += wsum[w];
+            total += wsum[w];
+        }
+        int run = carry + wpre + incl - sum;
+        for (int i = 0; i < 8; i++) {
+            int idx = c0 + 8 * t + i;
+            if (idx < ncell) starts[idx] = run;
+            run += v[i];
+        }
+        carry += total;
         __syncthreads();
     }
-    int run = part[t] - s;
-    for (int i = b; i < e; i++) { starts[i] = run; run += counts[i]; }
-    if (t == 1023) starts[ncell] = part[1023];
-    if (t == 0) *maxocc = mx[0];
+    for (int o = 32; o >= 1; o >>= 1) {
+        mx = max(mx, __shfl_xor(mx, o, 64));
+        ne += __shfl_xor(ne, o, 64);
+    }
+    if (lane == 0) { wmax[wv] = mx; wne[wv] = ne; }
+    __syncthreads();
+    if (t == 0) {
+        int m = 0, e = 0;
+        for (int w = 0; w < 16; w++) { m = max(m, wmax[w]); e += wne[w]; }
+        *maxocc = m;
+        *nonempty = e;
+        starts[ncell] = carry;
+    }
 }
 
 __global__ void k_cell_scatter(int n, const int* __restrict__ cell_tmp, const int* __restrict__ starts,
@@ -477,6 +509,8 @@ struct MoveParams {
     int ox, oy, oz;              // colour offset
     int p;                       // colour period (r+1)
     int nact[3];                 // active cells per axis
+    int trial_mode;              // 0: trials = cell occupancy; 1: dithered mean occupancy
+    int n;                       // particles
 };
 
 __device__ __forceinline__ V3<float> unit_from(float a, float b) {
@@ -486,71 +520,143 @@ __device__ __forceinline__ V3<float> unit_from(float a, float b) {
     return mk<float>(s * cosf(ph), s * sinf(ph), z);
 }
 
-// Energy of a moving particle (as part1) at the old and the new state against
-// every other staged particle, both in one pass over the stage.
+// Moving particle's derived vectors kept in LDS during a trial (27 floats per
+// state, DPart order: dir, pd[2], sd[4], ax[2]).
+#define MV_STRIDE 28
+
+__device__ __forceinline__ void mv_store(float* mv, const DPart<float>& p) {
+    const V3<float>* v[9] = {&p.dir, &p.pd[0], &p.pd[1], &p.sd[0], &p.sd[1], &p.sd[2], &p.sd[3], &p.ax[0], &p.ax[1]};
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        mv[3 * k] = v[k]->x;
+        mv[3 * k + 1] = v[k]->y;
+        mv[3 * k + 2] = v[k]->z;
+    }
+}
+__device__ __forceinline__ V3<float> mv_vec(const float* mv, int off) { return mk<float>(mv[off], mv[off + 1], mv[off + 2]); }
+__device__ __forceinline__ PatchView<float> mv_view(const float* mv, int k) {
+    PatchView<float> v;
+    v.pd = mv_vec(mv, 3 + 3 * k);
+    v.s0 = mv_vec(mv, 9 + 6 * k);
+    v.s1 = mv_vec(mv, 12 + 6 * k);
+    v.ax = mv_vec(mv, 21 + 3 * k);
+    return v;
+}
+__device__ __forceinline__ DPart<float> mv_part(const float* mv, int t, int g) {
+    DPart<float> p;
+    p.dir = mv_vec(mv, 0);
+    p.pd[0] = mv_vec(mv, 3); p.pd[1] = mv_vec(mv, 6);
+    p.sd[0] = mv_vec(mv, 9); p.sd[1] = mv_vec(mv, 12); p.sd[2] = mv_vec(mv, 15); p.sd[3] = mv_vec(mv, 18);
+    p.ax[0] = mv_vec(mv, 21); p.ax[1] = mv_vec(mv, 24);
+    p.t = t;
+    p.g = g;
+    return p;
+}
+
+// Energy of the moving particle (as part1, oneToAllTrial, totalenergycalculator.h:383-415)
+// at its old and new state against every other staged particle, in three
+// compacted phases so that every phase runs with full lanes:
+//   A  candidates inside sqmaxcut at either state -> near list
+//   B  (near, state) items: segment distance + WCA -> expensive list
+//   C  (expensive, patch pair) items: wedge intersections + attraction
+struct TrialScratch {
+    int* nl;     // near list, cap entries
+    int* xl;     // expensive list, 2*cap entries
+    float* mv;   // 2 * MV_STRIDE floats
+};
+
 __device__ void trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un, const DPart<float>& po,
                                const DPart<float>& pn, const SysConst& sc, const PP32* __restrict__ pp,
-                               const TP32* __restrict__ tp, int* queue, double& eo, double& en) {
+                               const TP32* __restrict__ tp, int ppmax, const TrialScratch& T, double& eo,
+                               double& en) {
     int lane = lane_id();
-    double ao = 0.0, an = 0.0;
-    int qn = 0;
     int t = uo.w;
+    bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
+    if (lane == 0) {
+        mv_store(T.mv, po);
+        mv_store(T.mv + MV_STRIDE, pn);
+    }
+    // phase A
+    int nn = 0;
     for (int base = 0; base < S.n; base += WAVE) {
-        int j = base + lane;
-        bool need0 = false, need1 = false;
+        int j = base + lane, fl = 0;
         if (j < S.n && j != k) {
             uint4 b = S.u[j];
-            const PP32& P2 = pp[t * sc.nt + b.w];
             V3<float> ro = fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-            V3<float> rn = fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-            float r2o = dot(ro, ro), r2n = dot(rn, rn);
-            if (r2o <= sc.sqmaxcut || r2n <= sc.sqmaxcut) {
-                DPart<float> pb;
-                pb.dir = normalised(mk<float>(S.o[j].x, S.o[j].y, S.o[j].z));
+            bool io = dot(ro, ro) <= sc.sqmaxcut, in_ = io;
+            if (moved) {
+                V3<float> rn = fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+                in_ = dot(rn, rn) <= sc.sqmaxcut;
+            }
+            fl = (io ? 1 : 0) | (in_ ? 2 : 0);
+        }
+        unsigned long long m = __ballot(fl != 0);
+        if (fl) T.nl[nn + wave_prefix(m)] = j | (fl << 28);
+        nn += __popcll(m);
+    }
+    __syncthreads();
+    // phase B
+    double ao = 0.0, an = 0.0;
+    int nx = 0;
+    V3<float> dold = po.dir, dnew = pn.dir;
+    for (int base = 0; base < 2 * nn; base += WAVE) {
+        int it = base + lane, jj = 0, which = 0;
+        bool need = false;
+        if (it < 2 * nn) {
+            int e = T.nl[it >> 1];
+            which = it & 1;
+            jj = e & 0x0fffffff;
+            if ((e >> (28 + which)) & 1) {
+                uint4 b = S.u[jj];
+                const PP32& P2 = pp[t * sc.nt + b.w];
+                V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
+                                    : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+                float ev = 0.0f;
                 if (P2.kind <= K_SCA && P2.kind >= 0) {
-                    float erep, d2;
-                    if (r2o <= sc.sqmaxcut) { need0 = scsc_cheap(P2, po, pb, ro, erep, d2); ao += (double)erep; }
-                    if (r2n <= sc.sqmaxcut) { need1 = scsc_cheap(P2, pn, pb, rn, erep, d2); an += (double)erep; }
+                    DPart<float> pa, pb;
+                    pa.dir = which ? dnew : dold;
+                    pb.dir = normalised(mk<float>(S.o[jj].x, S.o[jj].y, S.o[jj].z));
+                    float d2;
+                    need = scsc_cheap(P2, pa, pb, r, ev, d2);
                 } else if (P2.kind == K_SPH_WCA || P2.kind == K_SPH_TAYLOR) {
-                    if (r2o <= sc.sqmaxcut) ao += (double)sphere_pair(P2, sqrtf(r2o));
-                    if (r2n <= sc.sqmaxcut) an += (double)sphere_pair(P2, sqrtf(r2n));
+                    ev = sphere_pair(P2, sqrtf(dot(r, r)));
                 } else if (P2.kind != K_NONE) {
-                    need0 = r2o <= sc.sqmaxcut;
-                    need1 = r2n <= sc.sqmaxcut;
+                    need = true;
                 }
+                if (which) an += (double)ev; else ao += (double)ev;
             }
         }
-        unsigned long long m0 = __ballot(need0), m1 = __ballot(need1);
-        int c0 = __popcll(m0);
-        if (need0) queue[qn + wave_prefix(m0)] = j;
-        if (need1) queue[qn + c0 + wave_prefix(m1)] = j | (1 << 30);
-        qn += c0 + __popcll(m1);
-        __syncthreads();
-        bool last = base + WAVE >= S.n;
-        if (qn >= WAVE || (last && qn > 0)) {
-            int take = last ? qn : (qn / WAVE) * WAVE;
-            for (int e0 = 0; e0 < take; e0 += WAVE) {
-                int e = e0 + lane;
-                if (e < take) {
-                    int w = queue[e];
-                    int jj = w & 0x3fffffff;
-                    bool nw = (w >> 30) & 1;
-                    const uint4& u1 = nw ? un : uo;
-                    uint4 b = S.u[jj];
-                    V3<float> r = fx_image_c(u1.x, u1.y, u1.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-                    const PP32& P2 = pp[t * sc.nt + b.w];
+        unsigned long long m = __ballot(need);
+        if (need) T.xl[nx + wave_prefix(m)] = jj | (which << 30);
+        nx += __popcll(m);
+    }
+    __syncthreads();
+    // phase C
+    int items = nx * ppmax;
+    for (int base = 0; base < items; base += WAVE) {
+        int it = base + lane;
+        if (it < items) {
+            int e = it / ppmax, mpp = it - e * ppmax;
+            int w = T.xl[e];
+            int jj = w & 0x3fffffff, which = (w >> 30) & 1;
+            uint4 b = S.u[jj];
+            const PP32& P2 = pp[t * sc.nt + b.w];
+            V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
+                                : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+            const float* mv = T.mv + which * MV_STRIDE;
+            float ev = 0.0f;
+            if (P2.kind <= K_SCA) {
+                if (mpp < npatch_pairs(P2)) {
+                    int k1, k2;
+                    patch_pair_index(P2, mpp, k1, k2);
                     DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
-                    float ev = (P2.kind <= K_SCA) ? scsc_attr(P2, nw ? pn : po, pb, r) : spsc(P2, nw ? pn : po, pb, r);
-                    if (nw) an += (double)ev; else ao += (double)ev;
+                    ev = patch_pair_v<float>(P2, mv_view(mv, k1), view(pb, k2), r, k1, k2);
                 }
+            } else if (mpp == 0) {
+                DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
+                ev = spsc(P2, mv_part(mv, t, tp[t].geotype), pb, r);
             }
-            __syncthreads();
-            int rest = qn - take;
-            int carry = (lane < rest) ? queue[take + lane] : 0;
-            __syncthreads();
-            if (lane < rest) queue[lane] = carry;
-            __syncthreads();
-            qn = rest;
+            if (which) an += (double)ev; else ao += (double)ev;
         }
     }
     eo = wave_sum(ao);
@@ -561,6 +667,7 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
                                                      uint4* __restrict__ P, float4* __restrict__ O,
                                                      float2* __restrict__ Q, const int* __restrict__ ID,
                                                      const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
+                                                     int ppmax, const int* __restrict__ nonempty,
                                                      unsigned long long* __restrict__ stats, double* __restrict__ de_sum,
                                                      int* __restrict__ flag) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -575,21 +682,36 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
     S.o = (float4*)(S.u + cap);
     S.q = (float2*)(S.o + cap);
     S.id = (int*)(S.q + cap);
-    int* queue = S.id + cap;                 // 2*cap+64 entries
-    int* scratch = queue + 2 * cap + WAVE;
+    TrialScratch T;
+    T.nl = S.id + cap;
+    T.xl = T.nl + cap;
+    T.mv = (float*)(T.xl + 2 * cap);
+    int* scratch = (int*)(T.mv + 2 * MV_STRIDE);
     int lane = lane_id();
     if (!stage_load(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
         if (lane == 0) atomicOr(flag, 1);
         return;
     }
     const float Lx = sc.L[0], Ly = sc.L[1], Lz = sc.L[2];
+    U4 ctr;
+    ctr.y = (uint32_t)cell;
+    ctr.z = mp.pass_lo;
+    // trials in this cell: its occupancy (every particle once per sweep on
+    // average), or with trial_mode 1 the mean occupancy of the non-empty
+    // cells, dithered, so the longest chain of a pass is not the fullest cell
+    int ntrial = nh;
+    if (mp.trial_mode == 1) {
+        ctr.x = 0xffffffffu;
+        ctr.w = (mp.pass_hi << 2) | 3u;
+        U4 rd = philox(ctr, mp.key0, mp.key1);
+        float mean = (float)mp.n / (float)max(1, *nonempty);
+        int fl = (int)floorf(mean);
+        ntrial = fl + ((u01f(rd.x) < mean - (float)fl) ? 1 : 0);
+    }
     unsigned long long n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0;
     double dsum = 0.0;
-    for (int t = 0; t < nh; t++) {
-        U4 ctr;
+    for (int t = 0; t < ntrial; t++) {
         ctr.x = (uint32_t)t;
-        ctr.y = (uint32_t)cell;
-        ctr.z = mp.pass_lo;
         ctr.w = mp.pass_hi << 2;
         U4 ra = philox(ctr, mp.key0, mp.key1);
         int k = S.h0 + (int)(((uint64_t)ra.x * (uint64_t)nh) >> 32);
@@ -629,10 +751,12 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
             on = make_float4(d.x, d.y, d.z, pq.x);
             qn = make_float2(pq.y, pq.z);
         }
-        DPart<float> po = load_part<float>(oo, qo, uo.w, tp);
-        DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
         double eo, en;
-        trial_energies(S, k, uo, un, po, pn, sc, pp, tp, queue, eo, en);
+        {
+            DPart<float> po = load_part<float>(oo, qo, uo.w, tp);
+            DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
+            trial_energies(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, eo, en);
+        }
         double de = en - eo;
         bool acc = de <= 0.0;
         if (!acc) {
@@ -666,6 +790,11 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
         atomicAdd(&stats[4], n_left);
         atomicAdd(de_sum, dsum);
     }
+}
+
+// plain streaming copy: the calibration kernel for HBM byte counters
+__global__ void k_copy(long n16, const uint4* __restrict__ a, uint4* __restrict__ b) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n16; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
 
 // ------------------------------------------------------------------ context
@@ -712,6 +841,9 @@ struct sc_ctx {
     int d_out_cap = 0;
     int* d_flag = nullptr;
     int* d_maxocc = nullptr;
+    int* d_nonempty = nullptr;
+    int trial_mode = 0;
+    int ppmax = 1;
     unsigned long long* d_stats = nullptr;
     double* d_dsum = nullptr;
     Query* d_q = nullptr;
@@ -832,6 +964,9 @@ static int upload_tables(sc_ctx* ctx) {
             fill_pair(h64[a * nt + b], p, tr);
         }
     }
+    ctx->ppmax = 1;
+    for (const PP32& q : h32)
+        if (q.kind >= 0 && q.kind <= K_SCA) ctx->ppmax = std::max(ctx->ppmax, npatch_pairs(q));
     int rc;
     if ((rc = dalloc(ctx, &ctx->d_pp32, h32.size()))) return rc;
     if ((rc = dalloc(ctx, &ctx->d_pp64, h64.size()))) return rc;
@@ -890,7 +1025,8 @@ static int build_grid(sc_ctx* ctx, int r, const uint32_t shift[3], const double*
     HIPCK(hipMemsetAsync(ctx->counts, 0, sizeof(int) * ncell, ctx->stream));
     HIPCK(hipMemsetAsync(ctx->fill, 0, sizeof(int) * ncell, ctx->stream));
     if (n > 0) hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->P[c], g, ctx->cell_tmp, ctx->counts);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, ncell, ctx->counts, ctx->starts, ctx->d_maxocc);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, ncell, ctx->counts, ctx->starts, ctx->d_maxocc,
+                       ctx->d_nonempty);
     if (n > 0) {
         hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->cell_tmp, ctx->starts, ctx->fill, ctx->src_of_pos);
         hipLaunchKernelGGL(k_cell_sort, dim3((ncell + 255) / 256), dim3(256), 0, ctx->stream, ncell, ctx->starts, ctx->src_of_pos, ctx->ID[c]);
@@ -1001,6 +1137,7 @@ int sc_create(sc_ctx** out, int device) {
     hipEventCreate(&ctx->ev1);
     if (hipMalloc((void**)&ctx->d_flag, sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&ctx->d_maxocc, sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_nonempty, sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&ctx->d_stats, 8 * sizeof(unsigned long long)) != hipSuccess ||
         hipMalloc((void**)&ctx->d_dsum, sizeof(double)) != hipSuccess) {
         delete ctx;
@@ -1019,7 +1156,7 @@ void sc_destroy(sc_ctx* ctx) {
     void* ptrs[] = {ctx->d_pp32, ctx->d_pp64, ctx->d_tp32, ctx->d_tp64, ctx->P[0], ctx->P[1], ctx->O[0], ctx->O[1],
                     ctx->Q[0], ctx->Q[1], ctx->ID[0], ctx->ID[1], ctx->slot, ctx->cell_tmp, ctx->src_of_pos,
                     ctx->starts, ctx->counts, ctx->fill, ctx->partial, ctx->d_out, ctx->d_flag, ctx->d_maxocc,
-                    ctx->d_stats, ctx->d_dsum, ctx->d_q, ctx->d_i, ctx->d_j};
+                    ctx->d_nonempty, ctx->d_stats, ctx->d_dsum, ctx->d_q, ctx->d_i, ctx->d_j};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
@@ -1475,7 +1612,8 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
         if (rc) return rc;
         const Grid& g = ctx->g;
         int cap = stage_cap(ctx, g, ctx->box);
-        size_t lds = stage_bytes(cap) + sizeof(int) * (2 * cap + WAVE) + sizeof(int) * (2 * 125 + 2);
+        size_t lds = stage_bytes(cap) + sizeof(int) * (3 * cap) + sizeof(float) * 2 * MV_STRIDE +
+                     sizeof(int) * (2 * 125 + 2);
         // colour order: a random permutation of the p^3 colours
         int ncol = p * p * p;
         int order[27];
@@ -1497,6 +1635,8 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
             if (mp.ox >= pa[0] || mp.oy >= pa[1] || mp.oz >= pa[2]) continue;   // axis with a single cell
             mp.p = p;
             for (int k = 0; k < 3; k++) mp.nact[k] = g.nc[k] / pa[k];
+            mp.trial_mode = ctx->trial_mode;
+            mp.n = ctx->n;
             mp.transmx = (float)transmx;
             mp.rot_half = (float)rot_half;
             mp.temper = temper;
@@ -1517,7 +1657,7 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
             }
             hipLaunchKernelGGL(k_sweep_pass, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp, ctx->starts,
                                ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,
-                               ctx->d_stats, ctx->d_dsum, ctx->d_flag);
+                               ctx->ppmax, ctx->d_nonempty, ctx->d_stats, ctx->d_dsum, ctx->d_flag);
             if (ctx->profile) HIPCK(hipEventRecord(ctx->kev[2 * launches + 1], ctx->stream));
             launches++;
         }
@@ -1593,6 +1733,32 @@ int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double e
     }
     if ((rc = build_grid(ctx, 2, nullptr, ctx->box))) return rc;
     if (box_out) for (int k = 0; k < 3; k++) box_out[k] = ctx->box[k];
+    return SC_OK;
+}
+
+int sc_copy_probe(sc_ctx* ctx, long bytes, int reps) {
+    if (!ctx || bytes <= 0 || reps < 1) return SC_EINVAL;
+    if (ctx->device < 0) return fail(ctx, SC_ENODEV, "host-only context");
+    HIPCK(hipSetDevice(ctx->device));
+    long n16 = bytes / 16;
+    uint4 *a = nullptr, *b = nullptr;
+    HIPCK(hipMalloc((void**)&a, n16 * 16));
+    if (hipMalloc((void**)&b, n16 * 16) != hipSuccess) { hipFree(a); return fail(ctx, SC_EHIP, "copy probe alloc"); }
+    hipError_t e = hipMemsetAsync(a, 1, n16 * 16, ctx->stream);
+    for (int r = 0; r < reps && e == hipSuccess; r++) {
+        hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, ctx->stream, n16, a, b);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) return fail(ctx, SC_EHIP, hipGetErrorString(e));
+    return SC_OK;
+}
+
+int sc_set_trial_mode(sc_ctx* ctx, int mode) {
+    if (!ctx || mode < 0 || mode > 1) return SC_EINVAL;
+    ctx->trial_mode = mode;
     return SC_OK;
 }
 

@@ -342,38 +342,82 @@ template <typename R> SC_HD R cos_well(R dist, R pdis, R pswitch, R eps) {
     return e * (-e * eps);
 }
 
+// One patch of one particle as seen by the patch-pair functor: the axis the
+// patch is attached to, the patch direction and the two side vectors.
+template <typename R>
+struct PatchView {
+    V3<R> ax, pd, s0, s1;
+};
+
+template <typename R>
+SC_HD PatchView<R> view(const DPart<R>& p, int k) {
+    PatchView<R> v;
+    v.ax = k ? p.ax[1] : p.ax[0];
+    v.pd = k ? p.pd[1] : p.pd[0];
+    v.s0 = k ? p.sd[2] : p.sd[0];
+    v.s1 = k ? p.sd[3] : p.sd[1];
+    return v;
+}
+
 // One patch pair of two spherocylinders: Psc/CPsc/PscCPsc::operator()
 // (paire.h:469-484, :658-672, :864-889) and EPatch::atrE (paire.cpp:244-301).
-template <typename R, int k1, int k2>
-SC_HD R patch_pair(const PairParamsT<R>& P, const DPart<R>& a, const DPart<R>& b, const V3<R>& r) {
+// k1/k2 select the patch of each particle; they only pick parameter indices,
+// so one code path serves all four patch pairs (no divergence between lanes
+// evaluating different patch pairs).
+template <typename R>
+SC_HD R patch_pair_v(const PairParamsT<R>& P, const PatchView<R>& A, const PatchView<R>& Bv, const V3<R>& r, int k1,
+                     int k2) {
     bool c1 = (P.kind == K_CPSC) || (P.kind == K_MIXED && !is_psc(P.g0));
     bool c2 = (P.kind == K_CPSC) || (P.kind == K_MIXED && is_psc(P.g0));
+    R sw1 = k1 ? P.pcanglsw[2] : P.pcanglsw[0], pc1 = k1 ? P.pcangl[2] : P.pcangl[0];
+    R sw2 = k2 ? P.pcanglsw[3] : P.pcanglsw[1], pc2 = k2 ? P.pcangl[3] : P.pcangl[1];
     IQ<R> q;
-    q.p1 = a.ax[k1]; q.p2 = b.ax[k2]; q.pdir = a.pd[k1]; q.s0 = a.sd[2 * k1]; q.s1 = a.sd[2 * k1 + 1];
-    q.r = r; q.cossw = P.pcanglsw[2 * k1]; q.rcSq = P.rcutSq; q.h1 = P.hl0; q.h2 = P.hl1;
+    q.p1 = A.ax; q.p2 = Bv.ax; q.pdir = A.pd; q.s0 = A.s0; q.s1 = A.s1;
+    q.r = r; q.cossw = sw1; q.rcSq = P.rcutSq; q.h1 = P.hl0; q.h2 = P.hl1;
     R T1, T2, S1, S2;
     if (!intersect(q, c1, T1, T2)) return R(0);
-    q.p1 = b.ax[k2]; q.p2 = a.ax[k1]; q.pdir = b.pd[k2]; q.s0 = b.sd[2 * k2]; q.s1 = b.sd[2 * k2 + 1];
-    q.r = mk<R>(r.x * R(-1), r.y * R(-1), r.z * R(-1)); q.cossw = P.pcanglsw[2 * k2 + 1]; q.h1 = P.hl1; q.h2 = P.hl0;
+    q.p1 = Bv.ax; q.p2 = A.ax; q.pdir = Bv.pd; q.s0 = Bv.s0; q.s1 = Bv.s1;
+    q.r = mk<R>(r.x * R(-1), r.y * R(-1), r.z * R(-1)); q.cossw = sw2; q.h1 = P.hl1; q.h2 = P.hl0;
     if (!intersect(q, c2, S1, S2)) return R(0);
     // atrE
-    const V3<R>& d1 = a.ax[k1];
-    const V3<R>& d2 = b.ax[k2];
+    const V3<R>& d1 = A.ax;
+    const V3<R>& d2 = Bv.ax;
     R v1 = fabs(S1 - S2), v2 = fabs(T1 - T2), f0 = R(0.5) * (v1 + v2);
     R m1 = (S1 + S2) * R(0.5), m2 = (T1 + T2) * R(0.5);
     V3<R> vi = mk<R>(d2.x * m2 - d1.x * m1 - r.x, d2.y * m2 - d1.y * m1 - r.y, d2.z * m2 - d1.z * m1 - r.z);
     V3<R> vm = min_dist(d1, d2, v1, v2, vi);
     R e = cos_well<R>((R)sqrt(dot(vm, vm)), P.pdis, P.pswitch, P.eps);
     V3<R> w = perp(vi, d1);
-    R f1 = fangl(dot(w, a.pd[k1]) / len(w), P.pcangl[2 * k1], P.pcanglsw[2 * k1]);
+    R f1 = fangl(dot(w, A.pd) / len(w), pc1, sw1);
     w = perp(mk<R>(vi.x * R(-1), vi.y * R(-1), vi.z * R(-1)), d2);
-    R f2 = fangl(dot(w, b.pd[k2]) / len(w), P.pcangl[1 + 2 * k2], P.pcanglsw[1 + 2 * k2]);
+    R f2 = fangl(dot(w, Bv.pd) / len(w), pc2, sw2);
     R paral = R(1);
     if (P.parallel != R(0)) {
         R ca = dot(d1, d2);
         if ((P.parallel > R(0) && ca > R(0)) || (P.parallel < R(0) && ca < R(0))) paral = R(1) + P.parallel * ca;
     }
     return e * (f0 * f1 * f2 * paral);
+}
+
+template <typename R, int k1, int k2>
+SC_HD R patch_pair(const PairParamsT<R>& P, const DPart<R>& a, const DPart<R>& b, const V3<R>& r) {
+    return patch_pair_v<R>(P, view(a, k1), view(b, k2), r, k1, k2);
+}
+
+// number of patch pairs of an SC-SC pair and the (k1,k2) of the m-th, in the
+// order SpheroCylinder::operator() sums them (paire.h:1141-1170)
+template <typename R>
+SC_HD int npatch_pairs(const PairParamsT<R>& P) {
+    return 1 + (P.fT ? 1 : 0) + (P.sT ? 1 : 0) + ((P.fT && P.sT) ? 1 : 0);
+}
+template <typename R>
+SC_HD void patch_pair_index(const PairParamsT<R>& P, int m, int& k1, int& k2) {
+    // m = 0 -> (0,0); then (1,0) if fT; then (0,1) if sT; then (1,1)
+    k1 = 0; k2 = 0;
+    if (m == 0) return;
+    if (P.fT) { if (m == 1) { k1 = 1; return; } m--; }
+    if (P.sT) { if (m == 1) { k2 = 1; return; } m--; }
+    k1 = 1; k2 = 1;
 }
 
 // WCA on a squared distance, WcaTruncSq (paire.h:385-393)

@@ -242,34 +242,53 @@ def test_sweep_energy_drift_and_determinism(dev):
         np.testing.assert_array_equal(a, b)
 
 
-def test_sweep_ensemble_matches_cpu_oracle(dev):
-    """Mean potential energy per particle and nematic order of 500 PSC at
-    eta 0.2, T*=1, from the checkerboard GPU sweep and from the sequential
-    CPU oracle Metropolis: equal within 3 combined standard errors."""
-    say("case: ensemble GPU vs CPU oracle")
+@pytest.fixture(scope="module")
+def ensemble_system():
+    """500 PSC at eta 0.2, T*=1 and the CPU oracle's sequential Metropolis
+    reference: mean E/N over 40 blocks of 25 sweeps after 200 sweeps."""
     w = W.workload("c1_psc_1000", n_override=500)
-    g = sc_amd.GpuSystem(dev)
+    g = sc_amd.GpuSystem(0)
     g.load_topology(w["top"])
     pos, d, q = g.generate_rsa(w["types"], w["box"], seed=31)
     tp, st = gpu_state(g, w["top"], w["box"], pos, d, q, w["types"])
+    g.close()
     rot_half = 20.0 / 180 * np.pi / 4
-    g.sweep(200, 1.0, 0.3, rot_half, r=2, seed=5, counter=0)
-    eg = []
-    for blk in range(40):
-        g.sweep(25, 1.0, 0.3, rot_half, r=2, seed=5, counter=1000 + 25 * blk)
-        eg.append(g.allToAll() / st.n)
     stc, _, _, _ = po.mc_run(tp, st, 200, 1.0, 0.3, rot_half, seed=17)
     ec = []
     cur = stc
     for blk in range(40):
         cur, _, _, ef = po.mc_run(tp, cur, 25, 1.0, 0.3, rot_half, seed=100 + blk)
         ec.append(ef / st.n)
-    eg, ec = np.array(eg), np.array(ec)
+    return w, (pos, d, q), np.array(ec), rot_half
 
-    def blocked(x, nb=8):
-        b = x[: len(x) // nb * nb].reshape(nb, -1).mean(1)
-        return b.mean(), b.std(ddof=1) / np.sqrt(nb)
-    mg, sg = blocked(eg)
-    mc, scc = blocked(ec)
-    say("E/N gpu %.5f +- %.5f, cpu %.5f +- %.5f" % (mg, sg, mc, scc))
+
+def _blocked(x, nb=8):
+    b = x[: len(x) // nb * nb].reshape(nb, -1).mean(1)
+    return b.mean(), b.std(ddof=1) / np.sqrt(nb)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_sweep_ensemble_matches_cpu_oracle(dev, ensemble_system, mode):
+    """Mean potential energy per particle of 500 PSC at eta 0.2, T*=1, from
+    the checkerboard GPU sweep (trial mode 0 and 1) and from the sequential
+    CPU oracle Metropolis: equal within 3 combined standard errors."""
+    say("case: ensemble GPU vs CPU oracle, trial mode", mode)
+    w, (pos, d, q), ec, rot_half = ensemble_system
+    g = sc_amd.GpuSystem(dev)
+    g.load_topology(w["top"])
+    g.set_config(w["box"], pos, d, q, w["types"])
+    g.set_trial_mode(mode)
+    n = len(w["types"])
+    g.sweep(200, 1.0, 0.3, rot_half, r=2, seed=5, counter=0)
+    eg = []
+    tries = 0
+    for blk in range(40):
+        s = g.sweep(25, 1.0, 0.3, rot_half, r=2, seed=5, counter=1000 + 25 * blk)
+        tries += s["trans_try"] + s["rot_try"]
+        eg.append(g.allToAll() / n)
+    eg = np.array(eg)
+    mg, sg = _blocked(eg)
+    mc, scc = _blocked(ec)
+    say("E/N gpu %.5f +- %.5f, cpu %.5f +- %.5f, trials/sweep/N %.4f" % (mg, sg, mc, scc, tries / (1000.0 * n)))
     assert abs(mg - mc) <= 3.0 * np.hypot(sg, scc) + 0.01 * abs(mc)
+    assert abs(tries / (1000.0 * n) - 1.0) < 0.02
