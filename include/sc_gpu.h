@@ -9,7 +9,7 @@
  * Updater (updater.h:33), MoveCreator (movecreator.h:36) and ClusterSampler
  * (clust.h:23).  Each entry point below names the reference method it
  * replaces.  A C++ class with the reference's method set (GpuTotalE in
- * sc_amd/csrc/sc_host.h) wraps this ABI for in-process C++ callers.
+ * include/sc_gpu_totale.hpp) wraps this ABI for in-process C++ callers.
  *
  * Conventions:
  *   - every function returns 0 on success and a negative SC_E* code on

@@ -1,0 +1,113 @@
+// sc_gpu_totale.hpp -- drop-in TotalEnergyCalculator for robertvacha/SC
+// (scOOP 4.0) backed by the gfx950 library (include/sc_gpu.h).
+//
+// The reference selects its energy backend with one typedef
+// (scOOP/mc/totalenergycalculator.h:1225-1238):
+//     typedef TotalEMatrix<PairE> TotalEnergyCalculator;
+// GpuTotalE<Sim, Conf> has the method set its callers use
+// (Updater, updater.h:33; MoveCreator, movecreator.h:36; ClusterSampler,
+// clust.h:23) with the same call protocol:
+//   particle move: E_old = oneToAll(t); caller mutates conf->pvec[t];
+//                  E_new = oneToAllTrial(t); accept -> update(t);
+//                  reject -> caller restores pvec[t] (device state untouched)
+//                  (MoveCreator::partDisplace / partRotate,
+//                   movecreator.cpp:947-1028)
+//   volume move:   E_old = allToAll(); caller mutates conf->geo.box;
+//                  E_new = allToAllTrial(); accept -> update()
+//                  (MoveCreator::pressureMove, movecreator.cpp:330-421)
+// It only reads the members the reference's Particle / Conf carry (pos as box
+// fractions, dir, patchdir[0], type; geo.box; pvec.size()).  Errors are
+// reported on stderr and return 0.0 / leave the state, as the reference's
+// functors do for unprogrammed pairs (paire.h:207-212).
+#pragma once
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "sc_gpu.h"
+
+template <typename SimT, typename ConfT>
+class GpuTotalE {
+  public:
+    // top_init_text: the top.init the Inicializer read (the library derives
+    // the same pair table, Topo::genParamPairs / genTopoParams)
+    GpuTotalE(SimT* sim, ConfT* conf, const std::string& top_init_text, int device = 0)
+        : sim_(sim), conf_(conf) {
+        if (sc_create(&ctx_, device) != SC_OK) {
+            fprintf(stderr, "GpuTotalE: no usable gfx950 device %d\n", device);
+            ctx_ = nullptr;
+            return;
+        }
+        check(sc_load_topology(ctx_, top_init_text.c_str()), "sc_load_topology");
+    }
+    ~GpuTotalE() { sc_destroy(ctx_); }
+    GpuTotalE(const GpuTotalE&) = delete;
+    GpuTotalE& operator=(const GpuTotalE&) = delete;
+
+    // TotalEMatrix::initEM (totalenergycalculator.h:314): upload Conf::pvec
+    void initEM() {
+        size_t n = conf_->pvec.size();
+        std::vector<double> pos(3 * n), dir(3 * n), pd0(3 * n);
+        std::vector<int32_t> types(n);
+        for (size_t i = 0; i < n; i++) {
+            const auto& p = conf_->pvec[i];
+            pos[3 * i] = p.pos.x; pos[3 * i + 1] = p.pos.y; pos[3 * i + 2] = p.pos.z;
+            dir[3 * i] = p.dir.x; dir[3 * i + 1] = p.dir.y; dir[3 * i + 2] = p.dir.z;
+            pd0[3 * i] = p.patchdir[0].x; pd0[3 * i + 1] = p.patchdir[0].y; pd0[3 * i + 2] = p.patchdir[0].z;
+            types[i] = p.type;
+        }
+        double box[3] = {conf_->geo.box.x, conf_->geo.box.y, conf_->geo.box.z};
+        check(sc_set_config(ctx_, (int)n, box, pos.data(), dir.data(), pd0.data(), types.data()), "sc_set_config");
+    }
+    // TotalEMatrix::allToAll (:338-353)
+    double allToAll() { double e = 0; check(sc_energy_all(ctx_, &e), "sc_energy_all"); return e; }
+    // TotalEMatrix::allToAllTrial (:334-336): the box the caller just changed
+    double allToAllTrial() {
+        double box[3] = {conf_->geo.box.x, conf_->geo.box.y, conf_->geo.box.z}, e = 0;
+        check(sc_energy_all_trial(ctx_, box, &e), "sc_energy_all_trial");
+        return e;
+    }
+    // TotalEMatrix::oneToAll (:355-381)
+    double oneToAll(int target) { double e = 0; check(sc_energy_one(ctx_, target, &e), "sc_energy_one"); return e; }
+    // TotalEMatrix::oneToAllTrial (:383-415): the state the caller just wrote into pvec[target]
+    double oneToAllTrial(int target) {
+        double pos[3], dir[3], pd0[3], e = 0;
+        read(target, pos, dir, pd0);
+        check(sc_energy_one_trial(ctx_, target, pos, dir, pd0, &e), "sc_energy_one_trial");
+        return e;
+    }
+    // TotalEMatrix::update(int) (:326-328)
+    void update(int target) {
+        double pos[3], dir[3], pd0[3];
+        read(target, pos, dir, pd0);
+        check(sc_commit_particle(ctx_, target, pos, dir, pd0), "sc_commit_particle");
+    }
+    // TotalEMatrix::update() (:318-320): commit the box of a volume move
+    void update() {
+        double box[3] = {conf_->geo.box.x, conf_->geo.box.y, conf_->geo.box.z};
+        check(sc_commit_box(ctx_, box), "sc_commit_box");
+    }
+    // TotalE::p2p (:200-203)
+    double p2p(int part1, int part2) {
+        double e = 0;
+        int32_t i = part1, j = part2;
+        check(sc_pair_energies(ctx_, 1, &i, &j, &e, 0), "sc_pair_energies");
+        return e;
+    }
+    sc_ctx* context() { return ctx_; }
+
+  private:
+    void read(int t, double* pos, double* dir, double* pd0) const {
+        const auto& p = conf_->pvec[t];
+        pos[0] = p.pos.x; pos[1] = p.pos.y; pos[2] = p.pos.z;
+        dir[0] = p.dir.x; dir[1] = p.dir.y; dir[2] = p.dir.z;
+        pd0[0] = p.patchdir[0].x; pd0[1] = p.patchdir[0].y; pd0[2] = p.patchdir[0].z;
+    }
+    void check(int rc, const char* what) const {
+        if (rc != SC_OK) fprintf(stderr, "GpuTotalE: %s failed: %s\n", what, sc_last_error(ctx_));
+    }
+    SimT* sim_;
+    ConfT* conf_;
+    sc_ctx* ctx_ = nullptr;
+};

@@ -83,3 +83,20 @@ def test_rsa_configuration_has_no_overlap():
     assert mind >= 1.0 - 1e-12
     assert np.allclose(np.linalg.norm(d, axis=1), 1.0)
     assert np.allclose((d * q).sum(1), 0.0, atol=1e-12)
+
+
+def test_cpp_dropin_adapter_compiles_and_links(tmp_path):
+    """include/sc_gpu_totale.hpp (the TotalEnergyCalculator replacement)
+    compiles as C++11 against the reference's member layout and links the
+    library; without a GPU the constructor reports the missing device."""
+    import subprocess
+    exe = tmp_path / "totale_check"
+    src = os.path.join(ROOT, "tests", "native", "totale_compile.cpp")
+    r = subprocess.run(["g++", "-std=c++11", "-I" + os.path.join(ROOT, "include"), src,
+                        "-L" + os.path.join(ROOT, "sc_amd", "lib"), "-lscamd",
+                        "-Wl,-rpath," + os.path.join(ROOT, "sc_amd", "lib"), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0
+    assert "context" in r.stdout
