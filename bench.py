@@ -1,4 +1,5 @@
-"""MC sweeps/s of the checkerboard Monte Carlo sweep on one MI355X.
+"""MC sweeps/s of the checkerboard Monte Carlo sweep on one MI355X, exact and
+with the learned surrogate's delayed acceptance.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4_tchpsc_65536]
 
@@ -8,8 +9,12 @@ synthetic, seeded state resident in HBM.  Prints one JSON line.
 
 The default workload is 65,536 TCHPSC (L=4, two chiral patches) at packing
 fraction 0.25, T*=1: the system the delayed-acceptance metric is quoted on.
-The CPU baseline is the FP64 oracle's sequential Metropolis (one core) on a
-bounded sample of sweeps of the same state.
+Both sweeps start from the same state: first K exact sweeps, then the
+surrogate is trained on the GPU (untimed: 10M exact pair energies over
+segment distances 0-2.5 sigma, uniform orientations), then K
+delayed-acceptance sweeps.  `value` is the delayed-acceptance rate; the exact
+rate sits beside it.  The CPU baseline is the FP64 oracle's sequential
+Metropolis (one core) on a bounded sample of sweeps of the same state.
 """
 import argparse
 import json
@@ -28,10 +33,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary, if present (profiles/<kernel>_pmc.txt, line 'bytes_per_launch:')."""
-    p = os.path.join(ROOT, "profiles", "%s_pmc.txt" % kernel)
+def read_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary
+    profiles/<name>_pmc.txt (line 'bytes_per_launch:'), if present."""
+    p = os.path.join(ROOT, "profiles", "%s_pmc.txt" % name)
     if not os.path.exists(p):
         return None
     for line in open(p):
@@ -57,6 +62,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2025)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--samples", type=int, default=10_000_000, help="surrogate training pairs")
+    ap.add_argument("--epochs", type=int, default=10)
+    ap.add_argument("--no-da", action="store_true", help="exact sweep only")
+    ap.add_argument("--no-exact", action="store_true", help="delayed-acceptance sweep only")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding); "
@@ -80,38 +89,68 @@ def main():
         g.copy_probe(256 << 20, reps=3)
         log("copy probe: 3 x k_copy of 256 MiB")
     g.set_trial_mode(a.trial_mode)
-    g.sweep(a.warmup, temper, transmx, rot_half, r=a.r, seed=a.seed, counter=1)
-    g.set_profiling(True)
-    t1 = time.perf_counter()
-    st = g.sweep(a.steps, temper, transmx, rot_half, r=a.r, seed=a.seed, counter=10 ** 6)
-    t2 = time.perf_counter()
-    wall = t2 - t1
-    dev_ms, launches, kern_ms = g.last_timing()
-    # sweeps actually performed: trial moves / N (trial mode 1 makes N trials
-    # per sweep on average, not exactly)
-    done = (st["trans_try"] + st["rot_try"]) / float(n)
-    sweeps_per_s = done / wall
-    e_n = g.allToAll() / n
-    log("exact sweep: %.2f sweeps/s (%.3f ms/sweep wall, %.3f ms device), %d pass launches, kernel %.3f ms" % (
-        sweeps_per_s, 1e3 * wall / a.steps, dev_ms / a.steps, launches, kern_ms))
-
-    # roofline of the dominant kernel (k_sweep_pass): algorithmic bytes per
-    # launch = every record of the stencils read once (N * 44 B: 40-byte
-    # record + 4-byte index), the active cells written back (N/ncol * 40 B)
-    # and the cell CSR (4 B per cell)
+    box0, pos0, d0, q0 = g.get_config()
     _, start, _, nc = g.build_cells(r=a.r)
     ncell = int(np.prod(nc))
     ncol = (a.r + 1) ** 3
+    # algorithmic bytes per k_sweep_pass launch: every record of the stencils
+    # read once (N * 44 B: 40-byte record + 4-byte index), the active cells
+    # written back (N/ncol * 40 B) and the cell CSR (4 B per cell)
     bytes_launch = n * 44.0 + n / ncol * 40.0 + 4.0 * (ncell + 1)
-    avg_launch_s = kern_ms / 1e3 / max(launches, 1)
-    achieved = bytes_launch / avg_launch_s / 1e9
-    traffic = read_traffic("k_sweep_pass")
+
+    def leg(label, da):
+        g.set_config(box0, pos0, d0, q0, w["types"])
+        g.set_surrogate(da)
+        g.set_profiling(False)
+        g.sweep(a.warmup, temper, transmx, rot_half, r=a.r, seed=a.seed, counter=1)
+        g.set_profiling(True)
+        t1 = time.perf_counter()
+        st = g.sweep(a.steps, temper, transmx, rot_half, r=a.r, seed=a.seed, counter=10 ** 6)
+        t2 = time.perf_counter()
+        wall = t2 - t1
+        dev_ms, launches, kern_ms = g.last_timing()
+        # sweeps actually performed: trial moves / N (trial mode 1 makes N
+        # trials per sweep on average, not exactly)
+        done = (st["trans_try"] + st["rot_try"]) / float(n)
+        sps = done / wall
+        avg_launch_s = kern_ms / 1e3 / max(launches, 1)
+        achieved = bytes_launch / avg_launch_s / 1e9
+        tries = max(1, st["trans_try"] + st["rot_try"])
+        acc = st["trans_acc"] + st["rot_acc"]
+        r = {"sweeps_per_s": sps, "ms_per_step": 1e3 * wall / a.steps, "wall_s": wall, "device_ms": dev_ms,
+             "trial_moves_per_particle": done / a.steps,
+             "acceptance": {"translation": st["trans_acc"] / max(1, st["trans_try"]),
+                            "rotation": st["rot_acc"] / max(1, st["rot_try"]),
+                            "left_cell": st["left_cell"] / max(1, st["trans_try"]), "overall": acc / tries},
+             "energy_per_particle": g.allToAll() / n,
+             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": achieved / HBM_PEAK_GBS,
+                          "traffic": read_traffic("k_sweep_pass_da" if da else "k_sweep_pass_exact"),
+                          "kernel": "k_sweep_pass<%s>" % ("true" if da else "false"),
+                          "bytes_per_launch": bytes_launch, "avg_launch_ms": 1e3 * avg_launch_s,
+                          "launches": launches}}
+        if da:
+            r["stage1_acceptance"] = st["stage1_acc"] / max(1, st["stage1_try"])
+            r["stage2_acceptance"] = acc / max(1, st["stage1_acc"])
+        log("%s sweep: %.2f sweeps/s (%.3f ms/sweep wall, %.3f ms device), %d pass launches, kernel %.3f ms, "
+            "acceptance %.4f" % (label, sps, 1e3 * wall / a.steps, dev_ms / a.steps, launches, kern_ms, acc / tries))
+        return r
+
+    exact = None if a.no_exact else leg("exact", False)
+    da = None
+    train = None
+    if not a.no_da:
+        from sc_amd import surrogate
+        ptype = int(w["types"][0])
+        train = surrogate.train(g, ptype, nsamples=a.samples, epochs=a.epochs, seed=a.seed, log=log)
+        da = leg("delayed-acceptance", True)
+    main_leg = da if da is not None else exact
 
     cpu = None
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as po
-        box, pos, d, q = g.get_config()
+        box, pos, d, q = box0, pos0, d0, q0
         tp = po.Topo(w["top"])
         s = po.State(box, pos, d, q, w["types"])
         tc = time.perf_counter()
@@ -128,30 +167,28 @@ def main():
 
     out = {
         "metric": METRIC,
-        "value": sweeps_per_s,
+        "value": main_leg["sweeps_per_s"],
         "unit": "sweeps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": 1e3 * wall / a.steps,
+        "ms_per_step": main_leg["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair terms, fp64 sums",
-        "data": "synthetic seeded state (RSA + compression), no dataset",
+        "dtype": "fp32 pair terms, fp64 sums" + (", bf16 mfma surrogate (fp32 accumulate)" if da else ""),
+        "data": "synthetic seeded state (RSA + compression), no dataset; surrogate trained on synthetic exact pairs",
         "config": {"workload": a.workload, "n": n, "eta": w["eta"], "box": float(w["box"][0]), "temper": temper,
                    "translation": transmx, "max_rotation_deg": 10.0,
-                   "checkerboard": "%dx%dx%d" % ((a.r + 1,) * 3), "cells": list(nc), "sweep": "exact",
-                   "trial_mode": a.trial_mode, "trial_moves_per_particle": done / a.steps},
-        "exact_sweeps_per_s": sweeps_per_s,
-        "acceptance": {"translation": st["trans_acc"] / max(1, st["trans_try"]),
-                       "rotation": st["rot_acc"] / max(1, st["rot_try"]),
-                       "left_cell": st["left_cell"] / max(1, st["trans_try"])},
-        "energy_per_particle": e_n,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_sweep_pass", "bytes_per_launch": bytes_launch,
-                     "avg_launch_ms": 1e3 * avg_launch_s, "launches": launches},
+                   "checkerboard": "%dx%dx%d" % ((a.r + 1,) * 3), "cells": list(nc),
+                   "sweep": "delayed-acceptance" if da else "exact", "trial_mode": a.trial_mode},
+        "exact_sweeps_per_s": exact["sweeps_per_s"] if exact else None,
+        "da_sweeps_per_s": da["sweeps_per_s"] if da else None,
+        "da_over_exact": (da["sweeps_per_s"] / exact["sweeps_per_s"]) if (da and exact) else None,
+        "exact": exact,
+        "delayed_acceptance": da,
+        "surrogate_training": train,
+        "roofline": main_leg["roofline"],
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

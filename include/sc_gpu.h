@@ -159,6 +159,59 @@ int sc_set_trial_mode(sc_ctx* ctx, int mode);
 int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
                    uint64_t counter, int* accepted, double box_out[3]);
 
+/* --- learned surrogate and delayed acceptance ------------------------------ */
+/* No counterpart in the reference, which evaluates every trial move with the
+ * exact pair functors (paire.h:466-827, paire.cpp:244-301).  A small network
+ * (12 body-frame features -> 64 -> 64 -> 64 -> 1, SiLU) stands in for the
+ * patch attraction of one spherocylinder type with itself; the sweep then
+ * screens each move with exact repulsion + network attraction and evaluates
+ * the exact energy only for moves the screen accepts (two-stage delayed
+ * acceptance, Christen & Fox 2005), which leaves the stationary distribution
+ * exactly the reference's.
+ *
+ * Flat FP32 weight vector (SC_MLP_NPARAM floats), row-major [in][out]:
+ *   W1[12][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64] b4.
+ * Features of a pair (moving particle a as part1, neighbour b; v = pos_b -
+ * pos_a minimum image; e1 = dir_a, e2 = patchdir0_a, e3 = e1 x e2;
+ * lsc = half_len_a + half_len_b + rcut):
+ *   v.e1/lsc, v.e2/lsc, v.e3/lsc, dir_b.(e1,e2,e3), patchdir0_b.(e1,e2,e3),
+ *   d_seg/rcut, |v|/lsc, w(d_seg) (the cos^2 switch of the attraction). */
+#define SC_MLP_NPARAM 9217
+#define SC_MLP_NF 12
+/* Allocate the network for reference type `type` (same type both sides) with
+ * seeded random weights and reset the optimiser state. */
+int sc_mlp_init(sc_ctx* ctx, int type, uint64_t seed);
+int sc_mlp_set_weights(sc_ctx* ctx, const float* w, int n);
+int sc_mlp_get_weights(sc_ctx* ctx, float* w, int n);
+/* Generate ns training pairs on the device: uniform orientations, centre
+ * offsets with closest segment distance <= dmax (a fraction near_frac within
+ * 1.1 rcut); target = exact FP32 patch attraction (0 beyond rcut). */
+int sc_mlp_generate(sc_ctx* ctx, long ns, uint64_t seed, double dmax, double near_frac);
+/* copy rows [first, first+count) of the training set: features (count*12),
+ * target attraction and full exact pair energy; any pointer may be NULL */
+int sc_mlp_training_data(sc_ctx* ctx, long first, long count, float* feats, float* target, float* efull);
+/* Adam on mean squared error, FP32 forward/backward kernels, cosine learning
+ * rate from lr to lr_final; the last val_frac of the set is held out and its
+ * RMSE (BF16 MFMA inference) returned. */
+int sc_mlp_train(sc_ctx* ctx, int epochs, int batch, double lr, double lr_final, uint64_t seed, double val_frac,
+                 double* train_loss, double* val_rmse);
+/* Loss (mean squared error) and its gradient w.r.t. every weight for one batch
+ * (x: B*12, y: B) with the current weights, by the training kernels; grads
+ * holds ngrads = SC_MLP_NPARAM floats. */
+int sc_mlp_gradients(sc_ctx* ctx, int batch, const float* x, const float* y, float* grads, int ngrads, double* loss);
+/* SC_MLP_NPARAM as compiled into the library (bindings check it at load). */
+int sc_mlp_nparam(void);
+/* Network output for m feature rows: bf16 = 1 the MFMA tile the sweep uses,
+ * bf16 = 0 FP32 throughout. */
+int sc_mlp_forward(sc_ctx* ctx, int m, const float* x, float* out, int bf16);
+/* Features and exact FP32 patch attraction of pairs (i[k] as part1, j[k]) of
+ * the loaded configuration. */
+int sc_mlp_pair_features(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, float* feats, float* eattr);
+/* on = 1: sc_sweep uses the two-stage delayed acceptance (a one-type system of
+ * the network's type); sc_sweep_stats.stage1_acc / stage1_try count the
+ * screen.  on = 0: the exact sweep. */
+int sc_set_surrogate(sc_ctx* ctx, int on);
+
 /* --- synthetic inputs ----------------------------------------------------- */
 /* Seeded overlap-free configuration by random sequential addition (host,
  * FP64).  No counterpart in the scOOP library: the reference's generator

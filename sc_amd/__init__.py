@@ -36,6 +36,7 @@ _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int32)
 _U = ctypes.POINTER(ctypes.c_uint32)
 _U8 = ctypes.POINTER(ctypes.c_uint8)
+_F = ctypes.POINTER(ctypes.c_float)
 SIGNATURES = {
     "sc_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int]),
     "sc_destroy": (None, [_P]),
@@ -73,7 +74,22 @@ SIGNATURES = {
     "sc_set_profiling": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_copy_probe": (ctypes.c_int, [_P, ctypes.c_long, ctypes.c_int]),
     "sc_set_trial_mode": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_mlp_init": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint64]),
+    "sc_mlp_set_weights": (ctypes.c_int, [_P, _F, ctypes.c_int]),
+    "sc_mlp_get_weights": (ctypes.c_int, [_P, _F, ctypes.c_int]),
+    "sc_mlp_generate": (ctypes.c_int, [_P, ctypes.c_long, ctypes.c_uint64, ctypes.c_double, ctypes.c_double]),
+    "sc_mlp_training_data": (ctypes.c_int, [_P, ctypes.c_long, ctypes.c_long, _F, _F, _F]),
+    "sc_mlp_train": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                    ctypes.c_double, _D, _D]),
+    "sc_mlp_gradients": (ctypes.c_int, [_P, ctypes.c_int, _F, _F, _F, ctypes.c_int, _D]),
+    "sc_mlp_nparam": (ctypes.c_int, []),
+    "sc_mlp_forward": (ctypes.c_int, [_P, ctypes.c_int, _F, _F, ctypes.c_int]),
+    "sc_mlp_pair_features": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _F, _F]),
+    "sc_set_surrogate": (ctypes.c_int, [_P, ctypes.c_int]),
 }
+
+MLP_NPARAM = 9217   # W1[12][64] b1 W2[64][64] b2 W3[64][64] b3 W4[64] b4
+MLP_NF = 12
 
 _lib = None
 
@@ -89,6 +105,8 @@ def lib():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
+        if L.sc_mlp_nparam() != MLP_NPARAM:
+            raise SCError("library built with %d surrogate weights, binding expects %d" % (L.sc_mlp_nparam(), MLP_NPARAM))
         _lib = L
     return _lib
 
@@ -99,6 +117,10 @@ def _d(a):
 
 def _i(a):
     return a.ctypes.data_as(_I)
+
+
+def _f(a):
+    return a.ctypes.data_as(_F)
 
 
 def _vec3(x):
@@ -309,6 +331,66 @@ class GpuSystem:
                                         int(seed) & 0xFFFFFFFFFFFFFFFF, int(counter) & 0xFFFFFFFFFFFFFFFF,
                                         ctypes.byref(acc), _d(box)), "volume_move")
         return bool(acc.value), box
+
+    # ---- learned surrogate + delayed acceptance
+    def mlp_init(self, ptype, seed=1):
+        self._ck(self._L.sc_mlp_init(self._h, int(ptype), int(seed) & 0xFFFFFFFFFFFFFFFF), "mlp_init")
+
+    def mlp_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32).reshape(MLP_NPARAM)
+        self._ck(self._L.sc_mlp_set_weights(self._h, _f(w), MLP_NPARAM), "mlp_set_weights")
+
+    def mlp_get_weights(self):
+        w = np.zeros(MLP_NPARAM, np.float32)
+        self._ck(self._L.sc_mlp_get_weights(self._h, _f(w), MLP_NPARAM), "mlp_get_weights")
+        return w
+
+    def mlp_generate(self, nsamples, seed=1, dmax=2.5, near_frac=0.7):
+        self._ck(self._L.sc_mlp_generate(self._h, int(nsamples), int(seed) & 0xFFFFFFFFFFFFFFFF, float(dmax),
+                                         float(near_frac)), "mlp_generate")
+
+    def mlp_training_data(self, first, count):
+        x = np.zeros((count, MLP_NF), np.float32)
+        y = np.zeros(count, np.float32)
+        e = np.zeros(count, np.float32)
+        self._ck(self._L.sc_mlp_training_data(self._h, int(first), int(count), _f(x), _f(y), _f(e)),
+                 "mlp_training_data")
+        return x, y, e
+
+    def mlp_train(self, epochs, batch=4096, lr=2e-3, lr_final=5e-5, seed=1, val_frac=0.05):
+        tl, vr = ctypes.c_double(), ctypes.c_double()
+        self._ck(self._L.sc_mlp_train(self._h, int(epochs), int(batch), float(lr), float(lr_final),
+                                      int(seed) & 0xFFFFFFFFFFFFFFFF, float(val_frac), ctypes.byref(tl),
+                                      ctypes.byref(vr)), "mlp_train")
+        return tl.value, vr.value
+
+    def mlp_gradients(self, x, y):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_NF)
+        y = np.ascontiguousarray(y, np.float32).reshape(-1)
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = ctypes.c_double()
+        if len(x) != len(y):
+            raise ValueError("mlp_gradients: %d feature rows, %d targets" % (len(x), len(y)))
+        self._ck(self._L.sc_mlp_gradients(self._h, len(y), _f(x), _f(y), _f(g), MLP_NPARAM, ctypes.byref(loss)),
+                 "mlp_gradients")
+        return g, loss.value
+
+    def mlp_forward(self, x, bf16=True):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_NF)
+        out = np.zeros(len(x), np.float32)
+        self._ck(self._L.sc_mlp_forward(self._h, len(x), _f(x), _f(out), int(bool(bf16))), "mlp_forward")
+        return out
+
+    def mlp_pair_features(self, i, j):
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        f = np.zeros((len(i), MLP_NF), np.float32)
+        e = np.zeros(len(i), np.float32)
+        self._ck(self._L.sc_mlp_pair_features(self._h, len(i), _i(i), _i(j), _f(f), _f(e)), "mlp_pair_features")
+        return f, e
+
+    def set_surrogate(self, on=True):
+        self._ck(self._L.sc_set_surrogate(self._h, int(bool(on))), "set_surrogate")
 
     def set_trial_mode(self, mode):
         self._ck(self._L.sc_set_trial_mode(self._h, int(mode)), "set_trial_mode")

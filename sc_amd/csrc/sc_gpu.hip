@@ -21,6 +21,7 @@
 
 #include "../../include/sc_gpu.h"
 #include "sc_host.h"
+#include "sc_mlp.h"
 #include "sc_pair.h"
 #include "sc_rng.h"
 
@@ -562,13 +563,24 @@ __device__ __forceinline__ DPart<float> mv_part(const float* mv, int t, int g) {
 struct TrialScratch {
     int* nl;     // near list, cap entries
     int* xl;     // expensive list, 2*cap entries
+    float* xd;   // segment distance of each expensive entry, 2*cap
     float* mv;   // 2 * MV_STRIDE floats
 };
 
-__device__ void trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un, const DPart<float>& po,
-                               const DPart<float>& pn, const SysConst& sc, const PP32* __restrict__ pp,
-                               const TP32* __restrict__ tp, int ppmax, const TrialScratch& T, double& eo,
-                               double& en) {
+struct MlpConst {
+    float lsc, rcut, pdis, pswitch;
+};
+
+// With DA (delayed acceptance) a first stage accepts with the surrogate
+// energy change des (exact repulsion + network attraction); returns false when
+// it rejects.  Otherwise eo/en are the exact energies of the two states.
+template <bool DA>
+__device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un,
+                                               const DPart<float>& po, const DPart<float>& pn, const SysConst& sc,
+                                               const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
+                                               const TrialScratch& T, const float* ml, const MlpRegs& mr,
+                                               const MlpConst& mc, double temper, double u1, double& eo, double& en,
+                                               double& des) {
     int lane = lane_id();
     int t = uo.w;
     bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
@@ -602,6 +614,7 @@ __device__ void trial_energies(const Stage& S, int k, const uint4 uo, const uint
     for (int base = 0; base < 2 * nn; base += WAVE) {
         int it = base + lane, jj = 0, which = 0;
         bool need = false;
+        float dseg2 = 0.0f;
         if (it < 2 * nn) {
             int e = T.nl[it >> 1];
             which = it & 1;
@@ -616,8 +629,7 @@ __device__ void trial_energies(const Stage& S, int k, const uint4 uo, const uint
                     DPart<float> pa, pb;
                     pa.dir = which ? dnew : dold;
                     pb.dir = normalised(mk<float>(S.o[jj].x, S.o[jj].y, S.o[jj].z));
-                    float d2;
-                    need = scsc_cheap(P2, pa, pb, r, ev, d2);
+                    need = scsc_cheap(P2, pa, pb, r, ev, dseg2);
                 } else if (P2.kind == K_SPH_WCA || P2.kind == K_SPH_TAYLOR) {
                     ev = sphere_pair(P2, sqrtf(dot(r, r)));
                 } else if (P2.kind != K_NONE) {
@@ -627,10 +639,47 @@ __device__ void trial_energies(const Stage& S, int k, const uint4 uo, const uint
             }
         }
         unsigned long long m = __ballot(need);
-        if (need) T.xl[nx + wave_prefix(m)] = jj | (which << 30);
+        if (need) {
+            int pos = nx + wave_prefix(m);
+            T.xl[pos] = jj | (which << 30);
+            if (DA) T.xd[pos] = sqrtf(dseg2);
+        }
         nx += __popcll(m);
     }
     __syncthreads();
+    des = 0.0;
+    if (DA) {
+        // first stage: exact repulsion + surrogate attraction, tiles of 32 pairs on MFMA
+        double rep = wave_sum(an) - wave_sum(ao);
+        double so = 0.0, sn = 0.0;
+        for (int t0 = 0; t0 < nx; t0 += 32) {
+            int e = t0 + (lane & 31);
+            float f[MLP_NF];
+            int which = 0;
+            if (e < nx) {
+                int w = T.xl[e];
+                int jj = w & 0x3fffffff;
+                which = (w >> 30) & 1;
+                uint4 b = S.u[jj];
+                V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
+                                    : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+                const float* mv = T.mv + which * MV_STRIDE;
+                float4 ob = S.o[jj];
+                float2 qb = S.q[jj];
+                mlp_features(mv_vec(mv, 0), mv_vec(mv, 3), mk<float>(ob.x, ob.y, ob.z), mk<float>(ob.w, qb.x, qb.y), r,
+                             T.xd[e], mc.lsc, mc.rcut, mc.pdis, mc.pswitch, f);
+            } else {
+                for (int q = 0; q < MLP_NF; q++) f[q] = 0.0f;
+            }
+            float out = mlp_tile(mr, ml, mlp_slots(f, lane >> 5), lane);
+            if (lane < 32 && e < nx) {
+                if (which) sn += (double)out; else so += (double)out;
+            }
+        }
+        des = rep + wave_sum(sn) - wave_sum(so);
+        bool acc1 = des <= 0.0 || exp(-des / temper) > u1;
+        if (!acc1) return false;
+    }
     // phase C
     int items = nx * ppmax;
     for (int base = 0; base < items; base += WAVE) {
@@ -661,13 +710,16 @@ __device__ void trial_energies(const Stage& S, int k, const uint4 uo, const uint
     }
     eo = wave_sum(ao);
     en = wave_sum(an);
+    return true;
 }
 
+template <bool DA>
 __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
                                                      uint4* __restrict__ P, float4* __restrict__ O,
                                                      float2* __restrict__ Q, const int* __restrict__ ID,
                                                      const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
                                                      int ppmax, const int* __restrict__ nonempty,
+                                                     const unsigned char* __restrict__ mlp_img, MlpConst mc,
                                                      unsigned long long* __restrict__ stats, double* __restrict__ de_sum,
                                                      int* __restrict__ flag) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -685,9 +737,16 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
     TrialScratch T;
     T.nl = S.id + cap;
     T.xl = T.nl + cap;
-    T.mv = (float*)(T.xl + 2 * cap);
+    T.xd = (float*)(T.xl + 2 * cap);
+    T.mv = DA ? T.xd + 2 * cap : (float*)(T.xl + 2 * cap);
     int* scratch = (int*)(T.mv + 2 * MV_STRIDE);
     int lane = lane_id();
+    float* ml = nullptr;
+    MlpRegs mr;
+    if (DA) {
+        ml = (float*)(((uintptr_t)(scratch + 2 * 125 + 2) + 15) & ~(uintptr_t)15);
+        mlp_setup(mlp_img, ml, lane, mr);
+    }
     if (!stage_load(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
         if (lane == 0) atomicOr(flag, 1);
         return;
@@ -708,7 +767,7 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
         int fl = (int)floorf(mean);
         ntrial = fl + ((u01f(rd.x) < mean - (float)fl) ? 1 : 0);
     }
-    unsigned long long n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0;
+    unsigned long long n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0;
     double dsum = 0.0;
     for (int t = 0; t < ntrial; t++) {
         ctr.x = (uint32_t)t;
@@ -751,19 +810,26 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
             on = make_float4(d.x, d.y, d.z, pq.x);
             qn = make_float2(pq.y, pq.z);
         }
-        double eo, en;
+        double eo, en, des;
+        ctr.w = (mp.pass_hi << 2) | 2u;
+        U4 rc = philox(ctr, mp.key0, mp.key1);
+        bool stage1;
         {
             DPart<float> po = load_part<float>(oo, qo, uo.w, tp);
             DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
-            trial_energies(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, eo, en);
+            stage1 = trial_energies<DA>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, mp.temper,
+                                        u01d(rc.z, rc.w), eo, en, des);
+        }
+        if (DA) {
+            n_s1try++;
+            if (!stage1) { __syncthreads(); continue; }
+            n_s1acc++;
         }
         double de = en - eo;
-        bool acc = de <= 0.0;
-        if (!acc) {
-            ctr.w = (mp.pass_hi << 2) | 2u;
-            U4 rc = philox(ctr, mp.key0, mp.key1);
-            acc = exp(-de / mp.temper) > u01d(rc.x, rc.y);     // MoveCreator::moveTry, movecreator.h:175-187
-        }
+        // MoveCreator::moveTry (movecreator.h:175-187); with the surrogate the
+        // second stage accepts with min(1, exp(-(dE - dE_s)/T)) (Christen & Fox 2005)
+        double de2 = DA ? de - des : de;
+        bool acc = de2 <= 0.0 || exp(-de2 / mp.temper) > u01d(rc.x, rc.y);
         if (acc) {
             if (translate) n_tacc++; else n_racc++;
             dsum += de;
@@ -788,8 +854,214 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
         atomicAdd(&stats[2], n_racc);
         atomicAdd(&stats[3], n_rtry);
         atomicAdd(&stats[4], n_left);
+        atomicAdd(&stats[5], n_s1acc);
+        atomicAdd(&stats[6], n_s1try);
         atomicAdd(de_sum, dsum);
     }
+}
+
+// ---------------------------------------------------------------- surrogate
+// Training data: random pairs of one type pair with uniform orientations and
+// closest segment distance d in [0, dmax]; a fraction `near_frac` is drawn with
+// d <= 1.1 rcut where the attraction lives.  Target: the exact FP32 patch
+// attraction (0 beyond rcut); also the exact full pair energy.
+__global__ void k_mlp_gen(long ns, uint32_t key0, uint32_t key1, const PP32* __restrict__ ppair,
+                          const TP32* __restrict__ tpa, const TP32* __restrict__ tpb, MlpConst mc, float dmax,
+                          float near_frac, float* __restrict__ feats, float* __restrict__ target,
+                          float* __restrict__ efull) {
+    long s = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const PP32& P = *ppair;
+    float R = P.hl0 + P.hl1 + dmax;
+    U4 c;
+    c.x = (uint32_t)s; c.y = (uint32_t)(s >> 32); c.z = 0; c.w = 0x5a17u;
+    U4 h = philox(c, key0, key1);
+    bool want_near = u01f(h.x) < near_frac;
+    float dlim = want_near ? fminf(dmax, 1.1f * P.rcut) : dmax;
+    V3<float> r, da, qa, db, qb;
+    float d = 0.0f;
+    for (int attempt = 0; attempt < 256; attempt++) {
+        c.z = (uint32_t)attempt;
+        U4 u = philox(c, key0, key1);
+        c.w = 0x5a18u;
+        U4 w = philox(c, key0, key1);
+        c.w = 0x5a17u;
+        // centre offset uniform in a ball of radius R
+        V3<float> dir = unit_from(u01f(u.x), u01f(u.y));
+        float rad = R * cbrtf(u01f(u.z));
+        r = mk<float>(dir.x * rad, dir.y * rad, dir.z * rad);
+        da = unit_from(u01f(u.w), u01f(w.x));
+        db = unit_from(u01f(w.y), u01f(w.z));
+        // patch directions uniform around the axes
+        V3<float> ref = fabsf(da.x) < 0.9f ? mk<float>(1, 0, 0) : mk<float>(0, 1, 0);
+        V3<float> e1 = normalised(perp(ref, da)), e2 = cross(da, e1);
+        float ph = 6.283185307179586f * u01f(w.w);
+        qa = mk<float>(cosf(ph) * e1.x + sinf(ph) * e2.x, cosf(ph) * e1.y + sinf(ph) * e2.y, cosf(ph) * e1.z + sinf(ph) * e2.z);
+        ref = fabsf(db.x) < 0.9f ? mk<float>(1, 0, 0) : mk<float>(0, 1, 0);
+        e1 = normalised(perp(ref, db));
+        e2 = cross(db, e1);
+        ph = 6.283185307179586f * u01f(u.x ^ w.y);
+        qb = mk<float>(cosf(ph) * e1.x + sinf(ph) * e2.x, cosf(ph) * e1.y + sinf(ph) * e2.y, cosf(ph) * e1.z + sinf(ph) * e2.z);
+        V3<float> dv = min_dist(da, db, P.hl0, P.hl1, r);
+        d = sqrtf(dot(dv, dv));
+        if (d <= dlim) break;
+    }
+    DPart<float> pa, pb;
+    pa.dir = da; pa.pd[0] = qa; pa.pd[1] = qa; pa.sd[0] = pa.sd[1] = pa.sd[2] = pa.sd[3] = qa; pa.t = 0;
+    pb.dir = db; pb.pd[0] = qb; pb.pd[1] = qb; pb.sd[0] = pb.sd[1] = pb.sd[2] = pb.sd[3] = qb; pb.t = 0;
+    derive<float>(pa, *tpa);
+    derive<float>(pb, *tpb);
+    float erep, d2;
+    bool att = scsc_cheap(P, pa, pb, r, erep, d2);
+    float ea = att ? scsc_attr(P, pa, pb, r) : 0.0f;
+    float f[MLP_NF];
+    mlp_features(pa.dir, pa.pd[0], pb.dir, pb.pd[0], r, sqrtf(d2), mc.lsc, mc.rcut, mc.pdis, mc.pswitch, f);
+    for (int q = 0; q < MLP_NF; q++) feats[s * MLP_NF + q] = f[q];
+    target[s] = ea;
+    efull[s] = erep + ea;
+}
+
+// dense layer forward, one thread per (sample, unit): Z = X W + b, H = SiLU(Z)
+__global__ void k_dense_fwd(int B, int K, const float* __restrict__ X, const float* __restrict__ W,
+                            const float* __restrict__ b, float* __restrict__ Z, float* __restrict__ H) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * MLP_NH) return;
+    int s = t / MLP_NH, n = t - s * MLP_NH;
+    float z = b[n];
+    for (int k = 0; k < K; k++) z += X[s * K + k] * W[k * MLP_NH + n];
+    Z[t] = z;
+    H[t] = silu(z);
+}
+
+// output layer + squared-error loss: out = H3 w4 + b4, dOut = 2 (out - y) / B
+__global__ void k_out_loss(int B, const float* __restrict__ H3, const float* __restrict__ W, const float* __restrict__ y,
+                           float* __restrict__ out, float* __restrict__ dout, double* __restrict__ loss) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double l = 0.0;
+    if (s < B) {
+        float o = W[MLP_OFF_B4];
+        for (int k = 0; k < MLP_NH; k++) o += H3[s * MLP_NH + k] * W[MLP_OFF_W4 + k];
+        out[s] = o;
+        float e = o - y[s];
+        dout[s] = 2.0f * e / (float)B;
+        l = (double)e * (double)e;
+    }
+    l = wave_sum(l);
+    if (lane_id() == 0) atomicAdd(loss, l);
+}
+
+// dZ3 = dOut * w4 * SiLU'(Z3)
+__global__ void k_out_bwd(int B, const float* __restrict__ dout, const float* __restrict__ W, const float* __restrict__ Z3,
+                          float* __restrict__ dZ3) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * MLP_NH) return;
+    int s = t / MLP_NH, n = t - s * MLP_NH;
+    dZ3[t] = dout[s] * W[MLP_OFF_W4 + n] * silu_grad(Z3[t]);
+}
+
+// dZprev = (dZ W^T) * SiLU'(Zprev), one thread per (sample, input unit)
+__global__ void k_dense_bwd_x(int B, const float* __restrict__ dZ, const float* __restrict__ W,
+                              const float* __restrict__ Zprev, float* __restrict__ dZprev) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * MLP_NH) return;
+    int s = t / MLP_NH, k = t - s * MLP_NH;
+    float g = 0.0f;
+    for (int n = 0; n < MLP_NH; n++) g += dZ[s * MLP_NH + n] * W[k * MLP_NH + n];
+    dZprev[t] = g * silu_grad(Zprev[t]);
+}
+
+// weight gradients dW[k][n] = sum_s X[s][k] dZ[s][n], db[n] = sum_s dZ[s][n],
+// batch split over blocks (chunks of 256 samples) with atomic accumulation
+__global__ void k_dense_bwd_w(int B, int K, const float* __restrict__ X, const float* __restrict__ dZ,
+                              float* __restrict__ dW, float* __restrict__ db) {
+    int t = threadIdx.x + blockDim.x * blockIdx.x;   // (k, n) over gridDim.x blocks
+    int chunk = blockIdx.y;
+    if (t >= (K + 1) * MLP_NH) return;
+    int k = t / MLP_NH, n = t - k * MLP_NH;
+    int s0 = chunk * 256, s1 = min(B, s0 + 256);
+    float g = 0.0f;
+    if (k < K)
+        for (int s = s0; s < s1; s++) g += X[s * K + k] * dZ[s * MLP_NH + n];
+    else
+        for (int s = s0; s < s1; s++) g += dZ[s * MLP_NH + n];
+    if (k < K) atomicAdd(&dW[k * MLP_NH + n], g);
+    else atomicAdd(&db[n], g);
+}
+
+// w4 / b4 gradients: dW4[k] = sum_s H3[s][k] dOut[s]; db4 = sum dOut
+__global__ void k_out_bwd_w(int B, const float* __restrict__ H3, const float* __restrict__ dout, float* __restrict__ G) {
+    int k = threadIdx.x;   // 0..64 (64 = bias)
+    int chunk = blockIdx.x;
+    int s0 = chunk * 256, s1 = min(B, s0 + 256);
+    float g = 0.0f;
+    if (k < MLP_NH)
+        for (int s = s0; s < s1; s++) g += H3[s * MLP_NH + k] * dout[s];
+    else if (k == MLP_NH)
+        for (int s = s0; s < s1; s++) g += dout[s];
+    if (k < MLP_NH) atomicAdd(&G[MLP_OFF_W4 + k], g);
+    else if (k == MLP_NH) atomicAdd(&G[MLP_OFF_B4], g);
+}
+
+__global__ void k_adam(int n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// FP32 master weights -> packed BF16 MFMA image (operand order of mlp_tile)
+__global__ void k_mlp_pack(const float* __restrict__ W, unsigned char* __restrict__ img) {
+    int t = threadIdx.x;   // blockDim.x a multiple of 64: one thread per (operand, lane)
+    int lane = t & 63, m = lane & 31, kg = lane >> 5;
+    for (int q = t >> 6; q < 18; q += blockDim.x >> 6) {
+        bf16x8 v;
+        if (q < 2) {   // A1 block q (first-layer bias in K slot 12)
+            for (int j = 0; j < 8; j++) {
+                int k = 8 * kg + j, u = 32 * q + m;
+                float w = k < MLP_NF ? W[MLP_OFF_W1 + k * MLP_NH + u] : (k == MLP_NF ? W[MLP_OFF_B1 + u] : 0.0f);
+                v[j] = (__bf16)w;
+            }
+            ((bf16x8*)(img + MLP_A1_OFF))[q * 64 + lane] = v;
+        } else {
+            int qq = q - 2, layer = qq >> 3, blk = (qq >> 2) & 1, ks = qq & 3;
+            int off = layer == 0 ? MLP_OFF_W2 : MLP_OFF_W3;
+            for (int j = 0; j < 8; j++) v[j] = (__bf16)W[off + mlp_kperm(ks, kg, j) * MLP_NH + 32 * blk + m];
+            ((bf16x8*)(img + (layer == 0 ? MLP_A2_OFF : MLP_A3_OFF)))[(blk * 4 + ks) * 64 + lane] = v;
+        }
+    }
+    float* C = (float*)(img + MLP_C_OFF);
+    for (int i = t; i < 3 * 64; i += blockDim.x) {
+        int layer = i / 64, rest = i - 64 * layer;
+        int g = rest / 32, blk = (rest / 16) & 1, r = rest & 15;
+        int u = mlp_unit(g, blk, r);
+        C[i] = layer == 0 ? W[MLP_OFF_B2 + u] : (layer == 1 ? W[MLP_OFF_B3 + u] : W[MLP_OFF_W4 + u]);
+    }
+    if (t == 0) { C[192] = W[MLP_OFF_B4]; C[193] = C[194] = C[195] = 0.0f; }
+}
+
+// batched inference: one wave per tile of 32 feature rows; bf16 != 0 uses the
+// MFMA tile (the sweep's path), otherwise the FP32 per-sample forward
+__global__ __launch_bounds__(WAVE) void k_mlp_infer(int m, const float* __restrict__ X, const unsigned char* __restrict__ img,
+                                                    const float* __restrict__ W, int bf16, float* __restrict__ out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    int lane = lane_id();
+    int row = blockIdx.x * 32 + (lane & 31);
+    if (!bf16) {
+        if (lane < 32 && row < m) out[row] = mlp_forward_f32(W, X + (long)row * MLP_NF);
+        return;
+    }
+    MlpRegs R;
+    float* cst = (float*)smem;
+    mlp_setup(img, cst, lane, R);
+    float f[MLP_NF];
+    for (int q = 0; q < MLP_NF; q++) f[q] = row < m ? X[(long)row * MLP_NF + q] : 0.0f;
+    __syncthreads();
+    float o = mlp_tile(R, cst, mlp_slots(f, lane >> 5), lane);
+    if (lane < 32 && row < m) out[row] = o;
 }
 
 // plain streaming copy: the calibration kernel for HBM byte counters
@@ -802,6 +1074,7 @@ struct sc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    const char* last_call = "";   // the HIP call or kernel launch issued last
     schost::Topology topo;
     bool have_topo = false;
     schost::Options opts;
@@ -855,15 +1128,42 @@ struct sc_ctx {
     long last_launches = 0;
     bool profile = false;               // time every sweep-pass launch with events
     std::vector<hipEvent_t> kev;        // event pairs for the launches of one call
+    // surrogate
+    float* mlp_w = nullptr;             // FP32 master weights (MLP_NPARAM)
+    float* mlp_g = nullptr;
+    float* mlp_m = nullptr;
+    float* mlp_v = nullptr;
+    unsigned char* mlp_img = nullptr;   // packed BF16 image for MFMA
+    bool mlp_has = false;
+    int mlp_type = -1;                  // reference type the network was trained for
+    MlpConst mlp_c{};
+    long adam_t = 0;
+    float* tr_x = nullptr;
+    float* tr_y = nullptr;
+    float* tr_e = nullptr;
+    long tr_n = 0;
+    float* act = nullptr;               // Z1 H1 Z2 H2 Z3 H3 dZa dZb (batch x 64 each)
+    float* bx = nullptr;                // batch features / targets / out / dout
+    int act_batch = 0;
+    double* d_loss = nullptr;
+    bool use_surrogate = false;
 };
 
+// Every HIP call of a library function goes through HIPCK, which records the
+// call (for error messages) and turns a failure into SC_EHIP.
 #define HIPCK(x)                                                                       \
     do {                                                                               \
+        ctx->last_call = #x;                                                           \
         hipError_t e_ = (x);                                                           \
         if (e_ != hipSuccess) {                                                        \
             ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);                 \
             return SC_EHIP;                                                            \
         }                                                                              \
+    } while (0)
+#define SC_LAUNCH(kernel, ...)                                                         \
+    do {                                                                               \
+        ctx->last_call = #kernel;                                                      \
+        hipLaunchKernelGGL(kernel, __VA_ARGS__);                                       \
     } while (0)
 
 static int fail(sc_ctx* ctx, int code, const std::string& m) {
@@ -1112,9 +1412,25 @@ static int energy_all_impl(sc_ctx* ctx, const double* box, double* e) {
 }
 
 // ------------------------------------------------------------------ the ABI
+// No C++ exception may leave the C ABI (callers are C, ctypes, cgo...): each
+// entry point runs its body under a handler that turns one into SC_EHIP with
+// the exception's text and the HIP call that was in progress.
+static int guard_fail(sc_ctx* ctx, const char* fn, const char* what) {
+    if (ctx) ctx->err = std::string(fn) + ": C++ exception '" + what + "' during " + ctx->last_call;
+    return SC_EHIP;
+}
+#define SC_GUARD(ctx, call)                                                            \
+    try {                                                                              \
+        return call;                                                                   \
+    } catch (const std::exception& e_) {                                               \
+        return guard_fail(ctx, __func__, e_.what());                                   \
+    } catch (...) {                                                                    \
+        return guard_fail(ctx, __func__, "unknown");                                   \
+    }
+
 extern "C" {
 
-int sc_create(sc_ctx** out, int device) {
+static int sc_create_impl(sc_ctx** out, int device) {
     if (!out) return SC_EINVAL;
     *out = nullptr;
     if (device == -1) {
@@ -1148,7 +1464,11 @@ int sc_create(sc_ctx** out, int device) {
     return SC_OK;
 }
 
-void sc_destroy(sc_ctx* ctx) {
+int sc_create(sc_ctx** out, int device) {
+    SC_GUARD(nullptr, sc_create_impl(out, device))
+}
+
+static void sc_destroy_impl(sc_ctx* ctx) {
     if (!ctx) return;
     if (ctx->device < 0) { delete ctx; return; }
     hipSetDevice(ctx->device);
@@ -1156,7 +1476,9 @@ void sc_destroy(sc_ctx* ctx) {
     void* ptrs[] = {ctx->d_pp32, ctx->d_pp64, ctx->d_tp32, ctx->d_tp64, ctx->P[0], ctx->P[1], ctx->O[0], ctx->O[1],
                     ctx->Q[0], ctx->Q[1], ctx->ID[0], ctx->ID[1], ctx->slot, ctx->cell_tmp, ctx->src_of_pos,
                     ctx->starts, ctx->counts, ctx->fill, ctx->partial, ctx->d_out, ctx->d_flag, ctx->d_maxocc,
-                    ctx->d_nonempty, ctx->d_stats, ctx->d_dsum, ctx->d_q, ctx->d_i, ctx->d_j};
+                    ctx->d_nonempty, ctx->d_stats, ctx->d_dsum, ctx->d_q, ctx->d_i, ctx->d_j, ctx->mlp_w,
+                    ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, ctx->mlp_img, ctx->tr_x, ctx->tr_y, ctx->tr_e, ctx->act,
+                    ctx->bx, ctx->d_loss};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
@@ -1166,9 +1488,16 @@ void sc_destroy(sc_ctx* ctx) {
     delete ctx;
 }
 
+void sc_destroy(sc_ctx* ctx) {
+    try {
+        sc_destroy_impl(ctx);
+    } catch (...) {
+    }
+}
+
 const char* sc_last_error(const sc_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int sc_load_topology(sc_ctx* ctx, const char* text) {
+static int sc_load_topology_impl(sc_ctx* ctx, const char* text) {
     if (!ctx || !text) return SC_EINVAL;
     std::string e = ctx->topo.parse(text);
     if (!e.empty()) { ctx->have_topo = false; return fail(ctx, SC_EINVAL, e); }
@@ -1178,12 +1507,20 @@ int sc_load_topology(sc_ctx* ctx, const char* text) {
     return SC_OK;
 }
 
-int sc_load_options(sc_ctx* ctx, const char* text) {
+int sc_load_topology(sc_ctx* ctx, const char* text) {
+    SC_GUARD(ctx, sc_load_topology_impl(ctx, text))
+}
+
+static int sc_load_options_impl(sc_ctx* ctx, const char* text) {
     if (!ctx || !text) return SC_EINVAL;
     std::string e = ctx->opts.parse(text);
     if (!e.empty()) return fail(ctx, SC_EINVAL, e);
     ctx->have_opts = true;
     return SC_OK;
+}
+
+int sc_load_options(sc_ctx* ctx, const char* text) {
+    SC_GUARD(ctx, sc_load_options_impl(ctx, text))
 }
 
 int sc_get_option(const sc_ctx* ctx, const char* key, double* v) {
@@ -1212,7 +1549,7 @@ int sc_pair_params(const sc_ctx* ctx, int ti, int tj, double* out) {
     return SC_OK;
 }
 
-int sc_set_config(sc_ctx* ctx, int n, const double box[3], const double* pos, const double* dir, const double* pd0,
+static int sc_set_config_impl(sc_ctx* ctx, int n, const double box[3], const double* pos, const double* dir, const double* pd0,
                   const int32_t* types) {
     if (!ctx || n < 0 || !box || (n > 0 && (!pos || !dir || !pd0))) return SC_EINVAL;
     if (!ctx->have_topo) return fail(ctx, SC_ESTATE, "load a topology first");
@@ -1278,7 +1615,12 @@ int sc_set_config(sc_ctx* ctx, int n, const double box[3], const double* pos, co
     return build_grid(ctx, 2, nullptr, ctx->box);
 }
 
-int sc_load_config(sc_ctx* ctx, const char* text) {
+int sc_set_config(sc_ctx* ctx, int n, const double box[3], const double* pos, const double* dir, const double* pd0,
+                  const int32_t* types) {
+    SC_GUARD(ctx, sc_set_config_impl(ctx, n, box, pos, dir, pd0, types))
+}
+
+static int sc_load_config_impl(sc_ctx* ctx, const char* text) {
     if (!ctx || !text) return SC_EINVAL;
     if (!ctx->have_topo) return fail(ctx, SC_ESTATE, "load a topology first");
     int n = (int)ctx->ptypes.size();
@@ -1288,6 +1630,10 @@ int sc_load_config(sc_ctx* ctx, const char* text) {
     if (!e.empty()) return fail(ctx, SC_EINVAL, e);
     std::vector<int32_t> ty(ctx->ptypes.begin(), ctx->ptypes.end());
     return sc_set_config(ctx, n, box, pos.data(), dir.data(), pd0.data(), ty.data());
+}
+
+int sc_load_config(sc_ctx* ctx, const char* text) {
+    SC_GUARD(ctx, sc_load_config_impl(ctx, text))
 }
 
 static int download(sc_ctx* ctx, std::vector<uint4>& hp, std::vector<float4>& ho, std::vector<float2>& hq) {
@@ -1308,7 +1654,7 @@ static int download(sc_ctx* ctx, std::vector<uint4>& hp, std::vector<float4>& ho
     return SC_OK;
 }
 
-int sc_get_config(sc_ctx* ctx, double box[3], double* pos, double* dir, double* pd0) {
+static int sc_get_config_impl(sc_ctx* ctx, double box[3], double* pos, double* dir, double* pd0) {
     if (!ctx) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     std::vector<uint4> hp; std::vector<float4> ho; std::vector<float2> hq;
@@ -1324,7 +1670,11 @@ int sc_get_config(sc_ctx* ctx, double box[3], double* pos, double* dir, double* 
     return SC_OK;
 }
 
-int sc_get_fixed_positions(sc_ctx* ctx, uint32_t* u) {
+int sc_get_config(sc_ctx* ctx, double box[3], double* pos, double* dir, double* pd0) {
+    SC_GUARD(ctx, sc_get_config_impl(ctx, box, pos, dir, pd0))
+}
+
+static int sc_get_fixed_positions_impl(sc_ctx* ctx, uint32_t* u) {
     if (!ctx || !u) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     std::vector<uint4> hp; std::vector<float4> ho; std::vector<float2> hq;
@@ -1334,7 +1684,11 @@ int sc_get_fixed_positions(sc_ctx* ctx, uint32_t* u) {
     return SC_OK;
 }
 
-long sc_write_config(sc_ctx* ctx, char* buf, long cap) {
+int sc_get_fixed_positions(sc_ctx* ctx, uint32_t* u) {
+    SC_GUARD(ctx, sc_get_fixed_positions_impl(ctx, u))
+}
+
+static long sc_write_config_impl(sc_ctx* ctx, char* buf, long cap) {
     if (!ctx) return SC_EINVAL;
     int n = ctx->n;
     std::vector<double> pos(3 * n), dir(3 * n), pd0(3 * n);
@@ -1347,7 +1701,11 @@ long sc_write_config(sc_ctx* ctx, char* buf, long cap) {
     return (long)s.size();
 }
 
-int sc_build_cells(sc_ctx* ctx, int r, const uint32_t shift[3], int32_t* cell_of, int32_t* cell_start,
+long sc_write_config(sc_ctx* ctx, char* buf, long cap) {
+    SC_GUARD(ctx, sc_write_config_impl(ctx, buf, cap))
+}
+
+static int sc_build_cells_impl(sc_ctx* ctx, int r, const uint32_t shift[3], int32_t* cell_of, int32_t* cell_start,
                    int32_t* sorted, int32_t ncell_out[3]) {
     if (!ctx) return SC_EINVAL;
     HIPCK(hipSetDevice(ctx->device));
@@ -1367,7 +1725,12 @@ int sc_build_cells(sc_ctx* ctx, int r, const uint32_t shift[3], int32_t* cell_of
     return SC_OK;
 }
 
-int sc_neighbor_pairs(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* count) {
+int sc_build_cells(sc_ctx* ctx, int r, const uint32_t shift[3], int32_t* cell_of, int32_t* cell_start,
+                   int32_t* sorted, int32_t ncell_out[3]) {
+    SC_GUARD(ctx, sc_build_cells_impl(ctx, r, shift, cell_of, cell_start, sorted, ncell_out))
+}
+
+static int sc_neighbor_pairs_impl(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* count) {
     if (!ctx || !count) return SC_EINVAL;
     // Host-side enumeration over the device cell list would need a device
     // pair-compaction kernel; the list is built from the fixed-point state in
@@ -1395,7 +1758,11 @@ int sc_neighbor_pairs(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* co
     return SC_OK;
 }
 
-int sc_pair_energies(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64) {
+int sc_neighbor_pairs(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* count) {
+    SC_GUARD(ctx, sc_neighbor_pairs_impl(ctx, rc, pairs, cap, count))
+}
+
+static int sc_pair_energies_impl(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64) {
     if (!ctx || m < 0 || (m > 0 && (!i || !j || !out))) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     for (int k = 0; k < m; k++)
@@ -1421,14 +1788,22 @@ int sc_pair_energies(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, dou
     return SC_OK;
 }
 
-int sc_energy_all(sc_ctx* ctx, double* e) {
+int sc_pair_energies(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64) {
+    SC_GUARD(ctx, sc_pair_energies_impl(ctx, m, i, j, out, fp64))
+}
+
+static int sc_energy_all_impl(sc_ctx* ctx, double* e) {
     if (!ctx || !e) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     HIPCK(hipSetDevice(ctx->device));
     return energy_all_impl(ctx, ctx->box, e);
 }
 
-int sc_energy_all_trial(sc_ctx* ctx, const double box_new[3], double* e) {
+int sc_energy_all(sc_ctx* ctx, double* e) {
+    SC_GUARD(ctx, sc_energy_all_impl(ctx, e))
+}
+
+static int sc_energy_all_trial_impl(sc_ctx* ctx, const double box_new[3], double* e) {
     if (!ctx || !e || !box_new) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     for (int k = 0; k < 3; k++)
@@ -1437,13 +1812,21 @@ int sc_energy_all_trial(sc_ctx* ctx, const double box_new[3], double* e) {
     return energy_all_impl(ctx, box_new, e);
 }
 
-int sc_commit_box(sc_ctx* ctx, const double box_new[3]) {
+int sc_energy_all_trial(sc_ctx* ctx, const double box_new[3], double* e) {
+    SC_GUARD(ctx, sc_energy_all_trial_impl(ctx, box_new, e))
+}
+
+static int sc_commit_box_impl(sc_ctx* ctx, const double box_new[3]) {
     if (!ctx || !box_new) return SC_EINVAL;
     for (int k = 0; k < 3; k++) {
         if (!(box_new[k] > 0)) return fail(ctx, SC_EINVAL, "box edges must be positive");
         ctx->box[k] = box_new[k];
     }
     return build_grid(ctx, 2, nullptr, ctx->box);
+}
+
+int sc_commit_box(sc_ctx* ctx, const double box_new[3]) {
+    SC_GUARD(ctx, sc_commit_box_impl(ctx, box_new))
 }
 
 static int run_queries(sc_ctx* ctx, int nq, const Query* hq, const int* slots_dev, double* out) {
@@ -1464,7 +1847,7 @@ static int run_queries(sc_ctx* ctx, int nq, const Query* hq, const int* slots_de
     return SC_OK;
 }
 
-int sc_energy_one(sc_ctx* ctx, int target, double* e) {
+static int sc_energy_one_impl(sc_ctx* ctx, int target, double* e) {
     if (!ctx || !e) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     if (target < 0 || target >= ctx->n) return fail(ctx, SC_EINVAL, "target out of range");
@@ -1477,7 +1860,11 @@ int sc_energy_one(sc_ctx* ctx, int target, double* e) {
     return run_queries(ctx, 1, nullptr, ctx->d_i, e);
 }
 
-int sc_energy_per_particle(sc_ctx* ctx, double* out) {
+int sc_energy_one(sc_ctx* ctx, int target, double* e) {
+    SC_GUARD(ctx, sc_energy_one_impl(ctx, target, e))
+}
+
+static int sc_energy_per_particle_impl(sc_ctx* ctx, double* out) {
     if (!ctx || !out) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     HIPCK(hipSetDevice(ctx->device));
@@ -1494,6 +1881,10 @@ int sc_energy_per_particle(sc_ctx* ctx, double* out) {
     HIPCK(hipMemcpy(sl.data(), ctx->slot, n * sizeof(int), hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++) out[i] = bySlot[sl[i]];
     return SC_OK;
+}
+
+int sc_energy_per_particle(sc_ctx* ctx, double* out) {
+    SC_GUARD(ctx, sc_energy_per_particle_impl(ctx, out))
 }
 
 static Query make_query(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3]) {
@@ -1517,7 +1908,7 @@ static Query make_query(sc_ctx* ctx, int target, const double pos[3], const doub
     return q;
 }
 
-int sc_energy_one_trial(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3],
+static int sc_energy_one_trial_impl(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3],
                         double* e) {
     if (!ctx || !e || !pos || !dir || !pd0) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
@@ -1527,7 +1918,12 @@ int sc_energy_one_trial(sc_ctx* ctx, int target, const double pos[3], const doub
     return run_queries(ctx, 1, &q, nullptr, e);
 }
 
-int sc_commit_particle(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3]) {
+int sc_energy_one_trial(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3],
+                        double* e) {
+    SC_GUARD(ctx, sc_energy_one_trial_impl(ctx, target, pos, dir, pd0, e))
+}
+
+static int sc_commit_particle_impl(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3]) {
     if (!ctx || !pos || !dir || !pd0) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     if (target < 0 || target >= ctx->n) return fail(ctx, SC_EINVAL, "target out of range");
@@ -1543,7 +1939,11 @@ int sc_commit_particle(sc_ctx* ctx, int target, const double pos[3], const doubl
     return build_grid(ctx, ctx->grid_valid ? ctx->g.r : 2, nullptr, ctx->box);
 }
 
-int sc_overlap_flags(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, uint8_t* out, double* dmin) {
+int sc_commit_particle(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3]) {
+    SC_GUARD(ctx, sc_commit_particle_impl(ctx, target, pos, dir, pd0))
+}
+
+static int sc_overlap_flags_impl(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, uint8_t* out, double* dmin) {
     if (!ctx || m < 0 || (m > 0 && (!i || !j || !out))) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     for (int k = 0; k < m; k++)
@@ -1569,7 +1969,11 @@ int sc_overlap_flags(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, uin
     return SC_OK;
 }
 
-int sc_overlap_count(sc_ctx* ctx, long* count) {
+int sc_overlap_flags(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, uint8_t* out, double* dmin) {
+    SC_GUARD(ctx, sc_overlap_flags_impl(ctx, m, i, j, out, dmin))
+}
+
+static int sc_overlap_count_impl(sc_ctx* ctx, long* count) {
     if (!ctx || !count) return SC_EINVAL;
     long np = 0;
     int rc = sc_neighbor_pairs(ctx, ctx->maxcut, nullptr, 0, &np);
@@ -1586,6 +1990,10 @@ int sc_overlap_count(sc_ctx* ctx, long* count) {
     return SC_OK;
 }
 
+int sc_overlap_count(sc_ctx* ctx, long* count) {
+    SC_GUARD(ctx, sc_overlap_count_impl(ctx, count))
+}
+
 // host-side stream of the sweep schedule (grid shifts, colour orders)
 static uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
@@ -1594,7 +2002,7 @@ static uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
-int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot_half, int r, uint64_t seed,
+static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot_half, int r, uint64_t seed,
              uint64_t counter, sc_sweep_stats* stats) {
     if (!ctx || nsweeps < 0 || !(temper > 0) || r < 1 || r > 2) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
@@ -1612,8 +2020,16 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
         if (rc) return rc;
         const Grid& g = ctx->g;
         int cap = stage_cap(ctx, g, ctx->box);
-        size_t lds = stage_bytes(cap) + sizeof(int) * (3 * cap) + sizeof(float) * 2 * MV_STRIDE +
-                     sizeof(int) * (2 * 125 + 2);
+        bool da = ctx->use_surrogate;
+        if (da && (ctx->nt != 1 || ctx->dense2ref[0] != ctx->mlp_type))
+            return fail(ctx, SC_EINVAL, "delayed acceptance needs a one-type system of the network's type");
+        // LDS: stage + near/expensive lists + segment distances + moving particle
+        // + stencil scratch (+ the network and its activation tile)
+        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + (da ? MLP_LDS_BYTES + 16 : 0);
+        size_t per = stage_bytes(1) + sizeof(int) * 3 + (da ? sizeof(float) * 2 : 0);
+        int capmax = (int)((65536 - fixed) / per);
+        if (cap > capmax) cap = (capmax / 64) * 64;
+        size_t lds = per * cap + fixed;
         // colour order: a random permutation of the p^3 colours
         int ncol = p * p * p;
         int order[27];
@@ -1655,9 +2071,16 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
                 }
                 HIPCK(hipEventRecord(ctx->kev[2 * launches], ctx->stream));
             }
-            hipLaunchKernelGGL(k_sweep_pass, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp, ctx->starts,
-                               ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,
-                               ctx->ppmax, ctx->d_nonempty, ctx->d_stats, ctx->d_dsum, ctx->d_flag);
+            if (da)
+                hipLaunchKernelGGL(k_sweep_pass<true>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp, ctx->starts,
+                                   ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,
+                                   ctx->ppmax, ctx->d_nonempty, ctx->mlp_img, ctx->mlp_c, ctx->d_stats, ctx->d_dsum,
+                                   ctx->d_flag);
+            else
+                hipLaunchKernelGGL(k_sweep_pass<false>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
+                                   ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
+                                   cap, ctx->ppmax, ctx->d_nonempty, nullptr, ctx->mlp_c, ctx->d_stats, ctx->d_dsum,
+                                   ctx->d_flag);
             if (ctx->profile) HIPCK(hipEventRecord(ctx->kev[2 * launches + 1], ctx->stream));
             launches++;
         }
@@ -1687,13 +2110,19 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
         stats->rot_acc = hs[2];
         stats->rot_try = hs[3];
         stats->left_cell = hs[4];
-        stats->stage1_acc = stats->stage1_try = 0;
+        stats->stage1_acc = hs[5];
+        stats->stage1_try = hs[6];
         stats->energy_change = ds;
     }
     return SC_OK;
 }
 
-int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
+int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot_half, int r, uint64_t seed,
+             uint64_t counter, sc_sweep_stats* stats) {
+    SC_GUARD(ctx, sc_sweep_impl(ctx, nsweeps, temper, transmx, rot_half, r, seed, counter, stats))
+}
+
+static int sc_volume_move_impl(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
                    uint64_t counter, int* accepted, double box_out[3]) {
     if (!ctx || !accepted || !(temper > 0)) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
@@ -1736,7 +2165,12 @@ int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double e
     return SC_OK;
 }
 
-int sc_copy_probe(sc_ctx* ctx, long bytes, int reps) {
+int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
+                   uint64_t counter, int* accepted, double box_out[3]) {
+    SC_GUARD(ctx, sc_volume_move_impl(ctx, ptype, press, temper, edge_mx, seed, counter, accepted, box_out))
+}
+
+static int sc_copy_probe_impl(sc_ctx* ctx, long bytes, int reps) {
     if (!ctx || bytes <= 0 || reps < 1) return SC_EINVAL;
     if (ctx->device < 0) return fail(ctx, SC_ENODEV, "host-only context");
     HIPCK(hipSetDevice(ctx->device));
@@ -1756,16 +2190,426 @@ int sc_copy_probe(sc_ctx* ctx, long bytes, int reps) {
     return SC_OK;
 }
 
-int sc_set_trial_mode(sc_ctx* ctx, int mode) {
+int sc_copy_probe(sc_ctx* ctx, long bytes, int reps) {
+    SC_GUARD(ctx, sc_copy_probe_impl(ctx, bytes, reps))
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ surrogate ABI
+static_assert(MLP_NPARAM == SC_MLP_NPARAM && MLP_NF == SC_MLP_NF, "surrogate sizes of the public header");
+static MlpConst mlp_const_for(const schost::Ia& p) {
+    MlpConst c;
+    c.lsc = (float)(p.half_len[0] + p.half_len[1] + p.rcut);
+    c.rcut = (float)p.rcut;
+    c.pdis = (float)p.pdis;
+    c.pswitch = (float)p.pswitch;
+    return c;
+}
+
+static int mlp_alloc(sc_ctx* ctx) {
+    if (ctx->mlp_w) return SC_OK;
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->mlp_w, MLP_NPARAM)) || (rc = dalloc(ctx, &ctx->mlp_g, MLP_NPARAM)) ||
+        (rc = dalloc(ctx, &ctx->mlp_m, MLP_NPARAM)) || (rc = dalloc(ctx, &ctx->mlp_v, MLP_NPARAM)) ||
+        (rc = dalloc(ctx, &ctx->mlp_img, MLP_IMG_BYTES)) || (rc = dalloc(ctx, &ctx->d_loss, 1)))
+        return rc;
+    return SC_OK;
+}
+
+static int mlp_repack(sc_ctx* ctx) {
+    SC_LAUNCH(k_mlp_pack, dim3(1), dim3(256), 0, ctx->stream, ctx->mlp_w, ctx->mlp_img);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+static int sc_mlp_init_impl(sc_ctx* ctx, int type, uint64_t seed) {
+    if (!ctx) return SC_EINVAL;
+    if (ctx->device < 0) return fail(ctx, SC_ENODEV, "host-only context");
+    if (!ctx->have_topo || type < 0 || type >= schost::MAXT || ctx->topo.ia[type][type].geotype[0] == 0)
+        return fail(ctx, SC_EINVAL, "sc_mlp_init: type not defined in the topology");
+    {
+        PP32 hp;
+        fill_pair(hp, ctx->topo.ia[type][type], ctx->topo.ia[type][type]);
+        if (hp.kind != K_PSC && hp.kind != K_CPSC)
+            return fail(ctx, SC_EINVAL, "sc_mlp_init: the surrogate models patchy spherocylinders (PSC/CPSC families)");
+    }
+    HIPCK(hipSetDevice(ctx->device));
+    int rc = mlp_alloc(ctx);
+    if (rc) return rc;
+    // Kaiming-uniform weights (bound sqrt(6/fan_in) scaled for SiLU), zero biases
+    std::vector<float> w(MLP_NPARAM, 0.0f);
+    uint64_t h = mix64(seed + 99);
+    auto u = [&]() { h = mix64(h); return (double)(h >> 11) * (1.0 / 9007199254740992.0); };
+    auto fill = [&](int off, int fan_in, int count) {
+        double b = sqrt(6.0 / fan_in);
+        for (int i = 0; i < count; i++) w[off + i] = (float)((2.0 * u() - 1.0) * b);
+    };
+    fill(MLP_OFF_W1, MLP_NF, MLP_NF * MLP_NH);
+    fill(MLP_OFF_W2, MLP_NH, MLP_NH * MLP_NH);
+    fill(MLP_OFF_W3, MLP_NH, MLP_NH * MLP_NH);
+    fill(MLP_OFF_W4, MLP_NH, MLP_NH);
+    for (int i = 0; i < MLP_NH; i++) w[MLP_OFF_W4 + i] *= 0.1f;
+    HIPCK(hipMemcpy(ctx->mlp_w, w.data(), MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    HIPCK(hipMemset(ctx->mlp_m, 0, MLP_NPARAM * sizeof(float)));
+    HIPCK(hipMemset(ctx->mlp_v, 0, MLP_NPARAM * sizeof(float)));
+    ctx->adam_t = 0;
+    ctx->mlp_type = type;
+    ctx->mlp_c = mlp_const_for(ctx->topo.ia[type][type]);
+    ctx->mlp_has = true;
+    return mlp_repack(ctx);
+}
+
+extern "C" int sc_mlp_init(sc_ctx* ctx, int type, uint64_t seed) {
+    SC_GUARD(ctx, sc_mlp_init_impl(ctx, type, seed))
+}
+
+static int sc_mlp_set_weights_impl(sc_ctx* ctx, const float* w, int n) {
+    if (!ctx || !w || n != MLP_NPARAM) return SC_EINVAL;
+    if (!ctx->mlp_has) return fail(ctx, SC_ESTATE, "call sc_mlp_init first");
+    HIPCK(hipMemcpy(ctx->mlp_w, w, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+    return mlp_repack(ctx);
+}
+
+extern "C" int sc_mlp_set_weights(sc_ctx* ctx, const float* w, int n) {
+    SC_GUARD(ctx, sc_mlp_set_weights_impl(ctx, w, n))
+}
+
+static int sc_mlp_get_weights_impl(sc_ctx* ctx, float* w, int n) {
+    if (!ctx || !w || n != MLP_NPARAM) return SC_EINVAL;
+    if (!ctx->mlp_has) return fail(ctx, SC_ESTATE, "call sc_mlp_init first");
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    HIPCK(hipMemcpy(w, ctx->mlp_w, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
+extern "C" int sc_mlp_get_weights(sc_ctx* ctx, float* w, int n) {
+    SC_GUARD(ctx, sc_mlp_get_weights_impl(ctx, w, n))
+}
+
+static int single_pair_tables(sc_ctx* ctx, int type, PP32** pp, TP32** tp) {
+    PP32 hp;
+    TP32 ht;
+    fill_pair(hp, ctx->topo.ia[type][type], ctx->topo.ia[type][type]);
+    fill_type(ht, ctx->topo.ia[type][type]);
+    HIPCK(hipMalloc((void**)pp, sizeof(PP32)));
+    HIPCK(hipMalloc((void**)tp, sizeof(TP32)));
+    HIPCK(hipMemcpy(*pp, &hp, sizeof(PP32), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(*tp, &ht, sizeof(TP32), hipMemcpyHostToDevice));
+    return SC_OK;
+}
+
+static int sc_mlp_generate_impl(sc_ctx* ctx, long ns, uint64_t seed, double dmax, double near_frac) {
+    if (!ctx || ns <= 0 || !(dmax > 0)) return SC_EINVAL;
+    if (!ctx->mlp_has) return fail(ctx, SC_ESTATE, "call sc_mlp_init first");
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if (ns != ctx->tr_n) {
+        if ((rc = dalloc(ctx, &ctx->tr_x, ns * MLP_NF)) || (rc = dalloc(ctx, &ctx->tr_y, ns)) ||
+            (rc = dalloc(ctx, &ctx->tr_e, ns)))
+            return rc;
+        ctx->tr_n = ns;
+    }
+    PP32* pp = nullptr;
+    TP32* tp = nullptr;
+    if ((rc = single_pair_tables(ctx, ctx->mlp_type, &pp, &tp))) return rc;
+    SC_LAUNCH(k_mlp_gen, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, ctx->stream, ns, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), pp, tp, tp, ctx->mlp_c, (float)dmax, (float)near_frac, ctx->tr_x,
+                       ctx->tr_y, ctx->tr_e);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(pp);
+    hipFree(tp);
+    if (e != hipSuccess) return fail(ctx, SC_EHIP, hipGetErrorString(e));
+    return SC_OK;
+}
+
+extern "C" int sc_mlp_generate(sc_ctx* ctx, long ns, uint64_t seed, double dmax, double near_frac) {
+    SC_GUARD(ctx, sc_mlp_generate_impl(ctx, ns, seed, dmax, near_frac))
+}
+
+static int sc_mlp_training_data_impl(sc_ctx* ctx, long first, long count, float* feats, float* target, float* efull) {
+    if (!ctx || first < 0 || count < 0 || first + count > ctx->tr_n) return SC_EINVAL;
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (feats) HIPCK(hipMemcpy(feats, ctx->tr_x + first * MLP_NF, count * MLP_NF * sizeof(float), hipMemcpyDeviceToHost));
+    if (target) HIPCK(hipMemcpy(target, ctx->tr_y + first, count * sizeof(float), hipMemcpyDeviceToHost));
+    if (efull) HIPCK(hipMemcpy(efull, ctx->tr_e + first, count * sizeof(float), hipMemcpyDeviceToHost));
+    return SC_OK;
+}
+
+extern "C" int sc_mlp_training_data(sc_ctx* ctx, long first, long count, float* feats, float* target, float* efull) {
+    SC_GUARD(ctx, sc_mlp_training_data_impl(ctx, first, count, feats, target, efull))
+}
+
+static int ensure_act(sc_ctx* ctx, int B) {
+    if (B <= ctx->act_batch) return SC_OK;
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->act, (size_t)8 * B * MLP_NH)) || (rc = dalloc(ctx, &ctx->bx, (size_t)B * (MLP_NF + 3))))
+        return rc;
+    ctx->act_batch = B;
+    return SC_OK;
+}
+
+// forward + backward of one batch held in ctx->bx (x: B*12, then y: B), gradients into mlp_g
+static int mlp_fwd_bwd(sc_ctx* ctx, int B, double* loss) {
+    float* X = ctx->bx;
+    float* Y = X + (size_t)B * MLP_NF;
+    float* OUT = Y + B;
+    float* DOUT = OUT + B;
+    size_t L = (size_t)B * MLP_NH;
+    float *Z1 = ctx->act, *H1 = Z1 + L, *Z2 = H1 + L, *H2 = Z2 + L, *Z3 = H2 + L, *H3 = Z3 + L, *DA = H3 + L, *DB = DA + L;
+    const float* W = ctx->mlp_w;
+    float* G = ctx->mlp_g;
+    int nb = (int)((L + 255) / 256);
+    HIPCK(hipMemsetAsync(G, 0, MLP_NPARAM * sizeof(float), ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->d_loss, 0, sizeof(double), ctx->stream));
+    SC_LAUNCH(k_dense_fwd, dim3(nb), dim3(256), 0, ctx->stream, B, MLP_NF, X, W + MLP_OFF_W1, W + MLP_OFF_B1, Z1, H1);
+    SC_LAUNCH(k_dense_fwd, dim3(nb), dim3(256), 0, ctx->stream, B, MLP_NH, H1, W + MLP_OFF_W2, W + MLP_OFF_B2, Z2, H2);
+    SC_LAUNCH(k_dense_fwd, dim3(nb), dim3(256), 0, ctx->stream, B, MLP_NH, H2, W + MLP_OFF_W3, W + MLP_OFF_B3, Z3, H3);
+    SC_LAUNCH(k_out_loss, dim3((B + 255) / 256), dim3(256), 0, ctx->stream, B, H3, W, Y, OUT, DOUT, ctx->d_loss);
+    int nchunk = (B + 255) / 256;
+    SC_LAUNCH(k_out_bwd_w, dim3(nchunk), dim3(128), 0, ctx->stream, B, H3, DOUT, G);
+    SC_LAUNCH(k_out_bwd, dim3(nb), dim3(256), 0, ctx->stream, B, DOUT, W, Z3, DA);
+    SC_LAUNCH(k_dense_bwd_w, dim3((65 * MLP_NH + 255) / 256, nchunk), dim3(256), 0, ctx->stream, B, MLP_NH, H2, DA,
+                       G + MLP_OFF_W3, G + MLP_OFF_B3);
+    SC_LAUNCH(k_dense_bwd_x, dim3(nb), dim3(256), 0, ctx->stream, B, DA, W + MLP_OFF_W3, Z2, DB);
+    SC_LAUNCH(k_dense_bwd_w, dim3((65 * MLP_NH + 255) / 256, nchunk), dim3(256), 0, ctx->stream, B, MLP_NH, H1, DB,
+                       G + MLP_OFF_W2, G + MLP_OFF_B2);
+    SC_LAUNCH(k_dense_bwd_x, dim3(nb), dim3(256), 0, ctx->stream, B, DB, W + MLP_OFF_W2, Z1, DA);
+    SC_LAUNCH(k_dense_bwd_w, dim3((13 * MLP_NH + 255) / 256, nchunk), dim3(256), 0, ctx->stream, B, MLP_NF, X, DA,
+                       G + MLP_OFF_W1, G + MLP_OFF_B1);
+    HIPCK(hipGetLastError());
+    if (loss) {
+        HIPCK(hipMemcpyAsync(loss, ctx->d_loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCK(hipStreamSynchronize(ctx->stream));
+        *loss /= B;
+    }
+    return SC_OK;
+}
+
+static int sc_mlp_gradients_impl(sc_ctx* ctx, int B, const float* x, const float* y, float* grads, int ngrads,
+                                 double* loss) {
+    if (!ctx || B <= 0 || !x || !y || !grads || ngrads != MLP_NPARAM) return SC_EINVAL;
+    if (!ctx->mlp_has) return fail(ctx, SC_ESTATE, "call sc_mlp_init first");
+    HIPCK(hipSetDevice(ctx->device));
+    int rc = ensure_act(ctx, B);
+    if (rc) return rc;
+    HIPCK(hipMemcpy(ctx->bx, x, (size_t)B * MLP_NF * sizeof(float), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->bx + (size_t)B * MLP_NF, y, (size_t)B * sizeof(float), hipMemcpyHostToDevice));
+    double l = 0;
+    if ((rc = mlp_fwd_bwd(ctx, B, &l))) return rc;
+    HIPCK(hipMemcpy(grads, ctx->mlp_g, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    if (loss) *loss = l;
+    return SC_OK;
+}
+
+extern "C" int sc_mlp_gradients(sc_ctx* ctx, int B, const float* x, const float* y, float* grads, int ngrads,
+                                double* loss) {
+    SC_GUARD(ctx, sc_mlp_gradients_impl(ctx, B, x, y, grads, ngrads, loss))
+}
+
+extern "C" int sc_mlp_nparam(void) { return MLP_NPARAM; }
+
+// gather a batch: rows (a*i + b) mod nrows of the training set
+__global__ void k_gather(int B, long n, long a, long b, const float* __restrict__ X, const float* __restrict__ Y,
+                         float* __restrict__ bx) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B) return;
+    long r = (long)(((unsigned long long)a * (unsigned long long)i + (unsigned long long)b) % (unsigned long long)n);
+    for (int q = 0; q < MLP_NF; q++) bx[(size_t)i * MLP_NF + q] = X[r * MLP_NF + q];
+    bx[(size_t)B * MLP_NF + i] = Y[r];
+}
+
+static long gcdl(long a, long b) { while (b) { long t = a % b; a = b; b = t; } return a; }
+
+static int sc_mlp_train_impl(sc_ctx* ctx, int epochs, int batch, double lr, double lr_final, uint64_t seed,
+                            double val_frac, double* train_loss, double* val_rmse) {
+    if (!ctx || epochs < 1 || batch < 32 || !(lr > 0)) return SC_EINVAL;
+    if (!ctx->mlp_has || ctx->tr_n <= 0) return fail(ctx, SC_ESTATE, "generate training data first");
+    HIPCK(hipSetDevice(ctx->device));
+    long nval = (long)(ctx->tr_n * val_frac), ntr = ctx->tr_n - nval;
+    if (ntr < batch) return fail(ctx, SC_EINVAL, "training set smaller than a batch");
+    int rc = ensure_act(ctx, batch);
+    if (rc) return rc;
+    long steps_per_epoch = ntr / batch, total = steps_per_epoch * epochs, step = 0;
+    uint64_t h = mix64(seed);
+    double lsum = 0.0;
+    long lcount = 0;
+    for (int ep = 0; ep < epochs; ep++) {
+        h = mix64(h);
+        long a = (long)(h % (uint64_t)ntr) | 1;
+        while (gcdl(a, ntr) != 1) a += 2;
+        h = mix64(h);
+        long b0 = (long)(h % (uint64_t)ntr);
+        for (long st = 0; st < steps_per_epoch; st++, step++) {
+            SC_LAUNCH(k_gather, dim3((batch + 255) / 256), dim3(256), 0, ctx->stream, batch, ntr, a,
+                               (long)((b0 + (unsigned long long)a * (unsigned long long)(st * batch)) % (unsigned long long)ntr),
+                               ctx->tr_x, ctx->tr_y, ctx->bx);
+            bool last_epoch = ep == epochs - 1;
+            double l = 0;
+            if ((rc = mlp_fwd_bwd(ctx, batch, last_epoch ? &l : nullptr))) return rc;
+            if (last_epoch) { lsum += l; lcount++; }
+            // cosine decay from lr to lr_final
+            double lrs = lr_final + 0.5 * (lr - lr_final) * (1.0 + cos(3.141592653589793 * (double)step / (double)total));
+            ctx->adam_t++;
+            float c1 = (float)(1.0 - pow(0.9, (double)ctx->adam_t)), c2 = (float)(1.0 - pow(0.999, (double)ctx->adam_t));
+            SC_LAUNCH(k_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, ctx->stream, MLP_NPARAM, ctx->mlp_w,
+                               ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, (float)lrs, 0.9f, 0.999f, 1e-8f, c1, c2);
+        }
+    }
+    HIPCK(hipGetLastError());
+    if ((rc = mlp_repack(ctx))) return rc;
+    if (train_loss) *train_loss = lcount ? lsum / lcount : 0.0;
+    if (val_rmse) {
+        *val_rmse = 0.0;
+        if (nval > 0) {
+            std::vector<float> xv((size_t)nval * MLP_NF), yv(nval), ov(nval);
+            HIPCK(hipMemcpy(xv.data(), ctx->tr_x + ntr * MLP_NF, xv.size() * sizeof(float), hipMemcpyDeviceToHost));
+            HIPCK(hipMemcpy(yv.data(), ctx->tr_y + ntr, nval * sizeof(float), hipMemcpyDeviceToHost));
+            float* dx = nullptr;
+            float* dout = nullptr;
+            HIPCK(hipMalloc((void**)&dx, xv.size() * sizeof(float)));
+            HIPCK(hipMalloc((void**)&dout, nval * sizeof(float)));
+            HIPCK(hipMemcpy(dx, xv.data(), xv.size() * sizeof(float), hipMemcpyHostToDevice));
+            SC_LAUNCH(k_mlp_infer, dim3((unsigned)((nval + 31) / 32)), dim3(WAVE), MLP_LDS_BYTES, ctx->stream,
+                               (int)nval, dx, ctx->mlp_img, ctx->mlp_w, 1, dout);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipMemcpy(ov.data(), dout, nval * sizeof(float), hipMemcpyDeviceToHost);
+            hipFree(dx);
+            hipFree(dout);
+            if (e != hipSuccess) return fail(ctx, SC_EHIP, hipGetErrorString(e));
+            double ss = 0;
+            for (long i = 0; i < nval; i++) ss += (double)(ov[i] - yv[i]) * (ov[i] - yv[i]);
+            *val_rmse = sqrt(ss / nval);
+        }
+    }
+    return SC_OK;
+}
+
+extern "C" int sc_mlp_train(sc_ctx* ctx, int epochs, int batch, double lr, double lr_final, uint64_t seed,
+                            double val_frac, double* train_loss, double* val_rmse) {
+    SC_GUARD(ctx, sc_mlp_train_impl(ctx, epochs, batch, lr, lr_final, seed, val_frac, train_loss, val_rmse))
+}
+
+static int sc_mlp_forward_impl(sc_ctx* ctx, int m, const float* x, float* out, int bf16) {
+    if (!ctx || m <= 0 || !x || !out) return SC_EINVAL;
+    if (!ctx->mlp_has) return fail(ctx, SC_ESTATE, "call sc_mlp_init first");
+    HIPCK(hipSetDevice(ctx->device));
+    float* dx = nullptr;
+    float* dout = nullptr;
+    HIPCK(hipMalloc((void**)&dx, (size_t)m * MLP_NF * sizeof(float)));
+    if (hipMalloc((void**)&dout, (size_t)m * sizeof(float)) != hipSuccess) { hipFree(dx); return fail(ctx, SC_EHIP, "alloc"); }
+    hipError_t e = hipMemcpy(dx, x, (size_t)m * MLP_NF * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        SC_LAUNCH(k_mlp_infer, dim3((m + 31) / 32), dim3(WAVE), bf16 ? MLP_LDS_BYTES : 0, ctx->stream, m, dx,
+                           ctx->mlp_img, ctx->mlp_w, bf16, dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, (size_t)m * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(dx);
+    hipFree(dout);
+    if (e != hipSuccess) return fail(ctx, SC_EHIP, hipGetErrorString(e));
+    return SC_OK;
+}
+
+extern "C" int sc_mlp_forward(sc_ctx* ctx, int m, const float* x, float* out, int bf16) {
+    SC_GUARD(ctx, sc_mlp_forward_impl(ctx, m, x, out, bf16))
+}
+
+// features and exact attraction of pairs (i as part1) of the current state
+__global__ void k_pair_features(int m, const int* __restrict__ pi, const int* __restrict__ pj, const int* __restrict__ slot,
+                                SysConst sc, const uint4* __restrict__ P, const float4* __restrict__ O,
+                                const float2* __restrict__ Q, const PP32* __restrict__ pp, const TP32* __restrict__ tp,
+                                MlpConst mc, float* __restrict__ feats, float* __restrict__ eattr) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    int a = slot[pi[k]], b = slot[pj[k]];
+    uint4 ua = P[a], ub = P[b];
+    DPart<float> pa = load_part<float>(O[a], Q[a], ua.w, tp);
+    DPart<float> pb = load_part<float>(O[b], Q[b], ub.w, tp);
+    V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
+    const PP32& P2 = pp[ua.w * sc.nt + ub.w];
+    float erep = 0.0f, d2 = 0.0f, ea = 0.0f;
+    if (P2.kind <= K_SCA && P2.kind >= 0) {
+        bool att = scsc_cheap(P2, pa, pb, r, erep, d2);
+        if (att) ea = scsc_attr(P2, pa, pb, r);
+    }
+    float f[MLP_NF];
+    mlp_features(pa.dir, pa.pd[0], mk<float>(O[b].x, O[b].y, O[b].z), mk<float>(O[b].w, Q[b].x, Q[b].y), r, sqrtf(d2),
+                 mc.lsc, mc.rcut, mc.pdis, mc.pswitch, f);
+    for (int q = 0; q < MLP_NF; q++) feats[(long)k * MLP_NF + q] = f[q];
+    eattr[k] = ea;
+}
+
+static int sc_mlp_pair_features_impl(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, float* feats, float* eattr) {
+    if (!ctx || m <= 0 || !i || !j || !feats || !eattr) return SC_EINVAL;
+    if (!ctx->have_state || !ctx->mlp_has) return fail(ctx, SC_ESTATE, "needs a configuration and a network");
+    for (int k = 0; k < m; k++)
+        if (i[k] < 0 || i[k] >= ctx->n || j[k] < 0 || j[k] >= ctx->n) return fail(ctx, SC_EINVAL, "index out of range");
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = ensure_ij(ctx, m))) return rc;
+    float *df = nullptr, *de = nullptr;
+    HIPCK(hipMalloc((void**)&df, (size_t)m * MLP_NF * sizeof(float)));
+    if (hipMalloc((void**)&de, (size_t)m * sizeof(float)) != hipSuccess) { hipFree(df); return fail(ctx, SC_EHIP, "alloc"); }
+    hipError_t e = hipMemcpy(ctx->d_i, i, m * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(ctx->d_j, j, m * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        SysConst sc = sysconst(ctx, ctx->box);
+        int c = ctx->cur;
+        SC_LAUNCH(k_pair_features, dim3((m + 127) / 128), dim3(128), 0, ctx->stream, m, ctx->d_i, ctx->d_j,
+                           ctx->slot, sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp32, ctx->d_tp32, ctx->mlp_c, df, de);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpy(feats, df, (size_t)m * MLP_NF * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(eattr, de, (size_t)m * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(df);
+    hipFree(de);
+    if (e != hipSuccess) return fail(ctx, SC_EHIP, hipGetErrorString(e));
+    return SC_OK;
+}
+
+extern "C" int sc_mlp_pair_features(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, float* feats, float* eattr) {
+    SC_GUARD(ctx, sc_mlp_pair_features_impl(ctx, m, i, j, feats, eattr))
+}
+
+static int sc_set_surrogate_impl(sc_ctx* ctx, int on) {
+    if (!ctx) return SC_EINVAL;
+    if (on) {
+        if (!ctx->mlp_has) return fail(ctx, SC_ESTATE, "no surrogate network");
+        if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+        if (ctx->nt != 1 || ctx->dense2ref[0] != ctx->mlp_type)
+            return fail(ctx, SC_EINVAL, "delayed acceptance needs a one-type system of the network's type");
+    }
+    ctx->use_surrogate = on != 0;
+    return SC_OK;
+}
+
+extern "C" int sc_set_surrogate(sc_ctx* ctx, int on) {
+    SC_GUARD(ctx, sc_set_surrogate_impl(ctx, on))
+}
+
+extern "C" {
+
+static int sc_set_trial_mode_impl(sc_ctx* ctx, int mode) {
     if (!ctx || mode < 0 || mode > 1) return SC_EINVAL;
     ctx->trial_mode = mode;
     return SC_OK;
 }
 
-int sc_set_profiling(sc_ctx* ctx, int on) {
+int sc_set_trial_mode(sc_ctx* ctx, int mode) {
+    SC_GUARD(ctx, sc_set_trial_mode_impl(ctx, mode))
+}
+
+static int sc_set_profiling_impl(sc_ctx* ctx, int on) {
     if (!ctx) return SC_EINVAL;
     ctx->profile = on != 0;
     return SC_OK;
+}
+
+int sc_set_profiling(sc_ctx* ctx, int on) {
+    SC_GUARD(ctx, sc_set_profiling_impl(ctx, on))
 }
 
 int sc_last_timing(const sc_ctx* ctx, double* ms, long* launches, double* kernel_ms) {
@@ -1784,7 +2628,7 @@ int sc_last_timing(const sc_ctx* ctx, double* ms, long* launches, double* kernel
 // around the axis) and kept if their closest distance to every particle
 // already placed is >= sigma_ij (the hard-core criterion of sc_overlap_flags).
 // Used to build seeded, overlap-free starting configurations.
-extern "C" int sc_generate_rsa(sc_ctx* ctx, int n, const int32_t* types, const double box[3], uint64_t seed,
+static int sc_generate_rsa_impl(sc_ctx* ctx, int n, const int32_t* types, const double box[3], uint64_t seed,
                                long max_attempts, double* pos, double* dir, double* pd0) {
     if (!ctx || n < 0 || !types || !box || !pos || !dir || !pd0) return SC_EINVAL;
     if (!ctx->have_topo) return fail(ctx, SC_ESTATE, "load a topology first");
@@ -1863,4 +2707,9 @@ extern "C" int sc_generate_rsa(sc_ctx* ctx, int n, const int32_t* types, const d
         }
     }
     return SC_OK;
+}
+
+extern "C" int sc_generate_rsa(sc_ctx* ctx, int n, const int32_t* types, const double box[3], uint64_t seed,
+                               long max_attempts, double* pos, double* dir, double* pd0) {
+    SC_GUARD(ctx, sc_generate_rsa_impl(ctx, n, types, box, seed, max_attempts, pos, dir, pd0))
 }

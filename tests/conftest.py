@@ -16,3 +16,30 @@ def pytest_configure(config):
 @pytest.fixture(scope="session")
 def golden_dir():
     return os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.fixture(scope="session")
+def ensemble_system():
+    """500 PSC at eta 0.2, T*=1 and the CPU oracle's sequential Metropolis
+    reference: mean E/N over 40 blocks of 25 sweeps after 200 sweeps.
+    Returns (workload, (pos, dir, pd0) of the start, E/N per block, rot_half)."""
+    import numpy as np
+    import sc_amd
+    from sc_amd import workloads as W
+    import pyoracle as po
+    w = W.workload("c1_psc_1000", n_override=500)
+    g = sc_amd.GpuSystem(0)
+    g.load_topology(w["top"])
+    pos, d, q = g.generate_rsa(w["types"], w["box"], seed=31)
+    g.set_config(w["box"], pos, d, q, w["types"])
+    b, p, dd, qq = g.get_config()
+    g.close()
+    tp, st = po.Topo(w["top"]), po.State(b, p, dd, qq, w["types"])
+    rot_half = 20.0 / 180 * np.pi / 4
+    stc, _, _, _ = po.mc_run(tp, st, 200, 1.0, 0.3, rot_half, seed=17)
+    ec = []
+    cur = stc
+    for blk in range(40):
+        cur, _, _, ef = po.mc_run(tp, cur, 25, 1.0, 0.3, rot_half, seed=100 + blk)
+        ec.append(ef / st.n)
+    return w, (pos, d, q), np.array(ec), rot_half

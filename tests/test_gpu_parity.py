@@ -242,26 +242,6 @@ def test_sweep_energy_drift_and_determinism(dev):
         np.testing.assert_array_equal(a, b)
 
 
-@pytest.fixture(scope="module")
-def ensemble_system():
-    """500 PSC at eta 0.2, T*=1 and the CPU oracle's sequential Metropolis
-    reference: mean E/N over 40 blocks of 25 sweeps after 200 sweeps."""
-    w = W.workload("c1_psc_1000", n_override=500)
-    g = sc_amd.GpuSystem(0)
-    g.load_topology(w["top"])
-    pos, d, q = g.generate_rsa(w["types"], w["box"], seed=31)
-    tp, st = gpu_state(g, w["top"], w["box"], pos, d, q, w["types"])
-    g.close()
-    rot_half = 20.0 / 180 * np.pi / 4
-    stc, _, _, _ = po.mc_run(tp, st, 200, 1.0, 0.3, rot_half, seed=17)
-    ec = []
-    cur = stc
-    for blk in range(40):
-        cur, _, _, ef = po.mc_run(tp, cur, 25, 1.0, 0.3, rot_half, seed=100 + blk)
-        ec.append(ef / st.n)
-    return w, (pos, d, q), np.array(ec), rot_half
-
-
 def _blocked(x, nb=8):
     b = x[: len(x) // nb * nb].reshape(nb, -1).mean(1)
     return b.mean(), b.std(ddof=1) / np.sqrt(nb)
