@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=10)
     ap.add_argument("--no-da", action="store_true", help="exact sweep only")
     ap.add_argument("--no-exact", action="store_true", help="delayed-acceptance sweep only")
+    ap.add_argument("--equil", type=int, default=1000, help="untimed exact sweeps before the legs")
+    ap.add_argument("--ensemble", type=int, default=1000,
+                    help="untimed sweeps after each timed leg over which E/N is averaged")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding); "
@@ -89,6 +92,10 @@ def main():
         g.copy_probe(256 << 20, reps=3)
         log("copy probe: 3 x k_copy of 256 MiB")
     g.set_trial_mode(a.trial_mode)
+    if a.equil:
+        t0 = time.time()
+        g.sweep(a.equil, temper, transmx, rot_half, r=a.r, seed=a.seed + 1, counter=5 * 10 ** 6)
+        log("equilibrated: %d exact sweeps in %.1fs, E/N %.4f" % (a.equil, time.time() - t0, g.allToAll() / n))
     box0, pos0, d0, q0 = g.get_config()
     _, start, _, nc = g.build_cells(r=a.r)
     ncell = int(np.prod(nc))
@@ -98,9 +105,9 @@ def main():
     # written back (N/ncol * 40 B) and the cell CSR (4 B per cell)
     bytes_launch = n * 44.0 + n / ncol * 40.0 + 4.0 * (ncell + 1)
 
-    def leg(label, da):
+    def leg(label, mode):
         g.set_config(box0, pos0, d0, q0, w["types"])
-        g.set_surrogate(da)
+        g.set_surrogate(mode)
         g.set_profiling(False)
         g.sweep(a.warmup, temper, transmx, rot_half, r=a.r, seed=a.seed, counter=1)
         g.set_profiling(True)
@@ -117,6 +124,7 @@ def main():
         achieved = bytes_launch / avg_launch_s / 1e9
         tries = max(1, st["trans_try"] + st["rot_try"])
         acc = st["trans_acc"] + st["rot_acc"]
+        kname = {0: "k_sweep_pass<DA_OFF>", 1: "k_sweep_pass<DA_NET>", 2: "k_sweep_pass<DA_REP>"}[mode]
         r = {"sweeps_per_s": sps, "ms_per_step": 1e3 * wall / a.steps, "wall_s": wall, "device_ms": dev_ms,
              "trial_moves_per_particle": done / a.steps,
              "acceptance": {"translation": st["trans_acc"] / max(1, st["trans_try"]),
@@ -125,25 +133,41 @@ def main():
              "energy_per_particle": g.allToAll() / n,
              "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS,
-                          "traffic": read_traffic("k_sweep_pass_da" if da else "k_sweep_pass_exact"),
-                          "kernel": "k_sweep_pass<%s>" % ("true" if da else "false"),
-                          "bytes_per_launch": bytes_launch, "avg_launch_ms": 1e3 * avg_launch_s,
+                          "traffic": read_traffic({0: "k_sweep_pass_exact", 1: "k_sweep_pass_da_net",
+                                                   2: "k_sweep_pass_da_rep"}[mode]),
+                          "kernel": kname, "bytes_per_launch": bytes_launch, "avg_launch_ms": 1e3 * avg_launch_s,
                           "launches": launches}}
-        if da:
-            r["stage1_acceptance"] = st["stage1_acc"] / max(1, st["stage1_try"])
-            r["stage2_acceptance"] = acc / max(1, st["stage1_acc"])
+        # ensemble average of E/N over the following untimed sweeps (every 25)
+        es = []
+        for k in range(a.ensemble // 25):
+            g.sweep(25, temper, transmx, rot_half, r=a.r, seed=a.seed, counter=2 * 10 ** 6 + 25 * k)
+            es.append(g.allToAll() / n)
+        if es:
+            es = np.array(es)
+            nb = min(8, len(es))
+            blocks = es[: len(es) // nb * nb].reshape(nb, -1).mean(1)
+            r["ensemble_energy_per_particle"] = {"mean": float(es.mean()), "stderr": float(blocks.std(ddof=1) /
+                                                                                        np.sqrt(nb)) if nb > 1 else None,
+                                                 "sweeps": len(es) * 25}
+        if mode:
+            r["screen1_pass"] = st["stage1_acc"] / max(1, st["stage1_try"])
+            if mode == 1:
+                r["screen2_pass"] = st["stage2_acc"] / max(1, st["stage2_try"])
+            r["exact_stage_accept"] = acc / max(1, st["stage2_acc"] if mode == 1 else st["stage1_acc"])
         log("%s sweep: %.2f sweeps/s (%.3f ms/sweep wall, %.3f ms device), %d pass launches, kernel %.3f ms, "
             "acceptance %.4f" % (label, sps, 1e3 * wall / a.steps, dev_ms / a.steps, launches, kern_ms, acc / tries))
         return r
 
-    exact = None if a.no_exact else leg("exact", False)
+    exact = None if a.no_exact else leg("exact", 0)
     da = None
+    rep = None
     train = None
     if not a.no_da:
+        rep = leg("delayed-acceptance, cheap-terms screen only", 2)
         from sc_amd import surrogate
         ptype = int(w["types"][0])
         train = surrogate.train(g, ptype, nsamples=a.samples, epochs=a.epochs, seed=a.seed, log=log)
-        da = leg("delayed-acceptance", True)
+        da = leg("delayed-acceptance, network screen", 1)
     main_leg = da if da is not None else exact
 
     cpu = None
@@ -187,6 +211,11 @@ def main():
         "da_over_exact": (da["sweeps_per_s"] / exact["sweeps_per_s"]) if (da and exact) else None,
         "exact": exact,
         "delayed_acceptance": da,
+        "delayed_acceptance_without_network": rep,
+        "ensemble_rel_diff_vs_exact": {
+            k: (abs(v["ensemble_energy_per_particle"]["mean"] / exact["ensemble_energy_per_particle"]["mean"] - 1.0)
+                if (v and exact and "ensemble_energy_per_particle" in v and "ensemble_energy_per_particle" in exact)
+                else None) for k, v in (("delayed_acceptance", da), ("delayed_acceptance_without_network", rep))},
         "surrogate_training": train,
         "roofline": main_leg["roofline"],
         "cpu_baseline": cpu,

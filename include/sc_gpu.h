@@ -43,7 +43,8 @@ typedef struct sc_ctx sc_ctx;
 typedef struct {
     uint64_t trans_acc, trans_try, rot_acc, rot_try;   /* Statistics trans/rot acc/rej */
     uint64_t left_cell;       /* translations rejected for leaving their cell */
-    uint64_t stage1_acc, stage1_try;                    /* delayed acceptance, first stage */
+    uint64_t stage1_acc, stage1_try;                    /* delayed acceptance: screen on the exact cheap terms */
+    uint64_t stage2_acc, stage2_try;                    /* delayed acceptance: network screen */
     double energy_change;     /* sum of accepted dE (for the drift check, updater.cpp:122-150) */
 } sc_sweep_stats;
 
@@ -207,10 +208,15 @@ int sc_mlp_forward(sc_ctx* ctx, int m, const float* x, float* out, int bf16);
 /* Features and exact FP32 patch attraction of pairs (i[k] as part1, j[k]) of
  * the loaded configuration. */
 int sc_mlp_pair_features(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, float* feats, float* eattr);
-/* on = 1: sc_sweep uses the two-stage delayed acceptance (a one-type system of
- * the network's type); sc_sweep_stats.stage1_acc / stage1_try count the
- * screen.  on = 0: the exact sweep. */
-int sc_set_surrogate(sc_ctx* ctx, int on);
+/* Acceptance scheme of sc_sweep.  0: exact Metropolis.  1: delayed
+ * acceptance in three stages: a Metropolis screen on the exact cheap terms of
+ * the energy change (segment-distance repulsion, sphere pairs), a screen on
+ * the network's patch attraction (a one-type system of the network's type),
+ * then the exact energy, which accepts with min(1, exp(-(dE - screened)/T)).
+ * 2: the same without the network screen.  All three sample the reference's
+ * Boltzmann distribution; sc_sweep_stats.stage1_* / stage2_* count the
+ * screens. */
+int sc_set_surrogate(sc_ctx* ctx, int mode);
 
 /* --- synthetic inputs ----------------------------------------------------- */
 /* Seeded overlap-free configuration by random sequential addition (host,

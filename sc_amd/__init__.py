@@ -24,7 +24,8 @@ class SCError(RuntimeError):
 class SweepStats(ctypes.Structure):
     _fields_ = [("trans_acc", ctypes.c_uint64), ("trans_try", ctypes.c_uint64), ("rot_acc", ctypes.c_uint64),
                 ("rot_try", ctypes.c_uint64), ("left_cell", ctypes.c_uint64), ("stage1_acc", ctypes.c_uint64),
-                ("stage1_try", ctypes.c_uint64), ("energy_change", ctypes.c_double)]
+                ("stage1_try", ctypes.c_uint64), ("stage2_acc", ctypes.c_uint64), ("stage2_try", ctypes.c_uint64),
+                ("energy_change", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -389,8 +390,13 @@ class GpuSystem:
         self._ck(self._L.sc_mlp_pair_features(self._h, len(i), _i(i), _i(j), _f(f), _f(e)), "mlp_pair_features")
         return f, e
 
-    def set_surrogate(self, on=True):
-        self._ck(self._L.sc_set_surrogate(self._h, int(bool(on))), "set_surrogate")
+    DA_OFF, DA_NET, DA_REP = 0, 1, 2
+
+    def set_surrogate(self, mode=1):
+        """0 (or False): exact sweep; 1 (or True): delayed acceptance with the
+        network screen; 2: delayed acceptance screening only the exact cheap
+        terms."""
+        self._ck(self._L.sc_set_surrogate(self._h, int(mode)), "set_surrogate")
 
     def set_trial_mode(self, mode):
         self._ck(self._L.sc_set_trial_mode(self._h, int(mode)), "set_trial_mode")

@@ -567,20 +567,32 @@ struct TrialScratch {
     float* mv;   // 2 * MV_STRIDE floats
 };
 
+// sweep counters: STAT_REP replicas of 8 counters, STAT_STRIDE apart (128 B)
+#define STAT_REP 64
+#define STAT_STRIDE 16
+
+// delayed-acceptance modes of the sweep
+#define DA_OFF 0   // exact Metropolis
+#define DA_NET 1   // screens: exact cheap terms, then the network's attraction
+#define DA_REP 2   // screen: exact cheap terms only
+
 struct MlpConst {
     float lsc, rcut, pdis, pswitch;
 };
 
-// With DA (delayed acceptance) a first stage accepts with the surrogate
-// energy change des (exact repulsion + network attraction); returns false when
-// it rejects.  Otherwise eo/en are the exact energies of the two states.
-template <bool DA>
+// Delayed acceptance (Christen & Fox 2005, in stages): with DA != DA_OFF
+// the move first faces Metropolis screens on cheaper energy changes, each
+// accepting with min(1, exp(-(its change)/T)); returns false at the first
+// screen that rejects (stage = the number of screens passed).  Otherwise eo/en
+// are the exact energies of the two states and des the sum of the screened
+// changes, which the final stage divides out: min(1, exp(-(dE - des)/T)).
+template <int DA>
 __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un,
                                                const DPart<float>& po, const DPart<float>& pn, const SysConst& sc,
                                                const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
                                                const TrialScratch& T, const float* ml, const MlpRegs& mr,
-                                               const MlpConst& mc, double temper, double u1, double& eo, double& en,
-                                               double& des) {
+                                               const MlpConst& mc, double temper, double u1, double u2, double& eo,
+                                               double& en, double& des, int& stage) {
     int lane = lane_id();
     int t = uo.w;
     bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
@@ -642,43 +654,53 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         if (need) {
             int pos = nx + wave_prefix(m);
             T.xl[pos] = jj | (which << 30);
-            if (DA) T.xd[pos] = sqrtf(dseg2);
+            if constexpr (DA == DA_NET) T.xd[pos] = sqrtf(dseg2);
         }
         nx += __popcll(m);
     }
     __syncthreads();
+    // screens of the delayed acceptance: (1) the exact cheap terms already
+    // summed (repulsion, sphere pairs), (2) with DA_NET the network's patch
+    // attraction, tiles of 16 pairs on MFMA
     des = 0.0;
-    if (DA) {
-        // first stage: exact repulsion + surrogate attraction, tiles of 32 pairs on MFMA
+    stage = 2;
+    if constexpr (DA != DA_OFF) {
         double rep = wave_sum(an) - wave_sum(ao);
-        double so = 0.0, sn = 0.0;
-        for (int t0 = 0; t0 < nx; t0 += 32) {
-            int e = t0 + (lane & 31);
-            float f[MLP_NF];
-            int which = 0;
-            if (e < nx) {
-                int w = T.xl[e];
-                int jj = w & 0x3fffffff;
-                which = (w >> 30) & 1;
-                uint4 b = S.u[jj];
-                V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
-                                    : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-                const float* mv = T.mv + which * MV_STRIDE;
-                float4 ob = S.o[jj];
-                float2 qb = S.q[jj];
-                mlp_features(mv_vec(mv, 0), mv_vec(mv, 3), mk<float>(ob.x, ob.y, ob.z), mk<float>(ob.w, qb.x, qb.y), r,
-                             T.xd[e], mc.lsc, mc.rcut, mc.pdis, mc.pswitch, f);
-            } else {
-                for (int q = 0; q < MLP_NF; q++) f[q] = 0.0f;
+        stage = 0;
+        if (!(rep <= 0.0 || exp(-rep / temper) > u1)) return false;
+        double ds = 0.0;
+        if constexpr (DA == DA_NET) {
+            stage = 1;
+            double so = 0.0, sn = 0.0;
+            for (int t0 = 0; t0 < nx; t0 += MLP_TILE) {
+                int e = t0 + (lane & (MLP_TILE - 1));
+                float f[MLP_NF];
+                int which = 0;
+                if (e < nx) {
+                    int w = T.xl[e];
+                    int jj = w & 0x3fffffff;
+                    which = (w >> 30) & 1;
+                    uint4 b = S.u[jj];
+                    V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
+                                        : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+                    const float* mv = T.mv + which * MV_STRIDE;
+                    float4 ob = S.o[jj];
+                    float2 qb = S.q[jj];
+                    mlp_features(mv_vec(mv, 0), mv_vec(mv, 3), mk<float>(ob.x, ob.y, ob.z),
+                                 mk<float>(ob.w, qb.x, qb.y), r, T.xd[e], mc.lsc, mc.rcut, mc.pdis, mc.pswitch, f);
+                } else {
+                    for (int q = 0; q < MLP_NF; q++) f[q] = 0.0f;
+                }
+                float out = mlp_tile(mr, ml, mlp_slots(f, lane >> 4), lane);
+                if (lane < MLP_TILE && e < nx) {
+                    if (which) sn += (double)out; else so += (double)out;
+                }
             }
-            float out = mlp_tile(mr, ml, mlp_slots(f, lane >> 5), lane);
-            if (lane < 32 && e < nx) {
-                if (which) sn += (double)out; else so += (double)out;
-            }
+            ds = wave_sum(sn) - wave_sum(so);
+            if (!(ds <= 0.0 || exp(-ds / temper) > u2)) return false;
         }
-        des = rep + wave_sum(sn) - wave_sum(so);
-        bool acc1 = des <= 0.0 || exp(-des / temper) > u1;
-        if (!acc1) return false;
+        des = rep + ds;
+        stage = 2;
     }
     // phase C
     int items = nx * ppmax;
@@ -713,7 +735,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     return true;
 }
 
-template <bool DA>
+template <int DA>
 __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
                                                      uint4* __restrict__ P, float4* __restrict__ O,
                                                      float2* __restrict__ Q, const int* __restrict__ ID,
@@ -738,12 +760,12 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
     T.nl = S.id + cap;
     T.xl = T.nl + cap;
     T.xd = (float*)(T.xl + 2 * cap);
-    T.mv = DA ? T.xd + 2 * cap : (float*)(T.xl + 2 * cap);
+    T.mv = DA == DA_NET ? T.xd + 2 * cap : (float*)(T.xl + 2 * cap);
     int* scratch = (int*)(T.mv + 2 * MV_STRIDE);
     int lane = lane_id();
     float* ml = nullptr;
     MlpRegs mr;
-    if (DA) {
+    if constexpr (DA == DA_NET) {
         ml = (float*)(((uintptr_t)(scratch + 2 * 125 + 2) + 15) & ~(uintptr_t)15);
         mlp_setup(mlp_img, ml, lane, mr);
     }
@@ -767,7 +789,8 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
         int fl = (int)floorf(mean);
         ntrial = fl + ((u01f(rd.x) < mean - (float)fl) ? 1 : 0);
     }
-    unsigned long long n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0;
+    unsigned long long n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0,
+                       n_s2try = 0, n_s2acc = 0;
     double dsum = 0.0;
     for (int t = 0; t < ntrial; t++) {
         ctr.x = (uint32_t)t;
@@ -812,23 +835,37 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
         }
         double eo, en, des;
         ctr.w = (mp.pass_hi << 2) | 2u;
-        U4 rc = philox(ctr, mp.key0, mp.key1);
-        bool stage1;
+        double u1 = 0.0, u2 = 0.0;   // the screens' uniforms (drawn before the trial only with DA)
+        if constexpr (DA != DA_OFF) {
+            U4 rs = philox(ctr, mp.key0, mp.key1);
+            u1 = u01d(rs.z, rs.w);
+            U4 c2 = ctr;
+            c2.x |= 0x80000000u;
+            U4 rt = philox(c2, mp.key0, mp.key1);
+            u2 = u01d(rt.x, rt.y);
+        }
+        bool pass;
+        int stage;
         {
             DPart<float> po = load_part<float>(oo, qo, uo.w, tp);
             DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
-            stage1 = trial_energies<DA>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, mp.temper,
-                                        u01d(rc.z, rc.w), eo, en, des);
+            pass = trial_energies<DA>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, mp.temper, u1, u2, eo,
+                                      en, des, stage);
         }
-        if (DA) {
+        if constexpr (DA != DA_OFF) {
             n_s1try++;
-            if (!stage1) { __syncthreads(); continue; }
-            n_s1acc++;
+            if (stage >= 1) n_s1acc++;
+            if constexpr (DA == DA_NET) {
+                if (stage >= 1) n_s2try++;
+                if (stage >= 2) n_s2acc++;
+            }
+            if (!pass) { __syncthreads(); continue; }
         }
+        U4 rc = philox(ctr, mp.key0, mp.key1);
         double de = en - eo;
         // MoveCreator::moveTry (movecreator.h:175-187); with the surrogate the
         // second stage accepts with min(1, exp(-(dE - dE_s)/T)) (Christen & Fox 2005)
-        double de2 = DA ? de - des : de;
+        double de2 = de - des;
         bool acc = de2 <= 0.0 || exp(-de2 / mp.temper) > u01d(rc.x, rc.y);
         if (acc) {
             if (translate) n_tacc++; else n_racc++;
@@ -848,15 +885,16 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
         O[gs + e] = S.o[S.h0 + e];
         Q[gs + e] = S.q[S.h0 + e];
     }
-    if (lane == 0) {
-        atomicAdd(&stats[0], n_tacc);
-        atomicAdd(&stats[1], n_ttry);
-        atomicAdd(&stats[2], n_racc);
-        atomicAdd(&stats[3], n_rtry);
-        atomicAdd(&stats[4], n_left);
-        atomicAdd(&stats[5], n_s1acc);
-        atomicAdd(&stats[6], n_s1try);
-        atomicAdd(de_sum, dsum);
+    // counters: one atomic per lane into one of STAT_REP replicas (a cache
+    // line each), so a thousand waves ending together do not queue on one line
+    int rep = (int)(blockIdx.x & (STAT_REP - 1));
+    if (lane < (DA == DA_OFF ? 5 : DA == DA_REP ? 7 : 9)) {
+        unsigned long long v = lane == 0 ? n_tacc : lane == 1 ? n_ttry : lane == 2 ? n_racc : lane == 3 ? n_rtry
+                             : lane == 4 ? n_left : lane == 5 ? n_s1acc : lane == 6 ? n_s1try : lane == 7 ? n_s2acc
+                             : n_s2try;
+        atomicAdd(&stats[rep * STAT_STRIDE + lane], v);
+    } else if (lane == 9) {
+        atomicAdd(&de_sum[rep * STAT_STRIDE], dsum);
     }
 }
 
@@ -1016,42 +1054,42 @@ __global__ void k_adam(int n, float* __restrict__ w, const float* __restrict__ g
 // FP32 master weights -> packed BF16 MFMA image (operand order of mlp_tile)
 __global__ void k_mlp_pack(const float* __restrict__ W, unsigned char* __restrict__ img) {
     int t = threadIdx.x;   // blockDim.x a multiple of 64: one thread per (operand, lane)
-    int lane = t & 63, m = lane & 31, kg = lane >> 5;
-    for (int q = t >> 6; q < 18; q += blockDim.x >> 6) {
+    int lane = t & 63, m = lane & 15, kg = lane >> 4;
+    for (int q = t >> 6; q < 20; q += blockDim.x >> 6) {
         bf16x8 v;
-        if (q < 2) {   // A1 block q (first-layer bias in K slot 12)
+        if (q < 4) {   // A1 block q (first-layer bias in K slot 12)
             for (int j = 0; j < 8; j++) {
-                int k = 8 * kg + j, u = 32 * q + m;
+                int k = 8 * kg + j, u = 16 * q + m;
                 float w = k < MLP_NF ? W[MLP_OFF_W1 + k * MLP_NH + u] : (k == MLP_NF ? W[MLP_OFF_B1 + u] : 0.0f);
                 v[j] = (__bf16)w;
             }
             ((bf16x8*)(img + MLP_A1_OFF))[q * 64 + lane] = v;
         } else {
-            int qq = q - 2, layer = qq >> 3, blk = (qq >> 2) & 1, ks = qq & 3;
+            int qq = q - 4, layer = qq >> 3, blk = (qq >> 1) & 3, ks = qq & 1;
             int off = layer == 0 ? MLP_OFF_W2 : MLP_OFF_W3;
-            for (int j = 0; j < 8; j++) v[j] = (__bf16)W[off + mlp_kperm(ks, kg, j) * MLP_NH + 32 * blk + m];
-            ((bf16x8*)(img + (layer == 0 ? MLP_A2_OFF : MLP_A3_OFF)))[(blk * 4 + ks) * 64 + lane] = v;
+            for (int j = 0; j < 8; j++) v[j] = (__bf16)W[off + mlp_kperm(ks, kg, j) * MLP_NH + 16 * blk + m];
+            ((bf16x8*)(img + (layer == 0 ? MLP_A2_OFF : MLP_A3_OFF)))[(blk * 2 + ks) * 64 + lane] = v;
         }
     }
     float* C = (float*)(img + MLP_C_OFF);
     for (int i = t; i < 3 * 64; i += blockDim.x) {
         int layer = i / 64, rest = i - 64 * layer;
-        int g = rest / 32, blk = (rest / 16) & 1, r = rest & 15;
+        int g = rest / 16, blk = (rest / 4) & 3, r = rest & 3;
         int u = mlp_unit(g, blk, r);
         C[i] = layer == 0 ? W[MLP_OFF_B2 + u] : (layer == 1 ? W[MLP_OFF_B3 + u] : W[MLP_OFF_W4 + u]);
     }
     if (t == 0) { C[192] = W[MLP_OFF_B4]; C[193] = C[194] = C[195] = 0.0f; }
 }
 
-// batched inference: one wave per tile of 32 feature rows; bf16 != 0 uses the
+// batched inference: one wave per tile of 16 feature rows; bf16 != 0 uses the
 // MFMA tile (the sweep's path), otherwise the FP32 per-sample forward
 __global__ __launch_bounds__(WAVE) void k_mlp_infer(int m, const float* __restrict__ X, const unsigned char* __restrict__ img,
                                                     const float* __restrict__ W, int bf16, float* __restrict__ out) {
     extern __shared__ __align__(16) unsigned char smem[];
     int lane = lane_id();
-    int row = blockIdx.x * 32 + (lane & 31);
+    int row = blockIdx.x * MLP_TILE + (lane & (MLP_TILE - 1));
     if (!bf16) {
-        if (lane < 32 && row < m) out[row] = mlp_forward_f32(W, X + (long)row * MLP_NF);
+        if (lane < MLP_TILE && row < m) out[row] = mlp_forward_f32(W, X + (long)row * MLP_NF);
         return;
     }
     MlpRegs R;
@@ -1060,8 +1098,8 @@ __global__ __launch_bounds__(WAVE) void k_mlp_infer(int m, const float* __restri
     float f[MLP_NF];
     for (int q = 0; q < MLP_NF; q++) f[q] = row < m ? X[(long)row * MLP_NF + q] : 0.0f;
     __syncthreads();
-    float o = mlp_tile(R, cst, mlp_slots(f, lane >> 5), lane);
-    if (lane < 32 && row < m) out[row] = o;
+    float o = mlp_tile(R, cst, mlp_slots(f, lane >> 4), lane);
+    if (lane < MLP_TILE && row < m) out[row] = o;
 }
 
 // plain streaming copy: the calibration kernel for HBM byte counters
@@ -1146,7 +1184,7 @@ struct sc_ctx {
     float* bx = nullptr;                // batch features / targets / out / dout
     int act_batch = 0;
     double* d_loss = nullptr;
-    bool use_surrogate = false;
+    int da_mode = DA_OFF;              // delayed-acceptance mode of sc_sweep
 };
 
 // Every HIP call of a library function goes through HIPCK, which records the
@@ -1454,8 +1492,8 @@ static int sc_create_impl(sc_ctx** out, int device) {
     if (hipMalloc((void**)&ctx->d_flag, sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&ctx->d_maxocc, sizeof(int)) != hipSuccess ||
         hipMalloc((void**)&ctx->d_nonempty, sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_stats, 8 * sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_dsum, sizeof(double)) != hipSuccess) {
+        hipMalloc((void**)&ctx->d_stats, STAT_REP * STAT_STRIDE * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_dsum, STAT_REP * STAT_STRIDE * sizeof(double)) != hipSuccess) {
         delete ctx;
         return SC_EHIP;
     }
@@ -2007,8 +2045,8 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
     if (!ctx || nsweeps < 0 || !(temper > 0) || r < 1 || r > 2) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     HIPCK(hipSetDevice(ctx->device));
-    HIPCK(hipMemsetAsync(ctx->d_stats, 0, 8 * sizeof(unsigned long long), ctx->stream));
-    HIPCK(hipMemsetAsync(ctx->d_dsum, 0, sizeof(double), ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->d_stats, 0, STAT_REP * STAT_STRIDE * sizeof(unsigned long long), ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->d_dsum, 0, STAT_REP * STAT_STRIDE * sizeof(double), ctx->stream));
     int p = r + 1;
     long launches = 0;
     HIPCK(hipEventRecord(ctx->ev0, ctx->stream));
@@ -2020,9 +2058,10 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         if (rc) return rc;
         const Grid& g = ctx->g;
         int cap = stage_cap(ctx, g, ctx->box);
-        bool da = ctx->use_surrogate;
+        int mode = ctx->da_mode;
+        bool da = mode == DA_NET;
         if (da && (ctx->nt != 1 || ctx->dense2ref[0] != ctx->mlp_type))
-            return fail(ctx, SC_EINVAL, "delayed acceptance needs a one-type system of the network's type");
+            return fail(ctx, SC_EINVAL, "the network screen needs a one-type system of the network's type");
         // LDS: stage + near/expensive lists + segment distances + moving particle
         // + stencil scratch (+ the network and its activation tile)
         size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + (da ? MLP_LDS_BYTES + 16 : 0);
@@ -2071,13 +2110,18 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
                 }
                 HIPCK(hipEventRecord(ctx->kev[2 * launches], ctx->stream));
             }
-            if (da)
-                hipLaunchKernelGGL(k_sweep_pass<true>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp, ctx->starts,
-                                   ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,
-                                   ctx->ppmax, ctx->d_nonempty, ctx->mlp_img, ctx->mlp_c, ctx->d_stats, ctx->d_dsum,
+            if (mode == DA_NET)
+                hipLaunchKernelGGL(k_sweep_pass<DA_NET>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
+                                   ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
+                                   cap, ctx->ppmax, ctx->d_nonempty, ctx->mlp_img, ctx->mlp_c, ctx->d_stats,
+                                   ctx->d_dsum, ctx->d_flag);
+            else if (mode == DA_REP)
+                hipLaunchKernelGGL(k_sweep_pass<DA_REP>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
+                                   ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
+                                   cap, ctx->ppmax, ctx->d_nonempty, nullptr, ctx->mlp_c, ctx->d_stats, ctx->d_dsum,
                                    ctx->d_flag);
             else
-                hipLaunchKernelGGL(k_sweep_pass<false>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
+                hipLaunchKernelGGL(k_sweep_pass<DA_OFF>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
                                    ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
                                    cap, ctx->ppmax, ctx->d_nonempty, nullptr, ctx->mlp_c, ctx->d_stats, ctx->d_dsum,
                                    ctx->d_flag);
@@ -2087,12 +2131,20 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         HIPCK(hipGetLastError());
     }
     HIPCK(hipEventRecord(ctx->ev1, ctx->stream));
-    unsigned long long hs[8];
+    unsigned long long hs[STAT_STRIDE];
     double ds = 0;
-    HIPCK(hipMemcpyAsync(hs, ctx->d_stats, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCK(hipMemcpyAsync(&ds, ctx->d_dsum, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<unsigned long long> hrep(STAT_REP * STAT_STRIDE);
+    std::vector<double> drep(STAT_REP * STAT_STRIDE);
+    HIPCK(hipMemcpyAsync(hrep.data(), ctx->d_stats, hrep.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                         ctx->stream));
+    HIPCK(hipMemcpyAsync(drep.data(), ctx->d_dsum, drep.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     int rc = check_flag(ctx, "sweep");
     if (rc) return rc;
+    for (int j = 0; j < STAT_STRIDE; j++) {
+        hs[j] = 0;
+        for (int q = 0; q < STAT_REP; q++) hs[j] += hrep[q * STAT_STRIDE + j];
+    }
+    for (int q = 0; q < STAT_REP; q++) ds += drep[q * STAT_STRIDE];
     float ms = 0;
     HIPCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_ms = ms;
@@ -2112,6 +2164,8 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         stats->left_cell = hs[4];
         stats->stage1_acc = hs[5];
         stats->stage1_try = hs[6];
+        stats->stage2_acc = hs[7];
+        stats->stage2_try = hs[8];
         stats->energy_change = ds;
     }
     return SC_OK;
@@ -2472,7 +2526,7 @@ static int sc_mlp_train_impl(sc_ctx* ctx, int epochs, int batch, double lr, doub
             HIPCK(hipMalloc((void**)&dx, xv.size() * sizeof(float)));
             HIPCK(hipMalloc((void**)&dout, nval * sizeof(float)));
             HIPCK(hipMemcpy(dx, xv.data(), xv.size() * sizeof(float), hipMemcpyHostToDevice));
-            SC_LAUNCH(k_mlp_infer, dim3((unsigned)((nval + 31) / 32)), dim3(WAVE), MLP_LDS_BYTES, ctx->stream,
+            SC_LAUNCH(k_mlp_infer, dim3((unsigned)((nval + MLP_TILE - 1) / MLP_TILE)), dim3(WAVE), MLP_LDS_BYTES, ctx->stream,
                                (int)nval, dx, ctx->mlp_img, ctx->mlp_w, 1, dout);
             hipError_t e = hipGetLastError();
             if (e == hipSuccess) e = hipMemcpy(ov.data(), dout, nval * sizeof(float), hipMemcpyDeviceToHost);
@@ -2502,7 +2556,7 @@ static int sc_mlp_forward_impl(sc_ctx* ctx, int m, const float* x, float* out, i
     if (hipMalloc((void**)&dout, (size_t)m * sizeof(float)) != hipSuccess) { hipFree(dx); return fail(ctx, SC_EHIP, "alloc"); }
     hipError_t e = hipMemcpy(dx, x, (size_t)m * MLP_NF * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        SC_LAUNCH(k_mlp_infer, dim3((m + 31) / 32), dim3(WAVE), bf16 ? MLP_LDS_BYTES : 0, ctx->stream, m, dx,
+        SC_LAUNCH(k_mlp_infer, dim3((m + MLP_TILE - 1) / MLP_TILE), dim3(WAVE), bf16 ? MLP_LDS_BYTES : 0, ctx->stream, m, dx,
                            ctx->mlp_img, ctx->mlp_w, bf16, dout);
         e = hipGetLastError();
     }
@@ -2574,20 +2628,20 @@ extern "C" int sc_mlp_pair_features(sc_ctx* ctx, int m, const int32_t* i, const 
     SC_GUARD(ctx, sc_mlp_pair_features_impl(ctx, m, i, j, feats, eattr))
 }
 
-static int sc_set_surrogate_impl(sc_ctx* ctx, int on) {
-    if (!ctx) return SC_EINVAL;
-    if (on) {
+static int sc_set_surrogate_impl(sc_ctx* ctx, int mode) {
+    if (!ctx || mode < DA_OFF || mode > DA_REP) return SC_EINVAL;
+    if (mode == DA_NET) {
         if (!ctx->mlp_has) return fail(ctx, SC_ESTATE, "no surrogate network");
         if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
         if (ctx->nt != 1 || ctx->dense2ref[0] != ctx->mlp_type)
-            return fail(ctx, SC_EINVAL, "delayed acceptance needs a one-type system of the network's type");
+            return fail(ctx, SC_EINVAL, "the network screen needs a one-type system of the network's type");
     }
-    ctx->use_surrogate = on != 0;
+    ctx->da_mode = mode;
     return SC_OK;
 }
 
-extern "C" int sc_set_surrogate(sc_ctx* ctx, int on) {
-    SC_GUARD(ctx, sc_set_surrogate_impl(ctx, on))
+extern "C" int sc_set_surrogate(sc_ctx* ctx, int mode) {
+    SC_GUARD(ctx, sc_set_surrogate_impl(ctx, mode))
 }
 
 extern "C" {

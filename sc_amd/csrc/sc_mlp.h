@@ -94,47 +94,48 @@ SC_HD float mlp_forward_f32(const float* W, const float* x) {
 
 // ---------------------------------------------------------------- MFMA tile
 // Inference runs the transposed product H^T = W^T X^T on
-// v_mfma_f32_32x32x16_bf16 (D[i][j] = sum_k A[i][k] B[k][j], 32x32 output,
-// K = 16).  Operand layout: lane l holds A[l%32][8*(l/32)+0..7] and
-// B[8*(l/32)+0..7][l%32]; output register r of lane l is
-// D[8*(r/4) + 4*(l/32) + r%4][l%32].
-// With A = a 32-unit block of W^T and B = X^T (32 samples), lane l ends up
-// holding sample l%32 and hidden units u(kg, blk, r) = 32*blk + 8*(r/4) +
-// 4*kg + r%4 (kg = l/32).  Those are exactly the 8-value K slices the next
+// v_mfma_f32_16x16x32_bf16 (D[i][j] = sum_k A[i][k] B[k][j], 16x16 output,
+// K = 32), one tile of 16 samples per wave.  Operand layout: lane l holds
+// A[l%16][8*(l/16)+0..7] and B[8*(l/16)+0..7][l%16]; output register r of
+// lane l is D[4*(l/16) + r][l%16].
+// With A = a 16-unit block of W^T and B = X^T, lane l (group kg = l/16) ends
+// up holding sample l%16 and hidden units u(kg, blk, r) = 16*blk + 4*kg + r
+// of the four blocks.  Those are exactly the 8-value K slices the next
 // layer's B operand needs, if that layer's K index is permuted as
-//   k(ks, kg, j) = 32*(ks/2) + 8*(2*(ks%2) + j/4) + 4*kg + j%4
-// (K step ks = 0..3, slot j = 0..7): registers c_blk[0..7] and c_blk[8..15]
-// become the B operands of K steps 2*blk and 2*blk+1.  So activations never
-// leave registers; the weights are packed in that permuted order once, after
-// training.  The first layer's bias rides in feature slot 12 (= 1.0), the
-// other biases initialise the accumulators; the output layer is a per-lane
-// dot over the lane's 32 units plus one cross-half exchange.
+//   k(ks, kg, j) = 16*(2*ks + j/4) + 4*kg + j%4
+// (K step ks = 0..1, slot j = 0..7): the registers of blocks 2ks and 2ks+1
+// become the B operand of K step ks.  So activations never leave registers;
+// the weights are packed in that permuted order once, after training.  The
+// first layer's K is the 12 features, the bias (slot 12 = 1.0) and zeros;
+// the other biases initialise the accumulators; the output layer is a
+// per-lane dot over the lane's 16 units plus two cross-group exchanges.
 //
 // Packed image (global memory), all per-lane slices contiguous:
-//   A1 [blk 2][lane 64] bf16x8        W1ext[8*kg+j][32*blk+l%32]
-//   A2 [blk 2][ks 4][lane 64] bf16x8  W2[k(ks,kg,j)][32*blk+l%32]
+//   A1 [blk 4][lane 64] bf16x8        W1ext[8*kg+j][16*blk+l%16]
+//   A2 [blk 4][ks 2][lane 64] bf16x8  W2[k(ks,kg,j)][16*blk+l%16]
 //   A3 likewise for W3
-//   C  [3][kg 2][blk 2][16] float     b2[u], b3[u], w4[u] with u = u(kg,blk,r)
+//   C  [3][kg 4][blk 4][4] float      b2[u], b3[u], w4[u] with u = u(kg,blk,r)
 //   b4, 3 floats of padding
+constexpr int MLP_TILE = 16;   // samples per tile
 constexpr int MLP_A1_OFF = 0;
-constexpr int MLP_A2_OFF = MLP_A1_OFF + 2 * 64 * 16;
+constexpr int MLP_A2_OFF = MLP_A1_OFF + 4 * 64 * 16;
 constexpr int MLP_A3_OFF = MLP_A2_OFF + 8 * 64 * 16;
 constexpr int MLP_C_OFF = MLP_A3_OFF + 8 * 64 * 16;
 constexpr int MLP_C_BYTES = 4 * (3 * 64 + 4);
-constexpr int MLP_IMG_BYTES = MLP_C_OFF + MLP_C_BYTES;   // 19216
+constexpr int MLP_IMG_BYTES = MLP_C_OFF + MLP_C_BYTES;   // 21264
 constexpr int MLP_LDS_BYTES = MLP_C_BYTES;                // per wave: the constants
 static_assert(MLP_C_OFF % 16 == 0 && MLP_C_BYTES % 16 == 0, "16-byte copies");
 
-SC_HD int mlp_unit(int kg, int blk, int r) { return 32 * blk + 8 * (r >> 2) + 4 * kg + (r & 3); }
-SC_HD int mlp_kperm(int ks, int kg, int j) { return 32 * (ks >> 1) + 8 * (2 * (ks & 1) + (j >> 2)) + 4 * kg + (j & 3); }
+SC_HD int mlp_unit(int kg, int blk, int r) { return 16 * blk + 4 * kg + r; }
+SC_HD int mlp_kperm(int ks, int kg, int j) { return 16 * (2 * ks + (j >> 2)) + 4 * kg + (j & 3); }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // the A operands of one lane
 struct MlpRegs {
-    bf16x8 a1[2];
-    bf16x8 a2[8];   // [blk*4 + ks]
+    bf16x8 a1[4];   // [blk]
+    bf16x8 a2[8];   // [blk*2 + ks]
     bf16x8 a3[8];
 };
 
@@ -145,7 +146,7 @@ __device__ __forceinline__ void mlp_setup(const unsigned char* __restrict__ img,
     const bf16x8* a2 = (const bf16x8*)(img + MLP_A2_OFF);
     const bf16x8* a3 = (const bf16x8*)(img + MLP_A3_OFF);
 #pragma unroll
-    for (int b = 0; b < 2; b++) R.a1[b] = a1[b * 64 + lane];
+    for (int b = 0; b < 4; b++) R.a1[b] = a1[b * 64 + lane];
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         R.a2[q] = a2[q * 64 + lane];
@@ -158,53 +159,48 @@ __device__ __forceinline__ float silu_fast(float z) {
     return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
 }
 
-__device__ __forceinline__ bf16x8 mlp_pack8(const floatx16& c, int half) {
+__device__ __forceinline__ bf16x8 mlp_pack8(const floatx4& c0, const floatx4& c1) {
     bf16x8 v;
 #pragma unroll
-    for (int j = 0; j < 8; j++) v[j] = (__bf16)silu_fast(c[8 * half + j]);
+    for (int j = 0; j < 4; j++) {
+        v[j] = (__bf16)silu_fast(c0[j]);
+        v[4 + j] = (__bf16)silu_fast(c1[j]);
+    }
     return v;
 }
 
-__device__ __forceinline__ floatx16 mlp_load_c(const float* p) {
-    floatx16 c;
-    const float4* q = (const float4*)p;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        float4 v = q[i];
-        c[4 * i] = v.x; c[4 * i + 1] = v.y; c[4 * i + 2] = v.z; c[4 * i + 3] = v.w;
-    }
-    return c;
-}
-
-// Network output for sample l%32 of a 32-sample tile (returned on lanes l and
-// l+32).  x: this lane's feature slots 8*kg..8*kg+7 of its sample, BF16
-// (slot 12 = 1.0, slots 13-15 = 0).  cst: the constants in LDS.  Samples
-// are independent: a sample's output does not depend on its tile mates.
+// Network output for sample l%16 of a 16-sample tile (returned on lanes l,
+// l+16, l+32, l+48).  x: this lane's feature slots 8*kg..8*kg+7 of its
+// sample, BF16 (slot 12 = 1.0, the rest of 12..31 zero).  cst: the constants
+// in LDS.  Samples are independent: a sample's output does not depend on
+// its tile mates.
 __device__ __forceinline__ float mlp_tile(const MlpRegs& R, const float* cst, bf16x8 x, int lane) {
-    int kg = lane >> 5;
-    floatx16 z = {};
-    floatx16 c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(R.a1[0], x, z, 0, 0, 0);
-    floatx16 c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(R.a1[1], x, z, 0, 0, 0);
+    int kg = lane >> 4;
+    floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+    floatx4 c[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a1[b], x, z, 0, 0, 0);
 #pragma unroll
     for (int layer = 0; layer < 2; layer++) {
-        bf16x8 h0 = mlp_pack8(c0, 0), h1 = mlp_pack8(c0, 1), h2 = mlp_pack8(c1, 0), h3 = mlp_pack8(c1, 1);
+        bf16x8 h0 = mlp_pack8(c[0], c[1]), h1 = mlp_pack8(c[2], c[3]);
         const bf16x8* a = layer == 0 ? R.a2 : R.a3;
-        const float* cb = cst + layer * 64 + kg * 32;
-        c0 = mlp_load_c(cb);
-        c1 = mlp_load_c(cb + 16);
-        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], h0, c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[4], h0, c1, 0, 0, 0);
-        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], h1, c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[5], h1, c1, 0, 0, 0);
-        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2], h2, c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[6], h2, c1, 0, 0, 0);
-        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[3], h3, c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[7], h3, c1, 0, 0, 0);
+        const floatx4* cb = (const floatx4*)(cst + layer * 64 + kg * 16);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            c[b] = cb[b];
+            c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2 * b], h0, c[b], 0, 0, 0);
+            c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2 * b + 1], h1, c[b], 0, 0, 0);
+        }
     }
-    const float* w4 = cst + 128 + kg * 32;
+    const floatx4* w4 = (const floatx4*)(cst + 128 + kg * 16);
     float part = 0.0f;
 #pragma unroll
-    for (int r = 0; r < 16; r++) part += silu_fast(c0[r]) * w4[r] + silu_fast(c1[r]) * w4[16 + r];
+    for (int b = 0; b < 4; b++) {
+        floatx4 w = w4[b];
+#pragma unroll
+        for (int r = 0; r < 4; r++) part += silu_fast(c[b][r]) * w[r];
+    }
+    part += __shfl_xor(part, 16);
     part += __shfl_xor(part, 32);
     return part + cst[192];
 }
@@ -212,14 +208,15 @@ __device__ __forceinline__ float mlp_tile(const MlpRegs& R, const float* cst, bf
 // BF16 feature slice of this lane from the 12 features of its sample
 __device__ __forceinline__ bf16x8 mlp_slots(const float* f, int kg) {
     bf16x8 x;
+#pragma unroll
+    for (int j = 0; j < 8; j++) x[j] = (__bf16)0.0f;
     if (kg == 0) {
 #pragma unroll
         for (int j = 0; j < 8; j++) x[j] = (__bf16)f[j];
-    } else {
+    } else if (kg == 1) {
 #pragma unroll
         for (int j = 0; j < 4; j++) x[j] = (__bf16)f[8 + j];
         x[4] = (__bf16)1.0f;
-        x[5] = x[6] = x[7] = (__bf16)0.0f;
     }
     return x;
 }
