@@ -98,8 +98,9 @@ int sc_get_fixed_positions(sc_ctx* ctx, uint32_t* u);
 int sc_build_cells(sc_ctx* ctx, int r, const uint32_t shift[3], int32_t* cell_of, int32_t* cell_start,
                    int32_t* sorted, int32_t ncell_out[3]);
 /* Neighbour pairs (i<j) with minimum-image centre distance <= rc (FP64 from
- * the fixed-point positions); the GPU counterpart of genSimplePairList's
- * distance test (scOOP/mc/updater.cpp:484-552).  Pairs in row-major order. */
+ * the fixed-point positions, no FMA contraction); the GPU counterpart of
+ * genSimplePairList's distance test (scOOP/mc/updater.cpp:484-552),
+ * enumerated on the device over a cell list.  Pairs in row-major order. */
 int sc_neighbor_pairs(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* count);
 
 /* --- energies ------------------------------------------------------------- */
@@ -217,6 +218,19 @@ int sc_mlp_pair_features(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j,
  * Boltzmann distribution; sc_sweep_stats.stage1_* / stage2_* count the
  * screens. */
 int sc_set_surrogate(sc_ctx* ctx, int mode);
+
+/* --- observables ------------------------------------------------------------ */
+/* Cluster analysis of one-particle molecules (ClusterSampler::sameCluster /
+ * genClusterList, clust.cpp:51-163): particles i, j share a cluster when
+ * p2p(i, j) <= threshold (the reference uses -0.10); labels[i] = the smallest
+ * index of i's cluster (NULL allowed), nclusters = their number.  FP64 pair
+ * energies over the device pair list within maxcut, connected components on
+ * the device. */
+int sc_cluster_labels(sc_ctx* ctx, double threshold, int32_t* labels, int* nclusters);
+/* Nematic order of the particle axes: S2 = -2 x the middle eigenvalue of
+ * Q = (3<u u> - I)/2 (Tests/sc35.c:8563-8603); director = the eigenvector of
+ * the largest eigenvalue (may be NULL).  Q is reduced on the device in FP64. */
+int sc_nematic_order(sc_ctx* ctx, double* s2, double director[3]);
 
 /* --- synthetic inputs ----------------------------------------------------- */
 /* Seeded overlap-free configuration by random sequential addition (host,

@@ -272,6 +272,54 @@ def nematic_s2(dirs):
     return lib().orc_nematic_s2(len(d), _dp(d))
 
 
+def cluster_labels(topo, st, threshold=-0.10):
+    """Cluster labels of one-particle molecules (ClusterSampler::sameCluster,
+    clust.cpp:51-99: pair energy <= threshold; genClusterList,
+    clust.cpp:105-163: each particle ends labelled with the smallest index of
+    its component).  Pairs within maxcut (beyond it the energy is 0), FP64
+    oracle energies, union by smallest root."""
+    u = np.round(np.mod(st.pos, 1.0) * 4294967296.0).astype(np.uint64) % 4294967296
+    pr = neighbor_pairs(u.astype(np.uint32), st.box, topo.maxcut)
+    e = pair_energies(topo, st, pr[:, 0], pr[:, 1]) if len(pr) else np.zeros(0)
+    parent = np.arange(st.n)
+
+    def root(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for (a, b), ee in zip(pr, e):
+        if ee <= threshold:
+            ra, rb = root(a), root(b)
+            if ra != rb:
+                parent[max(ra, rb)] = min(ra, rb)
+    return np.array([root(a) for a in range(st.n)], np.int32)
+
+
+def cluster_labels_reference_loop(energy_of_pair, n, threshold=-0.10):
+    """Literal restatement of genClusterList's relabelling (clust.cpp:105-163)
+    over all pairs i < j, for small n: restart the sweep after every change."""
+    lab = list(range(n))
+    change = True
+    while change:
+        change = False
+        for i in range(n):
+            for j in range(i, n):
+                if i != j and energy_of_pair(i, j) <= threshold:
+                    if lab[i] < lab[j]:
+                        lab[j] = lab[i]
+                        change = True
+                        break
+                    if lab[j] < lab[i]:
+                        lab[i] = lab[j]
+                        change = True
+                        break
+            if change:
+                break
+    return np.array(lab, np.int32)
+
+
 # ------------------------------------------------------------------ reference harness
 
 OPTIONS_TEMPLATE = """ptype = 1

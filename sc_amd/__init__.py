@@ -87,6 +87,8 @@ SIGNATURES = {
     "sc_mlp_forward": (ctypes.c_int, [_P, ctypes.c_int, _F, _F, ctypes.c_int]),
     "sc_mlp_pair_features": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _F, _F]),
     "sc_set_surrogate": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_cluster_labels": (ctypes.c_int, [_P, ctypes.c_double, _I, ctypes.POINTER(ctypes.c_int)]),
+    "sc_nematic_order": (ctypes.c_int, [_P, _D, _D]),
 }
 
 MLP_NPARAM = 9217   # W1[12][64] b1 W2[64][64] b2 W3[64][64] b3 W4[64] b4
@@ -397,6 +399,21 @@ class GpuSystem:
         network screen; 2: delayed acceptance screening only the exact cheap
         terms."""
         self._ck(self._L.sc_set_surrogate(self._h, int(mode)), "set_surrogate")
+
+    # ---- observables
+    def cluster_labels(self, threshold=-0.10):
+        """(labels, number of clusters); labels[i] = smallest index of i's cluster"""
+        labels = np.zeros(self.n, np.int32)
+        nc = ctypes.c_int()
+        self._ck(self._L.sc_cluster_labels(self._h, float(threshold), _i(labels), ctypes.byref(nc)), "cluster_labels")
+        return labels, nc.value
+
+    def nematic_order(self):
+        """(S2, director)"""
+        s2 = ctypes.c_double()
+        d = np.zeros(3)
+        self._ck(self._L.sc_nematic_order(self._h, ctypes.byref(s2), _d(d)), "nematic_order")
+        return s2.value, d
 
     def set_trial_mode(self, mode):
         self._ck(self._L.sc_set_trial_mode(self._h, int(mode)), "set_trial_mode")

@@ -1102,6 +1102,110 @@ __global__ __launch_bounds__(WAVE) void k_mlp_infer(int m, const float* __restri
     if (lane < MLP_TILE && row < m) out[row] = o;
 }
 
+// ------------------------------------------------------- pairs and clusters
+// Squared minimum-image distance in FP64 from fixed-point fractions, with the
+// CPU brute force's operation order and no contraction into FMA, so the
+// neighbour test is bit-identical to it.
+__device__ __forceinline__ double fx_r2_fp64(uint4 a, uint4 b, double bx, double by, double bz) {
+#pragma clang fp contract(off)
+    const double s = 2.3283064365386962890625e-10;   // 2^-32
+    double dx = bx * ((double)(int32_t)(a.x - b.x) * s);
+    double dy = by * ((double)(int32_t)(a.y - b.y) * s);
+    double dz = bz * ((double)(int32_t)(a.z - b.z) * s);
+    double r2 = dx * dx;
+    r2 = r2 + dy * dy;
+    r2 = r2 + dz * dz;
+    return r2;
+}
+
+// neighbour cells of a cell along one axis without repeats (grids of 1 or 2
+// cells wrap onto themselves)
+__device__ __forceinline__ int axis_offsets(int nc, int* off) {
+    if (nc == 1) { off[0] = 0; return 1; }
+    if (nc == 2) { off[0] = 0; off[1] = 1; return 2; }
+    off[0] = -1; off[1] = 0; off[2] = 1;
+    return 3;
+}
+
+// Pairs (ID_i < ID_j) within rc over a cell list with cells >= rc wide: one
+// thread per particle (sorted position), counting (write == nullptr) or
+// writing at its offset.  Within a particle the pairs come in cell-list order.
+__global__ void k_pair_list(int n, const uint4* __restrict__ P, const int* __restrict__ ID, const int* __restrict__ starts,
+                            Grid g, double rc2, double bx, double by, double bz, int* __restrict__ count,
+                            const int* __restrict__ offset, int* __restrict__ pi, int* __restrict__ pj) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    uint4 ua = P[a];
+    int ida = ID[a];
+    int cx = axis_cell(ua.x, g.sh[0], g.nc[0]), cy = axis_cell(ua.y, g.sh[1], g.nc[1]), cz = axis_cell(ua.z, g.sh[2], g.nc[2]);
+    int ox[3], oy[3], oz[3];
+    int nx = axis_offsets(g.nc[0], ox), ny = axis_offsets(g.nc[1], oy), nz = axis_offsets(g.nc[2], oz);
+    int k = 0, base = offset ? offset[a] : 0;
+    for (int iz = 0; iz < nz; iz++)
+        for (int iy = 0; iy < ny; iy++)
+            for (int ix = 0; ix < nx; ix++) {
+                int c = ((cx + ox[ix] + g.nc[0]) % g.nc[0]) +
+                        g.nc[0] * (((cy + oy[iy] + g.nc[1]) % g.nc[1]) + g.nc[1] * ((cz + oz[iz] + g.nc[2]) % g.nc[2]));
+                for (int b = starts[c]; b < starts[c + 1]; b++) {
+                    int idb = ID[b];
+                    if (idb <= ida) continue;
+                    if (fx_r2_fp64(ua, P[b], bx, by, bz) <= rc2) {
+                        if (offset) {
+                            pi[base + k] = ida;
+                            pj[base + k] = idb;
+                        }
+                        k++;
+                    }
+                }
+            }
+    if (!offset) count[a] = k;
+}
+
+// Connected components of the graph {pairs with energy <= thr}: hook the
+// larger root under the smaller, then halve paths; repeated until no label
+// changes, each particle ends labelled with the smallest index of its
+// component (the fixed point ClusterSampler::genClusterList reaches,
+// clust.cpp:105-163).
+__global__ void k_cluster_hook(long m, const int* __restrict__ pi, const int* __restrict__ pj, const double* __restrict__ e,
+                               double thr, int* __restrict__ label, int* __restrict__ changed) {
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= m || !(e[k] <= thr)) return;
+    int a = label[pi[k]], b = label[pj[k]];
+    if (a == b) return;
+    int hi = a > b ? a : b, lo = a < b ? a : b;
+    atomicMin(&label[hi], lo);
+    *changed = 1;
+}
+__global__ void k_cluster_jump(int n, int* __restrict__ label, int* __restrict__ changed) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int l = label[i], ll = label[l];
+    if (ll != l) {
+        label[i] = ll;
+        *changed = 1;
+    }
+}
+__global__ void k_iota(int n, int* __restrict__ v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+
+// Q = sum over particles of u u^T (6 unique components, FP64), per block
+__global__ void k_nematic_q(int n, const float4* __restrict__ O, double* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double q[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        float4 o = O[i];
+        double x = o.x, y = o.y, z = o.z, inv = 1.0 / sqrt(x * x + y * y + z * z);
+        x *= inv; y *= inv; z *= inv;
+        q[0] = x * x; q[1] = y * y; q[2] = z * z; q[3] = x * y; q[4] = x * z; q[5] = y * z;
+    }
+    for (int c = 0; c < 6; c++) {
+        double v = wave_sum(q[c]);
+        if (lane_id() == 0) atomicAdd(&out[c], v);
+    }
+}
+
 // plain streaming copy: the calibration kernel for HBM byte counters
 __global__ void k_copy(long n16, const uint4* __restrict__ a, uint4* __restrict__ b) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n16; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
@@ -1185,6 +1289,17 @@ struct sc_ctx {
     int act_batch = 0;
     double* d_loss = nullptr;
     int da_mode = DA_OFF;              // delayed-acceptance mode of sc_sweep
+    // pair lists, clusters, order parameter
+    int* d_pcount = nullptr;            // per sorted particle: pair count, then offsets (n + 1)
+    int pcount_cap = 0;
+    int* d_pi = nullptr;
+    int* d_pj = nullptr;
+    double* d_pe = nullptr;
+    long pair_cap = 0;
+    int* d_label = nullptr;
+    int label_cap = 0;
+    double* d_q6 = nullptr;
+    int* d_changed = nullptr;
 };
 
 // Every HIP call of a library function goes through HIPCK, which records the
@@ -1333,15 +1448,23 @@ static SysConst sysconst(const sc_ctx* ctx, const double* box) {
 }
 
 // --- cell list --------------------------------------------------------------
+// Cell list with cells at least wmin wide; with period p > 1 the cell count
+// per axis is a multiple of p (the checkerboard's colour period) or 1.
+static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box, double wmin, int p);
+
 static int build_grid(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box) {
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     if (r < 1 || r > 2) return fail(ctx, SC_EINVAL, "stencil radius must be 1 or 2");
-    int p = r + 1;
-    double wmin = ctx->maxcut / r;
+    return build_grid_w(ctx, r, shift, box, ctx->maxcut / r, r + 1);
+}
+
+static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box, double wmin, int p) {
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     Grid g;
     g.r = r;
     for (int k = 0; k < 3; k++) {
         int nc = (int)floor(box[k] / wmin);
+        if (nc > 1024) nc = 1024;
         if (nc >= p) nc = (nc / p) * p;   // a multiple of the colour period
         else nc = 1;                      // too small for a checkerboard: one cell
         if (nc < 1) nc = 1;
@@ -1516,7 +1639,8 @@ static void sc_destroy_impl(sc_ctx* ctx) {
                     ctx->starts, ctx->counts, ctx->fill, ctx->partial, ctx->d_out, ctx->d_flag, ctx->d_maxocc,
                     ctx->d_nonempty, ctx->d_stats, ctx->d_dsum, ctx->d_q, ctx->d_i, ctx->d_j, ctx->mlp_w,
                     ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, ctx->mlp_img, ctx->tr_x, ctx->tr_y, ctx->tr_e, ctx->act,
-                    ctx->bx, ctx->d_loss};
+                    ctx->bx, ctx->d_loss, ctx->d_pcount, ctx->d_pi, ctx->d_pj, ctx->d_pe, ctx->d_label, ctx->d_q6,
+                    ctx->d_changed};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
@@ -1768,36 +1892,188 @@ int sc_build_cells(sc_ctx* ctx, int r, const uint32_t shift[3], int32_t* cell_of
     SC_GUARD(ctx, sc_build_cells_impl(ctx, r, shift, cell_of, cell_start, sorted, ncell_out))
 }
 
-static int sc_neighbor_pairs_impl(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* count) {
-    if (!ctx || !count) return SC_EINVAL;
-    // Host-side enumeration over the device cell list would need a device
-    // pair-compaction kernel; the list is built from the fixed-point state in
-    // FP64 with the same arithmetic as the CPU brute force.
+// All pairs within rc on the device: cell list with cells >= rc, count pass,
+// scan, write pass.  Leaves the pairs (IDs, i < j) in d_pi / d_pj.
+static int device_pairs(sc_ctx* ctx, double rc, long* total) {
     int n = ctx->n;
-    std::vector<uint32_t> u(3 * (size_t)n);
-    int r = sc_get_fixed_positions(ctx, u.data());
-    if (r) return r;
-    const double s32 = 1.0 / 4294967296.0, rc2 = rc * rc;
-    long cnt = 0;
-    for (int i = 0; i < n; i++)
-        for (int j = i + 1; j < n; j++) {
-            double dx = ctx->box[0] * ((double)(int32_t)(u[3 * i] - u[3 * j]) * s32);
-            double dy = ctx->box[1] * ((double)(int32_t)(u[3 * i + 1] - u[3 * j + 1]) * s32);
-            double dz = ctx->box[2] * ((double)(int32_t)(u[3 * i + 2] - u[3 * j + 2]) * s32);
-            double r2 = dx * dx;
-            r2 = r2 + dy * dy;
-            r2 = r2 + dz * dz;
-            if (r2 <= rc2) {
-                if (pairs && cnt < cap) { pairs[2 * cnt] = i; pairs[2 * cnt + 1] = j; }
-                cnt++;
+    *total = 0;
+    if (n < 2) return SC_OK;
+    HIPCK(hipSetDevice(ctx->device));
+    double vol = ctx->box[0] * ctx->box[1] * ctx->box[2];
+    double w = std::max(rc, cbrt(2.0 * vol / n));   // cells >= rc holding a few particles
+    uint32_t zero[3] = {0, 0, 0};
+    int rc_ = build_grid_w(ctx, 1, zero, ctx->box, w, 1);
+    if (rc_) return rc_;
+    if (n + 1 > ctx->pcount_cap) {
+        if ((rc_ = dalloc(ctx, &ctx->d_pcount, (size_t)n + 1)) || (rc_ = dalloc(ctx, &ctx->d_label, (size_t)n + 1)))
+            return rc_;
+        ctx->pcount_cap = n + 1;
+    }
+    int c = ctx->cur, nb = (n + 255) / 256;
+    const Grid& g = ctx->g;
+    double rc2 = rc * rc;
+    // counts into d_label (scratch), offsets into d_pcount
+    SC_LAUNCH(k_pair_list, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->P[c], ctx->ID[c], ctx->starts, g, rc2, ctx->box[0],
+              ctx->box[1], ctx->box[2], ctx->d_label, (const int*)nullptr, (int*)nullptr, (int*)nullptr);
+    SC_LAUNCH(k_scan, dim3(1), dim3(1024), 0, ctx->stream, n, ctx->d_label, ctx->d_pcount, ctx->d_maxocc,
+              ctx->d_nonempty);
+    int tot = 0;
+    HIPCK(hipMemcpyAsync(&tot, ctx->d_pcount + n, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (tot < 0) return fail(ctx, SC_ECAPACITY, "more than 2^31 pairs");
+    if (tot > ctx->pair_cap) {
+        if ((rc_ = dalloc(ctx, &ctx->d_pi, (size_t)tot)) || (rc_ = dalloc(ctx, &ctx->d_pj, (size_t)tot)) ||
+            (rc_ = dalloc(ctx, &ctx->d_pe, (size_t)tot)))
+            return rc_;
+        ctx->pair_cap = tot;
+    }
+    if (tot > 0)
+        SC_LAUNCH(k_pair_list, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->P[c], ctx->ID[c], ctx->starts, g, rc2,
+                  ctx->box[0], ctx->box[1], ctx->box[2], (int*)nullptr, (const int*)ctx->d_pcount, ctx->d_pi, ctx->d_pj);
+    HIPCK(hipGetLastError());
+    *total = tot;
+    return SC_OK;
+}
+
+static int sc_neighbor_pairs_impl(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* count) {
+    if (!ctx || !count || !(rc > 0)) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    long tot = 0;
+    int rc_ = device_pairs(ctx, rc, &tot);
+    if (rc_) return rc_;
+    *count = tot;
+    if (!pairs || tot == 0) return SC_OK;
+    // returned in lexicographic (i, j) order
+    std::vector<int32_t> hi(tot), hj(tot);
+    HIPCK(hipMemcpyAsync(hi.data(), ctx->d_pi, tot * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipMemcpyAsync(hj.data(), ctx->d_pj, tot * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    std::vector<uint64_t> key(tot);
+    for (long k = 0; k < tot; k++) key[k] = ((uint64_t)(uint32_t)hi[k] << 32) | (uint32_t)hj[k];
+    std::sort(key.begin(), key.end());
+    long m = std::min(cap, tot);
+    for (long k = 0; k < m; k++) {
+        pairs[2 * k] = (int32_t)(key[k] >> 32);
+        pairs[2 * k + 1] = (int32_t)(key[k] & 0xffffffffu);
+    }
+    return SC_OK;
+}
+
+// ClusterSampler::sameCluster (clust.cpp:51-99) for one-particle molecules:
+// two particles share a cluster when their pair energy is <= threshold
+// (-0.10 in the reference); labels as ClusterSampler::genClusterList
+// (clust.cpp:105-163) leaves them: the smallest index of the component.
+// Pair energies in FP64 (k_pairs<double>) over all pairs within maxcut.
+static int sc_cluster_labels_impl(sc_ctx* ctx, double threshold, int32_t* labels, int* nclusters) {
+    if (!ctx || !nclusters) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    int n = ctx->n;
+    long m = 0;
+    int rc_ = device_pairs(ctx, ctx->maxcut, &m);
+    if (rc_) return rc_;
+    if (n + 1 > ctx->pcount_cap) {
+        if ((rc_ = dalloc(ctx, &ctx->d_pcount, (size_t)n + 1)) || (rc_ = dalloc(ctx, &ctx->d_label, (size_t)n + 1)))
+            return rc_;
+        ctx->pcount_cap = n + 1;
+    }
+    if (!ctx->d_changed && (rc_ = dalloc(ctx, &ctx->d_changed, 1))) return rc_;
+    SysConst sc = sysconst(ctx, ctx->box);
+    int c = ctx->cur;
+    if (m > 0)
+        SC_LAUNCH(k_pairs<double>, dim3((unsigned)((m + 127) / 128)), dim3(128), 0, ctx->stream, (int)m, ctx->d_pi,
+                  ctx->d_pj, ctx->slot, sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp64, ctx->d_tp64, ctx->sqmaxcut,
+                  ctx->d_pe);
+    SC_LAUNCH(k_iota, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_label);
+    for (int it = 0; it < 4 * n + 64; it++) {
+        int changed = 0;
+        HIPCK(hipMemsetAsync(ctx->d_changed, 0, sizeof(int), ctx->stream));
+        if (m > 0)
+            SC_LAUNCH(k_cluster_hook, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, m, ctx->d_pi,
+                      ctx->d_pj, ctx->d_pe, threshold, ctx->d_label, ctx->d_changed);
+        SC_LAUNCH(k_cluster_jump, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_label, ctx->d_changed);
+        HIPCK(hipMemcpyAsync(&changed, ctx->d_changed, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCK(hipStreamSynchronize(ctx->stream));
+        if (!changed) break;
+    }
+    std::vector<int32_t> lab(n);
+    HIPCK(hipMemcpy(lab.data(), ctx->d_label, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int nc = 0;
+    for (int i = 0; i < n; i++) nc += lab[i] == i ? 1 : 0;
+    *nclusters = nc;
+    if (labels) memcpy(labels, lab.data(), n * sizeof(int32_t));
+    return SC_OK;
+}
+
+// nematic order parameter of the particle axes: Q = (3 <u u> - I) / 2 and
+// S2 = -2 x its middle eigenvalue (the convention of Tests/sc35.c:8563-8603);
+// the director is the eigenvector of the largest eigenvalue
+static void jacobi3(double a[3][3], double ev[3], double vec[3][3]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) vec[i][j] = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 64; sweep++) {
+        double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        if (off < 1e-30) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                if (fabs(a[p][q]) < 1e-300) continue;
+                double th = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < 3; k++) {
+                    double kp = a[k][p], kq = a[k][q];
+                    a[k][p] = cs * kp - sn * kq;
+                    a[k][q] = sn * kp + cs * kq;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double pk = a[p][k], qk = a[q][k];
+                    a[p][k] = cs * pk - sn * qk;
+                    a[q][k] = sn * pk + cs * qk;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double kp = vec[k][p], kq = vec[k][q];
+                    vec[k][p] = cs * kp - sn * kq;
+                    vec[k][q] = sn * kp + cs * kq;
+                }
             }
-        }
-    *count = cnt;
+    }
+    for (int i = 0; i < 3; i++) ev[i] = a[i][i];
+}
+
+static int sc_nematic_order_impl(sc_ctx* ctx, double* s2, double director[3]) {
+    if (!ctx || !s2) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    int n = ctx->n, rc_;
+    if (n <= 0) { *s2 = 0.0; return SC_OK; }
+    HIPCK(hipSetDevice(ctx->device));
+    if (!ctx->d_q6 && (rc_ = dalloc(ctx, &ctx->d_q6, 6))) return rc_;
+    HIPCK(hipMemsetAsync(ctx->d_q6, 0, 6 * sizeof(double), ctx->stream));
+    SC_LAUNCH(k_nematic_q, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->O[ctx->cur], ctx->d_q6);
+    HIPCK(hipGetLastError());
+    double q6[6];
+    HIPCK(hipMemcpyAsync(q6, ctx->d_q6, sizeof(q6), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    double q[3][3] = {{q6[0], q6[3], q6[4]}, {q6[3], q6[1], q6[5]}, {q6[4], q6[5], q6[2]}}, ev[3], vec[3][3];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) q[a][b] = (q[a][b] * 3.0 / n - (a == b ? 1.0 : 0.0)) / 2.0;
+    jacobi3(q, ev, vec);
+    int order[3] = {0, 1, 2};
+    std::sort(order, order + 3, [&](int x, int y) { return ev[x] < ev[y]; });
+    *s2 = -2.0 * ev[order[1]];
+    if (director)
+        for (int k = 0; k < 3; k++) director[k] = vec[k][order[2]];
     return SC_OK;
 }
 
 int sc_neighbor_pairs(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* count) {
     SC_GUARD(ctx, sc_neighbor_pairs_impl(ctx, rc, pairs, cap, count))
+}
+
+int sc_cluster_labels(sc_ctx* ctx, double threshold, int32_t* labels, int* nclusters) {
+    SC_GUARD(ctx, sc_cluster_labels_impl(ctx, threshold, labels, nclusters))
+}
+
+int sc_nematic_order(sc_ctx* ctx, double* s2, double director[3]) {
+    SC_GUARD(ctx, sc_nematic_order_impl(ctx, s2, director))
 }
 
 static int sc_pair_energies_impl(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64) {

@@ -21,8 +21,9 @@ def golden_dir():
 @pytest.fixture(scope="session")
 def ensemble_system():
     """500 PSC at eta 0.2, T*=1 and the CPU oracle's sequential Metropolis
-    reference: mean E/N over 40 blocks of 25 sweeps after 200 sweeps.
-    Returns (workload, (pos, dir, pd0) of the start, E/N per block, rot_half)."""
+    reference: E/N, nematic S2 and mean cluster size at the end of each of 40
+    blocks of 25 sweeps after 200 sweeps.  Returns (workload, (pos, dir, pd0)
+    of the start, [40 x 3] observables, rot_half)."""
     import numpy as np
     import sc_amd
     from sc_amd import workloads as W
@@ -41,5 +42,7 @@ def ensemble_system():
     cur = stc
     for blk in range(40):
         cur, _, _, ef = po.mc_run(tp, cur, 25, 1.0, 0.3, rot_half, seed=100 + blk)
-        ec.append(ef / st.n)
+        s2 = po.nematic_s2(cur.dir)
+        ncl = len(np.unique(po.cluster_labels(tp, cur)))
+        ec.append((ef / st.n, s2, st.n / ncl))
     return w, (pos, d, q), np.array(ec), rot_half

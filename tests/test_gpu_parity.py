@@ -247,11 +247,38 @@ def _blocked(x, nb=8):
     return b.mean(), b.std(ddof=1) / np.sqrt(nb)
 
 
+def ensemble_check(g, n, ec, rot_half, label):
+    """Run 200 + 40 x 25 sweeps on g and compare E/N, S2 and mean cluster size
+    block averages with the oracle's: equal within 3 combined standard errors
+    + 1% of the oracle's mean.  S2 of this isotropic fluid is O(N^-1/2) ~ 0.02
+    and decorrelates over ~400 sweeps (10 degree rotations), so the block
+    errors understate its spread and 1% of ~0 means nothing: its margin is
+    0.03 absolute instead.  Returns the counters summed over the blocks."""
+    g.sweep(200, 1.0, 0.3, rot_half, r=2, seed=5, counter=0)
+    obs = []
+    tot = {}
+    for blk in range(40):
+        s = g.sweep(25, 1.0, 0.3, rot_half, r=2, seed=5, counter=1000 + 25 * blk)
+        for k, v in s.items():
+            tot[k] = tot.get(k, 0) + v
+        _, ncl = g.cluster_labels()
+        obs.append((g.allToAll() / n, g.nematic_order()[0], n / ncl))
+    obs = np.array(obs)
+    for c, name in enumerate(("E/N", "S2", "mean cluster size")):
+        mg, sg = _blocked(obs[:, c])
+        mc, scc = _blocked(ec[:, c])
+        say("%s %s: gpu %.5f +- %.5f, cpu %.5f +- %.5f" % (label, name, mg, sg, mc, scc))
+        margin = 0.03 if name == "S2" else 0.01 * abs(mc)
+        assert abs(mg - mc) <= 3.0 * np.hypot(sg, scc) + margin, name
+    return tot
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_sweep_ensemble_matches_cpu_oracle(dev, ensemble_system, mode):
-    """Mean potential energy per particle of 500 PSC at eta 0.2, T*=1, from
-    the checkerboard GPU sweep (trial mode 0 and 1) and from the sequential
-    CPU oracle Metropolis: equal within 3 combined standard errors."""
+    """E/N, nematic S2 and mean cluster size of 500 PSC at eta 0.2, T*=1,
+    from the checkerboard GPU sweep (trial mode 0 and 1) and from the
+    sequential CPU oracle Metropolis: equal within 3 combined standard errors
+    (+1%)."""
     say("case: ensemble GPU vs CPU oracle, trial mode", mode)
     w, (pos, d, q), ec, rot_half = ensemble_system
     g = sc_amd.GpuSystem(dev)
@@ -259,16 +286,35 @@ def test_sweep_ensemble_matches_cpu_oracle(dev, ensemble_system, mode):
     g.set_config(w["box"], pos, d, q, w["types"])
     g.set_trial_mode(mode)
     n = len(w["types"])
-    g.sweep(200, 1.0, 0.3, rot_half, r=2, seed=5, counter=0)
-    eg = []
-    tries = 0
-    for blk in range(40):
-        s = g.sweep(25, 1.0, 0.3, rot_half, r=2, seed=5, counter=1000 + 25 * blk)
-        tries += s["trans_try"] + s["rot_try"]
-        eg.append(g.allToAll() / n)
-    eg = np.array(eg)
-    mg, sg = _blocked(eg)
-    mc, scc = _blocked(ec)
-    say("E/N gpu %.5f +- %.5f, cpu %.5f +- %.5f, trials/sweep/N %.4f" % (mg, sg, mc, scc, tries / (1000.0 * n)))
-    assert abs(mg - mc) <= 3.0 * np.hypot(sg, scc) + 0.01 * abs(mc)
+    tot = ensemble_check(g, n, ec, rot_half, "mode %d" % mode)
+    tries = tot["trans_try"] + tot["rot_try"]
+    say("trials/sweep/N %.4f" % (tries / (1000.0 * n)))
     assert abs(tries / (1000.0 * n) - 1.0) < 0.02
+
+
+def test_cluster_labels_and_nematic_order_match_oracle(dev):
+    """Device cluster labels (FP64 pair energies <= -0.10, connected
+    components) equal the oracle's, and S2 matches, on 1500 PSC after 60
+    sweeps at T*=0.6 (clusters of several particles)."""
+    say("case: cluster labels + nematic order vs oracle")
+    w = W.workload("c1_psc_1000", n_override=1500)
+    g = sc_amd.GpuSystem(dev)
+    box = w["box"] * (0.2 / 0.3) ** (1.0 / 3.0)
+    g.load_topology(w["top"])
+    pos, d, q = g.generate_rsa(w["types"], box, seed=8)
+    g.set_config(box, pos, d, q, w["types"])
+    g.sweep(60, 0.6, 0.3, 20.0 / 180 * np.pi / 4, r=2, seed=8, counter=0)
+    lab, ncl = g.cluster_labels()
+    b, p, dd, qq = g.get_config()
+    tp, st = po.Topo(w["top"]), po.State(b, p, dd, qq, w["types"])
+    ref = po.cluster_labels(tp, st)
+    sizes = np.bincount(np.unique(ref, return_counts=True)[1])
+    say("clusters: gpu %d, oracle %d; size histogram %s" % (ncl, len(np.unique(ref)), sizes.tolist()))
+    assert len(sizes) > 3
+    np.testing.assert_array_equal(lab, ref)
+    assert ncl == len(np.unique(ref))
+    s2, director = g.nematic_order()
+    s2r = po.nematic_s2(dd)
+    say("S2 gpu %.10f oracle %.10f" % (s2, s2r))
+    assert abs(s2 - s2r) <= 1e-6   # directions are FP32 on the device
+    assert abs(np.linalg.norm(director) - 1.0) < 1e-9

@@ -214,3 +214,35 @@ def test_nematic_order_limits():
     v = rng.normal(size=(200000, 3))
     v /= np.linalg.norm(v, axis=1)[:, None]
     assert abs(po.nematic_s2(v)) < 0.02
+
+
+def test_cluster_labels_match_literal_reference_loop():
+    """The oracle's union-find cluster labels equal a literal restatement of
+    genClusterList's restart-on-change relabelling over all pairs
+    (clust.cpp:105-163), on a dense attractive system after some Metropolis."""
+    from sc_amd import workloads as W
+    w = W.workload("c1_psc_1000", n_override=108)
+    tp = po.Topo(w["top"])
+    # parallel rods on a 6 x 6 x 3 lattice, 1.15 sigma apart side by side,
+    # patches random in the plane, then Metropolis at T = 0.5
+    box = np.array([6.9, 6.9, 12.6])
+    g = np.stack(np.meshgrid(np.arange(6), np.arange(6), np.arange(3), indexing="ij"), -1).reshape(-1, 3)
+    pos = (g + 0.5) / np.array([6, 6, 3])
+    rng = np.random.default_rng(3)
+    d = np.tile([0.0, 0.0, 1.0], (108, 1))
+    q = rng.normal(size=(108, 3))
+    q[:, 2] = 0.0
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    st = po.State(box, pos, d, q, w["types"])
+    st, _, _, _ = po.mc_run(tp, st, 30, 0.5, 0.3, 0.1, seed=5, energies=False)
+    n = st.n
+    ii, jj = np.triu_indices(n, 1)
+    e = po.pair_energies(tp, st, ii.astype(np.int32), jj.astype(np.int32))
+    emat = np.zeros((n, n))
+    emat[ii, jj] = e
+    emat[jj, ii] = e
+    assert np.count_nonzero(e <= -0.10) >= 10
+    lit = po.cluster_labels_reference_loop(lambda a, b: emat[a, b], n)
+    uf = po.cluster_labels(tp, st)
+    np.testing.assert_array_equal(uf, lit)
+    assert len(set(lit)) < n
