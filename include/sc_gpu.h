@@ -145,7 +145,7 @@ int sc_overlap_count(sc_ctx* ctx, long* count);
  * colours, 2: 3x3x3).  seed/counter key the counter-based RNG; the same
  * (seed, counter) reproduces a run bit for bit.  stats may be NULL. */
 int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot_half, int r, uint64_t seed,
-             uint64_t counter, sc_sweep_stats* stats);
+             uint64_t counter, sc_sweep_stats* stats);   /* r = 0: the autotuned radius (default 2) */
 /* How many trial moves an active cell makes in one checkerboard pass.
  * mode 0 (default): as many as the cell holds particles, so every particle is
  * chosen once per sweep on average as in Updater::simulate's step loop
@@ -218,6 +218,40 @@ int sc_mlp_pair_features(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j,
  * Boltzmann distribution; sc_sweep_stats.stage1_* / stage2_* count the
  * screens. */
 int sc_set_surrogate(sc_ctx* ctx, int mode);
+
+/* --- replica batches (parallel tempering) ------------------------------------ */
+/* The replicas of MoveCreator::replicaExchangeMove (movecreator.cpp:555-760),
+ * which the reference runs one per MPI process, held in one context: load
+ * their particles back to back with sc_set_config (positions as fractions of
+ * each replica's own box), then declare the cut.  Replica r gets its own cell
+ * grid; one checkerboard pass of every replica is one kernel launch.  A new
+ * sc_set_config ends the batch; single-system calls refuse a batch. */
+int sc_batch_set(sc_ctx* ctx, int nrep, const int32_t* n_per_rep, const double* boxes /* nrep x 3 */);
+/* nsweeps sweeps of every replica r at temperature temper[r] (r = 0: the
+ * autotuned stencil radius); stats has nrep entries (may be NULL).  Replica r
+ * draws from seed sc_replica_seed(seed, r), so a replica swept alone with that
+ * seed follows the same trajectory. */
+int sc_batch_sweep(sc_ctx* ctx, int nsweeps, const double* temper, double transmx, double rot_half, int r,
+                   uint64_t seed, uint64_t counter, sc_sweep_stats* stats);
+uint64_t sc_replica_seed(uint64_t seed, int r);
+/* whole-system energy of every replica (FP32 pair terms, FP64 sums) */
+int sc_batch_energies(sc_ctx* ctx, double* e);
+/* One replica-exchange step: replicas ordered by temperature, neighbours
+ * (k, k+1) with k % 2 == parity swap temperatures when
+ * change = (1/T_k - 1/T_k+1)(E_k - E_k+1) > 0 or U < exp(change)
+ * (movecreator.cpp:695-713, canonical ensemble).  temper is updated in place;
+ * accepted[k] (nrep - 1 entries, may be NULL) marks the swapped ladder pairs. */
+int sc_batch_exchange(sc_ctx* ctx, double* temper, const double* energy, int parity, uint64_t seed, uint64_t counter,
+                      int32_t* accepted);
+int sc_batch_get_config(sc_ctx* ctx, int r, double* pos, double* dir, double* pd0);
+/* Autotuner: times `sweeps` sweeps of the loaded system or batch in each of
+ * stencil radius 1 / 2 x trial mode 0 / 1 (state restored after each), keeps
+ * the fastest for later calls with r = 0; rates[4] (may be NULL) receives
+ * sweeps/s in the order (1,0), (1,1), (2,0), (2,1).  temper: one temperature
+ * for a single system, one per replica for a batch.  No counterpart in the
+ * reference. */
+int sc_autotune(sc_ctx* ctx, int sweeps, const double* temper, double transmx, double rot_half, uint64_t seed,
+                double* rates, int* r_best, int* mode_best);
 
 /* --- observables ------------------------------------------------------------ */
 /* Cluster analysis of one-particle molecules (ClusterSampler::sameCluster /

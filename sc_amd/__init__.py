@@ -88,6 +88,15 @@ SIGNATURES = {
     "sc_mlp_pair_features": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _F, _F]),
     "sc_set_surrogate": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_cluster_labels": (ctypes.c_int, [_P, ctypes.c_double, _I, ctypes.POINTER(ctypes.c_int)]),
+    "sc_batch_set": (ctypes.c_int, [_P, ctypes.c_int, _I, _D]),
+    "sc_batch_sweep": (ctypes.c_int, [_P, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(SweepStats)]),
+    "sc_replica_seed": (ctypes.c_uint64, [ctypes.c_uint64, ctypes.c_int]),
+    "sc_batch_energies": (ctypes.c_int, [_P, _D]),
+    "sc_batch_exchange": (ctypes.c_int, [_P, _D, _D, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, _I]),
+    "sc_batch_get_config": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D]),
+    "sc_autotune": (ctypes.c_int, [_P, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, _D,
+                                   ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     "sc_nematic_order": (ctypes.c_int, [_P, _D, _D]),
 }
 
@@ -399,6 +408,58 @@ class GpuSystem:
         network screen; 2: delayed acceptance screening only the exact cheap
         terms."""
         self._ck(self._L.sc_set_surrogate(self._h, int(mode)), "set_surrogate")
+
+    # ---- replica batch (parallel tempering)
+    def batch_set(self, n_per_rep, boxes):
+        ns = np.ascontiguousarray(n_per_rep, np.int32)
+        b = np.ascontiguousarray(boxes, np.float64).reshape(-1, 3)
+        if len(b) != len(ns):
+            raise ValueError("one box per replica")
+        self._ck(self._L.sc_batch_set(self._h, len(ns), _i(ns), _d(b)), "batch_set")
+        self.nrep = len(ns)
+        self.rep_n = ns.copy()
+
+    def batch_sweep(self, nsweeps, temper, transmx, rot_half, r=0, seed=1, counter=0):
+        t = np.ascontiguousarray(temper, np.float64)
+        st = (SweepStats * self.nrep)()
+        self._ck(self._L.sc_batch_sweep(self._h, int(nsweeps), _d(t), float(transmx), float(rot_half), int(r),
+                                        int(seed) & 0xFFFFFFFFFFFFFFFF, int(counter) & 0xFFFFFFFFFFFFFFFF, st),
+                 "batch_sweep")
+        return [x.as_dict() for x in st]
+
+    def batch_energies(self):
+        e = np.zeros(self.nrep)
+        self._ck(self._L.sc_batch_energies(self._h, _d(e)), "batch_energies")
+        return e
+
+    def batch_exchange(self, temper, energy, parity, seed=1, counter=0):
+        """returns (new temperatures, accepted flags of the nrep-1 ladder pairs)"""
+        t = np.array(temper, np.float64)
+        e = np.ascontiguousarray(energy, np.float64)
+        acc = np.zeros(max(self.nrep - 1, 1), np.int32)
+        self._ck(self._L.sc_batch_exchange(self._h, _d(t), _d(e), int(parity), int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                           int(counter) & 0xFFFFFFFFFFFFFFFF, _i(acc)), "batch_exchange")
+        return t, acc[: self.nrep - 1]
+
+    def batch_get_config(self, r):
+        m = int(self.rep_n[r])
+        pos, d, q = np.zeros((m, 3)), np.zeros((m, 3)), np.zeros((m, 3))
+        self._ck(self._L.sc_batch_get_config(self._h, int(r), _d(pos), _d(d), _d(q)), "batch_get_config")
+        return pos, d, q
+
+    @staticmethod
+    def replica_seed(seed, r):
+        return lib().sc_replica_seed(int(seed) & 0xFFFFFFFFFFFFFFFF, int(r))
+
+    def autotune(self, sweeps, temper, transmx, rot_half, seed=1):
+        """times the four launch shapes; returns (r, trial mode, rates[4])"""
+        t = np.ascontiguousarray(np.atleast_1d(temper), np.float64)
+        rates = np.zeros(4)
+        rb, mb = ctypes.c_int(), ctypes.c_int()
+        self._ck(self._L.sc_autotune(self._h, int(sweeps), _d(t), float(transmx), float(rot_half),
+                                     int(seed) & 0xFFFFFFFFFFFFFFFF, _d(rates), ctypes.byref(rb), ctypes.byref(mb)),
+                 "autotune")
+        return rb.value, mb.value, rates
 
     # ---- observables
     def cluster_labels(self, threshold=-0.10):

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -45,6 +46,15 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 __device__ __forceinline__ int wave_prefix(unsigned long long mask) {
     return (int)__popcll(mask & ((1ull << lane_id()) - 1ull));
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin by ID (XCD = ID mod 8), each
+// XCD with its own L2.  This bijection of [0, n) gives XCD k a contiguous run
+// of logical indices, so with z-major cell order each XCD sweeps one slab and
+// the overlapping stencils of its cells share its L2.
+__device__ __forceinline__ int xcd_slab_index(int b, int n) {
+    int q = n >> 3, rem = n & 7, k = b & 7;
+    return k * q + min(k, rem) + (b >> 3);
 }
 
 __device__ __forceinline__ int axis_cell(uint32_t u, uint32_t s, int nc) {
@@ -735,17 +745,18 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     return true;
 }
 
+// One active cell of a checkerboard pass (logical index b of the colour's
+// active cells): stage the stencil, run the cell's trial moves, write back.
 template <int DA>
-__global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
-                                                     uint4* __restrict__ P, float4* __restrict__ O,
-                                                     float2* __restrict__ Q, const int* __restrict__ ID,
-                                                     const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
-                                                     int ppmax, const int* __restrict__ nonempty,
-                                                     const unsigned char* __restrict__ mlp_img, MlpConst mc,
-                                                     unsigned long long* __restrict__ stats, double* __restrict__ de_sum,
-                                                     int* __restrict__ flag) {
+__device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst& sc, const MoveParams& mp,
+                                           const int* __restrict__ starts, uint4* __restrict__ P,
+                                           float4* __restrict__ O, float2* __restrict__ Q, const int* __restrict__ ID,
+                                           const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
+                                           int ppmax, const int* __restrict__ nonempty,
+                                           const unsigned char* __restrict__ mlp_img, const MlpConst& mc,
+                                           unsigned long long* __restrict__ stats, double* __restrict__ de_sum,
+                                           int* __restrict__ flag) {
     extern __shared__ __align__(16) unsigned char smem[];
-    int b = blockIdx.x;
     int ax = b % mp.nact[0], ay = (b / mp.nact[0]) % mp.nact[1], az = b / (mp.nact[0] * mp.nact[1]);
     int cx = mp.ox + mp.p * ax, cy = mp.oy + mp.p * ay, cz = mp.oz + mp.p * az;
     int cell = cx + g.nc[0] * (cy + g.nc[1] * cz);
@@ -895,6 +906,99 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
         atomicAdd(&stats[rep * STAT_STRIDE + lane], v);
     } else if (lane == 9) {
         atomicAdd(&de_sum[rep * STAT_STRIDE], dsum);
+    }
+}
+
+template <int DA>
+__global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
+                                                     uint4* __restrict__ P, float4* __restrict__ O,
+                                                     float2* __restrict__ Q, const int* __restrict__ ID,
+                                                     const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
+                                                     int ppmax, const int* __restrict__ nonempty,
+                                                     const unsigned char* __restrict__ mlp_img, MlpConst mc,
+                                                     unsigned long long* __restrict__ stats, double* __restrict__ de_sum,
+                                                     int* __restrict__ flag) {
+    sweep_cell<DA>(xcd_slab_index((int)blockIdx.x, (int)gridDim.x), g, sc, mp, starts, P, O, Q, ID, pp, tp, cap, ppmax,
+                   nonempty, mlp_img, mc, stats, de_sum, flag);
+}
+
+// Per-replica parameters of one checkerboard pass of a replica batch.
+struct RepPass {
+    Grid g;
+    SysConst sc;
+    MoveParams mp;
+    int cell_off;   // the replica's first cell in the batch's cell arrays
+    int blk_off;    // the replica's first block in this pass's launch
+    int rep;        // replica index (counters, non-empty cells)
+};
+
+// One colour pass of every replica of a batch in one launch: the launch's
+// blocks are the replicas' active cells back to back; a block finds its
+// replica by binary search over blk_off (a replica without active cells
+// shares its offset with the next one, and the search takes the last).
+template <int DA>
+__global__ __launch_bounds__(WAVE) void k_sweep_batch(const RepPass* __restrict__ rp, int nrep,
+                                                      const int* __restrict__ starts, uint4* __restrict__ P,
+                                                      float4* __restrict__ O, float2* __restrict__ Q,
+                                                      const int* __restrict__ ID, const PP32* __restrict__ pp,
+                                                      const TP32* __restrict__ tp, int cap, int ppmax,
+                                                      const int* __restrict__ nonempty,
+                                                      const unsigned char* __restrict__ mlp_img, MlpConst mc,
+                                                      unsigned long long* __restrict__ stats,
+                                                      double* __restrict__ de_sum, int* __restrict__ flag) {
+    int lb = xcd_slab_index((int)blockIdx.x, (int)gridDim.x);
+    int lo = 0, hi = nrep - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (rp[mid].blk_off <= lb) lo = mid; else hi = mid - 1;
+    }
+    const RepPass R = rp[lo];
+    sweep_cell<DA>(lb - R.blk_off, R.g, R.sc, R.mp, starts + R.cell_off, P, O, Q, ID, pp, tp, cap, ppmax,
+                   nonempty + R.rep, mlp_img, mc, stats + (size_t)R.rep * STAT_REP * STAT_STRIDE,
+                   de_sum + (size_t)R.rep * STAT_REP * STAT_STRIDE, flag);
+}
+
+// cell list of a replica batch: the cell of every particle in its replica's
+// grid, offset into the batch's cell arrays.  The replica of a particle is
+// found from its index (replicas are index ranges part_off..), by binary
+// search; after the build the replicas' cells, and so their particles, lie
+// contiguous and in replica order.
+struct RepGrid {
+    Grid g;
+    int cell_off;
+    int part_off;
+};
+__global__ void k_cell_count_batch(int n, int nrep, const RepGrid* __restrict__ rg, const uint4* __restrict__ P,
+                                   const int* __restrict__ ID, int* __restrict__ cell_tmp, int* __restrict__ counts) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int id = ID[k];
+    int lo = 0, hi = nrep - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (rg[mid].part_off <= id) lo = mid; else hi = mid - 1;
+    }
+    uint4 u = P[k];
+    int c = rg[lo].cell_off + cell_of_u(rg[lo].g, u.x, u.y, u.z);
+    cell_tmp[k] = c;
+    atomicAdd(&counts[c], 1);
+}
+
+// non-empty cells of each replica (one block per replica)
+__global__ void k_nonempty_batch(const RepGrid* __restrict__ rg, int nrep, int ncell_total,
+                                 const int* __restrict__ starts, int* __restrict__ nonempty) {
+    int r = blockIdx.x;
+    int c0 = rg[r].cell_off, c1 = r + 1 < nrep ? rg[r + 1].cell_off : ncell_total;
+    int cnt = 0;
+    for (int c = c0 + threadIdx.x; c < c1; c += blockDim.x) cnt += starts[c + 1] > starts[c] ? 1 : 0;
+    double v = wave_sum((double)cnt);
+    __shared__ int part[16];
+    if (lane_id() == 0) part[threadIdx.x >> 6] = (int)v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        nonempty[r] = t;
     }
 }
 
@@ -1300,6 +1404,22 @@ struct sc_ctx {
     int label_cap = 0;
     double* d_q6 = nullptr;
     int* d_changed = nullptr;
+    // replica batch (nrep > 0): the loaded particles are the replicas back to back
+    int nrep = 0;
+    std::vector<int> rep_n, rep_off;
+    std::vector<std::array<double, 3>> rep_box;
+    std::vector<RepGrid> rep_grid;
+    std::vector<int> rep_ncell;
+    int rep_ncell_total = 0;
+    RepGrid* d_rg = nullptr;
+    RepPass* d_rp = nullptr;
+    int rp_cap = 0;
+    int* d_rep_nonempty = nullptr;
+    unsigned long long* d_rep_stats = nullptr;
+    double* d_rep_dsum = nullptr;
+    double* d_rep_e = nullptr;
+    int rep_alloc = 0;
+    int tuned_r = 2;                    // stencil radius chosen by the autotuner
 };
 
 // Every HIP call of a library function goes through HIPCK, which records the
@@ -1502,6 +1622,18 @@ static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const doubl
 
 // stage capacity for a stencil: ~3x the mean occupancy (local packing cannot
 // exceed ~2.6x the mean at the fluid densities used), bounded by the system
+static int stage_cap_n(int n, double minvol, const Grid& g, const double* box) {
+    double cellvol = (box[0] / g.nc[0]) * (box[1] / g.nc[1]) * (box[2] / g.nc[2]);
+    int ns = std::min(2 * g.r + 1, g.nc[0]) * std::min(2 * g.r + 1, g.nc[1]) * std::min(2 * g.r + 1, g.nc[2]);
+    double mean = (double)n / (double)(g.nc[0] * g.nc[1] * g.nc[2]) * ns;
+    double pack = 0.74 * cellvol * ns / minvol;   // close packing of the stencil volume
+    int cap = (int)std::min(std::max(3.0 * mean + 64.0, 128.0), pack + 64.0);
+    cap = std::max(cap, 64);
+    cap = std::min(cap, n + 64);
+    cap = std::min(cap, 1152);   // keeps a wave's LDS image under 64 KiB
+    return ((cap + 63) / 64) * 64;
+}
+
 static int stage_cap(const sc_ctx* ctx, const Grid& g, const double* box) {
     double cellvol = (box[0] / g.nc[0]) * (box[1] / g.nc[1]) * (box[2] / g.nc[2]);
     int ns = std::min(2 * g.r + 1, g.nc[0]) * std::min(2 * g.r + 1, g.nc[1]) * std::min(2 * g.r + 1, g.nc[2]);
@@ -1548,6 +1680,7 @@ static int ensure_ij(sc_ctx* ctx, int m) {
 }
 
 static int energy_all_impl(sc_ctx* ctx, const double* box, double* e) {
+    if (ctx->nrep > 0) return fail(ctx, SC_ESTATE, "the context holds a replica batch: use sc_batch_energies");
     int rc = build_grid(ctx, 2, nullptr, box);
     if (rc) return rc;
     const Grid& g = ctx->g;
@@ -1640,7 +1773,8 @@ static void sc_destroy_impl(sc_ctx* ctx) {
                     ctx->d_nonempty, ctx->d_stats, ctx->d_dsum, ctx->d_q, ctx->d_i, ctx->d_j, ctx->mlp_w,
                     ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, ctx->mlp_img, ctx->tr_x, ctx->tr_y, ctx->tr_e, ctx->act,
                     ctx->bx, ctx->d_loss, ctx->d_pcount, ctx->d_pi, ctx->d_pj, ctx->d_pe, ctx->d_label, ctx->d_q6,
-                    ctx->d_changed};
+                    ctx->d_changed, ctx->d_rg, ctx->d_rp, ctx->d_rep_nonempty, ctx->d_rep_stats, ctx->d_rep_dsum,
+                    ctx->d_rep_e};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
@@ -1718,6 +1852,7 @@ static int sc_set_config_impl(sc_ctx* ctx, int n, const double box[3], const dou
     if (ctx->device < 0) return fail(ctx, SC_ENODEV, "host-only context: no device state");
     if (n >= (1 << 24)) return fail(ctx, SC_ECAPACITY, "too many particles");
     HIPCK(hipSetDevice(ctx->device));
+    ctx->nrep = 0;   // a new configuration ends any replica batch
     if (types) ctx->ptypes.assign(types, types + n);
     if ((int)ctx->ptypes.size() != n) return fail(ctx, SC_EINVAL, "particle count differs from the topology's [System]");
     int rc;
@@ -2316,10 +2451,33 @@ static uint64_t mix64(uint64_t z) {
     return z ^ (z >> 31);
 }
 
+// Grid shift and colour order of sweep `sw` under `seed`: the shift is three
+// random 32-bit fractions, the order a random permutation of the p^3 colours.
+struct SweepPlan {
+    uint32_t shift[3];
+    int order[27];
+};
+static void sweep_plan(uint64_t seed, uint64_t sw, int p, SweepPlan& pl) {
+    uint64_t h = mix64(seed ^ mix64(sw * 0x632BE59BD9B4E019ull));
+    pl.shift[0] = (uint32_t)h;
+    pl.shift[1] = (uint32_t)(h >> 32);
+    pl.shift[2] = (uint32_t)mix64(h);
+    int ncol = p * p * p;
+    for (int k = 0; k < ncol; k++) pl.order[k] = k;
+    uint64_t hs = mix64(h ^ 0xA5A5A5A5ull);
+    for (int k = ncol - 1; k > 0; k--) {
+        hs = mix64(hs);
+        int j = (int)(hs % (uint64_t)(k + 1));
+        std::swap(pl.order[k], pl.order[j]);
+    }
+}
+
 static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot_half, int r, uint64_t seed,
              uint64_t counter, sc_sweep_stats* stats) {
-    if (!ctx || nsweeps < 0 || !(temper > 0) || r < 1 || r > 2) return SC_EINVAL;
+    if (!ctx || nsweeps < 0 || !(temper > 0) || r < 0 || r > 2) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (ctx->nrep > 0) return fail(ctx, SC_ESTATE, "the context holds a replica batch: use sc_batch_sweep");
+    if (r == 0) r = ctx->tuned_r;
     HIPCK(hipSetDevice(ctx->device));
     HIPCK(hipMemsetAsync(ctx->d_stats, 0, STAT_REP * STAT_STRIDE * sizeof(unsigned long long), ctx->stream));
     HIPCK(hipMemsetAsync(ctx->d_dsum, 0, STAT_REP * STAT_STRIDE * sizeof(double), ctx->stream));
@@ -2328,8 +2486,9 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
     HIPCK(hipEventRecord(ctx->ev0, ctx->stream));
     for (int s = 0; s < nsweeps; s++) {
         uint64_t sw = counter + (uint64_t)s;
-        uint64_t h = mix64(seed ^ mix64(sw * 0x632BE59BD9B4E019ull));
-        uint32_t shift[3] = {(uint32_t)h, (uint32_t)(h >> 32), (uint32_t)mix64(h)};
+        SweepPlan plan;
+        sweep_plan(seed, sw, p, plan);
+        const uint32_t* shift = plan.shift;
         int rc = build_grid(ctx, r, shift, ctx->box);
         if (rc) return rc;
         const Grid& g = ctx->g;
@@ -2345,16 +2504,8 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         int capmax = (int)((65536 - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
         size_t lds = per * cap + fixed;
-        // colour order: a random permutation of the p^3 colours
         int ncol = p * p * p;
-        int order[27];
-        for (int k = 0; k < ncol; k++) order[k] = k;
-        uint64_t hs = mix64(h ^ 0xA5A5A5A5ull);
-        for (int k = ncol - 1; k > 0; k--) {
-            hs = mix64(hs);
-            int j = (int)(hs % (uint64_t)(k + 1));
-            std::swap(order[k], order[j]);
-        }
+        const int* order = plan.order;
         int pa[3];
         for (int k = 0; k < 3; k++) pa[k] = (g.nc[k] >= p) ? p : 1;
         SysConst sc = sysconst(ctx, ctx->box);
@@ -2452,10 +2603,406 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
     SC_GUARD(ctx, sc_sweep_impl(ctx, nsweeps, temper, transmx, rot_half, r, seed, counter, stats))
 }
 
+// ------------------------------------------------------------ replica batch
+// A batch is the loaded configuration cut into replicas of their own size and
+// box (particles back to back, positions as fractions of their own box): the
+// parallel-tempering replicas of MoveCreator::replicaExchangeMove
+// (movecreator.cpp:555-760), which the reference runs as one MPI process
+// each.  Every replica has its own cell grid; a checkerboard pass of all
+// replicas is one launch.
+
+// RNG key of replica r under a sweep seed (a replica swept on its own with
+// this seed follows the same trajectory)
+static uint64_t replica_seed(uint64_t seed, int r) { return mix64(seed + 0x9E3779B97F4A7C15ull * (uint64_t)(r + 1)); }
+
+static bool batch_active(const sc_ctx* ctx) { return ctx->nrep > 0; }
+
+static int batch_alloc(sc_ctx* ctx) {
+    int R = ctx->nrep, rc;
+    if (R > ctx->rep_alloc) {
+        if ((rc = dalloc(ctx, &ctx->d_rg, R)) || (rc = dalloc(ctx, &ctx->d_rep_nonempty, R)) ||
+            (rc = dalloc(ctx, &ctx->d_rep_stats, (size_t)R * STAT_REP * STAT_STRIDE)) ||
+            (rc = dalloc(ctx, &ctx->d_rep_dsum, (size_t)R * STAT_REP * STAT_STRIDE)) ||
+            (rc = dalloc(ctx, &ctx->d_rep_e, R)))
+            return rc;
+        ctx->rep_alloc = R;
+    }
+    return SC_OK;
+}
+
+// cell lists of every replica: grid of replica r shifted by shifts[r], cells
+// at least maxcut / r wide, a multiple of r + 1 per axis (or one cell)
+static int batch_build_grid(sc_ctx* ctx, int r, const std::vector<std::array<uint32_t, 3>>* shifts) {
+    int R = ctx->nrep, p = r + 1, rc;
+    double wmin = ctx->maxcut / r;
+    ctx->rep_grid.resize(R);
+    ctx->rep_ncell.resize(R);
+    int total = 0;
+    for (int q = 0; q < R; q++) {
+        Grid g;
+        g.r = r;
+        for (int k = 0; k < 3; k++) {
+            int nc = (int)floor(ctx->rep_box[q][k] / wmin);
+            if (nc > 1024) nc = 1024;
+            if (nc >= p) nc = (nc / p) * p;
+            else nc = 1;
+            g.nc[k] = nc;
+            g.sh[k] = shifts ? (*shifts)[q][k] : 0u;
+        }
+        ctx->rep_grid[q].g = g;
+        ctx->rep_grid[q].cell_off = total;
+        ctx->rep_grid[q].part_off = ctx->rep_off[q];
+        ctx->rep_ncell[q] = g.nc[0] * g.nc[1] * g.nc[2];
+        total += ctx->rep_ncell[q];
+    }
+    ctx->rep_ncell_total = total;
+    if (total > ctx->ncell_alloc) {
+        if ((rc = dalloc(ctx, &ctx->starts, total + 1)) || (rc = dalloc(ctx, &ctx->counts, total)) ||
+            (rc = dalloc(ctx, &ctx->fill, total)) || (rc = dalloc(ctx, &ctx->partial, total)))
+            return rc;
+        ctx->ncell_alloc = total;
+    }
+    if ((rc = batch_alloc(ctx))) return rc;
+    int n = ctx->n, c = ctx->cur, nb = (n + 255) / 256;
+    HIPCK(hipMemcpyAsync(ctx->d_rg, ctx->rep_grid.data(), R * sizeof(RepGrid), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->counts, 0, sizeof(int) * total, ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->fill, 0, sizeof(int) * total, ctx->stream));
+    SC_LAUNCH(k_cell_count_batch, dim3(nb), dim3(256), 0, ctx->stream, n, R, ctx->d_rg, ctx->P[c], ctx->ID[c],
+              ctx->cell_tmp, ctx->counts);
+    SC_LAUNCH(k_scan, dim3(1), dim3(1024), 0, ctx->stream, total, ctx->counts, ctx->starts, ctx->d_maxocc,
+              ctx->d_nonempty);
+    SC_LAUNCH(k_cell_scatter, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->cell_tmp, ctx->starts, ctx->fill,
+              ctx->src_of_pos);
+    SC_LAUNCH(k_cell_sort, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, total, ctx->starts, ctx->src_of_pos,
+              ctx->ID[c]);
+    SC_LAUNCH(k_permute, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->src_of_pos, ctx->P[c], ctx->O[c], ctx->Q[c],
+              ctx->ID[c], ctx->P[c ^ 1], ctx->O[c ^ 1], ctx->Q[c ^ 1], ctx->ID[c ^ 1], ctx->slot);
+    SC_LAUNCH(k_nonempty_batch, dim3(R), dim3(256), 0, ctx->stream, ctx->d_rg, R, total, ctx->starts,
+              ctx->d_rep_nonempty);
+    HIPCK(hipGetLastError());
+    ctx->cur ^= 1;
+    ctx->grid_valid = false;   // the single-system grid no longer describes the arrays
+    return SC_OK;
+}
+
+static SysConst sysconst_box(const sc_ctx* ctx, const std::array<double, 3>& b) {
+    double box[3] = {b[0], b[1], b[2]};
+    return sysconst(ctx, box);
+}
+
+static int sc_batch_set_impl(sc_ctx* ctx, int nrep, const int32_t* n_per_rep, const double* boxes) {
+    if (!ctx || nrep < 1 || !n_per_rep || !boxes) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "load the replicas' particles with sc_set_config first");
+    long tot = 0;
+    for (int q = 0; q < nrep; q++) {
+        if (n_per_rep[q] < 1) return fail(ctx, SC_EINVAL, "every replica needs a particle");
+        for (int k = 0; k < 3; k++)
+            if (!(boxes[3 * q + k] > 0)) return fail(ctx, SC_EINVAL, "box edges must be positive");
+        tot += n_per_rep[q];
+    }
+    if (tot != ctx->n) return fail(ctx, SC_EINVAL, "replica sizes do not add up to the loaded particles");
+    ctx->nrep = nrep;
+    ctx->rep_n.assign(n_per_rep, n_per_rep + nrep);
+    ctx->rep_off.resize(nrep);
+    ctx->rep_box.resize(nrep);
+    long off = 0;
+    for (int q = 0; q < nrep; q++) {
+        ctx->rep_off[q] = (int)off;
+        off += n_per_rep[q];
+        for (int k = 0; k < 3; k++) ctx->rep_box[q][k] = boxes[3 * q + k];
+    }
+    HIPCK(hipSetDevice(ctx->device));
+    return batch_build_grid(ctx, ctx->tuned_r, nullptr);
+}
+
+// nsweeps sweeps of every replica at its temperature temper[r]
+static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, double transmx, double rot_half, int r,
+                               uint64_t seed, uint64_t counter, sc_sweep_stats* stats) {
+    if (!ctx || nsweeps < 0 || !temper || r < 0 || r > 2) return SC_EINVAL;
+    if (!batch_active(ctx)) return fail(ctx, SC_ESTATE, "no replica batch (sc_batch_set)");
+    if (r == 0) r = ctx->tuned_r;
+    int R = ctx->nrep;
+    for (int q = 0; q < R; q++)
+        if (!(temper[q] > 0)) return fail(ctx, SC_EINVAL, "temperatures must be positive");
+    int mode = ctx->da_mode;
+    if (mode == DA_NET) return fail(ctx, SC_EINVAL, "the network screen runs on single systems only");
+    HIPCK(hipSetDevice(ctx->device));
+    int rc;
+    if ((rc = batch_alloc(ctx))) return rc;
+    HIPCK(hipMemsetAsync(ctx->d_rep_stats, 0, (size_t)R * STAT_REP * STAT_STRIDE * sizeof(unsigned long long),
+                         ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->d_rep_dsum, 0, (size_t)R * STAT_REP * STAT_STRIDE * sizeof(double), ctx->stream));
+    int p = r + 1, ncol = p * p * p;
+    if ((long)ncol * R > ctx->rp_cap) {
+        if ((rc = dalloc(ctx, &ctx->d_rp, (size_t)ncol * R))) return rc;
+        ctx->rp_cap = ncol * R;
+    }
+    std::vector<RepPass> hp((size_t)ncol * R);
+    std::vector<int> nblk(ncol);
+    std::vector<SweepPlan> plans(R);
+    std::vector<std::array<uint32_t, 3>> shifts(R);
+    long launches = 0;
+    HIPCK(hipEventRecord(ctx->ev0, ctx->stream));
+    for (int sidx = 0; sidx < nsweeps; sidx++) {
+        uint64_t sw = counter + (uint64_t)sidx;
+        for (int q = 0; q < R; q++) {
+            sweep_plan(replica_seed(seed, q), sw, p, plans[q]);
+            for (int k = 0; k < 3; k++) shifts[q][k] = plans[q].shift[k];
+        }
+        if ((rc = batch_build_grid(ctx, r, &shifts))) return rc;
+        int cap = 64;
+        for (int q = 0; q < R; q++) {
+            double b[3] = {ctx->rep_box[q][0], ctx->rep_box[q][1], ctx->rep_box[q][2]};
+            cap = std::max(cap, stage_cap_n(ctx->rep_n[q], ctx->minvol, ctx->rep_grid[q].g, b));
+        }
+        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2);
+        size_t per = stage_bytes(1) + sizeof(int) * 3;
+        int capmax = (int)((65536 - fixed) / per);
+        if (cap > capmax) cap = (capmax / 64) * 64;
+        size_t lds = per * cap + fixed;
+        for (int ci = 0; ci < ncol; ci++) {
+            int boff = 0;
+            for (int q = 0; q < R; q++) {
+                RepPass& P_ = hp[(size_t)ci * R + q];
+                const Grid& g = ctx->rep_grid[q].g;
+                P_.g = g;
+                P_.sc = sysconst_box(ctx, ctx->rep_box[q]);
+                P_.cell_off = ctx->rep_grid[q].cell_off;
+                P_.rep = q;
+                P_.blk_off = boff;
+                MoveParams& mp = P_.mp;
+                int col = plans[q].order[ci];
+                int pa[3];
+                for (int k = 0; k < 3; k++) pa[k] = (g.nc[k] >= p) ? p : 1;
+                mp.ox = col % p; mp.oy = (col / p) % p; mp.oz = col / (p * p);
+                mp.p = p;
+                bool skip = mp.ox >= pa[0] || mp.oy >= pa[1] || mp.oz >= pa[2];   // axis with a single cell
+                for (int k = 0; k < 3; k++) mp.nact[k] = g.nc[k] / pa[k];
+                mp.trial_mode = ctx->trial_mode;
+                mp.n = ctx->rep_n[q];
+                mp.transmx = (float)transmx;
+                mp.rot_half = (float)rot_half;
+                mp.temper = temper[q];
+                uint64_t rs = replica_seed(seed, q);
+                mp.key0 = (uint32_t)rs;
+                mp.key1 = (uint32_t)(rs >> 32);
+                uint64_t pass = sw * 32 + (uint64_t)ci;
+                mp.pass_lo = (uint32_t)pass;
+                mp.pass_hi = (uint32_t)(pass >> 32);
+                if (!skip) boff += mp.nact[0] * mp.nact[1] * mp.nact[2];
+            }
+            nblk[ci] = boff;
+        }
+        HIPCK(hipMemcpyAsync(ctx->d_rp, hp.data(), hp.size() * sizeof(RepPass), hipMemcpyHostToDevice, ctx->stream));
+        int c = ctx->cur;
+        for (int ci = 0; ci < ncol; ci++) {
+            if (nblk[ci] == 0) continue;
+            const RepPass* rp = ctx->d_rp + (size_t)ci * R;
+            if (mode == DA_REP)
+                SC_LAUNCH(k_sweep_batch<DA_REP>, dim3(nblk[ci]), dim3(WAVE), lds, ctx->stream, rp, R, ctx->starts,
+                          ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->ppmax,
+                          ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag);
+            else
+                SC_LAUNCH(k_sweep_batch<DA_OFF>, dim3(nblk[ci]), dim3(WAVE), lds, ctx->stream, rp, R, ctx->starts,
+                          ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->ppmax,
+                          ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag);
+            launches++;
+        }
+        HIPCK(hipGetLastError());
+    }
+    HIPCK(hipEventRecord(ctx->ev1, ctx->stream));
+    std::vector<unsigned long long> hrep((size_t)R * STAT_REP * STAT_STRIDE);
+    std::vector<double> drep((size_t)R * STAT_REP * STAT_STRIDE);
+    HIPCK(hipMemcpyAsync(hrep.data(), ctx->d_rep_stats, hrep.size() * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipMemcpyAsync(drep.data(), ctx->d_rep_dsum, drep.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         ctx->stream));
+    if ((rc = check_flag(ctx, "batch sweep"))) return rc;
+    float ms = 0;
+    HIPCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    ctx->last_launches = launches;
+    ctx->last_kernel_ms = 0;
+    if (stats)
+        for (int q = 0; q < R; q++) {
+            unsigned long long hs[STAT_STRIDE] = {0};
+            double ds = 0;
+            for (int j = 0; j < STAT_STRIDE; j++)
+                for (int t = 0; t < STAT_REP; t++) hs[j] += hrep[((size_t)q * STAT_REP + t) * STAT_STRIDE + j];
+            for (int t = 0; t < STAT_REP; t++) ds += drep[((size_t)q * STAT_REP + t) * STAT_STRIDE];
+            sc_sweep_stats& st = stats[q];
+            st.trans_acc = hs[0]; st.trans_try = hs[1]; st.rot_acc = hs[2]; st.rot_try = hs[3]; st.left_cell = hs[4];
+            st.stage1_acc = hs[5]; st.stage1_try = hs[6]; st.stage2_acc = hs[7]; st.stage2_try = hs[8];
+            st.energy_change = ds;
+        }
+    return SC_OK;
+}
+
+// whole-system energy of every replica (TotalEMatrix::allToAll per replica,
+// totalenergycalculator.h:338-353): one launch per replica over its cells
+static int sc_batch_energies_impl(sc_ctx* ctx, double* e) {
+    if (!ctx || !e) return SC_EINVAL;
+    if (!batch_active(ctx)) return fail(ctx, SC_ESTATE, "no replica batch (sc_batch_set)");
+    HIPCK(hipSetDevice(ctx->device));
+    int R = ctx->nrep, rc;
+    if ((rc = batch_alloc(ctx))) return rc;
+    int c = ctx->cur;
+    for (int q = 0; q < R; q++) {
+        const Grid& g = ctx->rep_grid[q].g;
+        double b[3] = {ctx->rep_box[q][0], ctx->rep_box[q][1], ctx->rep_box[q][2]};
+        int cap = stage_cap_n(ctx->rep_n[q], ctx->minvol, g, b);
+        size_t lds = stage_bytes(cap) + sizeof(int) * (2 * QCAP) + sizeof(int) * (2 * 125 + 2);
+        int co = ctx->rep_grid[q].cell_off, nc = ctx->rep_ncell[q];
+        SysConst sc = sysconst(ctx, b);
+        SC_LAUNCH(k_energy_all, dim3(nc), dim3(WAVE), lds, ctx->stream, g, sc, ctx->starts + co, ctx->P[c], ctx->O[c],
+                  ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->partial + co, ctx->d_flag);
+        SC_LAUNCH(k_reduce, dim3(1), dim3(1024), 0, ctx->stream, nc, ctx->partial + co, ctx->d_rep_e + q);
+    }
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(e, ctx->d_rep_e, R * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    return check_flag(ctx, "batch energies");
+}
+
+// One replica-exchange step (MoveCreator::replicaExchangeMove,
+// movecreator.cpp:555-760): replicas ordered by temperature; neighbours
+// (k, k+1) with k % 2 == parity attempt to swap temperatures, accepted if
+// change = (1/T_k - 1/T_k+1) (E_k - E_k+1) > 0 or U < exp(change) (the
+// canonical criterion of :695-713).  temper (per replica) is updated in
+// place; accepted[k] = 1 for the swapped pairs (by rank k in the ladder).
+static int sc_batch_exchange_impl(sc_ctx* ctx, double* temper, const double* energy, int parity, uint64_t seed,
+                                  uint64_t counter, int32_t* accepted) {
+    if (!ctx || !temper || !energy) return SC_EINVAL;
+    if (!batch_active(ctx)) return fail(ctx, SC_ESTATE, "no replica batch (sc_batch_set)");
+    int R = ctx->nrep;
+    std::vector<int> rank(R);
+    for (int q = 0; q < R; q++) rank[q] = q;
+    std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) { return temper[a] < temper[b]; });
+    uint64_t h = mix64(seed ^ mix64(counter * 0xD6E8FEB86659FD93ull + 3));
+    for (int k = 0; k + 1 < R; k++) {
+        if (accepted) accepted[k] = 0;
+        if ((k & 1) != (parity & 1)) continue;
+        int a = rank[k], b = rank[k + 1];
+        h = mix64(h);
+        double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+        double change = (1.0 / temper[a] - 1.0 / temper[b]) * (energy[a] - energy[b]);
+        if (change > 0 || u < exp(change)) {
+            std::swap(temper[a], temper[b]);
+            if (accepted) accepted[k] = 1;
+        }
+    }
+    return SC_OK;
+}
+
+// the configuration of replica r (positions as fractions of its own box)
+static int sc_batch_get_config_impl(sc_ctx* ctx, int r, double* pos, double* dir, double* pd0) {
+    if (!ctx || !pos || !dir || !pd0) return SC_EINVAL;
+    if (!batch_active(ctx) || r < 0 || r >= ctx->nrep) return fail(ctx, SC_EINVAL, "no such replica");
+    int n = ctx->n;
+    std::vector<double> P(3 * (size_t)n), D(3 * (size_t)n), Qv(3 * (size_t)n);
+    double box[3];
+    int rc = sc_get_config_impl(ctx, box, P.data(), D.data(), Qv.data());
+    if (rc) return rc;
+    size_t o = (size_t)ctx->rep_off[r] * 3, m = (size_t)ctx->rep_n[r] * 3;
+    memcpy(pos, P.data() + o, m * sizeof(double));
+    memcpy(dir, D.data() + o, m * sizeof(double));
+    memcpy(pd0, Qv.data() + o, m * sizeof(double));
+    return SC_OK;
+}
+
+int sc_batch_set(sc_ctx* ctx, int nrep, const int32_t* n_per_rep, const double* boxes) {
+    SC_GUARD(ctx, sc_batch_set_impl(ctx, nrep, n_per_rep, boxes))
+}
+int sc_batch_sweep(sc_ctx* ctx, int nsweeps, const double* temper, double transmx, double rot_half, int r,
+                   uint64_t seed, uint64_t counter, sc_sweep_stats* stats) {
+    SC_GUARD(ctx, sc_batch_sweep_impl(ctx, nsweeps, temper, transmx, rot_half, r, seed, counter, stats))
+}
+int sc_batch_energies(sc_ctx* ctx, double* e) { SC_GUARD(ctx, sc_batch_energies_impl(ctx, e)) }
+int sc_batch_exchange(sc_ctx* ctx, double* temper, const double* energy, int parity, uint64_t seed, uint64_t counter,
+                      int32_t* accepted) {
+    SC_GUARD(ctx, sc_batch_exchange_impl(ctx, temper, energy, parity, seed, counter, accepted))
+}
+int sc_batch_get_config(sc_ctx* ctx, int r, double* pos, double* dir, double* pd0) {
+    SC_GUARD(ctx, sc_batch_get_config_impl(ctx, r, pos, dir, pd0))
+}
+uint64_t sc_replica_seed(uint64_t seed, int r) { return replica_seed(seed, r); }
+
+// ------------------------------------------------------------ autotuner
+// Times the sweep in each launch shape on the loaded state (restored after
+// every trial) and keeps the fastest: stencil radius r = 1 (cells >= maxcut,
+// 2x2x2 colours) or 2 (cells >= maxcut/2, 3x3x3 colours), crossed with the
+// trial count per active cell (its occupancy, or the dithered mean).  Works
+// on a single system and on a replica batch (temper: one temperature, or one
+// per replica).
+static int sc_autotune_impl(sc_ctx* ctx, int sweeps, const double* temper, double transmx, double rot_half,
+                            uint64_t seed, double* rates, int* r_best, int* mode_best) {
+    if (!ctx || sweeps < 1 || !temper) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    HIPCK(hipSetDevice(ctx->device));
+    int n = ctx->n, c0 = ctx->cur, rc;
+    uint4* sp = nullptr; float4* so = nullptr; float2* sq = nullptr; int* si = nullptr; int* ss = nullptr;
+    HIPCK(hipMalloc((void**)&sp, n * sizeof(uint4)));
+    HIPCK(hipMalloc((void**)&so, n * sizeof(float4)));
+    HIPCK(hipMalloc((void**)&sq, n * sizeof(float2)));
+    HIPCK(hipMalloc((void**)&si, n * sizeof(int)));
+    HIPCK(hipMalloc((void**)&ss, n * sizeof(int)));
+    auto save = [&](bool back) -> int {
+        if (!back) ctx->cur = c0;
+        uint4* P_ = ctx->P[c0]; float4* O_ = ctx->O[c0]; float2* Q_ = ctx->Q[c0]; int* I_ = ctx->ID[c0];
+        hipMemcpyKind k = hipMemcpyDeviceToDevice;
+        HIPCK(hipMemcpyAsync(back ? sp : P_, back ? P_ : sp, n * sizeof(uint4), k, ctx->stream));
+        HIPCK(hipMemcpyAsync(back ? so : O_, back ? O_ : so, n * sizeof(float4), k, ctx->stream));
+        HIPCK(hipMemcpyAsync(back ? sq : Q_, back ? Q_ : sq, n * sizeof(float2), k, ctx->stream));
+        HIPCK(hipMemcpyAsync(back ? si : I_, back ? I_ : si, n * sizeof(int), k, ctx->stream));
+        HIPCK(hipMemcpyAsync(back ? ss : ctx->slot, back ? ctx->slot : ss, n * sizeof(int), k, ctx->stream));
+        HIPCK(hipStreamSynchronize(ctx->stream));
+        return SC_OK;
+    };
+    int saved_mode = ctx->trial_mode, best_r = ctx->tuned_r, best_m = saved_mode;
+    double best = -1.0;
+    rc = save(true);
+    for (int cand = 0; cand < 4 && !rc; cand++) {
+        int r = 1 + cand / 2, m = cand % 2;
+        ctx->trial_mode = m;
+        double tries = 0;
+        float ms = 0;
+        if (batch_active(ctx)) {
+            std::vector<sc_sweep_stats> st(ctx->nrep);
+            if ((rc = sc_batch_sweep_impl(ctx, 1, temper, transmx, rot_half, r, seed, 1, st.data()))) break;
+            if ((rc = sc_batch_sweep_impl(ctx, sweeps, temper, transmx, rot_half, r, seed, 2, st.data()))) break;
+            for (auto& x : st) tries += (double)(x.trans_try + x.rot_try);
+        } else {
+            sc_sweep_stats st;
+            if ((rc = sc_sweep_impl(ctx, 1, temper[0], transmx, rot_half, r, seed, 1, &st))) break;
+            if ((rc = sc_sweep_impl(ctx, sweeps, temper[0], transmx, rot_half, r, seed, 2, &st))) break;
+            tries = (double)(st.trans_try + st.rot_try);
+        }
+        ms = (float)ctx->last_ms;
+        double rate = tries / (double)n / (ms * 1e-3);
+        if (rates) rates[cand] = rate;
+        if (rate > best) { best = rate; best_r = r; best_m = m; }
+        if ((rc = save(false))) break;
+    }
+    hipFree(sp); hipFree(so); hipFree(sq); hipFree(si); hipFree(ss);
+    ctx->trial_mode = saved_mode;
+    if (rc) return rc;
+    ctx->tuned_r = best_r;
+    ctx->trial_mode = best_m;
+    if (r_best) *r_best = best_r;
+    if (mode_best) *mode_best = best_m;
+    if (batch_active(ctx)) return batch_build_grid(ctx, best_r, nullptr);
+    return build_grid(ctx, best_r, nullptr, ctx->box);
+}
+
+int sc_autotune(sc_ctx* ctx, int sweeps, const double* temper, double transmx, double rot_half, uint64_t seed,
+                double* rates, int* r_best, int* mode_best) {
+    SC_GUARD(ctx, sc_autotune_impl(ctx, sweeps, temper, transmx, rot_half, seed, rates, r_best, mode_best))
+}
+
 static int sc_volume_move_impl(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
                    uint64_t counter, int* accepted, double box_out[3]) {
     if (!ctx || !accepted || !(temper > 0)) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (ctx->nrep > 0) return fail(ctx, SC_ESTATE, "volume moves act on single systems, not replica batches");
     if (ptype != 0 && ptype != 1) return fail(ctx, SC_EINVAL, "volume move: ptype 0 and 1 are supported");
     // MoveCreator::pressureMove (movecreator.cpp:330-421)
     uint64_t h = mix64(seed ^ mix64(counter * 0xD1B54A32D192ED03ull + 7));
