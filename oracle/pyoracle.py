@@ -43,8 +43,20 @@ class _Vec(ctypes.Structure):
     _fields_ = [("x", ctypes.c_double), ("y", ctypes.c_double), ("z", ctypes.c_double)]
 
 
+MAXMT = 100
+MOL_KEYS = ("bond1c", "bond1eq", "bond2c", "bond2eq", "bonddc", "bonddeq", "bondhc", "bondheq",
+            "angle1c", "angle1eq", "angle2c", "angle2eq")
+DEGTORAD = 0.017453292519943295769236907683   # macros.h:67
+
+
+class _Mol(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int)] + [(k, ctypes.c_double) for k in MOL_KEYS]
+
+
 class _Topo(ctypes.Structure):
-    _fields_ = [("ia", (_IA * MAXT) * MAXT), ("sqmaxcut", ctypes.c_double), ("maxcut", ctypes.c_double)]
+    _fields_ = [("ia", (_IA * MAXT) * MAXT), ("sqmaxcut", ctypes.c_double), ("maxcut", ctypes.c_double),
+                ("mol", _Mol * MAXMT), ("nmol", ctypes.c_int), ("nchain", ctypes.c_int),
+                ("chain", ctypes.POINTER(ctypes.c_int))]
 
 
 _lib = None
@@ -64,6 +76,9 @@ def lib():
         L.orc_topo_add_type.argtypes = [T, ctypes.c_int, ctypes.c_int, D, ctypes.c_int]
         L.orc_topo_finalize.argtypes = [T, ctypes.c_void_p]
         L.orc_soa_pair_energies.argtypes = [T, D, ctypes.c_int, D, D, D, ctypes.c_void_p, I, ctypes.c_int, I, I, D]
+        L.orc_topo_set_mol.argtypes = [T, ctypes.c_int, ctypes.c_int, D]
+        L.orc_topo_set_chain.argtypes = [T, ctypes.c_int, ctypes.c_void_p]
+        L.orc_soa_bond_energies.argtypes = [T, D, ctypes.c_int, D, D, D, ctypes.c_void_p, I, ctypes.c_int, I, I, D]
         L.orc_soa_energy_all.argtypes = [T, D, ctypes.c_int, D, D, D, ctypes.c_void_p, I, ctypes.c_int]
         L.orc_soa_energy_all.restype = ctypes.c_double
         L.orc_soa_energy_one_each.argtypes = [T, D, ctypes.c_int, D, D, D, ctypes.c_void_p, I, ctypes.c_int, I, D]
@@ -136,8 +151,50 @@ def parse_exclusions(top_text):
     return ex
 
 
+def parse_molecules(top_text):
+    """[Molecules] and [System] (Inicializer::fillMol, inicializer.cpp:929-1108;
+    particle order of setParticlesParamss, :409-450).  Returns
+    ([(name, [types], {param: value})], [(name, count)]); angles in radians,
+    absent terms -1 (MoleculeParams defaults, moleculeparams.h:42-43)."""
+    mols, system, section, cur = [], [], None, None
+    for raw in top_text.splitlines():
+        line = raw.split("#", 1)[0].strip()
+        if not line:
+            continue
+        if line.startswith("["):
+            section = line[1:].split("]", 1)[0].strip().upper()
+            continue
+        if section == "MOLECULES":
+            rest = line
+            if cur is None:
+                name, _, rest = rest.partition(":")
+                cur = (name.strip(), [], {k: -1.0 for k in MOL_KEYS})
+                mols.append(cur)
+                rest = rest.partition("{")[2]
+            closes = "}" in rest
+            rest = rest.split("}", 1)[0].strip()
+            if rest:
+                cmd, _, val = rest.partition(":")
+                cmd, v = cmd.strip().upper(), val.split()
+                if cmd == "PARTICLES":
+                    cur[1].append(int(v[0]))
+                elif cmd in ("BOND1", "BOND2", "BONDD", "BONDH", "ANGLE1", "ANGLE2"):
+                    k = cmd.lower()
+                    c, eq = float(v[0]), float(v[1])
+                    if k.startswith("angle"):
+                        eq *= DEGTORAD
+                    cur[2][k + "c"], cur[2][k + "eq"] = c, eq
+            if closes:
+                cur = None
+        elif section == "SYSTEM":
+            f = line.split()
+            system.append((f[0], int(f[1])))
+    return mols, system
+
+
 class Topo:
-    """Oracle topology built from top.init text."""
+    """Oracle topology built from top.init text, with the chain layout of its
+    [System] section (molecule type and position of every particle)."""
 
     def __init__(self, top_text):
         self.t = _Topo()
@@ -149,6 +206,23 @@ class Topo:
                 raise ValueError("TOPOLOGY ERROR: wrong number of parameters for type %d" % num)
         ex = parse_exclusions(top_text)
         L.orc_topo_finalize(ctypes.byref(self.t), ex.ctypes.data_as(ctypes.c_void_p))
+        mols, system = parse_molecules(top_text)
+        names = [m[0] for m in mols]
+        for k, (name, types, prm) in enumerate(mols):
+            arr = np.array([prm[key] for key in MOL_KEYS], np.float64)
+            L.orc_topo_set_mol(ctypes.byref(self.t), k, max(1, len(types)), _dp(arr))
+        chain, ptypes = [], []
+        for name, count in system:
+            k = names.index(name)
+            for _ in range(count):
+                for pos, ty in enumerate(mols[k][1]):
+                    chain.append((k, pos))
+                    ptypes.append(ty)
+        self.types = np.array(ptypes, np.int32)
+        self.chain = np.ascontiguousarray(np.array(chain, np.int32).reshape(-1, 2))
+        self.has_chains = any(len(m[1]) > 1 for m in mols)
+        if self.has_chains:
+            L.orc_topo_set_chain(ctypes.byref(self.t), len(self.chain), self.chain.ctypes.data_as(ctypes.c_void_p))
 
     @property
     def ref(self):
@@ -197,6 +271,16 @@ def pair_energies(topo, st, pi, pj):
     pj = np.ascontiguousarray(pj, np.int32)
     out = np.zeros(len(pi), np.float64)
     lib().orc_soa_pair_energies(topo.ref, _dp(st.box), st.n, _dp(st.pos), _dp(st.dir), _dp(st.pd0), _opt(st.pd1),
+                                _ip(st.types), len(pi), _ip(pi), _ip(pj), _dp(out))
+    return out
+
+
+def bond_energies(topo, st, pi, pj):
+    """bond + angle part of the pair energies (i as part1)"""
+    pi = np.ascontiguousarray(pi, np.int32)
+    pj = np.ascontiguousarray(pj, np.int32)
+    out = np.zeros(len(pi), np.float64)
+    lib().orc_soa_bond_energies(topo.ref, _dp(st.box), st.n, _dp(st.pos), _dp(st.dir), _dp(st.pd0), _opt(st.pd1),
                                 _ip(st.types), len(pi), _ip(pi), _ip(pj), _dp(out))
     return out
 

@@ -17,6 +17,11 @@
  *   parameter derivation (fillTypes/genParamPairs/genTopoParams): the
  *   reference's own ia_params dumped into tests/golden/topo_golden.json.
  *   cell list / neighbour pairs: brute force over integers (bit-exact).
+ *   bonds and angles (HarmonicSc, HarmonicSp, AngleSc, getConlist): golden
+ *       vectors of five chain systems from the compiled reference
+ *       (tests/golden/chain_golden.npz: bond1/bond2/bondd/bondh/angle1/angle2,
+ *       spheres at either chain end) and closed forms of each term
+ *       (tests/test_oracle_chains.py).
  *   orc_mc_run: energy-drift invariant (E_end - E_start - sum dE = 0) and the
  *   acceptance rule restated from movecreator.h:175-187; its trajectories are
  *   not the reference's (own RNG) -- parity of sampled ensembles only.
@@ -604,8 +609,8 @@ static double wca_cos2_taylor(double dist, const orc_ia* ia) {
     return ia->A * pow(dist, -12) - ia->B * pow(dist, -6);
 }
 
-/* SpheroCylinder::operator() without bonds, paire.h:1122-1196 */
-static double sc_sc(int mode, const orc_ia* ia, const orc_part* p1, const orc_part* p2, orc_vec r_cm) {
+/* SpheroCylinder::operator(), paire.h:1122-1196; abE: its bond + angle terms */
+static double sc_sc(int mode, const orc_ia* ia, const orc_part* p1, const orc_part* p2, orc_vec r_cm, double abE) {
     double distSq, rep, atr = 0.0;
     orc_vec d1, d2;
     int fCH, sCH, fT, sT;
@@ -621,9 +626,9 @@ static double sc_sc(int mode, const orc_ia* ia, const orc_part* p1, const orc_pa
             double dist = sqrt(distSq);
             if (!(dist > ia->rcutwca)) atr = ia->A * pow(dist, -12) - ia->B * pow(dist, -6) + ia->epsilon;
         }
-        return rep + atr;
+        return abE + rep + atr;
     }
-    if (distSq > ia->rcutSq || ia->epsilon == 0.0 || ia->exclude) return rep;
+    if (distSq > ia->rcutSq || ia->epsilon == 0.0 || ia->exclude) return abE + rep + atr;
     /* getGeoTypes of Psc (:486-497), CPsc (:674-685), PscCPsc (:891-906) */
     if (mode == 0) {
         fCH = ia->geotype[0] == ORC_CHPSC || ia->geotype[0] == ORC_TCHPSC;
@@ -649,7 +654,7 @@ static double sc_sc(int mode, const orc_ia* ia, const orc_part* p1, const orc_pa
     if (fT) atr += patch_pair(mode, ia, fCH ? p1->chdir[1] : p1->dir, d2, &P11, &P20, r_cm, 1, 0);
     if (sT) atr += patch_pair(mode, ia, d1, sCH ? p2->chdir[1] : p2->dir, &P10, &P21, r_cm, 0, 1);
     if (fT && sT) atr += patch_pair(mode, ia, fCH ? p1->chdir[1] : p1->dir, sCH ? p2->chdir[1] : p2->dir, &P11, &P21, r_cm, 1, 1);
-    return rep + atr;
+    return abE + rep + atr;
 }
 
 /* MixSpSc::closestDist, paire.h:1077-1093 */
@@ -685,9 +690,9 @@ static double patch_to_sphere(int sca, double dist, double contt, orc_vec distve
     return e * (fangl_scale(a, ia->pcangl[0], ia->pcanglsw[0]) * f0);
 }
 
-/* MixSpSc::operator() without bonds, paire.h:1022-1067.  kind: 0 ScaSpa,
- * 1 PscSpa, 2 CPscSpa */
-static double sp_sc(int kind, const orc_topo* t, const orc_part* p1, const orc_part* p2, orc_vec r_cm) {
+/* MixSpSc::operator(), paire.h:1022-1067.  kind: 0 ScaSpa, 1 PscSpa,
+ * 2 CPscSpa; abE: its bond + angle terms */
+static double sp_sc(int kind, const orc_topo* t, const orc_part* p1, const orc_part* p2, orc_vec r_cm, double abE) {
     const orc_ia* ia12 = &t->ia[p1->type][p2->type];
     int isp1 = ia12->geotype[0] < ORC_SP;
     const orc_part* spc = isp1 ? p1 : p2;
@@ -717,19 +722,159 @@ static double sp_sc(int kind, const orc_topo* t, const orc_part* p1, const orc_p
                 atr += patch_to_sphere(kind == 0, dist, contt, distvec, ia, chiral ? spc->chdir[1] : spc->dir, spc->patchdir[1]);
         }
     }
-    return rep + atr;
+    return abE + rep + atr;
 }
 
-/* PairE::operator() (paire.h:1209-1220) with the functor table of
- * PairE::initIntFCE (paire.cpp:6-80): later rules override earlier ones. */
-double orc_pair_energy(const orc_topo* t, const double box[3], const orc_part* p1, const orc_part* p2) {
-    orc_vec r_cm = orc_image(box, p1->pos, p2->pos);
-    double dotrcm = DOT(r_cm, r_cm), dist;
+/* ------------------------------------------------------- bonds and angles */
+/* ConList (scOOP/structures/Conf.h:17-31) with particle indices in place of
+ * pointers (-1: none): [0] previous, [1] next, [2] second previous, [3]
+ * second next in the chain. */
+typedef struct {
+    int empty;
+    int cl[4];
+    double sp, mod[2], c[2], eq[2];
+} oconlist;
+
+int orc_topo_set_mol(orc_topo* t, int m, int size, const double* p) {
+    orc_mol* q;
+    if (m < 0 || m >= ORC_MAXMT || size < 1) return 0;
+    q = &t->mol[m];
+    q->size = size;
+    q->bond1c = p[0]; q->bond1eq = p[1]; q->bond2c = p[2]; q->bond2eq = p[3];
+    q->bonddc = p[4]; q->bonddeq = p[5]; q->bondhc = p[6]; q->bondheq = p[7];
+    q->angle1c = p[8]; q->angle1eq = p[9]; q->angle2c = p[10]; q->angle2eq = p[11];
+    if (m + 1 > t->nmol) t->nmol = m + 1;
+    return 1;
+}
+
+void orc_topo_set_chain(orc_topo* t, int n, const int* chain) {
+    t->nchain = chain ? n : 0;
+    t->chain = chain;
+}
+
+/* ParticleVector::getConlist, Conf.h:90-147 */
+static oconlist conlist_of(const orc_topo* t, const orc_part* p) {
+    oconlist c;
+    const orc_mol* m;
+    int i = p->idx, pos = p->molpos;
+    memset(&c, 0, sizeof(c));
+    c.empty = 1;
+    c.cl[0] = c.cl[1] = c.cl[2] = c.cl[3] = -1;
+    if (p->moltype < 0) return c;
+    m = &t->mol[p->moltype];
+    if (m->size == 1) return c;                                         /* isAtomic, :91-92 */
+    if (m->bond1c >= 0.0 || m->bonddc >= 0.0 || m->bondhc >= 0.0) {    /* :99-131 */
+        if (pos > 0) c.cl[0] = i - 1;
+        if (pos + 1 < m->size) c.cl[1] = i + 1;
+        if (m->bond1c >= 0.0) {
+            c.eq[0] = m->bond1eq; c.c[0] = m->bond1c;
+            c.mod[0] = 0.0; c.mod[1] = 0.0; c.sp = 0.0;
+        }
+        if (m->bonddc >= 0.0) {
+            c.eq[0] = 0.0; c.c[0] = m->bonddc;
+            c.mod[0] = m->bonddeq; c.mod[1] = 0.0; c.sp = 0.0;
+        }
+        if (m->bondhc >= 0.0) {
+            c.eq[0] = 0.0; c.c[0] = m->bondhc;
+            c.mod[0] = m->bondheq; c.mod[1] = m->bondheq; c.sp = m->bondheq;
+        }
+        c.empty = 0;
+    }
+    if (m->bond2c >= 0.0) {                                             /* :132-143 */
+        if (pos > 1) c.cl[2] = i - 2;
+        if (pos + 2 < m->size) c.cl[3] = i + 2;
+        c.eq[1] = m->bond2eq; c.c[1] = m->bond2c;
+        c.empty = 0;
+    }
+    return c;
+}
+
+/* EBond::harmonicPotential, paire.h:202-204 */
+static inline double harmonic(double a, double eq, double k) { return k * (a - eq) * (a - eq) * 0.5; }
+
+/* HarmonicSp, paire.h:221-236: centre distance for both neighbour shells */
+static double bond_sp(const oconlist* c, const orc_part* p2, double dist) {
+    if (p2->idx == c->cl[1] || p2->idx == c->cl[0]) return harmonic(dist, c->eq[0], c->c[0]);
+    if (p2->idx == c->cl[2] || p2->idx == c->cl[3]) return harmonic(dist, c->eq[1], c->c[1]);
+    return 0.0;
+}
+
+/* HarmonicSc, paire.h:239-280: nearest neighbours bond end to end (particle
+ * i's -dir end to i+1's +dir end; spheres bond at sp along dir), second
+ * neighbours by centre distance */
+static double bond_sc(const orc_topo* t, const double box[3], const orc_part* p1, const orc_part* p2,
+                      const oconlist* c, double dist) {
     const orc_ia* ia = &t->ia[p1->type][p2->type];
+    if (p2->idx == c->cl[1] || p2->idx == c->cl[0]) {
+        int prev = p2->idx == c->cl[0];
+        double h1, h2;
+        orc_vec v1, v2, b;
+        if (ia->geotype[0] < ORC_SP) h1 = (ia->half_len[0] + c->mod[prev ? 0 : 1]) * (prev ? 1.0 : -1.0);
+        else h1 = c->sp;
+        if (ia->geotype[1] < ORC_SP) h2 = (ia->half_len[1] + c->mod[prev ? 1 : 0]) * (prev ? -1.0 : 1.0);
+        else h2 = c->sp;
+        v1 = V(p1->pos.x + (p1->dir.x * h1 / box[0]), p1->pos.y + (p1->dir.y * h1 / box[1]),
+               p1->pos.z + (p1->dir.z * h1 / box[2]));
+        v2 = V(p2->pos.x + (p2->dir.x * h2 / box[0]), p2->pos.y + (p2->dir.y * h2 / box[1]),
+               p2->pos.z + (p2->dir.z * h2 / box[2]));
+        b = orc_image(box, v1, v2);
+        return harmonic(sqrt(DOT(b, b)), c->eq[0], c->c[0]);
+    }
+    if (p2->idx == c->cl[2] || p2->idx == c->cl[3]) return harmonic(dist, c->eq[1], c->c[1]);
+    return 0.0;
+}
+
+/* AngleSc::angleEnergyAngle2, paire.h:335-350: angle between the two patch
+ * directions projected on the planes normal to dir_a x dir_b */
+static double angle2_of(const orc_part* a, const orc_part* b) {
+    orc_vec ax = CROSS(a->dir, b->dir), x1 = CROSS(a->dir, ax), x2 = CROSS(b->dir, ax);
+    double v1x = DOT(x1, a->patchdir[0]), v1y = DOT(ax, a->patchdir[0]);
+    double v2x = DOT(x2, b->patchdir[0]), v2y = DOT(ax, b->patchdir[0]);
+    return acos((v1x * v2x + v1y * v2y) / sqrt((v1x * v1x + v1y * v1y) * (v2x * v2x + v2y * v2y)));
+}
+
+/* AngleSc::operator(), paire.h:283-333 */
+static double angle_sc(const orc_topo* t, const double box[3], const orc_part* p1, const orc_part* p2,
+                       const oconlist* c) {
+    const orc_ia* ia = &t->ia[p1->type][p2->type];
+    const orc_mol* m;
+    double e = 0.0;
+    int near = p2->idx == c->cl[0] || p2->idx == c->cl[1];
+    if (p1->moltype < 0 || !near) return 0.0;
+    m = &t->mol[p1->moltype];
+    if (m->angle1c >= 0) {
+        int prev = p2->idx == c->cl[0];
+        orc_vec v1, v2;
+        if (ia->geotype[0] < ORC_SP) v1 = p1->dir;
+        else {
+            double h = ia->half_len[1] * (prev ? -1.0 : 1.0);
+            v1 = V(p2->pos.x + p2->dir.x * h / box[0], p2->pos.y + p2->dir.y * h / box[1],
+                   p2->pos.z + p2->dir.z * h / box[2]);
+            v1 = NORM(orc_image(box, v1, p1->pos));
+        }
+        if (ia->geotype[1] < ORC_SP) v2 = p2->dir;
+        else {
+            double h = ia->half_len[0] * (prev ? 1.0 : -1.0);
+            v2 = V(p1->pos.x + p1->dir.x * h / box[0], p1->pos.y + p1->dir.y * h / box[1],
+                   p1->pos.z + p1->dir.z * h / box[2]);
+            v2 = NORM(orc_image(box, v2, p2->pos));
+        }
+        e += harmonic(acos(DOT(v1, v2)), m->angle1eq, m->angle1c);
+    }
+    if (m->angle2c >= 0 && ia->geotype[0] < ORC_SP && ia->geotype[1] < ORC_SP) {
+        double a = (p2->idx == c->cl[0]) ? angle2_of(p2, p1) : angle2_of(p1, p2);
+        e += harmonic(a, m->angle2eq, m->angle2c);
+    }
+    return e;
+}
+
+/* the functor of a type pair, PairE::initIntFCE (paire.cpp:6-80): later
+ * rules override earlier ones.  0..4 SpheroCylinder (Psc, CPsc, PscCPsc,
+ * Scn, Sca), 10 Sphere<WcaTrunc>, 11 Sphere<WcaCos2Taylor>, 20..22 MixSpSc
+ * (ScaSpa, PscSpa, CPscSpa), -1 EBasic */
+static int functor_kind(const orc_ia* ia) {
     int g = ia->geotype[0], o = ia->geotype[1];
-    int kind = -1; /* 0..4 sc_sc modes, 10 Sphere WcaTrunc, 11 Sphere Taylor, 20.. MixSpSc */
-    if (dotrcm > t->sqmaxcut) return 0.0;
-    dist = sqrt(dotrcm);
+    int kind = -1;
     if ((is_cpsc(g) && is_psc(o)) || (is_psc(g) && is_cpsc(o))) kind = 2;
     if (is_cpsc(g) && is_cpsc(o)) kind = 1;
     if (is_psc(g) && is_psc(o)) kind = 0;
@@ -740,15 +885,82 @@ double orc_pair_energy(const orc_topo* t, const double box[3], const orc_part* p
     if ((g == ORC_SCA && o == ORC_SPA) || (g == ORC_SPA && o == ORC_SCA)) kind = 20;
     if ((is_psc(g) && (o == ORC_SPA || o == ORC_SPN)) || ((g == ORC_SPA || g == ORC_SPN) && is_psc(o))) kind = 21;
     if ((is_cpsc(g) && (o == ORC_SPA || o == ORC_SPN)) || ((g == ORC_SPA || g == ORC_SPN) && is_cpsc(o))) kind = 22;
+    return kind;
+}
+
+/* bond + angle terms of the functor: HarmonicSc + AngleSc for
+ * SpheroCylinder and MixSpSc, HarmonicSp for Sphere, none for EBasic */
+static double bonds_of(int kind, const orc_topo* t, const double box[3], const orc_part* p1, const orc_part* p2,
+                       const oconlist* c, double dist) {
+    double ab;
+    if (c->empty) return 0.0;
+    if (kind == 10 || kind == 11) return bond_sp(c, p2, dist);
+    if (kind < 0) return 0.0;
+    ab = bond_sc(t, box, p1, p2, c, dist);
+    ab += angle_sc(t, box, p1, p2, c);
+    return ab;
+}
+
+/* the functor with its bond + angle part abE */
+static double functor(int kind, const orc_topo* t, const orc_ia* ia, const orc_part* p1, const orc_part* p2,
+                      orc_vec r_cm, double dist, double abE) {
     switch (kind) {
-    case 0: case 1: case 2: case 3: case 4: return sc_sc(kind, ia, p1, p2, r_cm);
-    case 10: return wca(dist, ia);
-    case 11: return wca_cos2_taylor(dist, ia);
-    case 20: return sp_sc(0, t, p1, p2, r_cm);
-    case 21: return sp_sc(1, t, p1, p2, r_cm);
-    case 22: return sp_sc(2, t, p1, p2, r_cm);
+    case 0: case 1: case 2: case 3: case 4: return sc_sc(kind, ia, p1, p2, r_cm, abE);
+    case 10: return abE + wca(dist, ia);
+    case 11: return abE + wca_cos2_taylor(dist, ia);
+    case 20: return sp_sc(0, t, p1, p2, r_cm, abE);
+    case 21: return sp_sc(1, t, p1, p2, r_cm, abE);
+    case 22: return sp_sc(2, t, p1, p2, r_cm, abE);
     default: return 0.0;   /* EBasic: "interaction not programmed" (paire.h:207-212) */
     }
+}
+
+/* PairE::operator(), paire.h:1209-1220, with p1's connectivity list
+ * (TotalEMatrix passes getConlist(part1)): the early exit holds only for an
+ * empty list. */
+double orc_pair_energy(const orc_topo* t, const double box[3], const orc_part* p1, const orc_part* p2) {
+    orc_vec r_cm = orc_image(box, p1->pos, p2->pos);
+    double dotrcm = DOT(r_cm, r_cm), dist;
+    const orc_ia* ia = &t->ia[p1->type][p2->type];
+    oconlist c = conlist_of(t, p1);
+    int kind;
+    if (dotrcm > t->sqmaxcut && c.empty) return 0.0;
+    dist = sqrt(dotrcm);
+    kind = functor_kind(ia);
+    return functor(kind, t, ia, p1, p2, r_cm, dist, bonds_of(kind, t, box, p1, p2, &c, dist));
+}
+
+/* non-bonded part alone (zero beyond sqmaxcut) */
+static double pair_nb(const orc_topo* t, const double box[3], const orc_part* p1, const orc_part* p2) {
+    orc_vec r_cm = orc_image(box, p1->pos, p2->pos);
+    double dotrcm = DOT(r_cm, r_cm);
+    const orc_ia* ia = &t->ia[p1->type][p2->type];
+    if (dotrcm > t->sqmaxcut) return 0.0;
+    return functor(functor_kind(ia), t, ia, p1, p2, r_cm, sqrt(dotrcm), 0.0);
+}
+
+double orc_bond_energy(const orc_topo* t, const double box[3], const orc_part* p1, const orc_part* p2) {
+    orc_vec r_cm = orc_image(box, p1->pos, p2->pos);
+    oconlist c = conlist_of(t, p1);
+    return bonds_of(functor_kind(&t->ia[p1->type][p2->type]), t, box, p1, p2, &c, sqrt(DOT(r_cm, r_cm)));
+}
+
+/* bond energy of p (as part1) with its chain partners: all of them, or only
+ * those of smaller index (each pair once in whole-system sums) */
+static double bonds_with_partners(const orc_topo* t, const double box[3], const orc_part* p, const orc_part* q,
+                                  int lower_only) {
+    oconlist c = conlist_of(t, q);
+    double e = 0.0;
+    int k;
+    if (c.empty) return 0.0;
+    for (k = 0; k < 4; k++) {
+        int j = c.cl[k];
+        orc_vec r_cm;
+        if (j < 0 || (lower_only && j > q->idx)) continue;
+        r_cm = orc_image(box, q->pos, p[j].pos);
+        e += bonds_of(functor_kind(&t->ia[q->type][p[j].type]), t, box, q, &p[j], &c, sqrt(DOT(r_cm, r_cm)));
+    }
+    return e;
 }
 
 /* TotalEMatrix::allToAll(EM*) order, totalenergycalculator.h:502-521:
@@ -837,8 +1049,9 @@ double orc_energy_all_cells(const orc_topo* t, const double box[3], const orc_pa
         for (q = 0; q < m; q++)
             for (k = g.head[st[q]]; k < g.head[st[q] + 1]; k++) {
                 int j = g.idx[k];
-                if (j < i) e += orc_pair_energy(t, box, &p[i], &p[j]);
+                if (j < i) e += pair_nb(t, box, &p[i], &p[j]);
             }
+        e += bonds_with_partners(t, box, p, &p[i], 1);
     }
     grid_free(&g);
     return e;
@@ -853,6 +1066,12 @@ static void load_part(const orc_topo* t, orc_part* q, int i, const double* pos, 
     q->patchdir[0] = V(pd0[3 * i], pd0[3 * i + 1], pd0[3 * i + 2]);
     if (pd1) q->patchdir[1] = V(pd1[3 * i], pd1[3 * i + 1], pd1[3 * i + 2]);
     q->type = type[i];
+    q->idx = i;
+    q->moltype = -1;
+    if (t->chain && i < t->nchain) {
+        q->moltype = t->chain[2 * i];
+        q->molpos = t->chain[2 * i + 1];
+    }
     orc_part_init(t, q);
 }
 
@@ -870,6 +1089,16 @@ int orc_soa_pair_energies(const orc_topo* t, const double box[3], int n, const d
     orc_part* p = load_all(t, n, pos, dir, pd0, pd1, type);
     int k;
     for (k = 0; k < m; k++) out[k] = orc_pair_energy(t, box, &p[pi[k]], &p[pj[k]]);
+    free(p);
+    return 0;
+}
+
+int orc_soa_bond_energies(const orc_topo* t, const double box[3], int n, const double* pos, const double* dir,
+                          const double* pd0, const double* pd1, const int* type, int m, const int* pi,
+                          const int* pj, double* out) {
+    orc_part* p = load_all(t, n, pos, dir, pd0, pd1, type);
+    int k;
+    for (k = 0; k < m; k++) out[k] = orc_bond_energy(t, box, &p[pi[k]], &p[pj[k]]);
     free(p);
     return 0;
 }
@@ -1013,9 +1242,9 @@ static double grid_one(const orc_topo* t, const double box[3], const ogrid* g, c
     for (a = 0; a < m; a++)
         for (k = g->head[st[a]]; k < g->head[st[a] + 1]; k++) {
             int j = g->idx[k];
-            if (j != self) e += orc_pair_energy(t, box, q, &p[j]);
+            if (j != self) e += pair_nb(t, box, q, &p[j]);
         }
-    return e;
+    return e + bonds_with_partners(t, box, p, q, 0);
 }
 
 /* move particle i between cells of the CSR grid by re-building only if its cell changed */

@@ -53,10 +53,28 @@ typedef struct {
     int exclude;
 } orc_ia;
 
+#define ORC_MAXMT 100        /* scOOP/structures/macros.h:88 MAXMT */
+
+/* Bond and angle parameters of one molecule type, the fields of
+ * MoleculeParams (scOOP/structures/moleculeparams.h:16-32) that the energy
+ * reads; -1 marks an absent term (the constructor's default, :42-43).
+ * Angles in radians. */
+typedef struct {
+    int size;              /* particles per molecule (molSize) */
+    double bond1c, bond1eq, bond2c, bond2eq, bonddc, bonddeq, bondhc, bondheq;
+    double angle1c, angle1eq, angle2c, angle2eq;
+} orc_mol;
+
 typedef struct {
     orc_ia ia[ORC_MAXT][ORC_MAXT];
     double sqmaxcut;   /* squared after finalize, as Topo::sqmaxcut */
     double maxcut;
+    orc_mol mol[ORC_MAXMT];
+    int nmol;
+    /* per particle (index order): molecule type, position in its molecule.
+     * Caller-owned; NULL or nchain == 0 means every particle is atomic. */
+    int nchain;
+    const int* chain;
 } orc_topo;
 
 typedef struct {
@@ -66,6 +84,8 @@ typedef struct {
     orc_vec patchsides[4];
     orc_vec chdir[2];
     int type;
+    int idx;              /* index in the configuration (identity for the connectivity list) */
+    int moltype, molpos;  /* molecule type and position in the molecule; moltype -1: atomic */
 } orc_part;
 
 /* ---- topology ---- */
@@ -77,12 +97,23 @@ int orc_topo_add_type(orc_topo* t, int type, int geotype, const double* params, 
 void orc_topo_finalize(orc_topo* t, const unsigned char* exclusions);
 const orc_ia* orc_topo_ia(const orc_topo* t, int i, int j);
 
+/* ---- molecules ----
+ * m: molecule type; p: size then the 12 parameters in orc_mol order.  Returns
+ * 1, or 0 if m is out of range. */
+int orc_topo_set_mol(orc_topo* t, int m, int size, const double* p);
+/* chain: 2 ints per particle (molecule type, position); kept by pointer. */
+void orc_topo_set_chain(orc_topo* t, int n, const int* chain);
+
 /* ---- particles ---- */
 void orc_part_init(const orc_topo* t, orc_part* p);
 orc_vec orc_image(const double box[3], orc_vec r1, orc_vec r2);
 
 /* ---- energies ---- */
+/* PairE::operator() with p1's connectivity list: bonds, angles and the
+ * non-bonded functor */
 double orc_pair_energy(const orc_topo* t, const double box[3], const orc_part* p1, const orc_part* p2);
+/* the bond and angle part alone (HarmonicSc/HarmonicSp + AngleSc) */
+double orc_bond_energy(const orc_topo* t, const double box[3], const orc_part* p1, const orc_part* p2);
 orc_vec orc_min_dist_segments(orc_vec segA, orc_vec segB, double halfl1, double halfl2, orc_vec r_cm);
 double orc_energy_all(const orc_topo* t, const double box[3], const orc_part* p, int n);
 double orc_energy_one(const orc_topo* t, const double box[3], const orc_part* p, int n, int target);
@@ -99,6 +130,10 @@ int orc_soa_pair_energies(const orc_topo* t, const double box[3], int n,
 double orc_soa_energy_all(const orc_topo* t, const double box[3], int n,
                           const double* pos, const double* dir, const double* pd0, const double* pd1,
                           const int* type, int use_cells);
+/* bond + angle energy of each pair (i[k], j[k]) with i[k] as part1 */
+int orc_soa_bond_energies(const orc_topo* t, const double box[3], int n,
+                          const double* pos, const double* dir, const double* pd0, const double* pd1,
+                          const int* type, int m, const int* pi, const int* pj, double* out);
 int orc_soa_energy_one_each(const orc_topo* t, const double box[3], int n,
                             const double* pos, const double* dir, const double* pd0, const double* pd1,
                             const int* type, int m, const int* targets, double* out);

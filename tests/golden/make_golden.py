@@ -17,6 +17,12 @@ Sets:
   derived_golden.npz  patch sides / chiral axes / second patch after
                       Particle::init for the mixed configuration
   mindist_golden.npz  EPatch::minDistSegments on random and degenerate inputs
+  chain_golden.npz    p2p(i,j) over all ordered pairs, allToAll and oneToAll(i)
+                      of chain systems covering every bond and angle term
+                      (bond1, bond2, bondd, bondh, angle1, angle2; spheres at
+                      either end, PSC and CPSC segments)
+
+`python tests/golden/make_golden.py chains` rewrites chain_golden.npz only.
 """
 import json
 import os
@@ -207,9 +213,146 @@ def run_mindist(cases):
         return np.loadtxt(out).reshape(-1, 3)
 
 
+CHAIN_TYPES = """[Types]
+H 4 SPA 1.0 1.2 1.0 0.2
+T 5 PSC 1.0 1.0 1.0 0.2 80.0 5.0 3.0 0.0
+C 6 CPSC 1.0 1.0 1.2 0.2 90.0 5.0 3.0 0.0
+"""
+SP_TYPES = """[Types]
+N 7 SPN 1.0 0.95
+S 8 SPA 1.0 1.0 1.12246205 1.6
+"""
+# name: (types text, molecules text, [(molecule, count)], box edge)
+CHAIN_SETS = {
+    "c5": (CHAIN_TYPES, "CH: {\nbond1: 100 1\nparticles: 4\nparticles: 5\nparticles: 5\n}\n",
+           [("CH", 16)], 12.0),
+    "angles": (CHAIN_TYPES, "K: {\nbond1: 80 0.8\nbond2: 5 3.0\nangle1: 20 30\nangle2: 15 60\nparticles: 4\n"
+                            "particles: 5\nparticles: 6\nparticles: 6\nparticles: 4\n}\nA: {\nparticles: 4\n}\n",
+               [("K", 8), ("A", 6)], 12.0),
+    "bondd": (CHAIN_TYPES, "D: {\nbondd: 50 0.5\nparticles: 5\nparticles: 5\nparticles: 6\n}\n", [("D", 12)], 11.0),
+    "bondh": (CHAIN_TYPES, "B: {\nbondh: 50 0.3\nparticles: 4\nparticles: 5\nparticles: 5\nparticles: 4\n}\n",
+              [("B", 10)], 11.0),
+    "sp": (SP_TYPES, "W: {\nbond1: 90 1.0\nbond2: 10 2.2\nparticles: 7\nparticles: 8\nparticles: 8\n}\n",
+           [("W", 20)], 9.0),
+}
+
+
+def chain_top(name):
+    types, mols, system, _ = CHAIN_SETS[name]
+    return types + "[Molecules]\n" + mols + "[System]\n" + "".join("%s %d\n" % s for s in system)
+
+
+def _seg_dist(a, b):
+    """closest distance of two beads (pos, dir, half length)"""
+    r = b[0] - a[0]
+    v = po.min_dist_segments(a[1], b[1], a[2], b[2], -r)
+    return np.linalg.norm(v)
+
+
+def make_chain_config(name, rng):
+    """Chains grown bead by bead: each bond vector between the attachment
+    points (HarmonicSc, paire.h:239-280: bead i's -dir end to bead i+1's +dir
+    end, spheres at sp along dir; mod = bondd/bondh extension) has a random
+    direction and a length within +-35 % of equilibrium; beads keep a segment
+    distance >= 0.6 sigma from their bonded neighbour, >= 0.8 sigma from the
+    rest of their chain and >= 0.85 sigma from earlier chains (retries), so
+    every energy is finite and moderate and the bond terms are not lost in
+    the rounding of a large repulsion."""
+    tp = po.Topo(chain_top(name))
+    box = np.full(3, CHAIN_SETS[name][3])
+    types = tp.types
+    n = len(types)
+    half = {t: tp.ia(t, t).half_len[0] for t in set(types.tolist())}
+    sph = {t: tp.ia(t, t).geotype[0] >= 30 for t in set(types.tolist())}
+    pos, dirs = np.zeros((n, 3)), np.zeros((n, 3))
+    i = 0
+    while i < n:
+        mt, mp = tp.chain[i]
+        m = tp.t.mol[mt]
+        size = m.size
+        for attempt in range(200):
+            beads = []
+            for k in range(size):
+                t = types[i + k]
+                d = rng.normal(size=3)
+                d /= np.linalg.norm(d)
+                if k == 0:
+                    c = rng.random(3) * box
+                else:
+                    pc, pd, pt = beads[-1]
+                    # previous bead's attachment towards its next partner
+                    if sph[pt]:
+                        sp = m.bondheq if m.bondhc >= 0 else 0.0
+                        a0 = pc + pd * sp
+                    else:
+                        mod1 = m.bondheq if m.bondhc >= 0 else 0.0
+                        a0 = pc - pd * (half[pt] + mod1)
+                    eq = m.bond1eq if m.bond1c >= 0 else (0.3 if m.bonddc >= 0 or m.bondhc >= 0 else 1.0)
+                    u = rng.normal(size=3)
+                    u /= np.linalg.norm(u)
+                    a1 = a0 + u * eq * rng.uniform(0.65, 1.35) if eq > 0 else a0 + u * rng.uniform(0.0, 0.4)
+                    if not sph[t]:   # body runs from the +dir end along -dir: away from the bond
+                        d = -u + 0.6 * d
+                        d /= np.linalg.norm(d)
+                    if sph[t]:
+                        sp = m.bondheq if m.bondhc >= 0 else 0.0
+                        c = a1 - d * sp
+                    else:
+                        mod0 = m.bonddeq if m.bonddc >= 0 else (m.bondheq if m.bondhc >= 0 else 0.0)
+                        c = a1 - d * (half[t] + mod0)
+                beads.append((c, d, t))
+            ok = True
+            for k, (c, d, t) in enumerate(beads):
+                for k2 in range(k):
+                    c2, d2, t2 = beads[k2]
+                    if _seg_dist((c2, d2, half[t2]), (c, d, half[t])) < (0.6 if k2 == k - 1 else 0.8):
+                        ok = False
+                for j in range(i):
+                    r = c - pos[j]
+                    r -= box * np.rint(r / box)
+                    if np.linalg.norm(r) > 6.0:
+                        continue
+                    if _seg_dist((pos[j], dirs[j], half[types[j]]), (pos[j] + r, d, half[t])) < 0.85:
+                        ok = False
+                        break
+                if not ok:
+                    break
+            if ok:
+                break
+        if not ok:
+            raise RuntimeError("could not place chain at particle %d" % i)
+        for k, (c, d, t) in enumerate(beads):
+            pos[i + k] = c - box * np.floor(c / box)
+            dirs[i + k] = d
+        i += size
+    q = rng.normal(size=(n, 3))
+    return tp, box, pos - box / 2, dirs, q, types
+
+
+def make_chains(rng):
+    kw = {}
+    for name in CHAIN_SETS:
+        tp, box, pos, d, q, types = make_chain_config(name, rng)
+        top = chain_top(name)
+        n = len(types)
+        pi, pj = np.nonzero(~np.eye(n, dtype=bool))
+        pi, pj = pi.astype(np.int32), pj.astype(np.int32)
+        e = pairs_query(top, box, pos, d, q, pi, pj)
+        txt = po.ref_query("total", top, box, pos, d, q)
+        vals = np.array([float(x) for x in txt.split()])
+        kw.update({name + "_top": top, name + "_box": box, name + "_pos": pos, name + "_dir": d, name + "_pd0": q,
+                   name + "_types": types, name + "_i": pi, name + "_j": pj, name + "_e": e,
+                   name + "_eall": vals[0], name + "_eone": vals[1:]})
+        print("chains %s: %d particles, %d pairs, E_all %.6f" % (name, n, len(e), vals[0]))
+    np.savez_compressed(os.path.join(HERE, "chain_golden.npz"), **kw)
+
+
 def main():
     if not po.ref_available():
         sys.exit("oracle/_ref/scref missing: run `make -C oracle ref` first")
+    if sys.argv[1:] == ["chains"]:
+        make_chains(np.random.default_rng(20241101))
+        return
     rng = np.random.default_rng(20240601)
     # topology dump
     top = mixed_top([1] * 12)
@@ -258,6 +401,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "mindist_golden.npz"), cases=cases, result=res)
     print("pairs mixed %d (nonzero %d), grid %d; totals %s" % (
         len(e), np.count_nonzero(e), sum(len(g[6]) for g in grid), [(t[0], t[7]) for t in tot]))
+    make_chains(np.random.default_rng(20241101))
 
 
 if __name__ == "__main__":
