@@ -46,6 +46,7 @@ typedef struct {
     uint64_t stage1_acc, stage1_try;                    /* delayed acceptance: screen on the exact cheap terms */
     uint64_t stage2_acc, stage2_try;                    /* delayed acceptance: network screen */
     double energy_change;     /* sum of accepted dE (for the drift check, updater.cpp:122-150) */
+    uint64_t bond_rejected;   /* moves rejected because a bonded chain partner lay outside the stencil */
 } sc_sweep_stats;
 
 /* --- lifetime --------------------------------------------------------- */

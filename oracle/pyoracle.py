@@ -357,11 +357,11 @@ def nematic_s2(dirs):
 
 
 def cluster_labels(topo, st, threshold=-0.10):
-    """Cluster labels of one-particle molecules (ClusterSampler::sameCluster,
-    clust.cpp:51-99: pair energy <= threshold; genClusterList,
-    clust.cpp:105-163: each particle ends labelled with the smallest index of
-    its component).  Pairs within maxcut (beyond it the energy is 0), FP64
-    oracle energies, union by smallest root."""
+    """Cluster labels (ClusterSampler::sameCluster, clust.cpp:51-99: the same
+    molecule, or pair energy <= threshold; genClusterList, clust.cpp:105-163:
+    each particle ends labelled with the smallest index of its component).
+    Pairs within maxcut (beyond it the non-bonded energy is 0), FP64 oracle
+    energies, union by smallest root."""
     u = np.round(np.mod(st.pos, 1.0) * 4294967296.0).astype(np.uint64) % 4294967296
     pr = neighbor_pairs(u.astype(np.uint32), st.box, topo.maxcut)
     e = pair_energies(topo, st, pr[:, 0], pr[:, 1]) if len(pr) else np.zeros(0)
@@ -373,11 +373,15 @@ def cluster_labels(topo, st, threshold=-0.10):
             a = parent[a]
         return a
 
-    for (a, b), ee in zip(pr, e):
-        if ee <= threshold:
-            ra, rb = root(a), root(b)
-            if ra != rb:
-                parent[max(ra, rb)] = min(ra, rb)
+    links = [(a, b) for (a, b), ee in zip(pr, e) if ee <= threshold]
+    if topo.has_chains:
+        sizes = np.array([topo.t.mol[k].size for k in range(max(1, topo.t.nmol))])
+        c = topo.chain
+        links += [(i, i + 1) for i in range(st.n - 1) if c[i, 1] + 1 < sizes[c[i, 0]]]
+    for a, b in links:
+        ra, rb = root(a), root(b)
+        if ra != rb:
+            parent[max(ra, rb)] = min(ra, rb)
     return np.array([root(a) for a in range(st.n)], np.int32)
 
 

@@ -25,7 +25,7 @@ class SweepStats(ctypes.Structure):
     _fields_ = [("trans_acc", ctypes.c_uint64), ("trans_try", ctypes.c_uint64), ("rot_acc", ctypes.c_uint64),
                 ("rot_try", ctypes.c_uint64), ("left_cell", ctypes.c_uint64), ("stage1_acc", ctypes.c_uint64),
                 ("stage1_try", ctypes.c_uint64), ("stage2_acc", ctypes.c_uint64), ("stage2_try", ctypes.c_uint64),
-                ("energy_change", ctypes.c_double)]
+                ("energy_change", ctypes.c_double), ("bond_rejected", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

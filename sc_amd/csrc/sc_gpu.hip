@@ -282,6 +282,133 @@ struct SysConst {
     int nt;
 };
 
+// ------------------------------------------------------------ bonded chains
+// Constants of a molecule type's connectivity list (ParticleVector::getConlist,
+// scOOP/structures/Conf.h:97-147) and its angle terms (MoleculeParams,
+// moleculeparams.h:16-32).  A negative angle constant marks an absent term.
+struct MolP {
+    double c0, eq0, mod0, mod1, sp;   // nearest-neighbour bond (bond1 / bondd / bondh)
+    double c1, eq1;                   // second-neighbour bond (bond2)
+    double a1c, a1eq, a2c, a2eq;      // angle1, angle2
+};
+
+// Chain word of a particle, by ID: bit p (0..3) set when it is bonded to the
+// particle at ID + cw_rel(p); CW_SAME_NEXT when ID + 1 belongs to the same
+// molecule; bits 8.. the molecule type.  0 for atomic molecules.
+#define CW_SAME_NEXT 16
+__host__ __device__ __forceinline__ int cw_rel(int p) { return p == 0 ? -1 : p == 1 ? 1 : p == 2 ? -2 : 2; }
+__host__ __device__ __forceinline__ int cw_bit(int rel) {
+    return rel == -1 ? 0 : rel == 1 ? 1 : rel == -2 ? 2 : rel == 2 ? 3 : -1;
+}
+
+// chain tables handed to the energy and sweep kernels (cw == nullptr: no bonds)
+struct ChainTab {
+    const int* cw;
+    const MolP* mol;
+    const PP64* pp64;
+};
+
+// EBond::harmonicPotential, paire.h:202-204
+__device__ __forceinline__ double harm(double a, double eq, double k) { return k * (a - eq) * (a - eq) * 0.5; }
+
+// pos_a - pos_b, minimum image, FP64 from the fixed-point fractions
+__device__ __forceinline__ V3<double> fx_r12(uint4 a, uint4 b, const double* L) {
+    return fx_image<double>(a.x, a.y, a.z, b.x, b.y, b.z, L[0], L[1], L[2]);
+}
+
+__device__ __forceinline__ V3<double> unit64(float x, float y, float z) { return normalised(mk<double>(x, y, z)); }
+
+// AngleSc::angleEnergyAngle2 (paire.h:335-350): the angle between the patch
+// directions of a and b projected on the planes normal to dir_a x dir_b.
+// Parallel axes leave the angle undefined (0/0 in the reference): no term.
+__device__ __forceinline__ bool angle2_of(const V3<double>& da, const V3<double>& pa, const V3<double>& db,
+                                          const V3<double>& pb, double& ang) {
+    V3<double> ax = cross(da, db), x1 = cross(da, ax), x2 = cross(db, ax);
+    double v1x = dot(x1, pa), v1y = dot(ax, pa), v2x = dot(x2, pb), v2y = dot(ax, pb);
+    double den = sqrt((v1x * v1x + v1y * v1y) * (v2x * v2x + v2y * v2y));
+    if (!(den > 0.0)) return false;
+    ang = acos(fmin(1.0, fmax(-1.0, (v1x * v2x + v1y * v2y) / den)));
+    return true;
+}
+
+// Bond + angle energy of particle 1 with its chain partner 2 (ID_2 = ID_1 +
+// rel) for part1 = 1: HarmonicSc + AngleSc for the SpheroCylinder and MixSpSc
+// functors, HarmonicSp for Sphere, nothing for a pair without a functor
+// (paire.h:221-353, functor table paire.cpp:6-80).  r12 = pos1 - pos2; d, q
+// are unit axes and patch directions.  FP64.  Where rounding takes the
+// argument of an acos outside [-1, 1] (NaN in the reference) it is clamped.
+__device__ double bond_energy(const MolP& m, const PP64& P, int rel, const V3<double>& r12, const V3<double>& d1,
+                              const V3<double>& q1, const V3<double>& d2, const V3<double>& q2) {
+    bool nearp = rel == 1 || rel == -1;
+    bool sph = P.kind == K_SPH_WCA || P.kind == K_SPH_TAYLOR;
+    bool scf = (P.kind >= 0 && P.kind <= K_SCA) || P.kind >= K_SCASPA;
+    if (!sph && !scf) return 0.0;
+    if (!nearp || sph) {   // HarmonicSp; HarmonicSc for second neighbours: centre distance
+        double dist = sqrt(dot(r12, r12));
+        return nearp ? harm(dist, m.eq0, m.c0) : harm(dist, m.eq1, m.c1);
+    }
+    bool prev = rel == -1;
+    // HarmonicSc: the bonded ends (part1's +dir end to a previous partner's
+    // -dir end, and the reverse), spheres at sp along their axis
+    double h1 = P.g0 < 30 ? (P.hl0 + (prev ? m.mod0 : m.mod1)) * (prev ? 1.0 : -1.0) : m.sp;
+    double h2 = P.g1 < 30 ? (P.hl1 + (prev ? m.mod1 : m.mod0)) * (prev ? -1.0 : 1.0) : m.sp;
+    V3<double> b = mk<double>(r12.x + d1.x * h1 - d2.x * h2, r12.y + d1.y * h1 - d2.y * h2,
+                              r12.z + d1.z * h1 - d2.z * h2);
+    double e = harm(len(b), m.eq0, m.c0);
+    if (m.a1c >= 0) {   // AngleSc angle1: a sphere's axis points from its centre to the partner's bonded end
+        V3<double> v1 = d1, v2 = d2;
+        if (P.g0 >= 30) {
+            double h = P.hl1 * (prev ? -1.0 : 1.0);
+            v1 = normalised(mk<double>(d2.x * h - r12.x, d2.y * h - r12.y, d2.z * h - r12.z));
+        }
+        if (P.g1 >= 30) {
+            double h = P.hl0 * (prev ? 1.0 : -1.0);
+            v2 = normalised(mk<double>(r12.x + d1.x * h, r12.y + d1.y * h, r12.z + d1.z * h));
+        }
+        e += harm(acos(fmin(1.0, fmax(-1.0, dot(v1, v2)))), m.a1eq, m.a1c);
+    }
+    if (m.a2c >= 0 && P.g0 < 30 && P.g1 < 30) {   // AngleSc angle2, (earlier, later) in chain order
+        double ang;
+        bool ok = prev ? angle2_of(d2, q2, d1, q1, ang) : angle2_of(d1, q1, d2, q2, ang);
+        if (ok) e += harm(ang, m.a2eq, m.a2c);
+    }
+    return e;
+}
+
+// Bond energy of the particle at slot a (ID ia, record ua/oa/qa) with those of
+// its partners given by `sel` (bits of its chain word), partners read from
+// global memory through slot[].  part1 = the particle.
+__device__ double bonds_global(const ChainTab& ct, int nt, const double* L, int ia, uint4 ua, float4 oa, float2 qa,
+                               int sel, const int* __restrict__ slot, const uint4* __restrict__ P,
+                               const float4* __restrict__ O, const float2* __restrict__ Q) {
+    int cw = ct.cw[ia];
+    int mask = cw & sel & 15;
+    if (!mask) return 0.0;
+    const MolP& m = ct.mol[cw >> 8];
+    V3<double> d1 = unit64(oa.x, oa.y, oa.z), q1 = unit64(oa.w, qa.x, qa.y);
+    double e = 0.0;
+    for (int p = 0; p < 4; p++) {
+        if (!((mask >> p) & 1)) continue;
+        int b = slot[ia + cw_rel(p)];
+        uint4 ub = P[b];
+        float4 ob = O[b];
+        float2 qb = Q[b];
+        e += bond_energy(m, ct.pp64[ua.w * nt + ub.w], cw_rel(p), fx_r12(ua, ub, L), d1, q1, unit64(ob.x, ob.y, ob.z),
+                         unit64(ob.w, qb.x, qb.y));
+    }
+    return e;
+}
+
+// Whole-system bond energy: one thread per ID, its partners of smaller ID
+// (part1 = the larger ID, as TotalEMatrix::allToAll pairs (i, j < i)).
+__global__ void k_bond_all(int n, ChainTab ct, SysConst sc, const int* __restrict__ slot, const uint4* __restrict__ P,
+                           const float4* __restrict__ O, const float2* __restrict__ Q, double* __restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int a = slot[i];
+    out[i] = bonds_global(ct, sc.nt, sc.Ld, i, P[a], O[a], Q[a], 1 | 4, slot, P, O, Q);
+}
+
 // ------------------------------------------------------------ energy (all)
 // One wave per cell; pairs (i in home cell, j in stencil) with ID_j < ID_i
 // flattened over lanes.  The cheap stage (image, cutoff, segment distance,
@@ -419,6 +546,7 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
                                                        const uint4* __restrict__ P, const float4* __restrict__ O,
                                                        const float2* __restrict__ Q, const int* __restrict__ ID,
                                                        const PP32* __restrict__ pp, const TP32* __restrict__ tp,
+                                                       ChainTab ct, const int* __restrict__ slot,
                                                        double* __restrict__ out) {
     int qi = blockIdx.x;
     if (qi >= nq) return;
@@ -450,16 +578,22 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
             acc += (double)pair_energy<float>(pp[q.t * sc.nt + pb4.w], pa, pb, r, sc.sqmaxcut);
         }
     }
+    // chain partners, wherever they are (the query's ID is `exclude`)
+    if (ct.cw && lane == 0)
+        acc += bonds_global(ct, sc.nt, sc.Ld, q.exclude, make_uint4(q.u[0], q.u[1], q.u[2], (uint32_t)q.t), q.o, q.q,
+                            15, slot, P, O, Q);
     double s = wave_sum(acc);
     if (lane == 0) out[qi] = s;
 }
 
 // ------------------------------------------------------------ pair energies
+// p2p(i, j) with i's connectivity list: the non-bonded functor, plus the bond
+// and angle terms when j is one of i's chain partners
 template <typename R>
 __global__ void k_pairs(int m, const int* __restrict__ pi, const int* __restrict__ pj, const int* __restrict__ slot,
                         SysConst sc, const uint4* __restrict__ P, const float4* __restrict__ O,
                         const float2* __restrict__ Q, const PairParamsT<R>* __restrict__ pp,
-                        const TypeParamsT<R>* __restrict__ tp, R sqmaxcut, double* __restrict__ out) {
+                        const TypeParamsT<R>* __restrict__ tp, R sqmaxcut, ChainTab ct, double* __restrict__ out) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
     int a = slot[pi[k]], b = slot[pj[k]];
@@ -473,7 +607,15 @@ __global__ void k_pairs(int m, const int* __restrict__ pi, const int* __restrict
         V3<float> rf = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
         r = mk<R>((R)rf.x, (R)rf.y, (R)rf.z);
     }
-    out[k] = (double)pair_energy<R>(pp[ua.w * sc.nt + ub.w], pa, pb, r, sqmaxcut);
+    double e = (double)pair_energy<R>(pp[ua.w * sc.nt + ub.w], pa, pb, r, sqmaxcut);
+    if (ct.cw) {
+        int bit = cw_bit(pj[k] - pi[k]), cw = ct.cw[pi[k]];
+        if (bit >= 0 && ((cw >> bit) & 1))
+            e += bond_energy(ct.mol[cw >> 8], ct.pp64[ua.w * sc.nt + ub.w], cw_rel(bit), fx_r12(ua, ub, sc.Ld),
+                             unit64(O[a].x, O[a].y, O[a].z), unit64(O[a].w, Q[a].x, Q[a].y),
+                             unit64(O[b].x, O[b].y, O[b].z), unit64(O[b].w, Q[b].x, Q[b].y));
+    }
+    out[k] = e;
 }
 
 // ---------------------------------------------------------------- overlaps
@@ -575,6 +717,7 @@ struct TrialScratch {
     int* xl;     // expensive list, 2*cap entries
     float* xd;   // segment distance of each expensive entry, 2*cap
     float* mv;   // 2 * MV_STRIDE floats
+    int* bp;     // staged index of each chain partner (4 ints)
 };
 
 // sweep counters: STAT_REP replicas of 8 counters, STAT_STRIDE apart (128 B)
@@ -596,20 +739,32 @@ struct MlpConst {
 // screen that rejects (stage = the number of screens passed).  Otherwise eo/en
 // are the exact energies of the two states and des the sum of the screened
 // changes, which the final stage divides out: min(1, exp(-(dE - des)/T)).
+// Chain bonds are exact cheap terms (FP64, in the first screen); a move whose
+// bonded partner lies outside the staged stencil is rejected (stage -1): the
+// partner might be moving in another cell of the pass, and the rejection does
+// not depend on the move (the particle stays in its cell).
 template <int DA>
 __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un,
                                                const DPart<float>& po, const DPart<float>& pn, const SysConst& sc,
                                                const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
                                                const TrialScratch& T, const float* ml, const MlpRegs& mr,
-                                               const MlpConst& mc, double temper, double u1, double u2, double& eo,
-                                               double& en, double& des, int& stage) {
+                                               const MlpConst& mc, const ChainTab& ct, double temper, double u1,
+                                               double u2, double& eo, double& en, double& des, int& stage) {
     int lane = lane_id();
     int t = uo.w;
     bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
+    int idk = 0, cwk = 0;
+    if (ct.cw) {
+        idk = S.id[k];
+        cwk = ct.cw[idk];
+    }
+    int bmask = cwk & 15;
     if (lane == 0) {
         mv_store(T.mv, po);
         mv_store(T.mv + MV_STRIDE, pn);
     }
+    if (bmask && lane < 4) T.bp[lane] = -1;
+    if (bmask) __syncthreads();
     // phase A
     int nn = 0;
     for (int base = 0; base < S.n; base += WAVE) {
@@ -623,14 +778,39 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                 in_ = dot(rn, rn) <= sc.sqmaxcut;
             }
             fl = (io ? 1 : 0) | (in_ ? 2 : 0);
+            if (bmask) {
+                int bit = cw_bit(S.id[j] - idk);
+                if (bit >= 0 && ((bmask >> bit) & 1)) T.bp[bit] = j;
+            }
         }
         unsigned long long m = __ballot(fl != 0);
         if (fl) T.nl[nn + wave_prefix(m)] = j | (fl << 28);
         nn += __popcll(m);
     }
     __syncthreads();
-    // phase B
     double ao = 0.0, an = 0.0;
+    if (bmask) {
+        int found = 0;
+        for (int q = 0; q < 4; q++) found |= (T.bp[q] >= 0 ? 1 : 0) << q;
+        if (found != bmask) {
+            stage = -1;
+            return false;
+        }
+        // lanes 0-3: the old state with partners 0-3, lanes 4-7 the new one
+        int q = lane & 3, which = lane >> 2;
+        if (lane < 8 && ((bmask >> q) & 1)) {
+            int j = T.bp[q];
+            uint4 b = S.u[j];
+            float4 ob = S.o[j];
+            float2 qb = S.q[j];
+            const float* mv = T.mv + which * MV_STRIDE;
+            double eb = bond_energy(ct.mol[cwk >> 8], ct.pp64[t * sc.nt + b.w], cw_rel(q),
+                                    fx_r12(which ? un : uo, b, sc.Ld), unit64(mv[0], mv[1], mv[2]),
+                                    unit64(mv[3], mv[4], mv[5]), unit64(ob.x, ob.y, ob.z), unit64(ob.w, qb.x, qb.y));
+            if (which) an += eb; else ao += eb;
+        }
+    }
+    // phase B
     int nx = 0;
     V3<float> dold = po.dir, dnew = pn.dir;
     for (int base = 0; base < 2 * nn; base += WAVE) {
@@ -754,8 +934,8 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
                                            const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
                                            int ppmax, const int* __restrict__ nonempty,
                                            const unsigned char* __restrict__ mlp_img, const MlpConst& mc,
-                                           unsigned long long* __restrict__ stats, double* __restrict__ de_sum,
-                                           int* __restrict__ flag) {
+                                           const ChainTab& ct, unsigned long long* __restrict__ stats,
+                                           double* __restrict__ de_sum, int* __restrict__ flag) {
     extern __shared__ __align__(16) unsigned char smem[];
     int ax = b % mp.nact[0], ay = (b / mp.nact[0]) % mp.nact[1], az = b / (mp.nact[0] * mp.nact[1]);
     int cx = mp.ox + mp.p * ax, cy = mp.oy + mp.p * ay, cz = mp.oz + mp.p * az;
@@ -773,11 +953,12 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     T.xd = (float*)(T.xl + 2 * cap);
     T.mv = DA == DA_NET ? T.xd + 2 * cap : (float*)(T.xl + 2 * cap);
     int* scratch = (int*)(T.mv + 2 * MV_STRIDE);
+    T.bp = scratch + 2 * 125 + 2;
     int lane = lane_id();
     float* ml = nullptr;
     MlpRegs mr;
     if constexpr (DA == DA_NET) {
-        ml = (float*)(((uintptr_t)(scratch + 2 * 125 + 2) + 15) & ~(uintptr_t)15);
+        ml = (float*)(((uintptr_t)(T.bp + 4) + 15) & ~(uintptr_t)15);
         mlp_setup(mlp_img, ml, lane, mr);
     }
     if (!stage_load(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
@@ -801,7 +982,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         ntrial = fl + ((u01f(rd.x) < mean - (float)fl) ? 1 : 0);
     }
     unsigned long long n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0,
-                       n_s2try = 0, n_s2acc = 0;
+                       n_s2try = 0, n_s2acc = 0, n_bond = 0;
     double dsum = 0.0;
     for (int t = 0; t < ntrial; t++) {
         ctr.x = (uint32_t)t;
@@ -860,8 +1041,13 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         {
             DPart<float> po = load_part<float>(oo, qo, uo.w, tp);
             DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
-            pass = trial_energies<DA>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, mp.temper, u1, u2, eo,
-                                      en, des, stage);
+            pass = trial_energies<DA>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, mp.temper, u1, u2,
+                                      eo, en, des, stage);
+        }
+        if (stage < 0) {   // a chain partner outside the stencil
+            n_bond++;
+            __syncthreads();
+            continue;
         }
         if constexpr (DA != DA_OFF) {
             n_s1try++;
@@ -899,12 +1085,12 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     // counters: one atomic per lane into one of STAT_REP replicas (a cache
     // line each), so a thousand waves ending together do not queue on one line
     int rep = (int)(blockIdx.x & (STAT_REP - 1));
-    if (lane < (DA == DA_OFF ? 5 : DA == DA_REP ? 7 : 9)) {
+    if (lane < 10) {
         unsigned long long v = lane == 0 ? n_tacc : lane == 1 ? n_ttry : lane == 2 ? n_racc : lane == 3 ? n_rtry
                              : lane == 4 ? n_left : lane == 5 ? n_s1acc : lane == 6 ? n_s1try : lane == 7 ? n_s2acc
-                             : n_s2try;
-        atomicAdd(&stats[rep * STAT_STRIDE + lane], v);
-    } else if (lane == 9) {
+                             : lane == 8 ? n_s2try : n_bond;
+        if (v) atomicAdd(&stats[rep * STAT_STRIDE + lane], v);
+    } else if (lane == 10) {
         atomicAdd(&de_sum[rep * STAT_STRIDE], dsum);
     }
 }
@@ -916,10 +1102,10 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
                                                      const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
                                                      int ppmax, const int* __restrict__ nonempty,
                                                      const unsigned char* __restrict__ mlp_img, MlpConst mc,
-                                                     unsigned long long* __restrict__ stats, double* __restrict__ de_sum,
-                                                     int* __restrict__ flag) {
+                                                     ChainTab ct, unsigned long long* __restrict__ stats,
+                                                     double* __restrict__ de_sum, int* __restrict__ flag) {
     sweep_cell<DA>(xcd_slab_index((int)blockIdx.x, (int)gridDim.x), g, sc, mp, starts, P, O, Q, ID, pp, tp, cap, ppmax,
-                   nonempty, mlp_img, mc, stats, de_sum, flag);
+                   nonempty, mlp_img, mc, ct, stats, de_sum, flag);
 }
 
 // Per-replica parameters of one checkerboard pass of a replica batch.
@@ -944,7 +1130,7 @@ __global__ __launch_bounds__(WAVE) void k_sweep_batch(const RepPass* __restrict_
                                                       const TP32* __restrict__ tp, int cap, int ppmax,
                                                       const int* __restrict__ nonempty,
                                                       const unsigned char* __restrict__ mlp_img, MlpConst mc,
-                                                      unsigned long long* __restrict__ stats,
+                                                      ChainTab ct, unsigned long long* __restrict__ stats,
                                                       double* __restrict__ de_sum, int* __restrict__ flag) {
     int lb = xcd_slab_index((int)blockIdx.x, (int)gridDim.x);
     int lo = 0, hi = nrep - 1;
@@ -954,7 +1140,7 @@ __global__ __launch_bounds__(WAVE) void k_sweep_batch(const RepPass* __restrict_
     }
     const RepPass R = rp[lo];
     sweep_cell<DA>(lb - R.blk_off, R.g, R.sc, R.mp, starts + R.cell_off, P, O, Q, ID, pp, tp, cap, ppmax,
-                   nonempty + R.rep, mlp_img, mc, stats + (size_t)R.rep * STAT_REP * STAT_STRIDE,
+                   nonempty + R.rep, mlp_img, mc, ct, stats + (size_t)R.rep * STAT_REP * STAT_STRIDE,
                    de_sum + (size_t)R.rep * STAT_REP * STAT_STRIDE, flag);
 }
 
@@ -1280,6 +1466,18 @@ __global__ void k_cluster_hook(long m, const int* __restrict__ pi, const int* __
     atomicMin(&label[hi], lo);
     *changed = 1;
 }
+// ClusterSampler::sameCluster (clust.cpp:86-90): particles of one molecule
+// share a cluster; consecutive members are hooked like a bonded pair.
+__global__ void k_cluster_hook_mol(int n, const int* __restrict__ cw, int* __restrict__ label,
+                                   int* __restrict__ changed) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1 || !(cw[i] & CW_SAME_NEXT)) return;
+    int a = label[i], b = label[i + 1];
+    if (a == b) return;
+    int hi = a > b ? a : b, lo = a < b ? a : b;
+    atomicMin(&label[hi], lo);
+    *changed = 1;
+}
 __global__ void k_cluster_jump(int n, int* __restrict__ label, int* __restrict__ changed) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -1420,7 +1618,22 @@ struct sc_ctx {
     double* d_rep_e = nullptr;
     int rep_alloc = 0;
     int tuned_r = 2;                    // stencil radius chosen by the autotuner
+    // bonded chains ([Molecules] with more than one particle)
+    bool bonds = false;                 // some particle has a chain partner
+    bool molecules = false;             // some molecule has more than one particle
+    int* d_cw = nullptr;                // chain word per ID
+    MolP* d_mol = nullptr;              // per molecule type
+    double* d_bond = nullptr;           // per-ID bond energies (n)
+    int bond_cap = 0;
 };
+
+static ChainTab chain_tab(const sc_ctx* ctx) {
+    ChainTab t;
+    t.cw = ctx->bonds ? ctx->d_cw : nullptr;
+    t.mol = ctx->d_mol;
+    t.pp64 = ctx->d_pp64;
+    return t;
+}
 
 // Every HIP call of a library function goes through HIPCK, which records the
 // call (for error messages) and turns a failure into SC_EHIP.
@@ -1551,6 +1764,62 @@ static int upload_tables(sc_ctx* ctx) {
     HIPCK(hipMemcpy(ctx->d_tp64, t64.data(), t64.size() * sizeof(TP64), hipMemcpyHostToDevice));
     ctx->maxcut = ctx->topo.maxcut;
     ctx->sqmaxcut = ctx->topo.sqmaxcut;
+    return SC_OK;
+}
+
+// Chain words and molecule constants from the topology: particles in
+// [System] order, molecule after molecule (Inicializer::setParticlesParamss,
+// inicializer.cpp:409-450); the connectivity of ParticleVector::getConlist
+// (Conf.h:97-147).  A configuration of another layout cannot carry chains.
+static int upload_chains(sc_ctx* ctx) {
+    ctx->bonds = ctx->molecules = false;
+    const schost::Topology& T = ctx->topo;
+    bool multi = false;
+    for (const auto& m : T.molecules) multi |= m.types.size() > 1;
+    if (!multi) return SC_OK;
+    std::vector<int> cw;
+    std::vector<int> types;
+    for (const auto& sy : T.system)
+        for (size_t mi = 0; mi < T.molecules.size(); mi++) {
+            const schost::Molecule& m = T.molecules[mi];
+            if (m.name != sy.first) continue;
+            int size = (int)m.types.size();
+            bool nearb = size > 1 && (m.bond1c >= 0.0 || m.bonddc >= 0.0 || m.bondhc >= 0.0);
+            bool second = size > 1 && m.bond2c >= 0.0;
+            for (long c = 0; c < sy.second; c++)
+                for (int pos = 0; pos < size; pos++) {
+                    int w = (int)mi << 8;
+                    if (nearb && pos > 0) w |= 1;
+                    if (nearb && pos + 1 < size) w |= 2;
+                    if (second && pos > 1) w |= 4;
+                    if (second && pos + 2 < size) w |= 8;
+                    if (pos + 1 < size) w |= CW_SAME_NEXT;
+                    cw.push_back(w);
+                    types.push_back(m.types[pos]);
+                }
+        }
+    if ((int)cw.size() != ctx->n || types != ctx->ptypes)
+        return fail(ctx, SC_EINVAL, "the topology has chains: the configuration must follow its [System] layout");
+    std::vector<MolP> mp(T.molecules.size());
+    for (size_t mi = 0; mi < T.molecules.size(); mi++) {
+        const schost::Molecule& m = T.molecules[mi];
+        MolP& q = mp[mi];
+        q.c0 = q.eq0 = q.mod0 = q.mod1 = q.sp = q.c1 = q.eq1 = 0.0;
+        if (m.bond1c >= 0.0) { q.eq0 = m.bond1eq; q.c0 = m.bond1c; }
+        if (m.bonddc >= 0.0) { q.eq0 = 0.0; q.c0 = m.bonddc; q.mod0 = m.bonddeq; }
+        if (m.bondhc >= 0.0) { q.eq0 = 0.0; q.c0 = m.bondhc; q.mod0 = q.mod1 = q.sp = m.bondheq; }
+        if (m.bond2c >= 0.0) { q.eq1 = m.bond2eq; q.c1 = m.bond2c; }
+        q.a1c = m.angle1c; q.a1eq = m.angle1eq;
+        q.a2c = m.angle2c; q.a2eq = m.angle2eq;
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_cw, cw.size())) || (rc = dalloc(ctx, &ctx->d_mol, mp.size())) ||
+        (rc = dalloc(ctx, &ctx->d_bond, cw.size())))
+        return rc;
+    HIPCK(hipMemcpy(ctx->d_cw, cw.data(), cw.size() * sizeof(int), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->d_mol, mp.data(), mp.size() * sizeof(MolP), hipMemcpyHostToDevice));
+    ctx->molecules = true;
+    for (int w : cw) ctx->bonds |= (w & 15) != 0;
     return SC_OK;
 }
 
@@ -1686,7 +1955,7 @@ static int energy_all_impl(sc_ctx* ctx, const double* box, double* e) {
     const Grid& g = ctx->g;
     int cap = stage_cap(ctx, g, box);
     size_t lds = stage_bytes(cap) + sizeof(int) * (2 * QCAP) + sizeof(int) * (2 * 125 + 2);
-    if ((rc = ensure_out(ctx, 1))) return rc;
+    if ((rc = ensure_out(ctx, 2))) return rc;
     SysConst sc = sysconst(ctx, box);
     int c = ctx->cur;
     HIPCK(hipEventRecord(ctx->ev0, ctx->stream));
@@ -1694,9 +1963,16 @@ static int energy_all_impl(sc_ctx* ctx, const double* box, double* e) {
                        ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->partial, ctx->d_flag);
     HIPCK(hipEventRecord(ctx->ev1, ctx->stream));
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, ctx->stream, ctx->ncell, ctx->partial, ctx->d_out);
+    double eb[2] = {0.0, 0.0};
+    if (ctx->bonds) {   // chain bonds: FP64 per particle, its partners of smaller ID
+        hipLaunchKernelGGL(k_bond_all, dim3((ctx->n + 255) / 256), dim3(256), 0, ctx->stream, ctx->n, chain_tab(ctx),
+                           sc, ctx->slot, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_bond);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, ctx->stream, ctx->n, ctx->d_bond, ctx->d_out + 1);
+    }
     HIPCK(hipGetLastError());
-    HIPCK(hipMemcpyAsync(e, ctx->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipMemcpyAsync(eb, ctx->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if ((rc = check_flag(ctx, "energy_all"))) return rc;
+    *e = ctx->bonds ? eb[0] + eb[1] : eb[0];
     float ms = 0;
     HIPCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_ms = ms;
@@ -1774,7 +2050,7 @@ static void sc_destroy_impl(sc_ctx* ctx) {
                     ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, ctx->mlp_img, ctx->tr_x, ctx->tr_y, ctx->tr_e, ctx->act,
                     ctx->bx, ctx->d_loss, ctx->d_pcount, ctx->d_pi, ctx->d_pj, ctx->d_pe, ctx->d_label, ctx->d_q6,
                     ctx->d_changed, ctx->d_rg, ctx->d_rp, ctx->d_rep_nonempty, ctx->d_rep_stats, ctx->d_rep_dsum,
-                    ctx->d_rep_e};
+                    ctx->d_rep_e, ctx->d_cw, ctx->d_mol, ctx->d_bond};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
@@ -1907,6 +2183,7 @@ static int sc_set_config_impl(sc_ctx* ctx, int n, const double box[3], const dou
         HIPCK(hipMemcpy(ctx->ID[0], hid.data(), n * sizeof(int), hipMemcpyHostToDevice));
         HIPCK(hipMemcpy(ctx->slot, hid.data(), n * sizeof(int), hipMemcpyHostToDevice));
     }
+    if ((rc = upload_chains(ctx))) return rc;
     ctx->have_state = true;
     ctx->grid_valid = false;
     return build_grid(ctx, 2, nullptr, ctx->box);
@@ -2094,9 +2371,9 @@ static int sc_neighbor_pairs_impl(sc_ctx* ctx, double rc, int32_t* pairs, long c
     return SC_OK;
 }
 
-// ClusterSampler::sameCluster (clust.cpp:51-99) for one-particle molecules:
-// two particles share a cluster when their pair energy is <= threshold
-// (-0.10 in the reference); labels as ClusterSampler::genClusterList
+// ClusterSampler::sameCluster (clust.cpp:51-99): two particles share a
+// cluster when they belong to one molecule or their pair energy is <=
+// threshold (-0.10 in the reference); labels as ClusterSampler::genClusterList
 // (clust.cpp:105-163) leaves them: the smallest index of the component.
 // Pair energies in FP64 (k_pairs<double>) over all pairs within maxcut.
 static int sc_cluster_labels_impl(sc_ctx* ctx, double threshold, int32_t* labels, int* nclusters) {
@@ -2117,7 +2394,7 @@ static int sc_cluster_labels_impl(sc_ctx* ctx, double threshold, int32_t* labels
     if (m > 0)
         SC_LAUNCH(k_pairs<double>, dim3((unsigned)((m + 127) / 128)), dim3(128), 0, ctx->stream, (int)m, ctx->d_pi,
                   ctx->d_pj, ctx->slot, sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp64, ctx->d_tp64, ctx->sqmaxcut,
-                  ctx->d_pe);
+                  chain_tab(ctx), ctx->d_pe);
     SC_LAUNCH(k_iota, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_label);
     for (int it = 0; it < 4 * n + 64; it++) {
         int changed = 0;
@@ -2125,6 +2402,9 @@ static int sc_cluster_labels_impl(sc_ctx* ctx, double threshold, int32_t* labels
         if (m > 0)
             SC_LAUNCH(k_cluster_hook, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, m, ctx->d_pi,
                       ctx->d_pj, ctx->d_pe, threshold, ctx->d_label, ctx->d_changed);
+        if (ctx->molecules)
+            SC_LAUNCH(k_cluster_hook_mol, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_cw, ctx->d_label,
+                      ctx->d_changed);
         SC_LAUNCH(k_cluster_jump, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, ctx->d_label, ctx->d_changed);
         HIPCK(hipMemcpyAsync(&changed, ctx->d_changed, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         HIPCK(hipStreamSynchronize(ctx->stream));
@@ -2226,11 +2506,12 @@ static int sc_pair_energies_impl(sc_ctx* ctx, int m, const int32_t* i, const int
     int c = ctx->cur;
     if (fp64)
         hipLaunchKernelGGL(k_pairs<double>, dim3((m + 127) / 128), dim3(128), 0, ctx->stream, m, ctx->d_i, ctx->d_j,
-                           ctx->slot, sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp64, ctx->d_tp64, ctx->sqmaxcut, ctx->d_out);
+                           ctx->slot, sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp64, ctx->d_tp64, ctx->sqmaxcut,
+                           chain_tab(ctx), ctx->d_out);
     else
         hipLaunchKernelGGL(k_pairs<float>, dim3((m + 127) / 128), dim3(128), 0, ctx->stream, m, ctx->d_i, ctx->d_j,
                            ctx->slot, sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp32, ctx->d_tp32, (float)ctx->sqmaxcut,
-                           ctx->d_out);
+                           chain_tab(ctx), ctx->d_out);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(out, ctx->d_out, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCK(hipStreamSynchronize(ctx->stream));
@@ -2289,7 +2570,7 @@ static int run_queries(sc_ctx* ctx, int nq, const Query* hq, const int* slots_de
     int c = ctx->cur;
     hipLaunchKernelGGL(k_energy_query, dim3(nq), dim3(WAVE), 0, ctx->stream, nq, hq ? ctx->d_q : nullptr, slots_dev,
                        ctx->g, sc, ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
-                       ctx->d_out);
+                       chain_tab(ctx), ctx->slot, ctx->d_out);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync(out, ctx->d_out, nq * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCK(hipStreamSynchronize(ctx->stream));
@@ -2499,7 +2780,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
             return fail(ctx, SC_EINVAL, "the network screen needs a one-type system of the network's type");
         // LDS: stage + near/expensive lists + segment distances + moving particle
         // + stencil scratch (+ the network and its activation tile)
-        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + (da ? MLP_LDS_BYTES + 16 : 0);
+        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2 + 4) + (da ? MLP_LDS_BYTES + 16 : 0);
         size_t per = stage_bytes(1) + sizeof(int) * 3 + (da ? sizeof(float) * 2 : 0);
         int capmax = (int)((65536 - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
@@ -2540,18 +2821,18 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
             if (mode == DA_NET)
                 hipLaunchKernelGGL(k_sweep_pass<DA_NET>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
                                    ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
-                                   cap, ctx->ppmax, ctx->d_nonempty, ctx->mlp_img, ctx->mlp_c, ctx->d_stats,
-                                   ctx->d_dsum, ctx->d_flag);
+                                   cap, ctx->ppmax, ctx->d_nonempty, ctx->mlp_img, ctx->mlp_c, chain_tab(ctx),
+                                   ctx->d_stats, ctx->d_dsum, ctx->d_flag);
             else if (mode == DA_REP)
                 hipLaunchKernelGGL(k_sweep_pass<DA_REP>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
                                    ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
-                                   cap, ctx->ppmax, ctx->d_nonempty, nullptr, ctx->mlp_c, ctx->d_stats, ctx->d_dsum,
-                                   ctx->d_flag);
+                                   cap, ctx->ppmax, ctx->d_nonempty, nullptr, ctx->mlp_c, chain_tab(ctx), ctx->d_stats,
+                                   ctx->d_dsum, ctx->d_flag);
             else
                 hipLaunchKernelGGL(k_sweep_pass<DA_OFF>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
                                    ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
-                                   cap, ctx->ppmax, ctx->d_nonempty, nullptr, ctx->mlp_c, ctx->d_stats, ctx->d_dsum,
-                                   ctx->d_flag);
+                                   cap, ctx->ppmax, ctx->d_nonempty, nullptr, ctx->mlp_c, chain_tab(ctx), ctx->d_stats,
+                                   ctx->d_dsum, ctx->d_flag);
             if (ctx->profile) HIPCK(hipEventRecord(ctx->kev[2 * launches + 1], ctx->stream));
             launches++;
         }
@@ -2594,6 +2875,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         stats->stage2_acc = hs[7];
         stats->stage2_try = hs[8];
         stats->energy_change = ds;
+        stats->bond_rejected = hs[9];
     }
     return SC_OK;
 }
@@ -2701,6 +2983,7 @@ static int sc_batch_set_impl(sc_ctx* ctx, int nrep, const int32_t* n_per_rep, co
         tot += n_per_rep[q];
     }
     if (tot != ctx->n) return fail(ctx, SC_EINVAL, "replica sizes do not add up to the loaded particles");
+    if (ctx->molecules) return fail(ctx, SC_EINVAL, "replica batches hold atomic molecules only");
     ctx->nrep = nrep;
     ctx->rep_n.assign(n_per_rep, n_per_rep + nrep);
     ctx->rep_off.resize(nrep);
@@ -2755,7 +3038,7 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             double b[3] = {ctx->rep_box[q][0], ctx->rep_box[q][1], ctx->rep_box[q][2]};
             cap = std::max(cap, stage_cap_n(ctx->rep_n[q], ctx->minvol, ctx->rep_grid[q].g, b));
         }
-        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2);
+        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2 + 4);
         size_t per = stage_bytes(1) + sizeof(int) * 3;
         int capmax = (int)((65536 - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
@@ -2801,11 +3084,13 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             if (mode == DA_REP)
                 SC_LAUNCH(k_sweep_batch<DA_REP>, dim3(nblk[ci]), dim3(WAVE), lds, ctx->stream, rp, R, ctx->starts,
                           ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->ppmax,
-                          ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag);
+                          ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ChainTab{nullptr, nullptr, ctx->d_pp64},
+                          ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag);
             else
                 SC_LAUNCH(k_sweep_batch<DA_OFF>, dim3(nblk[ci]), dim3(WAVE), lds, ctx->stream, rp, R, ctx->starts,
                           ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->ppmax,
-                          ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag);
+                          ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ChainTab{nullptr, nullptr, ctx->d_pp64},
+                          ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag);
             launches++;
         }
         HIPCK(hipGetLastError());
@@ -2834,6 +3119,7 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             st.trans_acc = hs[0]; st.trans_try = hs[1]; st.rot_acc = hs[2]; st.rot_try = hs[3]; st.left_cell = hs[4];
             st.stage1_acc = hs[5]; st.stage1_try = hs[6]; st.stage2_acc = hs[7]; st.stage2_try = hs[8];
             st.energy_change = ds;
+            st.bond_rejected = hs[9];
         }
     return SC_OK;
 }

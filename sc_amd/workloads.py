@@ -70,6 +70,63 @@ def workload(name, n_override=None):
     return {"top": top_init(lines, counts), "box": np.array([L, L, L]), "types": types, "eta": eta, "name": name}
 
 
+# Self-assembling chains: one SPA head (sigma 1.2) bonded to two PSC tails
+# (L = 3) by harmonic bonds k = 100 kT/sigma^2, l0 = 1 sigma (bond1: the
+# head centre to the first tail's +dir end, each tail's -dir end to the next
+# tail's +dir end, HarmonicSc, paire.h:239-280).
+CHAIN_TYPES = ["H 1 SPA 1.0 1.2 1.0 0.2", "T 2 PSC 1.0 1.0 1.0 0.2 80.0 5.0 3.0 0.0"]
+CHAIN_MOLECULE = "CH: {\nbond1: 100 1.0\nparticles: 1\nparticles: 2\nparticles: 2\n}"
+
+
+def chain_topology(nchains):
+    return "\n".join(["[Types]"] + CHAIN_TYPES + ["[Molecules]", CHAIN_MOLECULE, "[System]", "CH %d" % nchains]) + "\n"
+
+
+def chain_volume():
+    """geometric volume of one head + two tails"""
+    return particle_volume(1.2, 0.0) + 2 * particle_volume(1.0, 3.0)
+
+
+def chain_lattice(nchains, eta, seed):
+    """Straight chains (head, tail, tail along +-z, bonds near their rest
+    length) on a lattice of a cubic box at packing fraction eta, with random
+    sense, patch directions, in-plane jitter and small tilts.  Returns (box, box-fraction
+    positions, dirs, patch dirs, types) in [System] order."""
+    rng = np.random.default_rng(seed)
+    L = (nchains * chain_volume() / eta) ** (1.0 / 3.0)
+    span = 9.1                                # head top (+0.6) to the far tail's cap (-8.5)
+    nz = max(1, int(L // (span + 0.2)))
+    nxy = int(np.ceil(np.sqrt(nchains / nz)))
+    a_xy, a_z = L / nxy, L / nz
+    if a_xy < 1.25:
+        raise ValueError("packing fraction too high for the chain lattice")
+    sites = [(ix, iy, iz) for iz in range(nz) for iy in range(nxy) for ix in range(nxy)]
+    pick = rng.permutation(len(sites))[:nchains]
+    pos, dirs, pd = [], [], []
+    for k in pick:
+        ix, iy, iz = sites[k]
+        u = 1.0 if rng.random() < 0.5 else -1.0
+        jit = (a_xy - 1.22) / 2
+        x = (ix + 0.5) * a_xy + rng.uniform(-jit, jit)
+        y = (iy + 0.5) * a_xy + rng.uniform(-jit, jit)
+        zb = iz * a_z + (a_z - span) / 2
+        c0 = zb + (8.5 if u > 0 else 0.6)     # head centre
+        for off in (0.0, -2.5, -6.5):           # head, tail 1, tail 2 centres along u
+            # small lateral jitter and tilt: no bead sits exactly on another's axis
+            jx, jy = rng.normal(0.0, 0.03, 2)
+            pos.append([x + jx, y + jy, c0 + u * off])
+            d = np.array([rng.normal(0.0, 0.03), rng.normal(0.0, 0.03), u])
+            d /= np.linalg.norm(d)
+            dirs.append(d)
+            ph = rng.uniform(0, 2 * np.pi)
+            q = np.array([np.cos(ph), np.sin(ph), 0.0])
+            q -= d * (q @ d)
+            pd.append(q / np.linalg.norm(q))
+    pos = np.array(pos) / L
+    types = np.tile(np.array([1, 2, 2], np.int32), nchains)
+    return np.full(3, L), pos - np.floor(pos), np.array(dirs), np.array(pd), types
+
+
 OPTIONS = """ptype = 1
 press = 0
 paralpress = 0
