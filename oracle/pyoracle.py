@@ -95,6 +95,10 @@ def lib():
         L.orc_mc_run.argtypes = [T, D, ctypes.c_int, D, D, D, ctypes.c_void_p, I, ctypes.c_long, ctypes.c_double,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(ctypes.c_long), D]
         L.orc_mc_run.restype = ctypes.c_double
+        L.orc_mc_run_npt.argtypes = [T, D, ctypes.c_int, D, D, D, ctypes.c_void_p, I, ctypes.c_long, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(ctypes.c_long), D,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_mc_run_npt.restype = ctypes.c_double
         L.orc_nematic_s2.argtypes = [ctypes.c_int, D]
         L.orc_nematic_s2.restype = ctypes.c_double
         _lib = L
@@ -348,6 +352,20 @@ def mc_run(topo, st, nsweeps, temper, transmx, rot_half, seed, energies=True):
     drift = lib().orc_mc_run(topo.ref, _dp(s.box), s.n, _dp(s.pos), _dp(s.dir), _dp(s.pd0), _opt(s.pd1),
                              _ip(s.types), int(nsweeps), float(temper), float(transmx), float(rot_half),
                              int(seed) & 0xFFFFFFFFFFFFFFFF, stats, ctypes.byref(ef) if energies else None)
+    return s, drift, list(stats), ef.value
+
+
+def mc_run_npt(topo, st, nsweeps, temper, transmx, rot_half, seed, ptype, press, edge_mx):
+    """mc_run plus one volume move per sweep; returns (new_state, drift,
+    stats [trans acc, try, rot acc, try, volume acc, try], e_final)."""
+    s = State(st.box.copy(), st.pos.copy(), st.dir.copy(), st.pd0.copy(), st.types.copy(),
+              None if st.pd1 is None else st.pd1.copy())
+    stats = (ctypes.c_long * 6)()
+    ef = ctypes.c_double(0.0)
+    drift = lib().orc_mc_run_npt(topo.ref, _dp(s.box), s.n, _dp(s.pos), _dp(s.dir), _dp(s.pd0), _opt(s.pd1),
+                                 _ip(s.types), int(nsweeps), float(temper), float(transmx), float(rot_half),
+                                 int(seed) & 0xFFFFFFFFFFFFFFFF, stats, ctypes.byref(ef), int(ptype), float(press),
+                                 float(edge_mx))
     return s, drift, list(stats), ef.value
 
 

@@ -1272,18 +1272,76 @@ static void grid_move(ogrid* g, const orc_part* p, int n, int i) {
     g->cell[i] = nc;
 }
 
+static double mc_run(const orc_topo* t, double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
+                     const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
+                     long* stats, double* e_final, int ptype, double press, double edge_mx);
+
 double orc_mc_run(const orc_topo* t, const double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
                   const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
                   long* stats, double* e_final) {
+    double b[3] = {box[0], box[1], box[2]};
+    return mc_run(t, b, n, pos, dir, pd0, pd1, type, nsweeps, temper, transmx, rot_half, seed, stats, e_final, -1,
+                  0.0, 0.0);
+}
+
+double orc_mc_run_npt(const orc_topo* t, double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
+                      const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
+                      long* stats, double* e_final, int ptype, double press, double edge_mx) {
+    return mc_run(t, box, n, pos, dir, pd0, pd1, type, nsweeps, temper, transmx, rot_half, seed, stats, e_final,
+                  ptype, press, edge_mx);
+}
+
+/* MoveCreator::pressureMove, movecreator.cpp:330-421 (ptype 0: one random
+ * edge, 1: isotropic); box fractions stay, so every position scales.
+ * Returns the change of the energy (without the pressure-volume terms) of an
+ * accepted move, 0 on rejection; *e_cur is the current whole-system energy. */
+static double pressure_move(const orc_topo* t, double box[3], const orc_part* p, int n, int ptype, double press,
+                            double temper, double edge_mx, uint64_t* rs, double* e_cur, long* vstats) {
+    double nb[3] = {box[0], box[1], box[2]}, enermove, e_new;
+    int k;
+    if (ptype == 0) {
+        double rsave = urand(rs), area, old;
+        int side = rsave < 1.0 / 3.0 ? 0 : (rsave < 2.0 / 3.0 ? 1 : 2);
+        area = nb[(side + 1) % 3] * nb[(side + 2) % 3];
+        old = nb[side];
+        nb[side] += edge_mx * (urand(rs) - 0.5);
+        if (nb[side] <= 0.0) { vstats[1]++; return 0.0; }
+        enermove = press * area * (nb[side] - old) - (double)n * temper * log(nb[side] / old);
+    } else {
+        double psch = edge_mx * (urand(rs) - 0.5), pvol = nb[0] * nb[1] * nb[2], pvoln;
+        for (k = 0; k < 3; k++) nb[k] += psch;
+        pvoln = nb[0] * nb[1] * nb[2];
+        enermove = press * (pvoln - pvol) - (double)n * temper * log(pvoln / pvol);
+    }
+    vstats[1]++;
+    /* the cell grid needs edges above the cutoff (the device's limit too) */
+    for (k = 0; k < 3; k++) if (!(nb[k] > t->maxcut)) return 0.0;
+    e_new = orc_energy_all_cells(t, nb, p, n);
+    enermove += e_new;
+    /* moveTry, movecreator.h:175-187 */
+    if (!(enermove <= *e_cur) && !(exp(-1.0 * (enermove - *e_cur) / temper) > urand(rs))) return 0.0;
+    vstats[0]++;
+    {
+        double de = e_new - *e_cur;
+        for (k = 0; k < 3; k++) box[k] = nb[k];
+        *e_cur = e_new;
+        return de;
+    }
+}
+
+static double mc_run(const orc_topo* t, double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
+                     const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
+                     long* stats, double* e_final, int ptype, double press, double edge_mx) {
     orc_part* p = load_all(t, n, pos, dir, pd0, pd1, type);
     ogrid g;
     uint64_t rs = seed * 0x2545F4914F6CDD1Dull + 12345u;
-    double e_start, e_end, edrift = 0.0;
+    double e_start, e_end, edrift = 0.0, e_cur = 0.0;
     long sweep, step;
     int i;
     stats[0] = stats[1] = stats[2] = stats[3] = 0;
+    if (ptype >= 0) stats[4] = stats[5] = 0;
     grid_build(&g, p, n, box, t->maxcut);
-    e_start = e_final ? orc_energy_all_cells(t, box, p, n) : 0.0;
+    e_start = (e_final || ptype >= 0) ? orc_energy_all_cells(t, box, p, n) : 0.0;
     for (sweep = 0; sweep < nsweeps; sweep++) {
         for (step = 0; step < n; step++) {
             /* MoveCreator::particleMove, movecreator.cpp:16-29 */
@@ -1328,8 +1386,17 @@ double orc_mc_run(const orc_topo* t, const double box[3], int n, double* pos, do
                 if (translate) grid_move(&g, p, n, target);
             }
         }
+        if (ptype >= 0) {   /* one volume move at the end of every sweep (shave = 1) */
+            long acc0 = stats[4];
+            e_cur = orc_energy_all_cells(t, box, p, n);
+            edrift += pressure_move(t, box, p, n, ptype, press, temper, edge_mx, &rs, &e_cur, stats + 4);
+            if (stats[4] != acc0) {
+                grid_free(&g);
+                grid_build(&g, p, n, box, t->maxcut);
+            }
+        }
     }
-    e_end = e_final ? orc_energy_all_cells(t, box, p, n) : 0.0;
+    e_end = (e_final || ptype >= 0) ? orc_energy_all_cells(t, box, p, n) : 0.0;
     if (e_final) *e_final = e_end;
     for (i = 0; i < n; i++) {
         pos[3 * i] = p[i].pos.x; pos[3 * i + 1] = p[i].pos.y; pos[3 * i + 2] = p[i].pos.z;

@@ -171,6 +171,15 @@ double orc_mc_run(const orc_topo* t, const double box[3], int n,
                   long nsweeps, double temper, double transmx, double rot_half,
                   uint64_t seed, long* stats, double* e_final);
 
+/* NPT: as orc_mc_run plus one volume move (MoveCreator::pressureMove,
+ * movecreator.cpp:330-421; ptype 0 one random edge, 1 isotropic) at the end
+ * of every sweep; box is updated in place; stats[4..5] = volume acc, tries.
+ * The drift check covers the energy changes of both kinds of move. */
+double orc_mc_run_npt(const orc_topo* t, double box[3], int n,
+                      double* pos, double* dir, double* pd0, double* pd1, const int* type,
+                      long nsweeps, double temper, double transmx, double rot_half,
+                      uint64_t seed, long* stats, double* e_final, int ptype, double press, double edge_mx);
+
 /* ---- observables ---- */
 /* nematic order S2 = -2 * middle eigenvalue of Q (Tests/sc35.c:8563-8603) */
 double orc_nematic_s2(int n, const double* dir);

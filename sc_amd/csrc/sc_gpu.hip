@@ -731,6 +731,7 @@ struct TrialScratch {
 
 struct MlpConst {
     float lsc, rcut, pdis, pswitch;
+    int tnet;   // dense type of the network's pair (the screen's other pairs count 0)
 };
 
 // Delayed acceptance (Christen & Fox 2005, in stages): with DA != DA_OFF
@@ -866,23 +867,28 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                 int e = t0 + (lane & (MLP_TILE - 1));
                 float f[MLP_NF];
                 int which = 0;
+                bool net = false;   // a pair of the network's type; other pairs screen with 0
                 if (e < nx) {
                     int w = T.xl[e];
                     int jj = w & 0x3fffffff;
                     which = (w >> 30) & 1;
                     uint4 b = S.u[jj];
-                    V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
-                                        : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-                    const float* mv = T.mv + which * MV_STRIDE;
-                    float4 ob = S.o[jj];
-                    float2 qb = S.q[jj];
-                    mlp_features(mv_vec(mv, 0), mv_vec(mv, 3), mk<float>(ob.x, ob.y, ob.z),
-                                 mk<float>(ob.w, qb.x, qb.y), r, T.xd[e], mc.lsc, mc.rcut, mc.pdis, mc.pswitch, f);
-                } else {
-                    for (int q = 0; q < MLP_NF; q++) f[q] = 0.0f;
+                    net = t == mc.tnet && (int)b.w == mc.tnet;
+                    if (net) {
+                        V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
+                                            : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+                        const float* mv = T.mv + which * MV_STRIDE;
+                        float4 ob = S.o[jj];
+                        float2 qb = S.q[jj];
+                        mlp_features(mv_vec(mv, 0), mv_vec(mv, 3), mk<float>(ob.x, ob.y, ob.z),
+                                     mk<float>(ob.w, qb.x, qb.y), r, T.xd[e], mc.lsc, mc.rcut, mc.pdis, mc.pswitch,
+                                     f);
+                    }
                 }
+                if (!net)
+                    for (int q = 0; q < MLP_NF; q++) f[q] = 0.0f;
                 float out = mlp_tile(mr, ml, mlp_slots(f, lane >> 4), lane);
-                if (lane < MLP_TILE && e < nx) {
+                if (lane < MLP_TILE && e < nx && net) {
                     if (which) sn += (double)out; else so += (double)out;
                 }
             }
@@ -2776,8 +2782,10 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         int cap = stage_cap(ctx, g, ctx->box);
         int mode = ctx->da_mode;
         bool da = mode == DA_NET;
-        if (da && (ctx->nt != 1 || ctx->dense2ref[0] != ctx->mlp_type))
-            return fail(ctx, SC_EINVAL, "the network screen needs a one-type system of the network's type");
+        if (da && (ctx->mlp_type < 0 || ctx->ref2dense[ctx->mlp_type] < 0))
+            return fail(ctx, SC_EINVAL, "the network screen needs the network's particle type in the system");
+        MlpConst mlpc = ctx->mlp_c;
+        mlpc.tnet = ctx->mlp_type >= 0 ? ctx->ref2dense[ctx->mlp_type] : -1;
         // LDS: stage + near/expensive lists + segment distances + moving particle
         // + stencil scratch (+ the network and its activation tile)
         size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2 + 4) + (da ? MLP_LDS_BYTES + 16 : 0);
@@ -2821,7 +2829,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
             if (mode == DA_NET)
                 hipLaunchKernelGGL(k_sweep_pass<DA_NET>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
                                    ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
-                                   cap, ctx->ppmax, ctx->d_nonempty, ctx->mlp_img, ctx->mlp_c, chain_tab(ctx),
+                                   cap, ctx->ppmax, ctx->d_nonempty, ctx->mlp_img, mlpc, chain_tab(ctx),
                                    ctx->d_stats, ctx->d_dsum, ctx->d_flag);
             else if (mode == DA_REP)
                 hipLaunchKernelGGL(k_sweep_pass<DA_REP>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
@@ -3367,6 +3375,7 @@ static MlpConst mlp_const_for(const schost::Ia& p) {
     c.rcut = (float)p.rcut;
     c.pdis = (float)p.pdis;
     c.pswitch = (float)p.pswitch;
+    c.tnet = -1;
     return c;
 }
 
@@ -3742,8 +3751,8 @@ static int sc_set_surrogate_impl(sc_ctx* ctx, int mode) {
     if (mode == DA_NET) {
         if (!ctx->mlp_has) return fail(ctx, SC_ESTATE, "no surrogate network");
         if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
-        if (ctx->nt != 1 || ctx->dense2ref[0] != ctx->mlp_type)
-            return fail(ctx, SC_EINVAL, "the network screen needs a one-type system of the network's type");
+        if (ctx->mlp_type < 0 || ctx->ref2dense[ctx->mlp_type] < 0)
+            return fail(ctx, SC_EINVAL, "the network screen needs the network's particle type in the system");
     }
     ctx->da_mode = mode;
     return SC_OK;

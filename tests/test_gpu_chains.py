@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import sc_amd
+from sc_amd import surrogate
 from sc_amd import workloads as W
 import pyoracle as po
 
@@ -177,36 +178,60 @@ def _bond_energy_per_bond(tp, st):
     return po.bond_energies(tp, st, i[nxt], i[nxt] - 1).mean()
 
 
-def test_chain_ensemble_matches_cpu_oracle():
-    """60 chains (180 beads) at eta 0.15, T* = 1.1, NVT: E/N, the mean bond
-    energy and the mean cluster size (molecules + attraction) of the GPU
-    checkerboard sweep and of the oracle's sequential Metropolis agree within
-    3 combined standard errors + 1 %."""
-    say("case: chain ensemble GPU vs CPU oracle")
+@pytest.fixture(scope="module")
+def chain_nvt_oracle():
+    """60 chains (180 beads) at eta 0.15, T* = 1.1: the oracle's sequential
+    Metropolis after 400 sweeps, then E/N, the mean bond energy and the mean
+    cluster size at the end of 40 blocks of 25 sweeps."""
     nch = 60
-    g = sc_amd.GpuSystem(0)
-    tp, st = chain_system(g, nch, 0.15, 11)
-    n = st.n
-    rot = ROT_HALF
-    # oracle
-    cur, _, _, _ = po.mc_run(tp, st, 400, 1.1, 0.3, rot, seed=21)
+    top = W.chain_topology(nch)
+    tp = po.Topo(top)
+    box, pos, d, q, t = W.chain_lattice(nch, 0.15, 11)
+    st = po.State(box, pos, d, q, t)
+    cur, _, _, _ = po.mc_run(tp, st, 400, 1.1, 0.3, ROT_HALF, seed=21)
     ec = []
     for blk in range(40):
-        cur, _, _, ef = po.mc_run(tp, cur, 25, 1.1, 0.3, rot, seed=200 + blk)
-        ec.append((ef / n, _bond_energy_per_bond(tp, cur), n / len(np.unique(po.cluster_labels(tp, cur)))))
-    ec = np.array(ec)
-    # device
-    g.sweep(400, 1.1, 0.3, rot, r=2, seed=5, counter=0)
+        cur, _, _, ef = po.mc_run(tp, cur, 25, 1.1, 0.3, ROT_HALF, seed=200 + blk)
+        ec.append((ef / st.n, _bond_energy_per_bond(tp, cur), st.n / len(np.unique(po.cluster_labels(tp, cur)))))
+    return nch, (box, pos, d, q, t), np.array(ec)
+
+
+@pytest.mark.parametrize("mode", [0, 2, 1])
+def test_chain_ensemble_matches_cpu_oracle(chain_nvt_oracle, mode):
+    """E/N, the mean bond energy and the mean cluster size (molecules +
+    attraction) of the GPU checkerboard sweep -- exact (0), with the
+    cheap-terms screen (2) or with the network screen trained on the tails'
+    PSC pair, heads screening with 0 (1) -- and of the oracle's sequential
+    Metropolis agree within 3 combined standard errors + 1 %."""
+    say("case: chain ensemble GPU vs CPU oracle, surrogate mode", mode)
+    nch, (box, pos, d, q, t), ec = chain_nvt_oracle
+    g = sc_amd.GpuSystem(0)
+    tp = po.Topo(W.chain_topology(nch))
+    g.load_topology(W.chain_topology(nch))
+    if mode == 1:
+        g.set_config(box, pos, d, q, t)
+        res = surrogate.train(g, 2, nsamples=1_000_000, epochs=4, seed=3)
+        say("network: held-out rmse %.4f (target std %.4f)" % (res["val_rmse"], res["target_std"]))
+    g.set_config(box, pos, d, q, t)
+    g.set_surrogate(mode)
+    n = len(t)
+    g.sweep(400, 1.1, 0.3, ROT_HALF, r=2, seed=5, counter=0)
     obs = []
-    brej = 0
+    tot = {}
     for blk in range(40):
-        s = g.sweep(25, 1.1, 0.3, rot, r=2, seed=5, counter=1000 + 25 * blk)
-        brej += s["bond_rejected"]
-        stg = dev_state(g, st.types)
+        s = g.sweep(25, 1.1, 0.3, ROT_HALF, r=2, seed=5, counter=1000 + 25 * blk)
+        for k, v in s.items():
+            tot[k] = tot.get(k, 0) + v
+        stg = dev_state(g, t)
         _, ncl = g.cluster_labels()
         obs.append((g.allToAll() / n, _bond_energy_per_bond(tp, stg), n / ncl))
     obs = np.array(obs)
-    say("bond-rejected moves: %d" % brej)
+    say("bond-rejected moves %d; screens %d/%d, %d/%d" % (tot["bond_rejected"], tot["stage1_acc"], tot["stage1_try"],
+                                                          tot["stage2_acc"], tot["stage2_try"]))
+    if mode:
+        assert tot["stage1_try"] > 0 and tot["stage1_acc"] < tot["stage1_try"]
+    if mode == 1:
+        assert tot["stage2_try"] > 0
     for c, name in enumerate(("E/N", "bond energy", "mean cluster size")):
         mg, sg = _blocked(obs[:, c])
         mc, scc = _blocked(ec[:, c])
@@ -234,4 +259,52 @@ def test_batches_refuse_chains():
     chain_system(g, 10, 0.1, 1)
     with pytest.raises(RuntimeError):
         g.batch_set(np.array([15, 15], np.int32), np.full((2, 3), 20.0))
+    g.close()
+
+
+def test_chain_npt_ensemble_matches_cpu_oracle():
+    """NPT (P* = 0.5, T* = 1.1, one isotropic volume move per sweep) of 60
+    chains: packing fraction and E/N of the GPU sweep + device volume moves
+    against the oracle's sequential Metropolis + pressureMove, both continuing
+    from one oracle-compressed state: equal within 3 combined standard errors
+    + 1 %."""
+    say("case: chain NPT ensemble GPU vs CPU oracle")
+    nch, press, temper, edge = 60, 0.5, 1.1, 0.3
+    top = W.chain_topology(nch)
+    tp = po.Topo(top)
+    box, pos, d, q, t = W.chain_lattice(nch, 0.15, 17)
+    st, _, _, _ = po.mc_run_npt(tp, po.State(box, pos, d, q, t), 1500, temper, 0.3, ROT_HALF, 5, 1, press, edge)
+    vchain = nch * W.chain_volume()
+    n = st.n
+    # oracle
+    cur, drift, _, _ = po.mc_run_npt(tp, st, 400, temper, 0.3, ROT_HALF, 7, 1, press, edge)
+    assert abs(drift) < 1e-8
+    ec = []
+    vacc = [0, 0]
+    for blk in range(40):
+        cur, _, s, ef = po.mc_run_npt(tp, cur, 25, temper, 0.3, ROT_HALF, 300 + blk, 1, press, edge)
+        vacc[0] += s[4]
+        vacc[1] += s[5]
+        ec.append((vchain / np.prod(cur.box), ef / n))
+    ec = np.array(ec)
+    # device
+    g = sc_amd.GpuSystem(0)
+    g.load_topology(top)
+    g.set_config(st.box, st.pos, st.dir, st.pd0, st.types)
+    gacc = 0
+    obs = []
+    for sw in range(1400):
+        g.sweep(1, temper, 0.3, ROT_HALF, r=2, seed=11, counter=sw)
+        a, b = g.volume_move(1, press, temper, edge, seed=12, counter=sw)
+        if sw >= 400:
+            gacc += a
+            if (sw - 400) % 25 == 24:
+                obs.append((vchain / np.prod(b), g.allToAll() / n))
+    obs = np.array(obs)
+    say("volume acceptance: gpu %.3f, cpu %.3f" % (gacc / 1000.0, vacc[0] / max(1, vacc[1])))
+    for c, name in enumerate(("packing fraction", "E/N")):
+        mg, sg = _blocked(obs[:, c])
+        mc, scc = _blocked(ec[:, c])
+        say("%s: gpu %.5f +- %.5f, cpu %.5f +- %.5f" % (name, mg, sg, mc, scc))
+        assert abs(mg - mc) <= 3.0 * np.hypot(sg, scc) + 0.01 * abs(mc), name
     g.close()
