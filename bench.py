@@ -48,6 +48,71 @@ def read_traffic(name):
     return None
 
 
+def npt_leg(a, transmx, rot_half):
+    """NPT self-assembly of bonded chains: SPA head + two PSC tails, harmonic
+    bonds, P* = 0.5, T* = 1.1, one isotropic volume move (two whole-system
+    energies) after every sweep; exact sweep, the cheap-terms screen and the
+    network screen (trained on the tails' PSC pair).  A step is one sweep
+    plus one volume move."""
+    import numpy as np
+    import sc_amd
+    from sc_amd import workloads as W
+    gn = sc_amd.GpuSystem(0)
+    P_, T_ = 0.5, 1.1
+    nch = a.npt_chains
+    boxc, posc, dc, qc, tc_ = W.chain_lattice(nch, 0.25, a.seed)
+    gn.load_topology(W.chain_topology(nch))
+    nb_ = len(tc_)
+
+    def npt_steps(k, mode, counter):
+        gn.set_surrogate(mode)
+        acc, tot = 0, {"sweep_s": 0.0, "volume_s": 0.0}
+        for s_ in range(k):
+            t_ = time.perf_counter()
+            st_ = gn.sweep(1, T_, transmx, rot_half, r=a.r, seed=a.seed + 3, counter=counter + s_)
+            t2_ = time.perf_counter()
+            for kk, v in st_.items():
+                tot[kk] = tot.get(kk, 0) + v
+            ok, bx = gn.volume_move(1, P_, T_, a.npt_edge, seed=a.seed + 4, counter=counter + s_)
+            tot["sweep_s"] += t2_ - t_
+            tot["volume_s"] += time.perf_counter() - t2_
+            acc += ok
+        return acc, tot, bx
+
+    npt = {"chains": nch, "particles": nb_, "press": P_, "temper": T_, "volume_move": "isotropic, every sweep",
+           "edge_mx": a.npt_edge, "start": "straight chains on a lattice at packing fraction 0.25"}
+    gn.set_config(boxc, posc, dc, qc, tc_)
+    npt_steps(a.npt_warmup, 0, 0)
+    bx0, p0, d0_, q0_ = gn.get_config()
+    npt["eta_start"] = nch * W.chain_volume() / float(np.prod(bx0))
+    legs_ = [("exact", 0)] + ([] if a.no_da else [("cheap_terms_screen", 2), ("network_screen", 1)])
+    for label, mode in legs_:
+        gn.set_config(bx0, p0, d0_, q0_, tc_)
+        if mode == 1:   # the tails' pair (PSC-PSC); pairs with a head screen with 0
+            from sc_amd import surrogate
+            npt["surrogate_training"] = surrogate.train(gn, 2, nsamples=a.samples, epochs=a.epochs,
+                                                        seed=a.seed + 5, log=log)
+            gn.set_config(bx0, p0, d0_, q0_, tc_)
+        npt_steps(5, mode, 10 ** 5)
+        gn.set_config(bx0, p0, d0_, q0_, tc_)
+        t1 = time.perf_counter()
+        acc, tot, bx = npt_steps(a.npt_steps, mode, 10 ** 6)
+        dt = time.perf_counter() - t1
+        tries = max(1, tot["trans_try"] + tot["rot_try"])
+        npt[label] = {"sweeps_per_s": a.npt_steps / dt, "ms_per_step": 1e3 * dt / a.npt_steps,
+                      "ms_sweep": 1e3 * tot["sweep_s"] / a.npt_steps, "ms_volume_move": 1e3 * tot["volume_s"] / a.npt_steps,
+                      "volume_acceptance": acc / float(a.npt_steps),
+                      "move_acceptance": (tot["trans_acc"] + tot["rot_acc"]) / tries,
+                      "bond_rejected": int(tot["bond_rejected"]), "left_cell": int(tot["left_cell"]),
+                      "eta_end": nch * W.chain_volume() / float(np.prod(bx)),
+                      "energy_per_particle": gn.allToAll() / nb_}
+        log("npt chains %s: %.2f steps/s (sweep + volume move), volume acc %.3f, eta %.4f -> %.4f" % (
+            label, npt[label]["sweeps_per_s"], npt[label]["volume_acceptance"], npt["eta_start"],
+            npt[label]["eta_end"]))
+    gn.close()
+    return npt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,9 +139,15 @@ def main():
     ap.add_argument("--npt-steps", type=int, default=200, help="timed NPT steps (one sweep + one volume move)")
     ap.add_argument("--npt-warmup", type=int, default=50)
     ap.add_argument("--npt-edge", type=float, default=0.1, help="edge_mx of the isotropic volume move")
+    ap.add_argument("--npt-only", action="store_true",
+                    help="run the NPT chain leg alone and print its result (profiling runs)")
     ap.add_argument("--ensemble", type=int, default=1000,
                     help="untimed sweeps after each timed leg over which E/N is averaged")
     a = ap.parse_args()
+    if a.npt_only:
+        import numpy as np
+        print(json.dumps({"npt_chains": npt_leg(a, 0.3, 20.0 / 180 * np.pi / 4)}), flush=True)
+        return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding); "
               "run with --gpus 1", file=sys.stderr)
@@ -205,60 +276,7 @@ def main():
             tried.sum()))
         gp.close()
 
-    # NPT self-assembly of bonded chains: SPA head + two PSC tails, harmonic
-    # bonds, P* = 0.5, T* = 1.1, one isotropic volume move (two whole-system
-    # energies) after every sweep; exact sweep, the cheap-terms screen and the
-    # network screen (trained on the tails' PSC pair)
-    npt = None
-    if not a.no_npt:
-        gn = sc_amd.GpuSystem(0)
-        P_, T_ = 0.5, 1.1
-        nch = a.npt_chains
-        boxc, posc, dc, qc, tc_ = W.chain_lattice(nch, 0.25, a.seed)
-        gn.load_topology(W.chain_topology(nch))
-        nb_ = len(tc_)
-
-        def npt_steps(k, mode, counter):
-            gn.set_surrogate(mode)
-            acc, tot = 0, {}
-            for s_ in range(k):
-                st_ = gn.sweep(1, T_, transmx, rot_half, r=a.r, seed=a.seed + 3, counter=counter + s_)
-                for kk, v in st_.items():
-                    tot[kk] = tot.get(kk, 0) + v
-                ok, bx = gn.volume_move(1, P_, T_, a.npt_edge, seed=a.seed + 4, counter=counter + s_)
-                acc += ok
-            return acc, tot, bx
-
-        npt = {"chains": nch, "particles": nb_, "press": P_, "temper": T_, "volume_move": "isotropic, every sweep",
-               "edge_mx": a.npt_edge, "start": "straight chains on a lattice at packing fraction 0.25"}
-        gn.set_config(boxc, posc, dc, qc, tc_)
-        npt_steps(a.npt_warmup, 0, 0)
-        bx0, p0, d0_, q0_ = gn.get_config()
-        npt["eta_start"] = nch * W.chain_volume() / float(np.prod(bx0))
-        legs_ = [("exact", 0)] + ([] if a.no_da else [("cheap_terms_screen", 2), ("network_screen", 1)])
-        for label, mode in legs_:
-            gn.set_config(bx0, p0, d0_, q0_, tc_)
-            if mode == 1:   # the tails' pair (PSC-PSC); pairs with a head screen with 0
-                from sc_amd import surrogate
-                npt["surrogate_training"] = surrogate.train(gn, 2, nsamples=a.samples, epochs=a.epochs,
-                                                            seed=a.seed + 5, log=log)
-                gn.set_config(bx0, p0, d0_, q0_, tc_)
-            npt_steps(5, mode, 10 ** 5)
-            gn.set_config(bx0, p0, d0_, q0_, tc_)
-            t1 = time.perf_counter()
-            acc, tot, bx = npt_steps(a.npt_steps, mode, 10 ** 6)
-            dt = time.perf_counter() - t1
-            tries = max(1, tot["trans_try"] + tot["rot_try"])
-            npt[label] = {"sweeps_per_s": a.npt_steps / dt, "ms_per_step": 1e3 * dt / a.npt_steps,
-                          "volume_acceptance": acc / float(a.npt_steps),
-                          "move_acceptance": (tot["trans_acc"] + tot["rot_acc"]) / tries,
-                          "bond_rejected": int(tot["bond_rejected"]), "left_cell": int(tot["left_cell"]),
-                          "eta_end": nch * W.chain_volume() / float(np.prod(bx)),
-                          "energy_per_particle": gn.allToAll() / nb_}
-            log("npt chains %s: %.2f steps/s (sweep + volume move), volume acc %.3f, eta %.4f -> %.4f" % (
-                label, npt[label]["sweeps_per_s"], npt[label]["volume_acceptance"], npt["eta_start"],
-                npt[label]["eta_end"]))
-        gn.close()
+    npt = None if a.no_npt else npt_leg(a, transmx, rot_half)
 
     cpu = None
     if not a.no_cpu:

@@ -72,6 +72,24 @@ int sc_load_config(sc_ctx* ctx, const char* config_init_text);
  * returns the length written (or needed, when buf is NULL / cap too small). */
 long sc_write_config(sc_ctx* ctx, char* buf, long cap);
 
+/* The run's report files, text per frame (the caller appends it to the file);
+ * each returns the length written (or needed, when buf is NULL / cap too small):
+ *   movie frame        Updater::dumpMovie (mc/updater.h:82-93): "N\nsweep s; box x y z\n" + Conf::draw
+ *   stat.dat line      " s; volume\n"                      (mc/updater.cpp:322)
+ *   energy.dat line    " s; allToAll  alignmentOrder \n"   (mc/updater.cpp:323, :466-482) */
+long sc_write_movie_frame(sc_ctx* ctx, long sweep, char* buf, long cap);
+long sc_write_stat_line(sc_ctx* ctx, long sweep, char* buf, long cap);
+long sc_write_energy_line(sc_ctx* ctx, long sweep, char* buf, long cap);
+/* Updater::alignmentOrder (mc/updater.cpp:466-482): sum of dir_i . dir_j over
+ * pairs with centre distance < 1.5 */
+int sc_alignment_order(sc_ctx* ctx, double* out);
+/* ClusterSampler::writeCluster without decoration (mc/clust.cpp:7-49): the
+ * frames of cluster_stat.dat (size histogram) and cluster.dat (clusters in
+ * order of their smallest member, with their energy = sum of p2p over their
+ * pairs, clust.h:100-111, and 1-based members); lengths as above. */
+int sc_write_clusters(sc_ctx* ctx, double threshold, long sweep, char* stat_buf, long stat_cap, char* cl_buf,
+                      long cl_cap, long* stat_len, long* cl_len);
+
 /* --- topology queries ---------------------------------------------------- */
 int sc_num_particles(const sc_ctx* ctx);
 int sc_particle_types(const sc_ctx* ctx, int32_t* types, int cap);
@@ -157,8 +175,11 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
 int sc_set_trial_mode(sc_ctx* ctx, int mode);
 /* Volume move (MoveCreator::pressureMove, movecreator.cpp:330-550) for
  * ptype 0 (one random edge) and 1 (isotropic); returns accepted=1/0 and the
- * new box.  Whole-system energies are FP32 pair terms with FP64 reduction; dE
- * is FP64. */
+ * new box.  Whole-system energies are FP32 pair terms (FP64 bonds) with FP64
+ * reduction; dE is FP64.  E_old is the energy carried through the sweeps
+ * since the last full evaluation (the reference's allToAll sums its cached
+ * pair energies likewise), re-evaluated every 16 volume moves and after any
+ * other change of the state; E_new is always evaluated in full. */
 int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
                    uint64_t counter, int* accepted, double box_out[3]);
 

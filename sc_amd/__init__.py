@@ -47,6 +47,13 @@ SIGNATURES = {
     "sc_get_option": (ctypes.c_int, [_P, ctypes.c_char_p, _D]),
     "sc_load_config": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "sc_write_config": (ctypes.c_long, [_P, ctypes.c_char_p, ctypes.c_long]),
+    "sc_write_movie_frame": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
+    "sc_write_stat_line": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
+    "sc_write_energy_line": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
+    "sc_alignment_order": (ctypes.c_int, [_P, _D]),
+    "sc_write_clusters": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_long, ctypes.c_char_p, ctypes.c_long,
+                                         ctypes.c_char_p, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
+                                         ctypes.POINTER(ctypes.c_long)]),
     "sc_num_particles": (ctypes.c_int, [_P]),
     "sc_particle_types": (ctypes.c_int, [_P, _I, ctypes.c_int]),
     "sc_maxcut": (ctypes.c_int, [_P, _D]),
@@ -198,6 +205,41 @@ class GpuSystem:
         buf = ctypes.create_string_buffer(n + 1)
         self._L.sc_write_config(self._h, buf, n + 1)
         return buf.value.decode()
+
+    def _text(self, fn, what, *args):
+        n = fn(self._h, *args, None, 0)
+        if n < 0:
+            self._ck(int(n), what)
+        buf = ctypes.create_string_buffer(n + 1)
+        fn(self._h, *args, buf, n + 1)
+        return buf.value.decode()
+
+    def movie_frame(self, sweep):
+        """one frame of the movie file (Updater::dumpMovie)"""
+        return self._text(self._L.sc_write_movie_frame, "write_movie_frame", int(sweep))
+
+    def stat_line(self, sweep):
+        """one line of stat.dat: sweep; volume"""
+        return self._text(self._L.sc_write_stat_line, "write_stat_line", int(sweep))
+
+    def energy_line(self, sweep):
+        """one line of energy.dat: sweep; allToAll  alignmentOrder"""
+        return self._text(self._L.sc_write_energy_line, "write_energy_line", int(sweep))
+
+    def alignment_order(self):
+        v = ctypes.c_double()
+        self._ck(self._L.sc_alignment_order(self._h, ctypes.byref(v)), "alignment_order")
+        return v.value
+
+    def cluster_files(self, sweep, threshold=-0.10):
+        """(cluster_stat.dat frame, cluster.dat frame) of ClusterSampler::writeCluster"""
+        ls, lc = ctypes.c_long(), ctypes.c_long()
+        self._ck(self._L.sc_write_clusters(self._h, float(threshold), int(sweep), None, 0, None, 0, ctypes.byref(ls),
+                                           ctypes.byref(lc)), "write_clusters")
+        bs, bc = ctypes.create_string_buffer(ls.value + 1), ctypes.create_string_buffer(lc.value + 1)
+        self._ck(self._L.sc_write_clusters(self._h, float(threshold), int(sweep), bs, ls.value + 1, bc, lc.value + 1,
+                                           ctypes.byref(ls), ctypes.byref(lc)), "write_clusters")
+        return bs.value.decode(), bc.value.decode()
 
     def particle_types(self):
         n = self._L.sc_num_particles(self._h)

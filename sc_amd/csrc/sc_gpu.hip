@@ -1498,6 +1498,23 @@ __global__ void k_iota(int n, int* __restrict__ v) {
     if (i < n) v[i] = i;
 }
 
+// Updater::alignmentOrder (updater.cpp:466-482): per pair within the list,
+// dir_i . dir_j when the centre distance is < 1.5 (strict), else 0; FP64
+// axes renormalised from the stored FP32 ones
+__global__ void k_align_pairs(long m, const int* __restrict__ pi, const int* __restrict__ pj,
+                              const int* __restrict__ slot, const uint4* __restrict__ P, const float4* __restrict__ O,
+                              double bx, double by, double bz, double* __restrict__ out) {
+    long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    int a = slot[pi[k]], b = slot[pj[k]];
+    double v = 0.0;
+    if (fx_r2_fp64(P[a], P[b], bx, by, bz) < 2.25) {
+        float4 oa = O[a], ob = O[b];
+        v = dot(unit64(oa.x, oa.y, oa.z), unit64(ob.x, ob.y, ob.z));
+    }
+    out[k] = v;
+}
+
 // Q = sum over particles of u u^T (6 unique components, FP64), per block
 __global__ void k_nematic_q(int n, const float4* __restrict__ O, double* __restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1631,6 +1648,13 @@ struct sc_ctx {
     MolP* d_mol = nullptr;              // per molecule type
     double* d_bond = nullptr;           // per-ID bond energies (n)
     int bond_cap = 0;
+    // whole-system energy carried from move to move (the counterpart of
+    // TotalEMatrix::allToAll, which sums the cached pair energies): set by a
+    // full evaluation, advanced by the sweeps' accepted dE, dropped when the
+    // state changes otherwise
+    bool e_valid = false;
+    double e_track = 0.0;
+    int e_age = 0;                      // volume moves since the last full evaluation
 };
 
 static ChainTab chain_tab(const sc_ctx* ctx) {
@@ -1853,17 +1877,22 @@ static int build_grid(sc_ctx* ctx, int r, const uint32_t shift[3], const double*
     return build_grid_w(ctx, r, shift, box, ctx->maxcut / r, r + 1);
 }
 
+// cells per axis of a grid with cells >= wmin wide and colour period p
+static int grid_axis_cells(double edge, double wmin, int p) {
+    int nc = (int)floor(edge / wmin);
+    if (nc > 1024) nc = 1024;
+    if (nc >= p) nc = (nc / p) * p;   // a multiple of the colour period
+    else nc = 1;                      // too small for a checkerboard: one cell
+    if (nc < 1) nc = 1;
+    return nc;
+}
+
 static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box, double wmin, int p) {
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     Grid g;
     g.r = r;
     for (int k = 0; k < 3; k++) {
-        int nc = (int)floor(box[k] / wmin);
-        if (nc > 1024) nc = 1024;
-        if (nc >= p) nc = (nc / p) * p;   // a multiple of the colour period
-        else nc = 1;                      // too small for a checkerboard: one cell
-        if (nc < 1) nc = 1;
-        g.nc[k] = nc;
+        g.nc[k] = grid_axis_cells(box[k], wmin, p);
         g.sh[k] = shift ? shift[k] : 0u;
     }
     int ncell = g.nc[0] * g.nc[1] * g.nc[2];
@@ -1954,9 +1983,28 @@ static int ensure_ij(sc_ctx* ctx, int m) {
     return SC_OK;
 }
 
+// Stencil radius of the whole-system energy's grid.  Radius 1 (cells >=
+// maxcut, 27-cell stencil) stages each record for ~16 home particles of a
+// cell; radius 2 (cells >= maxcut/2) culls more candidate pairs but a record
+// staged in LDS serves only ~2 home particles.
+#ifndef ENERGY_R
+#define ENERGY_R 1
+#endif
+
+// The grid of the whole-system energy for box `box`.  Positions are box
+// fractions: an unshifted grid of that radius with the cell counts the box
+// asks for already holds the right cells.
+static int energy_grid(sc_ctx* ctx, const double* box) {
+    bool reuse = ctx->grid_valid && ctx->g.r == ENERGY_R;
+    for (int k = 0; k < 3 && reuse; k++)
+        reuse = ctx->g.sh[k] == 0u &&
+                ctx->g.nc[k] == grid_axis_cells(box[k], ctx->maxcut / ENERGY_R, ENERGY_R + 1);
+    return reuse ? SC_OK : build_grid(ctx, ENERGY_R, nullptr, box);
+}
+
 static int energy_all_impl(sc_ctx* ctx, const double* box, double* e) {
     if (ctx->nrep > 0) return fail(ctx, SC_ESTATE, "the context holds a replica batch: use sc_batch_energies");
-    int rc = build_grid(ctx, 2, nullptr, box);
+    int rc = energy_grid(ctx, box);
     if (rc) return rc;
     const Grid& g = ctx->g;
     int cap = stage_cap(ctx, g, box);
@@ -2190,6 +2238,7 @@ static int sc_set_config_impl(sc_ctx* ctx, int n, const double box[3], const dou
         HIPCK(hipMemcpy(ctx->slot, hid.data(), n * sizeof(int), hipMemcpyHostToDevice));
     }
     if ((rc = upload_chains(ctx))) return rc;
+    ctx->e_valid = false;
     ctx->have_state = true;
     ctx->grid_valid = false;
     return build_grid(ctx, 2, nullptr, ctx->box);
@@ -2497,6 +2546,140 @@ int sc_nematic_order(sc_ctx* ctx, double* s2, double director[3]) {
     SC_GUARD(ctx, sc_nematic_order_impl(ctx, s2, director))
 }
 
+static int sc_alignment_order_impl(sc_ctx* ctx, double* out) {
+    if (!ctx || !out) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    long m = 0;
+    int rc = device_pairs(ctx, 1.5, &m);
+    if (rc) return rc;
+    *out = 0.0;
+    if (m == 0) return SC_OK;
+    if ((rc = ensure_out(ctx, 1))) return rc;
+    int c = ctx->cur;
+    SC_LAUNCH(k_align_pairs, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, ctx->stream, m, ctx->d_pi, ctx->d_pj,
+              ctx->slot, ctx->P[c], ctx->O[c], ctx->box[0], ctx->box[1], ctx->box[2], ctx->d_pe);
+    SC_LAUNCH(k_reduce, dim3(1), dim3(1024), 0, ctx->stream, (int)m, ctx->d_pe, ctx->d_out);
+    HIPCK(hipMemcpyAsync(out, ctx->d_out, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+int sc_alignment_order(sc_ctx* ctx, double* out) {
+    SC_GUARD(ctx, sc_alignment_order_impl(ctx, out))
+}
+
+static long put_text(const std::string& s, char* buf, long cap) {
+    if (buf && cap > (long)s.size()) memcpy(buf, s.c_str(), s.size() + 1);
+    return (long)s.size();
+}
+
+static long sc_write_movie_frame_impl(sc_ctx* ctx, long sweep, char* buf, long cap) {
+    if (!ctx) return SC_EINVAL;
+    int n = ctx->n;
+    std::vector<double> pos(3 * n), dir(3 * n), pd0(3 * n);
+    double box[3];
+    int rc = sc_get_config(ctx, box, pos.data(), dir.data(), pd0.data());
+    if (rc) return rc;
+    return put_text(schost::movieFrame(sweep, box, pos, dir, pd0, ctx->topo.particleMolecules()), buf, cap);
+}
+
+long sc_write_movie_frame(sc_ctx* ctx, long sweep, char* buf, long cap) {
+    SC_GUARD(ctx, sc_write_movie_frame_impl(ctx, sweep, buf, cap))
+}
+
+static long sc_write_stat_line_impl(sc_ctx* ctx, long sweep, char* buf, long cap) {
+    if (!ctx) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    return put_text(schost::statLine(sweep, ctx->box[0] * ctx->box[1] * ctx->box[2]), buf, cap);
+}
+
+long sc_write_stat_line(sc_ctx* ctx, long sweep, char* buf, long cap) {
+    SC_GUARD(ctx, sc_write_stat_line_impl(ctx, sweep, buf, cap))
+}
+
+static long sc_write_energy_line_impl(sc_ctx* ctx, long sweep, char* buf, long cap) {
+    if (!ctx) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    double e = 0, al = 0;
+    int rc = energy_all_impl(ctx, ctx->box, &e);
+    if (rc) return rc;
+    if ((rc = sc_alignment_order_impl(ctx, &al))) return rc;
+    return put_text(schost::energyLine(sweep, e, al), buf, cap);
+}
+
+long sc_write_energy_line(sc_ctx* ctx, long sweep, char* buf, long cap) {
+    SC_GUARD(ctx, sc_write_energy_line_impl(ctx, sweep, buf, cap))
+}
+
+static int sc_pair_energies_impl(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64);
+
+// ClusterSampler::writeCluster (clust.cpp:7-49): labels as sc_cluster_labels;
+// cluster energies as calcClusterEnergies (clust.h:100-111), the sum of p2p
+// (smaller index as part1) over the pairs of each cluster: pairs within
+// maxcut plus bonded pairs beyond it (FP64 pair energies).
+static int sc_write_clusters_impl(sc_ctx* ctx, double threshold, long sweep, char* stat_buf, long stat_cap,
+                                  char* cl_buf, long cl_cap, long* stat_len, long* cl_len) {
+    if (!ctx || !stat_len || !cl_len) return SC_EINVAL;
+    int n = ctx->n, ncl = 0;
+    std::vector<int32_t> lab(n);
+    int rc = sc_cluster_labels_impl(ctx, threshold, lab.data(), &ncl);
+    if (rc) return rc;
+    // pairs within maxcut and their FP64 energies, as the labelling left them
+    long m = 0;
+    if ((rc = device_pairs(ctx, ctx->maxcut, &m))) return rc;
+    std::vector<int32_t> hi(m), hj(m);
+    std::vector<double> he(m);
+    SysConst sc = sysconst(ctx, ctx->box);
+    int c = ctx->cur;
+    if (m > 0) {
+        SC_LAUNCH(k_pairs<double>, dim3((unsigned)((m + 127) / 128)), dim3(128), 0, ctx->stream, (int)m, ctx->d_pi,
+                  ctx->d_pj, ctx->slot, sc, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp64, ctx->d_tp64, ctx->sqmaxcut,
+                  chain_tab(ctx), ctx->d_pe);
+        HIPCK(hipMemcpyAsync(hi.data(), ctx->d_pi, m * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCK(hipMemcpyAsync(hj.data(), ctx->d_pj, m * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCK(hipMemcpyAsync(he.data(), ctx->d_pe, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCK(hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<double> eroot(n, 0.0);
+    std::vector<char> listed;
+    std::vector<int32_t> bi, bj;
+    if (ctx->bonds) {
+        // bonded pairs (i, i+1), (i, i+2) that the list lacks
+        std::vector<int> cw(n);
+        HIPCK(hipMemcpy(cw.data(), ctx->d_cw, n * sizeof(int), hipMemcpyDeviceToHost));
+        listed.assign(2 * (size_t)n, 0);
+        for (long k = 0; k < m; k++) {
+            int d = hj[k] - hi[k];
+            if (d == 1 || d == 2) listed[2 * (size_t)hi[k] + (d - 1)] = 1;
+        }
+        for (int i = 0; i < n; i++)
+            for (int d = 1; d <= 2; d++)
+                if (((cw[i] >> cw_bit(d)) & 1) && !listed[2 * (size_t)i + (d - 1)]) {
+                    bi.push_back(i);
+                    bj.push_back(i + d);
+                }
+    }
+    for (long k = 0; k < m; k++)
+        if (lab[hi[k]] == lab[hj[k]]) eroot[lab[hi[k]]] += he[k];
+    if (!bi.empty()) {
+        std::vector<double> be(bi.size());
+        if ((rc = sc_pair_energies_impl(ctx, (int)bi.size(), bi.data(), bj.data(), be.data(), 1))) return rc;
+        for (size_t k = 0; k < bi.size(); k++)
+            if (lab[bi[k]] == lab[bj[k]]) eroot[lab[bi[k]]] += be[k];
+    }
+    std::vector<int> labv(lab.begin(), lab.end());
+    std::string st, cl;
+    schost::clusterFiles(sweep, labv, eroot, st, cl);
+    *stat_len = put_text(st, stat_buf, stat_cap);
+    *cl_len = put_text(cl, cl_buf, cl_cap);
+    return SC_OK;
+}
+
+int sc_write_clusters(sc_ctx* ctx, double threshold, long sweep, char* stat_buf, long stat_cap, char* cl_buf,
+                      long cl_cap, long* stat_len, long* cl_len) {
+    SC_GUARD(ctx, sc_write_clusters_impl(ctx, threshold, sweep, stat_buf, stat_cap, cl_buf, cl_cap, stat_len, cl_len))
+}
+
 static int sc_pair_energies_impl(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64) {
     if (!ctx || m < 0 || (m > 0 && (!i || !j || !out))) return SC_EINVAL;
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
@@ -2558,6 +2741,7 @@ static int sc_commit_box_impl(sc_ctx* ctx, const double box_new[3]) {
         if (!(box_new[k] > 0)) return fail(ctx, SC_EINVAL, "box edges must be positive");
         ctx->box[k] = box_new[k];
     }
+    ctx->e_valid = false;
     return build_grid(ctx, 2, nullptr, ctx->box);
 }
 
@@ -2671,6 +2855,7 @@ static int sc_commit_particle_impl(sc_ctx* ctx, int target, const double pos[3],
     HIPCK(hipMemcpy(ctx->P[c] + s, &p, sizeof(uint4), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(ctx->O[c] + s, &q.o, sizeof(float4), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(ctx->Q[c] + s, &q.q, sizeof(float2), hipMemcpyHostToDevice));
+    ctx->e_valid = false;
     // the particle may have changed cell: rebuild the cell list
     return build_grid(ctx, ctx->grid_valid ? ctx->g.r : 2, nullptr, ctx->box);
 }
@@ -2766,6 +2951,8 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
     if (ctx->nrep > 0) return fail(ctx, SC_ESTATE, "the context holds a replica batch: use sc_batch_sweep");
     if (r == 0) r = ctx->tuned_r;
     HIPCK(hipSetDevice(ctx->device));
+    bool carried = ctx->e_valid;   // the carried energy survives only a sweep that completes
+    ctx->e_valid = false;
     HIPCK(hipMemsetAsync(ctx->d_stats, 0, STAT_REP * STAT_STRIDE * sizeof(unsigned long long), ctx->stream));
     HIPCK(hipMemsetAsync(ctx->d_dsum, 0, STAT_REP * STAT_STRIDE * sizeof(double), ctx->stream));
     int p = r + 1;
@@ -2861,6 +3048,10 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         for (int q = 0; q < STAT_REP; q++) hs[j] += hrep[q * STAT_STRIDE + j];
     }
     for (int q = 0; q < STAT_REP; q++) ds += drep[q * STAT_STRIDE];
+    if (carried) {
+        ctx->e_track += ds;
+        ctx->e_valid = true;
+    }
     float ms = 0;
     HIPCK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     ctx->last_ms = ms;
@@ -2992,6 +3183,7 @@ static int sc_batch_set_impl(sc_ctx* ctx, int nrep, const int32_t* n_per_rep, co
     }
     if (tot != ctx->n) return fail(ctx, SC_EINVAL, "replica sizes do not add up to the loaded particles");
     if (ctx->molecules) return fail(ctx, SC_EINVAL, "replica batches hold atomic molecules only");
+    ctx->e_valid = false;
     ctx->nrep = nrep;
     ctx->rep_n.assign(n_per_rep, n_per_rep + nrep);
     ctx->rep_off.resize(nrep);
@@ -3302,8 +3494,19 @@ static int sc_volume_move_impl(sc_ctx* ctx, int ptype, double press, double temp
     uint64_t h = mix64(seed ^ mix64(counter * 0xD1B54A32D192ED03ull + 7));
     auto u01 = [&]() { h = mix64(h); return (double)(h >> 11) * (1.0 / 9007199254740992.0); };
     double e_old = 0, e_new = 0;
-    int rc = energy_all_impl(ctx, ctx->box, &e_old);
-    if (rc) return rc;
+    int rc = SC_OK;
+    // E_old: the energy carried through the sweeps (as the reference's
+    // allToAll sums its cached pair energies), re-evaluated in full every 16
+    // volume moves and whenever it is not known
+    if (ctx->e_valid && ctx->e_age < 16) {
+        e_old = ctx->e_track;
+        ctx->e_age++;
+    } else {
+        if ((rc = energy_all_impl(ctx, ctx->box, &e_old))) return rc;
+        ctx->e_track = e_old;
+        ctx->e_valid = true;
+        ctx->e_age = 0;
+    }
     double nb[3] = {ctx->box[0], ctx->box[1], ctx->box[2]};
     double enermove;
     double npart = (double)ctx->n;
@@ -3330,8 +3533,10 @@ static int sc_volume_move_impl(sc_ctx* ctx, int ptype, double press, double temp
     *accepted = acc ? 1 : 0;
     if (acc) {
         for (int k = 0; k < 3; k++) ctx->box[k] = nb[k];
+        ctx->e_track = e_new;
+        ctx->e_age = 0;
     }
-    if ((rc = build_grid(ctx, 2, nullptr, ctx->box))) return rc;
+    if ((rc = energy_grid(ctx, ctx->box))) return rc;
     if (box_out) for (int k = 0; k < 3; k++) box_out[k] = ctx->box[k];
     return SC_OK;
 }

@@ -1,6 +1,7 @@
 // sc_host.cpp -- SC input formats and FP64 pair-parameter derivation (host).
 #include "sc_host.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -426,12 +427,11 @@ std::string parseConfig(const std::string& text, int n, double box[3], std::vect
     return "";
 }
 
-std::string drawConfig(const double box[3], const std::vector<double>& pos, const std::vector<double>& dir,
-                       const std::vector<double>& pd0, const std::vector<int>& mol) {
+// Conf::draw (structures/Conf.h:413-421): one line per particle
+std::string drawParticles(const double box[3], const std::vector<double>& pos, const std::vector<double>& dir,
+                          const std::vector<double>& pd0, const std::vector<int>& mol) {
     std::string out;
     char buf[512];
-    snprintf(buf, sizeof buf, "%.8f %.8f %.8f\n", box[0], box[1], box[2]);
-    out += buf;
     size_t n = pos.size() / 3;
     for (size_t i = 0; i < n; i++) {
         double r[3];
@@ -442,6 +442,80 @@ std::string drawConfig(const double box[3], const std::vector<double>& pos, cons
         out += buf;
     }
     return out;
+}
+
+// config.last (main.cpp:304-310): the box line, then Conf::draw
+std::string drawConfig(const double box[3], const std::vector<double>& pos, const std::vector<double>& dir,
+                       const std::vector<double>& pd0, const std::vector<int>& mol) {
+    char buf[128];
+    snprintf(buf, sizeof buf, "%15.8e %15.8e %15.8e\n", box[0], box[1], box[2]);
+    return std::string(buf) + drawParticles(box, pos, dir, pd0, mol);
+}
+
+// Updater::dumpMovie (mc/updater.h:82-93): particle count, sweep and box, then Conf::draw
+std::string movieFrame(long sweep, const double box[3], const std::vector<double>& pos, const std::vector<double>& dir,
+                       const std::vector<double>& pd0, const std::vector<int>& mol) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "%ld\nsweep %ld; box %.10f %.10f %.10f\n", (long)(pos.size() / 3), sweep, box[0], box[1],
+             box[2]);
+    return std::string(buf) + drawParticles(box, pos, dir, pd0, mol);
+}
+
+// stat.dat and energy.dat lines of Updater::simulate (mc/updater.cpp:322-323)
+std::string statLine(long sweep, double volume) {
+    char buf[128];
+    snprintf(buf, sizeof buf, " %ld; %.10f\n", sweep, volume);
+    return buf;
+}
+std::string energyLine(long sweep, double energy, double alignment) {
+    char buf[160];
+    snprintf(buf, sizeof buf, " %ld; %.10f  %f \n", sweep, energy, alignment);
+    return buf;
+}
+
+// ClusterSampler::writeCluster without decoration (mc/clust.cpp:7-49):
+// clusters in order of their smallest member (sortClusterList, :165-244),
+// cluster_stat.dat = "Sweep: s | Maximal size: m" + the size histogram
+// (printClusterStat, clust.h:161-176), cluster.dat = "Sweep: s | Number of
+// clusters: c" + per cluster its energy and 1-based members (printClusters,
+// clust.h:138-158).  labels[i] = the smallest index of i's cluster.
+void clusterFiles(long sweep, const std::vector<int>& labels, const std::vector<double>& energy_of_root,
+                  std::string& stat, std::string& clusters) {
+    size_t n = labels.size();
+    std::vector<std::vector<int>> members;
+    std::vector<int> root_of_cluster;
+    std::vector<int> cluster_of_root(n, -1);
+    for (size_t i = 0; i < n; i++) {
+        int r = labels[i];
+        if (cluster_of_root[r] < 0) {
+            cluster_of_root[r] = (int)members.size();
+            members.emplace_back();
+            root_of_cluster.push_back(r);
+        }
+        members[cluster_of_root[r]].push_back((int)i);
+    }
+    long maxc = 0;
+    for (auto& m : members) maxc = std::max(maxc, (long)m.size());
+    std::vector<long> hist(maxc, 0);
+    for (auto& m : members) hist[m.size() - 1]++;
+    char buf[128];
+    snprintf(buf, sizeof buf, "Sweep: %ld | Maximal size: %ld\n", sweep, maxc);
+    stat = buf;
+    for (long i = 0; i < maxc; i++) {
+        snprintf(buf, sizeof buf, "%5d\t%5ld\n", (int)(i + 1), hist[i]);
+        stat += buf;
+    }
+    snprintf(buf, sizeof buf, "Sweep: %ld | Number of clusters: %ld\n", sweep, (long)members.size());
+    clusters = buf;
+    for (size_t c = 0; c < members.size(); c++) {
+        snprintf(buf, sizeof buf, "%3d(%f):", (int)(c + 1), energy_of_root[root_of_cluster[c]]);
+        clusters += buf;
+        for (int p : members[c]) {
+            snprintf(buf, sizeof buf, "%5ld", (long)p + 1);
+            clusters += buf;
+        }
+        clusters += "\n";
+    }
 }
 
 void iaRow(const Ia& p, double* o) {

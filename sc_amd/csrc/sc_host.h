@@ -70,9 +70,23 @@ struct Options {
 // orthogonalised to dir and normalised.
 std::string parseConfig(const std::string& text, int n, double box[3], std::vector<double>& pos,
                         std::vector<double>& dir, std::vector<double>& pd0);
-// Conf::draw (structures/Conf.h:388-425) for one state.
+// Conf::draw (structures/Conf.h:388-425) for one state: the particle lines;
+// drawConfig prefixes the box line of config.last (main.cpp:304-310),
+// movieFrame the header of Updater::dumpMovie (mc/updater.h:82-93).
+std::string drawParticles(const double box[3], const std::vector<double>& pos, const std::vector<double>& dir,
+                          const std::vector<double>& pd0, const std::vector<int>& mol);
 std::string drawConfig(const double box[3], const std::vector<double>& pos, const std::vector<double>& dir,
-                       const std::vector<double>& pd0, const std::vector<int>& switched);
+                       const std::vector<double>& pd0, const std::vector<int>& mol);
+std::string movieFrame(long sweep, const double box[3], const std::vector<double>& pos, const std::vector<double>& dir,
+                       const std::vector<double>& pd0, const std::vector<int>& mol);
+// stat.dat / energy.dat lines (mc/updater.cpp:322-323)
+std::string statLine(long sweep, double volume);
+std::string energyLine(long sweep, double energy, double alignment);
+// cluster_stat.dat and cluster.dat frames of ClusterSampler::writeCluster
+// (mc/clust.cpp:7-49); energy_of_root[r] = energy of the cluster whose
+// smallest member is r
+void clusterFiles(long sweep, const std::vector<int>& labels, const std::vector<double>& energy_of_root,
+                  std::string& stat, std::string& clusters);
 
 // row of 44 doubles describing ia[ti][tj] (for parity dumps)
 void iaRow(const Ia& p, double* out44);
