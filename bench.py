@@ -58,6 +58,8 @@ def npt_leg(a, transmx, rot_half):
     import sc_amd
     from sc_amd import workloads as W
     gn = sc_amd.GpuSystem(0)
+    gn.set_block_waves(a.waves)
+    gn.set_trial_mode(a.trial_mode)
     P_, T_ = 0.5, 1.1
     nch = a.npt_chains
     boxc, posc, dc, qc, tc_ = W.chain_lattice(nch, 0.25, a.seed)
@@ -119,7 +121,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--workload", default="c4_tchpsc_65536")
-    ap.add_argument("--r", type=int, default=2, help="checkerboard stencil radius (1: 2^3 colours, 2: 3^3)")
+    ap.add_argument("--r", type=int, default=1, help="checkerboard stencil radius (1: 2^3 colours, 2: 3^3)")
+    ap.add_argument("--waves", type=int, default=0,
+                    help="waves per active cell in the sweep kernels (1, 4; 0: 4, and 1 with the network screen)")
     ap.add_argument("--trial-mode", type=int, default=1,
                     help="trials per active cell: 0 = its occupancy, 1 = dithered mean occupancy")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -170,6 +174,7 @@ def main():
         g.copy_probe(256 << 20, reps=3)
         log("copy probe: 3 x k_copy of 256 MiB")
     g.set_trial_mode(a.trial_mode)
+    g.set_block_waves(a.waves)
     if a.equil:
         t0 = time.time()
         g.sweep(a.equil, temper, transmx, rot_half, r=a.r, seed=a.seed + 1, counter=5 * 10 ** 6)
@@ -254,6 +259,7 @@ def main():
     if not a.no_pt:
         from sc_amd import pt
         gp = sc_amd.GpuSystem(0)
+        gp.set_block_waves(a.waves)
         t0 = time.time()
         nrep_n, temps, _ = pt.c3_batch(gp)
         tb = time.time() - t0

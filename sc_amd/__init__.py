@@ -47,6 +47,7 @@ SIGNATURES = {
     "sc_get_option": (ctypes.c_int, [_P, ctypes.c_char_p, _D]),
     "sc_load_config": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "sc_write_config": (ctypes.c_long, [_P, ctypes.c_char_p, ctypes.c_long]),
+    "sc_set_block_waves": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_write_movie_frame": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
     "sc_write_stat_line": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
     "sc_write_energy_line": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
@@ -517,6 +518,10 @@ class GpuSystem:
         d = np.zeros(3)
         self._ck(self._L.sc_nematic_order(self._h, ctypes.byref(s2), _d(d)), "nematic_order")
         return s2.value, d
+
+    def set_block_waves(self, nw):
+        """waves per active cell in the sweep kernels (1 or 4)"""
+        self._ck(self._L.sc_set_block_waves(self._h, int(nw)), "set_block_waves")
 
     def set_trial_mode(self, mode):
         self._ck(self._L.sc_set_trial_mode(self._h, int(mode)), "set_trial_mode")

@@ -34,6 +34,10 @@ typedef TypeParamsT<float> TP32;
 typedef TypeParamsT<double> TP64;
 
 #define WAVE 64
+#ifndef SWEEP_TABLES_IN_LDS
+#define SWEEP_TABLES_IN_LDS 1
+#endif
+#define SWEEP_TABLE_MAXT 4   // systems of up to this many types keep the tables in LDS
 #define MAX_TYPES 16
 
 // ------------------------------------------------------------------ helpers
@@ -46,6 +50,52 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 __device__ __forceinline__ int wave_prefix(unsigned long long mask) {
     return (int)__popcll(mask & ((1ull << lane_id()) - 1ull));
+}
+
+// Helpers for kernels whose blocks are NW waves working on one task (NW = 1:
+// plain wave code, no barriers).  Both take every thread of the block.
+// Sum over the block, the same value on every thread; waves combine in a
+// fixed order, so the result is deterministic.  red: NW doubles of LDS.
+template <int NW>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    if constexpr (NW == 1) {
+        return v;
+    } else {
+        if (lane_id() == 0) red[threadIdx.x >> 6] = v;
+        __syncthreads();
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; k++) s += red[k];
+        __syncthreads();
+        return s;
+    }
+}
+// Stable compaction step over a chunk of NW*64 items: the output index of
+// this thread's item (when flag), in thread order; total advances by the
+// chunk's count.  cnt: NW ints of LDS.
+template <int NW>
+__device__ __forceinline__ int block_compact(bool flag, int& total, int* cnt) {
+    unsigned long long m = __ballot(flag);
+    if constexpr (NW == 1) {
+        int pos = total + wave_prefix(m);
+        total += __popcll(m);
+        return pos;
+    } else {
+        int w = threadIdx.x >> 6;
+        if (lane_id() == 0) cnt[w] = __popcll(m);
+        __syncthreads();
+        int base = total, all = 0;
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            int c = cnt[k];
+            base += k < w ? c : 0;
+            all += c;
+        }
+        __syncthreads();
+        total += all;
+        return base + wave_prefix(m);
+    }
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin by ID (XCD = ID mod 8), each
@@ -220,44 +270,52 @@ __device__ __forceinline__ int stencil_size(const Grid& g) {
     return min(2 * g.r + 1, g.nc[0]) * min(2 * g.r + 1, g.nc[1]) * min(2 * g.r + 1, g.nc[2]);
 }
 
-// cooperative copy by one wave; scratch holds 2*ns+1 ints. Returns false on overflow.
+// cooperative copy by the block (NW waves; the stencil prefix by wave 0);
+// scratch holds 2*ns+2 ints.  Returns false on overflow.
+template <int NW>
 __device__ bool stage_load(Stage& S, int cap, const Grid& g, int cx, int cy, int cz, const int* __restrict__ starts,
                            const uint4* __restrict__ P, const float4* __restrict__ O, const float2* __restrict__ Q,
                            const int* __restrict__ ID, int* scratch) {
     int ns = stencil_size(g);
     int* pre = scratch;          // ns+1 prefix of counts
     int* cst = scratch + ns + 1; // cell start per stencil entry
-    int lane = lane_id();
-    int home_idx = -1;
-    // counts -> prefix (serial over chunks of 64)
-    int carry = 0;
-    for (int b = 0; b < ns; b += WAVE) {
-        int i = b + lane, cnt = 0, st = 0, hm = 0;
-        if (i < ns) {
-            int c = stencil_cell(g, cx, cy, cz, i, hm);
-            st = starts[c];
-            cnt = starts[c + 1] - st;
-            cst[i] = st;
+    int tid = (int)threadIdx.x;
+    if (tid < WAVE) {
+        int lane = tid;
+        int home_idx = -1;
+        // counts -> prefix (serial over chunks of 64)
+        int carry = 0;
+        for (int b = 0; b < ns; b += WAVE) {
+            int i = b + lane, cnt = 0, st = 0, hm = 0;
+            if (i < ns) {
+                int c = stencil_cell(g, cx, cy, cz, i, hm);
+                st = starts[c];
+                cnt = starts[c + 1] - st;
+                cst[i] = st;
+            }
+            // inclusive wave scan of cnt
+            int v = cnt;
+            for (int o = 1; o < WAVE; o <<= 1) {
+                int t = __shfl_up(v, o, WAVE);
+                if (lane >= o) v += t;
+            }
+            if (i < ns) pre[i] = carry + v - cnt;
+            unsigned long long hmask = __ballot(i < ns && hm);
+            if (hmask) home_idx = b + __ffsll((long long)hmask) - 1;
+            carry += __shfl(v, WAVE - 1, WAVE);
         }
-        // inclusive wave scan of cnt
-        int v = cnt;
-        for (int o = 1; o < WAVE; o <<= 1) {
-            int t = __shfl_up(v, o, WAVE);
-            if (lane >= o) v += t;
+        if (lane == 0) {
+            pre[ns] = carry;
+            scratch[2 * ns + 1] = home_idx;
         }
-        if (i < ns) pre[i] = carry + v - cnt;
-        unsigned long long hmask = __ballot(i < ns && hm);
-        if (hmask) home_idx = b + __ffsll((long long)hmask) - 1;
-        carry += __shfl(v, WAVE - 1, WAVE);
     }
-    if (lane == 0) pre[ns] = carry;
     __syncthreads();
-    int total = carry;
+    int total = pre[ns], home_idx = scratch[2 * ns + 1];
     S.n = total;
     S.h0 = pre[home_idx];
     S.h1 = pre[home_idx + 1];
     if (total > cap) return false;
-    for (int e = lane; e < total; e += WAVE) {
+    for (int e = tid; e < total; e += WAVE * NW) {
         // binary search the stencil entry holding element e
         int lo = 0, hi = ns - 1;
         while (lo < hi) {
@@ -436,7 +494,7 @@ __global__ __launch_bounds__(WAVE) void k_energy_all(Grid g, SysConst sc, const 
         if (lane == 0) partial[cell] = 0.0;
         return;
     }
-    if (!stage_load(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
+    if (!stage_load<1>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
         if (lane == 0) { atomicOr(flag, 1); partial[cell] = 0.0; }
         return;
     }
@@ -718,6 +776,8 @@ struct TrialScratch {
     float* xd;   // segment distance of each expensive entry, 2*cap
     float* mv;   // 2 * MV_STRIDE floats
     int* bp;     // staged index of each chain partner (4 ints)
+    int* cnt;    // block compaction counts (4 ints)
+    double* red; // block sums (4 doubles)
 };
 
 // sweep counters: STAT_REP replicas of 8 counters, STAT_STRIDE apart (128 B)
@@ -744,14 +804,14 @@ struct MlpConst {
 // bonded partner lies outside the staged stencil is rejected (stage -1): the
 // partner might be moving in another cell of the pass, and the rejection does
 // not depend on the move (the particle stays in its cell).
-template <int DA>
+template <int DA, int NW>
 __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un,
                                                const DPart<float>& po, const DPart<float>& pn, const SysConst& sc,
                                                const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
                                                const TrialScratch& T, const float* ml, const MlpRegs& mr,
                                                const MlpConst& mc, const ChainTab& ct, double temper, double u1,
                                                double u2, double& eo, double& en, double& des, int& stage) {
-    int lane = lane_id();
+    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
     int t = uo.w;
     bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
     int idk = 0, cwk = 0;
@@ -760,16 +820,16 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         cwk = ct.cw[idk];
     }
     int bmask = cwk & 15;
-    if (lane == 0) {
+    if (tid == 0) {
         mv_store(T.mv, po);
         mv_store(T.mv + MV_STRIDE, pn);
     }
-    if (bmask && lane < 4) T.bp[lane] = -1;
+    if (bmask && tid < 4) T.bp[tid] = -1;
     if (bmask) __syncthreads();
     // phase A
     int nn = 0;
-    for (int base = 0; base < S.n; base += WAVE) {
-        int j = base + lane, fl = 0;
+    for (int base = 0; base < S.n; base += BS) {
+        int j = base + tid, fl = 0;
         if (j < S.n && j != k) {
             uint4 b = S.u[j];
             V3<float> ro = fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
@@ -784,9 +844,8 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                 if (bit >= 0 && ((bmask >> bit) & 1)) T.bp[bit] = j;
             }
         }
-        unsigned long long m = __ballot(fl != 0);
-        if (fl) T.nl[nn + wave_prefix(m)] = j | (fl << 28);
-        nn += __popcll(m);
+        int pos = block_compact<NW>(fl != 0, nn, T.cnt);
+        if (fl) T.nl[pos] = j | (fl << 28);
     }
     __syncthreads();
     double ao = 0.0, an = 0.0;
@@ -797,9 +856,9 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             stage = -1;
             return false;
         }
-        // lanes 0-3: the old state with partners 0-3, lanes 4-7 the new one
-        int q = lane & 3, which = lane >> 2;
-        if (lane < 8 && ((bmask >> q) & 1)) {
+        // threads 0-3: the old state with partners 0-3, threads 4-7 the new one
+        int q = tid & 3, which = (tid >> 2) & 1;
+        if (tid < 8 && ((bmask >> q) & 1)) {
             int j = T.bp[q];
             uint4 b = S.u[j];
             float4 ob = S.o[j];
@@ -814,8 +873,8 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     // phase B
     int nx = 0;
     V3<float> dold = po.dir, dnew = pn.dir;
-    for (int base = 0; base < 2 * nn; base += WAVE) {
-        int it = base + lane, jj = 0, which = 0;
+    for (int base = 0; base < 2 * nn; base += BS) {
+        int it = base + tid, jj = 0, which = 0;
         bool need = false;
         float dseg2 = 0.0f;
         if (it < 2 * nn) {
@@ -841,13 +900,11 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                 if (which) an += (double)ev; else ao += (double)ev;
             }
         }
-        unsigned long long m = __ballot(need);
+        int pos = block_compact<NW>(need, nx, T.cnt);
         if (need) {
-            int pos = nx + wave_prefix(m);
             T.xl[pos] = jj | (which << 30);
             if constexpr (DA == DA_NET) T.xd[pos] = sqrtf(dseg2);
         }
-        nx += __popcll(m);
     }
     __syncthreads();
     // screens of the delayed acceptance: (1) the exact cheap terms already
@@ -856,14 +913,15 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     des = 0.0;
     stage = 2;
     if constexpr (DA != DA_OFF) {
-        double rep = wave_sum(an) - wave_sum(ao);
+        double rn = block_sum<NW>(an, T.red);
+        double rep = rn - block_sum<NW>(ao, T.red);
         stage = 0;
         if (!(rep <= 0.0 || exp(-rep / temper) > u1)) return false;
         double ds = 0.0;
         if constexpr (DA == DA_NET) {
             stage = 1;
             double so = 0.0, sn = 0.0;
-            for (int t0 = 0; t0 < nx; t0 += MLP_TILE) {
+            for (int t0 = (tid >> 6) * MLP_TILE; t0 < nx; t0 += MLP_TILE * NW) {
                 int e = t0 + (lane & (MLP_TILE - 1));
                 float f[MLP_NF];
                 int which = 0;
@@ -892,7 +950,8 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                     if (which) sn += (double)out; else so += (double)out;
                 }
             }
-            ds = wave_sum(sn) - wave_sum(so);
+            double dn = block_sum<NW>(sn, T.red);
+            ds = dn - block_sum<NW>(so, T.red);
             if (!(ds <= 0.0 || exp(-ds / temper) > u2)) return false;
         }
         des = rep + ds;
@@ -900,8 +959,8 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     }
     // phase C
     int items = nx * ppmax;
-    for (int base = 0; base < items; base += WAVE) {
-        int it = base + lane;
+    for (int base = 0; base < items; base += BS) {
+        int it = base + tid;
         if (it < items) {
             int e = it / ppmax, mpp = it - e * ppmax;
             int w = T.xl[e];
@@ -926,14 +985,14 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             if (which) an += (double)ev; else ao += (double)ev;
         }
     }
-    eo = wave_sum(ao);
-    en = wave_sum(an);
+    eo = block_sum<NW>(ao, T.red);
+    en = block_sum<NW>(an, T.red);
     return true;
 }
 
 // One active cell of a checkerboard pass (logical index b of the colour's
 // active cells): stage the stencil, run the cell's trial moves, write back.
-template <int DA>
+template <int DA, int NW>
 __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst& sc, const MoveParams& mp,
                                            const int* __restrict__ starts, uint4* __restrict__ P,
                                            float4* __restrict__ O, float2* __restrict__ Q, const int* __restrict__ ID,
@@ -960,15 +1019,32 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     T.mv = DA == DA_NET ? T.xd + 2 * cap : (float*)(T.xl + 2 * cap);
     int* scratch = (int*)(T.mv + 2 * MV_STRIDE);
     T.bp = scratch + 2 * 125 + 2;
-    int lane = lane_id();
+    T.cnt = T.bp + 4;
+    T.red = (double*)(((uintptr_t)(T.cnt + 4) + 7) & ~(uintptr_t)7);
+    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1);
     float* ml = nullptr;
     MlpRegs mr;
+    uintptr_t tail = (uintptr_t)(T.red + 4);
     if constexpr (DA == DA_NET) {
-        ml = (float*)(((uintptr_t)(T.bp + 4) + 15) & ~(uintptr_t)15);
+        ml = (float*)((tail + 15) & ~(uintptr_t)15);
         mlp_setup(mlp_img, ml, lane, mr);
+        tail = (uintptr_t)ml + MLP_LDS_BYTES;
     }
-    if (!stage_load(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
-        if (lane == 0) atomicOr(flag, 1);
+#if SWEEP_TABLES_IN_LDS
+    // the pair and type parameter tables in LDS: every pair term reads them,
+    // and a global (L1/L2) load per field would stall the few waves of a CU
+    if (sc.nt <= SWEEP_TABLE_MAXT) {
+        PP32* ppl = (PP32*)((tail + 15) & ~(uintptr_t)15);
+        TP32* tpl = (TP32*)(ppl + sc.nt * sc.nt);
+        const int w4 = (int)(sizeof(PP32) / 4), t4 = (int)(sizeof(TP32) / 4);
+        for (int e = tid; e < sc.nt * sc.nt * w4; e += WAVE * NW) ((int*)ppl)[e] = ((const int*)pp)[e];
+        for (int e = tid; e < sc.nt * t4; e += WAVE * NW) ((int*)tpl)[e] = ((const int*)tp)[e];
+        pp = ppl;
+        tp = tpl;
+    }
+#endif
+    if (!stage_load<NW>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
+        if (tid == 0) atomicOr(flag, 1);
         return;
     }
     const float Lx = sc.L[0], Ly = sc.L[1], Lz = sc.L[2];
@@ -1047,8 +1123,8 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         {
             DPart<float> po = load_part<float>(oo, qo, uo.w, tp);
             DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
-            pass = trial_energies<DA>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, mp.temper, u1, u2,
-                                      eo, en, des, stage);
+            pass = trial_energies<DA, NW>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, mp.temper, u1,
+                                          u2, eo, en, des, stage);
         }
         if (stage < 0) {   // a chain partner outside the stencil
             n_bond++;
@@ -1073,7 +1149,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         if (acc) {
             if (translate) n_tacc++; else n_racc++;
             dsum += de;
-            if (lane == 0) {
+            if (tid == 0) {
                 S.u[k] = un;
                 S.o[k] = on;
                 S.q[k] = qn;
@@ -1083,7 +1159,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     }
     // write the home cell back (same slots it was staged from)
     int gs = starts[cell];
-    for (int e = lane; e < nh; e += WAVE) {
+    for (int e = tid; e < nh; e += WAVE * NW) {
         P[gs + e] = S.u[S.h0 + e];
         O[gs + e] = S.o[S.h0 + e];
         Q[gs + e] = S.q[S.h0 + e];
@@ -1091,18 +1167,19 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     // counters: one atomic per lane into one of STAT_REP replicas (a cache
     // line each), so a thousand waves ending together do not queue on one line
     int rep = (int)(blockIdx.x & (STAT_REP - 1));
-    if (lane < 10) {
-        unsigned long long v = lane == 0 ? n_tacc : lane == 1 ? n_ttry : lane == 2 ? n_racc : lane == 3 ? n_rtry
-                             : lane == 4 ? n_left : lane == 5 ? n_s1acc : lane == 6 ? n_s1try : lane == 7 ? n_s2acc
-                             : lane == 8 ? n_s2try : n_bond;
-        if (v) atomicAdd(&stats[rep * STAT_STRIDE + lane], v);
-    } else if (lane == 10) {
+    if (tid < 10) {
+        unsigned long long v = tid == 0 ? n_tacc : tid == 1 ? n_ttry : tid == 2 ? n_racc : tid == 3 ? n_rtry
+                             : tid == 4 ? n_left : tid == 5 ? n_s1acc : tid == 6 ? n_s1try : tid == 7 ? n_s2acc
+                             : tid == 8 ? n_s2try : n_bond;
+        if (v) atomicAdd(&stats[rep * STAT_STRIDE + tid], v);
+    } else if (tid == 10) {
         atomicAdd(&de_sum[rep * STAT_STRIDE], dsum);
     }
 }
 
-template <int DA>
-__global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
+// NW waves per block work on one active cell (NW = 1 or 4)
+template <int DA, int NW>
+__global__ __launch_bounds__(WAVE * NW) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
                                                      uint4* __restrict__ P, float4* __restrict__ O,
                                                      float2* __restrict__ Q, const int* __restrict__ ID,
                                                      const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
@@ -1110,8 +1187,8 @@ __global__ __launch_bounds__(WAVE) void k_sweep_pass(Grid g, SysConst sc, MovePa
                                                      const unsigned char* __restrict__ mlp_img, MlpConst mc,
                                                      ChainTab ct, unsigned long long* __restrict__ stats,
                                                      double* __restrict__ de_sum, int* __restrict__ flag) {
-    sweep_cell<DA>(xcd_slab_index((int)blockIdx.x, (int)gridDim.x), g, sc, mp, starts, P, O, Q, ID, pp, tp, cap, ppmax,
-                   nonempty, mlp_img, mc, ct, stats, de_sum, flag);
+    sweep_cell<DA, NW>(xcd_slab_index((int)blockIdx.x, (int)gridDim.x), g, sc, mp, starts, P, O, Q, ID, pp, tp, cap,
+                       ppmax, nonempty, mlp_img, mc, ct, stats, de_sum, flag);
 }
 
 // Per-replica parameters of one checkerboard pass of a replica batch.
@@ -1128,8 +1205,8 @@ struct RepPass {
 // blocks are the replicas' active cells back to back; a block finds its
 // replica by binary search over blk_off (a replica without active cells
 // shares its offset with the next one, and the search takes the last).
-template <int DA>
-__global__ __launch_bounds__(WAVE) void k_sweep_batch(const RepPass* __restrict__ rp, int nrep,
+template <int DA, int NW>
+__global__ __launch_bounds__(WAVE * NW) void k_sweep_batch(const RepPass* __restrict__ rp, int nrep,
                                                       const int* __restrict__ starts, uint4* __restrict__ P,
                                                       float4* __restrict__ O, float2* __restrict__ Q,
                                                       const int* __restrict__ ID, const PP32* __restrict__ pp,
@@ -1145,9 +1222,9 @@ __global__ __launch_bounds__(WAVE) void k_sweep_batch(const RepPass* __restrict_
         if (rp[mid].blk_off <= lb) lo = mid; else hi = mid - 1;
     }
     const RepPass R = rp[lo];
-    sweep_cell<DA>(lb - R.blk_off, R.g, R.sc, R.mp, starts + R.cell_off, P, O, Q, ID, pp, tp, cap, ppmax,
-                   nonempty + R.rep, mlp_img, mc, ct, stats + (size_t)R.rep * STAT_REP * STAT_STRIDE,
-                   de_sum + (size_t)R.rep * STAT_REP * STAT_STRIDE, flag);
+    sweep_cell<DA, NW>(lb - R.blk_off, R.g, R.sc, R.mp, starts + R.cell_off, P, O, Q, ID, pp, tp, cap, ppmax,
+                       nonempty + R.rep, mlp_img, mc, ct, stats + (size_t)R.rep * STAT_REP * STAT_STRIDE,
+                       de_sum + (size_t)R.rep * STAT_REP * STAT_STRIDE, flag);
 }
 
 // cell list of a replica batch: the cell of every particle in its replica's
@@ -1655,6 +1732,7 @@ struct sc_ctx {
     bool e_valid = false;
     double e_track = 0.0;
     int e_age = 0;                      // volume moves since the last full evaluation
+    int sweep_nw = 0;                   // waves per active cell in the sweep kernels (1, 4; 0 = by mode)
 };
 
 static ChainTab chain_tab(const sc_ctx* ctx) {
@@ -1951,6 +2029,12 @@ static int stage_cap(const sc_ctx* ctx, const Grid& g, const double* box) {
 }
 
 static size_t stage_bytes(int cap) { return (size_t)cap * (16 + 16 + 8 + 4); }
+
+// LDS for the sweep's copy of the parameter tables (with alignment slack)
+static size_t sweep_table_bytes(int nt) {
+    if (!SWEEP_TABLES_IN_LDS || nt > SWEEP_TABLE_MAXT) return 0;
+    return (size_t)nt * nt * sizeof(PP32) + (size_t)nt * sizeof(TP32) + 16;
+}
 
 static int check_flag(sc_ctx* ctx, const char* what) {
     int f = 0;
@@ -2975,7 +3059,9 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         mlpc.tnet = ctx->mlp_type >= 0 ? ctx->ref2dense[ctx->mlp_type] : -1;
         // LDS: stage + near/expensive lists + segment distances + moving particle
         // + stencil scratch (+ the network and its activation tile)
-        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2 + 4) + (da ? MLP_LDS_BYTES + 16 : 0);
+        // (+ 80 B: chain partners, block compaction counts and block sums)
+        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + 80 + (da ? MLP_LDS_BYTES + 16 : 0) +
+                       sweep_table_bytes(ctx->nt);
         size_t per = stage_bytes(1) + sizeof(int) * 3 + (da ? sizeof(float) * 2 : 0);
         int capmax = (int)((65536 - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
@@ -3013,21 +3099,24 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
                 }
                 HIPCK(hipEventRecord(ctx->kev[2 * launches], ctx->stream));
             }
-            if (mode == DA_NET)
-                hipLaunchKernelGGL(k_sweep_pass<DA_NET>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
-                                   ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
-                                   cap, ctx->ppmax, ctx->d_nonempty, ctx->mlp_img, mlpc, chain_tab(ctx),
-                                   ctx->d_stats, ctx->d_dsum, ctx->d_flag);
-            else if (mode == DA_REP)
-                hipLaunchKernelGGL(k_sweep_pass<DA_REP>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
-                                   ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
-                                   cap, ctx->ppmax, ctx->d_nonempty, nullptr, ctx->mlp_c, chain_tab(ctx), ctx->d_stats,
-                                   ctx->d_dsum, ctx->d_flag);
-            else
-                hipLaunchKernelGGL(k_sweep_pass<DA_OFF>, dim3(nblk), dim3(WAVE), lds, ctx->stream, g, sc, mp,
-                                   ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,
-                                   cap, ctx->ppmax, ctx->d_nonempty, nullptr, ctx->mlp_c, chain_tab(ctx), ctx->d_stats,
-                                   ctx->d_dsum, ctx->d_flag);
+#define SWEEP_PASS(DAM, NWV, IMG, MC)                                                                          \
+    hipLaunchKernelGGL((k_sweep_pass<DAM, NWV>), dim3(nblk), dim3(WAVE * NWV), lds, ctx->stream, g, sc, mp,         \
+                       ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,     \
+                       ctx->ppmax, ctx->d_nonempty, IMG, MC, chain_tab(ctx), ctx->d_stats, ctx->d_dsum, ctx->d_flag)
+            // by mode: the network screen holds its MFMA operands in registers, which
+            // leaves room for one wave per SIMD, so it runs one wave per cell
+            int nw = ctx->sweep_nw ? ctx->sweep_nw : (mode == DA_NET ? 1 : 4);
+            if (mode == DA_NET) {
+                if (nw == 4) SWEEP_PASS(DA_NET, 4, ctx->mlp_img, mlpc);
+                else SWEEP_PASS(DA_NET, 1, ctx->mlp_img, mlpc);
+            } else if (mode == DA_REP) {
+                if (nw == 4) SWEEP_PASS(DA_REP, 4, nullptr, ctx->mlp_c);
+                else SWEEP_PASS(DA_REP, 1, nullptr, ctx->mlp_c);
+            } else {
+                if (nw == 4) SWEEP_PASS(DA_OFF, 4, nullptr, ctx->mlp_c);
+                else SWEEP_PASS(DA_OFF, 1, nullptr, ctx->mlp_c);
+            }
+#undef SWEEP_PASS
             if (ctx->profile) HIPCK(hipEventRecord(ctx->kev[2 * launches + 1], ctx->stream));
             launches++;
         }
@@ -3238,7 +3327,7 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             double b[3] = {ctx->rep_box[q][0], ctx->rep_box[q][1], ctx->rep_box[q][2]};
             cap = std::max(cap, stage_cap_n(ctx->rep_n[q], ctx->minvol, ctx->rep_grid[q].g, b));
         }
-        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2 + 4);
+        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + 80 + sweep_table_bytes(ctx->nt);
         size_t per = stage_bytes(1) + sizeof(int) * 3;
         int capmax = (int)((65536 - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
@@ -3281,16 +3370,20 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
         for (int ci = 0; ci < ncol; ci++) {
             if (nblk[ci] == 0) continue;
             const RepPass* rp = ctx->d_rp + (size_t)ci * R;
-            if (mode == DA_REP)
-                SC_LAUNCH(k_sweep_batch<DA_REP>, dim3(nblk[ci]), dim3(WAVE), lds, ctx->stream, rp, R, ctx->starts,
-                          ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->ppmax,
-                          ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ChainTab{nullptr, nullptr, ctx->d_pp64},
-                          ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag);
-            else
-                SC_LAUNCH(k_sweep_batch<DA_OFF>, dim3(nblk[ci]), dim3(WAVE), lds, ctx->stream, rp, R, ctx->starts,
-                          ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->ppmax,
-                          ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ChainTab{nullptr, nullptr, ctx->d_pp64},
-                          ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag);
+#define SWEEP_BATCH(DAM, NWV)                                                                                  \
+    SC_LAUNCH((k_sweep_batch<DAM, NWV>), dim3(nblk[ci]), dim3(WAVE * NWV), lds, ctx->stream, rp, R, ctx->starts,     \
+              ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->ppmax,                \
+              ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ChainTab{nullptr, nullptr, ctx->d_pp64}, ctx->d_rep_stats,  \
+              ctx->d_rep_dsum, ctx->d_flag)
+            int nw = ctx->sweep_nw ? ctx->sweep_nw : 4;
+            if (mode == DA_REP) {
+                if (nw == 4) SWEEP_BATCH(DA_REP, 4);
+                else SWEEP_BATCH(DA_REP, 1);
+            } else {
+                if (nw == 4) SWEEP_BATCH(DA_OFF, 4);
+                else SWEEP_BATCH(DA_OFF, 1);
+            }
+#undef SWEEP_BATCH
             launches++;
         }
         HIPCK(hipGetLastError());
@@ -3977,6 +4070,16 @@ static int sc_set_trial_mode_impl(sc_ctx* ctx, int mode) {
 
 int sc_set_trial_mode(sc_ctx* ctx, int mode) {
     SC_GUARD(ctx, sc_set_trial_mode_impl(ctx, mode))
+}
+
+static int sc_set_block_waves_impl(sc_ctx* ctx, int nw) {
+    if (!ctx || (nw != 0 && nw != 1 && nw != 4)) return SC_EINVAL;
+    ctx->sweep_nw = nw;
+    return SC_OK;
+}
+
+int sc_set_block_waves(sc_ctx* ctx, int nw) {
+    SC_GUARD(ctx, sc_set_block_waves_impl(ctx, nw))
 }
 
 static int sc_set_profiling_impl(sc_ctx* ctx, int on) {

@@ -318,3 +318,32 @@ def test_cluster_labels_and_nematic_order_match_oracle(dev):
     say("S2 gpu %.10f oracle %.10f" % (s2, s2r))
     assert abs(s2 - s2r) <= 1e-6   # directions are FP32 on the device
     assert abs(np.linalg.norm(director) - 1.0) < 1e-9
+
+
+@pytest.mark.parametrize("mode", [0, 2])
+def test_block_waves_reproduce_wave_sweep(dev, mode):
+    """The sweep with 4 waves per active cell follows the 1-wave sweep move
+    for move: the same trial moves, pair lists and decisions; only the order of
+    the FP64 sums differs (~1e-16 relative), which could flip a Metropolis
+    decision only for a uniform within that of its threshold."""
+    say("case: 4-wave vs 1-wave sweep, surrogate mode", mode)
+    w = W.workload("c1_psc_1000")
+    out = []
+    for nw in (1, 4):
+        g = sc_amd.GpuSystem(dev)
+        g.load_topology(w["top"])
+        pos, d, q = g.generate_rsa(w["types"], w["box"], seed=23)
+        g.set_config(w["box"], pos, d, q)
+        g.set_block_waves(nw)
+        g.set_surrogate(mode)
+        s = g.sweep(10, 1.0, 0.3, 20.0 / 180 * np.pi / 4, r=1, seed=5, counter=0)
+        _, p, dd, qq = g.get_config()
+        out.append((s, p, dd, qq))
+        g.close()
+    (s1, p1, d1, q1), (s4, p4, d4, q4) = out
+    for k in ("trans_acc", "trans_try", "rot_acc", "rot_try", "left_cell", "stage1_acc", "stage1_try"):
+        assert s1[k] == s4[k], k
+    assert abs(s1["energy_change"] - s4["energy_change"]) <= 1e-9 * max(1.0, abs(s1["energy_change"]))
+    np.testing.assert_array_equal(p1, p4)
+    np.testing.assert_array_equal(d1, d4)
+    np.testing.assert_array_equal(q1, q4)
