@@ -2002,6 +2002,13 @@ static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const doubl
     return SC_OK;
 }
 
+// gfx950 gives a workgroup up to 160 KiB of LDS (as much as a CU has).  The
+// sweep keeps a block's image under 80 KiB, so two blocks share a CU (the
+// register budget allows two waves per SIMD); a stencil holds at most
+// STAGE_CAP_MAX records (~2.5x the mean of the densest systems built here).
+#define SWEEP_LDS_BUDGET 81920
+#define STAGE_CAP_MAX 1408
+
 // stage capacity for a stencil: ~3x the mean occupancy (local packing cannot
 // exceed ~2.6x the mean at the fluid densities used), bounded by the system
 static int stage_cap_n(int n, double minvol, const Grid& g, const double* box) {
@@ -2012,7 +2019,7 @@ static int stage_cap_n(int n, double minvol, const Grid& g, const double* box) {
     int cap = (int)std::min(std::max(3.0 * mean + 64.0, 128.0), pack + 64.0);
     cap = std::max(cap, 64);
     cap = std::min(cap, n + 64);
-    cap = std::min(cap, 1152);   // keeps a wave's LDS image under 64 KiB
+    cap = std::min(cap, STAGE_CAP_MAX);
     return ((cap + 63) / 64) * 64;
 }
 
@@ -2024,11 +2031,12 @@ static int stage_cap(const sc_ctx* ctx, const Grid& g, const double* box) {
     int cap = (int)std::min(std::max(3.0 * mean + 64.0, 128.0), pack + 64.0);
     cap = std::max(cap, 64);
     cap = std::min(cap, ctx->n + 64);
-    cap = std::min(cap, 1152);   // keeps a wave's LDS image under 64 KiB
+    cap = std::min(cap, STAGE_CAP_MAX);
     return ((cap + 63) / 64) * 64;
 }
 
 static size_t stage_bytes(int cap) { return (size_t)cap * (16 + 16 + 8 + 4); }
+
 
 // LDS for the sweep's copy of the parameter tables (with alignment slack)
 static size_t sweep_table_bytes(int nt) {
@@ -3063,7 +3071,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + 80 + (da ? MLP_LDS_BYTES + 16 : 0) +
                        sweep_table_bytes(ctx->nt);
         size_t per = stage_bytes(1) + sizeof(int) * 3 + (da ? sizeof(float) * 2 : 0);
-        int capmax = (int)((65536 - fixed) / per);
+        int capmax = (int)((SWEEP_LDS_BUDGET - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
         size_t lds = per * cap + fixed;
         int ncol = p * p * p;
@@ -3329,7 +3337,7 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
         }
         size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + 80 + sweep_table_bytes(ctx->nt);
         size_t per = stage_bytes(1) + sizeof(int) * 3;
-        int capmax = (int)((65536 - fixed) / per);
+        int capmax = (int)((SWEEP_LDS_BUDGET - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
         size_t lds = per * cap + fixed;
         for (int ci = 0; ci < ncol; ci++) {
