@@ -153,7 +153,7 @@ def main():
         print(json.dumps({"npt_chains": npt_leg(a, 0.3, 20.0 / 180 * np.pi / 4)}), flush=True)
         return
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding); "
+        print("bench.py runs on one GPU only: the sweep has no multi-GPU path (spatial sharding is designed, not built); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
