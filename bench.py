@@ -137,7 +137,7 @@ def main():
     ap.add_argument("--no-exact", action="store_true", help="delayed-acceptance sweep only")
     ap.add_argument("--no-pt", action="store_true", help="skip the batched parallel-tempering leg")
     ap.add_argument("--pt-sweeps", type=int, default=100)
-    ap.add_argument("--equil", type=int, default=1000, help="untimed exact sweeps before the legs")
+    ap.add_argument("--equil", type=int, default=4000, help="untimed exact sweeps before the legs")
     ap.add_argument("--no-npt", action="store_true", help="skip the NPT chain leg")
     ap.add_argument("--npt-chains", type=int, default=43690, help="3-bead chains of the NPT leg (131,070 beads)")
     ap.add_argument("--npt-steps", type=int, default=200, help="timed NPT steps (one sweep + one volume move)")

@@ -2006,12 +2006,15 @@ static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const doubl
 // sweep keeps a block's image under 80 KiB, so two blocks share a CU (the
 // register budget allows two waves per SIMD); a stencil holds at most
 // STAGE_CAP_MAX records (~2.5x the mean of the densest systems built here).
+// The one-wave whole-system energy kernel keeps to ENERGY_CAP_MAX records
+// (~52 KiB), so three of its blocks share a CU.
 #define SWEEP_LDS_BUDGET 81920
 #define STAGE_CAP_MAX 1408
+#define ENERGY_CAP_MAX 1152
 
 // stage capacity for a stencil: ~3x the mean occupancy (local packing cannot
 // exceed ~2.6x the mean at the fluid densities used), bounded by the system
-static int stage_cap_n(int n, double minvol, const Grid& g, const double* box) {
+static int stage_cap_n(int n, double minvol, const Grid& g, const double* box, int capmax = STAGE_CAP_MAX) {
     double cellvol = (box[0] / g.nc[0]) * (box[1] / g.nc[1]) * (box[2] / g.nc[2]);
     int ns = std::min(2 * g.r + 1, g.nc[0]) * std::min(2 * g.r + 1, g.nc[1]) * std::min(2 * g.r + 1, g.nc[2]);
     double mean = (double)n / (double)(g.nc[0] * g.nc[1] * g.nc[2]) * ns;
@@ -2019,11 +2022,11 @@ static int stage_cap_n(int n, double minvol, const Grid& g, const double* box) {
     int cap = (int)std::min(std::max(3.0 * mean + 64.0, 128.0), pack + 64.0);
     cap = std::max(cap, 64);
     cap = std::min(cap, n + 64);
-    cap = std::min(cap, STAGE_CAP_MAX);
+    cap = std::min(cap, capmax);
     return ((cap + 63) / 64) * 64;
 }
 
-static int stage_cap(const sc_ctx* ctx, const Grid& g, const double* box) {
+static int stage_cap(const sc_ctx* ctx, const Grid& g, const double* box, int capmax = STAGE_CAP_MAX) {
     double cellvol = (box[0] / g.nc[0]) * (box[1] / g.nc[1]) * (box[2] / g.nc[2]);
     int ns = std::min(2 * g.r + 1, g.nc[0]) * std::min(2 * g.r + 1, g.nc[1]) * std::min(2 * g.r + 1, g.nc[2]);
     double mean = (double)ctx->n / (double)(g.nc[0] * g.nc[1] * g.nc[2]) * ns;
@@ -2031,7 +2034,7 @@ static int stage_cap(const sc_ctx* ctx, const Grid& g, const double* box) {
     int cap = (int)std::min(std::max(3.0 * mean + 64.0, 128.0), pack + 64.0);
     cap = std::max(cap, 64);
     cap = std::min(cap, ctx->n + 64);
-    cap = std::min(cap, STAGE_CAP_MAX);
+    cap = std::min(cap, capmax);
     return ((cap + 63) / 64) * 64;
 }
 
@@ -2099,7 +2102,7 @@ static int energy_all_impl(sc_ctx* ctx, const double* box, double* e) {
     int rc = energy_grid(ctx, box);
     if (rc) return rc;
     const Grid& g = ctx->g;
-    int cap = stage_cap(ctx, g, box);
+    int cap = stage_cap(ctx, g, box, ENERGY_CAP_MAX);
     size_t lds = stage_bytes(cap) + sizeof(int) * (2 * QCAP) + sizeof(int) * (2 * 125 + 2);
     if ((rc = ensure_out(ctx, 2))) return rc;
     SysConst sc = sysconst(ctx, box);
@@ -3437,7 +3440,7 @@ static int sc_batch_energies_impl(sc_ctx* ctx, double* e) {
     for (int q = 0; q < R; q++) {
         const Grid& g = ctx->rep_grid[q].g;
         double b[3] = {ctx->rep_box[q][0], ctx->rep_box[q][1], ctx->rep_box[q][2]};
-        int cap = stage_cap_n(ctx->rep_n[q], ctx->minvol, g, b);
+        int cap = stage_cap_n(ctx->rep_n[q], ctx->minvol, g, b, ENERGY_CAP_MAX);
         size_t lds = stage_bytes(cap) + sizeof(int) * (2 * QCAP) + sizeof(int) * (2 * 125 + 2);
         int co = ctx->rep_grid[q].cell_off, nc = ctx->rep_ncell[q];
         SysConst sc = sysconst(ctx, b);

@@ -347,3 +347,36 @@ def test_block_waves_reproduce_wave_sweep(dev, mode):
     np.testing.assert_array_equal(p1, p4)
     np.testing.assert_array_equal(d1, d4)
     np.testing.assert_array_equal(q1, q4)
+
+
+def test_volume_moves_one_edge(dev):
+    """ptype 0 (pressureMove, movecreator.cpp:350-389): an accepted move changes
+    one edge; the energy carried through sweeps and volume moves matches a
+    full evaluation and the oracle at the final box."""
+    say("case: anisotropic volume moves")
+    w = W.workload("c1_psc_1000")
+    g = sc_amd.GpuSystem(dev)
+    g.load_topology(w["top"])
+    pos, d, q = g.generate_rsa(w["types"], w["box"], seed=29)
+    g.set_config(w["box"], pos, d, q)
+    box = np.array(w["box"], float)
+    acc = 0
+    for k in range(40):
+        g.sweep(1, 1.0, 0.3, 20.0 / 180 * np.pi / 4, r=1, seed=3, counter=k)
+        a, b = g.volume_move(0, 2.0, 1.0, 1.0, seed=4, counter=k)
+        changed = np.nonzero(b != box)[0]
+        if a:
+            acc += 1
+            assert len(changed) == 1
+        else:
+            assert len(changed) == 0
+        box = b
+    b, p, dd, qq = g.get_config()
+    np.testing.assert_array_equal(b, box)
+    tp, st = po.Topo(w["top"]), po.State(b, p, dd, qq, w["types"])
+    eo = po.energy_all(tp, st, use_cells=True)
+    eg = g.allToAll()
+    say("accepted %d/40, box %s, E gpu %.6f oracle %.6f" % (acc, np.round(b, 3).tolist(), eg, eo))
+    assert acc > 0
+    assert abs(eg - eo) <= 1e-5 * max(1.0, abs(eo))
+    g.close()
