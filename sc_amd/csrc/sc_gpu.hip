@@ -468,17 +468,30 @@ __global__ void k_bond_all(int n, ChainTab ct, SysConst sc, const int* __restric
 }
 
 // ------------------------------------------------------------ energy (all)
-// One wave per cell; pairs (i in home cell, j in stencil) with ID_j < ID_i
-// flattened over lanes.  The cheap stage (image, cutoff, segment distance,
-// WCA) runs for every candidate; pairs inside the attraction range are queued
-// in LDS and evaluated 64 at a time so the branch-heavy patch code runs with
-// full lanes.  FP32 pair terms, FP64 accumulation; per-cell partial sums.
+// One block of NW waves per cell; pairs (i in home cell, j in stencil) with
+// ID_j < ID_i flattened over the block's threads (wave w takes the chunks
+// w, w + NW, ...).  The cheap stage (image, cutoff, segment distance, WCA)
+// runs for every candidate; pairs inside the attraction range are queued in
+// the wave's LDS queue and evaluated 64 at a time so the branch-heavy patch
+// code runs with full lanes.  FP32 pair terms, FP64 accumulation; per-cell
+// partial sums combined in a fixed order.
 #define QCAP 128
-__global__ __launch_bounds__(WAVE) void k_energy_all(Grid g, SysConst sc, const int* __restrict__ starts,
-                                                     const uint4* __restrict__ P, const float4* __restrict__ O,
-                                                     const float2* __restrict__ Q, const int* __restrict__ ID,
-                                                     const PP32* __restrict__ pp, const TP32* __restrict__ tp,
-                                                     int cap, double* __restrict__ partial, int* __restrict__ flag) {
+#define ENERGY_NW 4
+
+// the LDS writes of a wave are visible to its own later reads (a wave's LDS
+// operations complete in order; this keeps the compiler from moving them)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NW>
+__global__ __launch_bounds__(WAVE * NW) void k_energy_all(Grid g, SysConst sc, const int* __restrict__ starts,
+                                                          const uint4* __restrict__ P, const float4* __restrict__ O,
+                                                          const float2* __restrict__ Q, const int* __restrict__ ID,
+                                                          const PP32* __restrict__ pp, const TP32* __restrict__ tp,
+                                                          int cap, double* __restrict__ partial, int* __restrict__ flag) {
     extern __shared__ __align__(16) unsigned char smem[];
     int cell = blockIdx.x;
     int cx = cell % g.nc[0], cy = (cell / g.nc[0]) % g.nc[1], cz = cell / (g.nc[0] * g.nc[1]);
@@ -487,22 +500,23 @@ __global__ __launch_bounds__(WAVE) void k_energy_all(Grid g, SysConst sc, const 
     S.o = (float4*)(S.u + cap);
     S.q = (float2*)(S.o + cap);
     S.id = (int*)(S.q + cap);
-    int* queue = S.id + cap;                 // QCAP entries: (hi<<16)|j packed as two ints
-    int* scratch = queue + 2 * QCAP;         // 2*125+1 ints
-    int lane = lane_id();
+    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), w = tid >> 6;
+    int* scratch = S.id + cap;                           // 2*125+2 ints
+    double* red = (double*)(scratch + 2 * 125 + 2);      // NW doubles (8-byte aligned: cap is a multiple of 64)
+    int* queue = (int*)(red + NW) + w * 2 * QCAP;        // this wave's QCAP (hi, j) entries
     if (starts[cell + 1] == starts[cell]) {
-        if (lane == 0) partial[cell] = 0.0;
+        if (tid == 0) partial[cell] = 0.0;
         return;
     }
-    if (!stage_load<1>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
-        if (lane == 0) { atomicOr(flag, 1); partial[cell] = 0.0; }
+    if (!stage_load<NW>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
+        if (tid == 0) { atomicOr(flag, 1); partial[cell] = 0.0; }
         return;
     }
     int nh = S.h1 - S.h0, ns = S.n;
     long total = (long)nh * ns;
     double acc = 0.0;
-    int qn = 0;   // uniform queue length
-    for (long base = 0; base < total; base += WAVE) {
+    int qn = 0;   // wave-uniform queue length
+    for (long base = (long)w * WAVE; base < total; base += (long)WAVE * NW) {
         long pidx = base + lane;
         bool need = false;
         int hi_ = 0, j = 0;
@@ -540,8 +554,8 @@ __global__ __launch_bounds__(WAVE) void k_energy_all(Grid g, SysConst sc, const 
             queue[2 * pos + 1] = j;
         }
         qn += __popcll(m);
-        __syncthreads();
-        bool last = base + WAVE >= total;
+        wave_lds_sync();
+        bool last = base + (long)WAVE * NW >= total;
         if (qn >= WAVE || (last && qn > 0)) {
             // drain whole batches of 64 (everything at the end)
             int take = last ? qn : (qn / WAVE) * WAVE;
@@ -558,18 +572,18 @@ __global__ __launch_bounds__(WAVE) void k_energy_all(Grid g, SysConst sc, const 
                     else acc += (double)spsc(P2, pa, pb, r);
                 }
             }
-            __syncthreads();
+            wave_lds_sync();
             int rest = qn - take;   // < 64
             int c0 = 0, c1 = 0;
             if (lane < rest) { c0 = queue[2 * (take + lane)]; c1 = queue[2 * (take + lane) + 1]; }
-            __syncthreads();
+            wave_lds_sync();
             if (lane < rest) { queue[2 * lane] = c0; queue[2 * lane + 1] = c1; }
-            __syncthreads();
+            wave_lds_sync();
             qn = rest;
         }
     }
-    double s = wave_sum(acc);
-    if (lane == 0) partial[cell] = s;
+    double s = block_sum<NW>(acc, red);
+    if (tid == 0) partial[cell] = s;
 }
 
 // deterministic final reduction of per-cell partial sums
@@ -2040,6 +2054,13 @@ static int stage_cap(const sc_ctx* ctx, const Grid& g, const double* box, int ca
 
 static size_t stage_bytes(int cap) { return (size_t)cap * (16 + 16 + 8 + 4); }
 
+// LDS of k_energy_all: the stage, the stencil scratch, the block sums and one
+// pair queue per wave
+static size_t energy_lds_bytes(int cap) {
+    return stage_bytes(cap) + sizeof(int) * (2 * 125 + 2) + sizeof(double) * ENERGY_NW +
+           sizeof(int) * 2 * QCAP * ENERGY_NW;
+}
+
 
 // LDS for the sweep's copy of the parameter tables (with alignment slack)
 static size_t sweep_table_bytes(int nt) {
@@ -2103,12 +2124,12 @@ static int energy_all_impl(sc_ctx* ctx, const double* box, double* e) {
     if (rc) return rc;
     const Grid& g = ctx->g;
     int cap = stage_cap(ctx, g, box, ENERGY_CAP_MAX);
-    size_t lds = stage_bytes(cap) + sizeof(int) * (2 * QCAP) + sizeof(int) * (2 * 125 + 2);
+    size_t lds = energy_lds_bytes(cap);
     if ((rc = ensure_out(ctx, 2))) return rc;
     SysConst sc = sysconst(ctx, box);
     int c = ctx->cur;
     HIPCK(hipEventRecord(ctx->ev0, ctx->stream));
-    hipLaunchKernelGGL(k_energy_all, dim3(ctx->ncell), dim3(WAVE), lds, ctx->stream, g, sc, ctx->starts, ctx->P[c],
+    hipLaunchKernelGGL(k_energy_all<ENERGY_NW>, dim3(ctx->ncell), dim3(WAVE * ENERGY_NW), lds, ctx->stream, g, sc, ctx->starts, ctx->P[c],
                        ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->partial, ctx->d_flag);
     HIPCK(hipEventRecord(ctx->ev1, ctx->stream));
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, ctx->stream, ctx->ncell, ctx->partial, ctx->d_out);
@@ -3441,10 +3462,10 @@ static int sc_batch_energies_impl(sc_ctx* ctx, double* e) {
         const Grid& g = ctx->rep_grid[q].g;
         double b[3] = {ctx->rep_box[q][0], ctx->rep_box[q][1], ctx->rep_box[q][2]};
         int cap = stage_cap_n(ctx->rep_n[q], ctx->minvol, g, b, ENERGY_CAP_MAX);
-        size_t lds = stage_bytes(cap) + sizeof(int) * (2 * QCAP) + sizeof(int) * (2 * 125 + 2);
+        size_t lds = energy_lds_bytes(cap);
         int co = ctx->rep_grid[q].cell_off, nc = ctx->rep_ncell[q];
         SysConst sc = sysconst(ctx, b);
-        SC_LAUNCH(k_energy_all, dim3(nc), dim3(WAVE), lds, ctx->stream, g, sc, ctx->starts + co, ctx->P[c], ctx->O[c],
+        SC_LAUNCH(k_energy_all<ENERGY_NW>, dim3(nc), dim3(WAVE * ENERGY_NW), lds, ctx->stream, g, sc, ctx->starts + co, ctx->P[c], ctx->O[c],
                   ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->partial + co, ctx->d_flag);
         SC_LAUNCH(k_reduce, dim3(1), dim3(1024), 0, ctx->stream, nc, ctx->partial + co, ctx->d_rep_e + q);
     }
