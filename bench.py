@@ -123,7 +123,7 @@ def main():
     ap.add_argument("--workload", default="c4_tchpsc_65536")
     ap.add_argument("--r", type=int, default=1, help="checkerboard stencil radius (1: 2^3 colours, 2: 3^3)")
     ap.add_argument("--waves", type=int, default=0,
-                    help="waves per active cell in the sweep kernels (1, 4; 0: 4, and 1 with the network screen)")
+                    help="waves per active cell in the sweep kernels (1, 2, 4; 0: 4, and 2 with the network screen)")
     ap.add_argument("--trial-mode", type=int, default=1,
                     help="trials per active cell: 0 = its occupancy, 1 = dithered mean occupancy")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -173,12 +173,13 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
  * pass waits for its fullest cell.  Both are Metropolis chains with the same
  * stationary distribution (the count never depends on the moves of the pass). */
 int sc_set_trial_mode(sc_ctx* ctx, int mode);
-/* Waves that work together on one active cell of a sweep pass: 1, 4, or 0
- * (default: 4, and 1 for the network screen, whose MFMA operands leave room
- * for one wave per SIMD).  With 4 the stencil scan, the pair lists and the
- * patch terms of every trial are spread over 256 threads; the chain of
- * trials stays sequential.  Results are deterministic for each setting; they
- * differ between settings only by the order of FP64 sums. */
+/* Waves that work together on one active cell of a sweep pass: 1, 2, 4, or
+ * 0 (default: 4, and 2 for the network screen, whose MFMA operands leave room
+ * for one wave per SIMD, so two 2-wave blocks fill a CU).  The stencil scan,
+ * the pair lists and the patch terms of every trial are spread over the
+ * block's threads; the chain of trials stays sequential.  Results are
+ * deterministic for each setting; they differ between settings only by the
+ * order of FP64 sums. */
 int sc_set_block_waves(sc_ctx* ctx, int nw);
 /* Volume move (MoveCreator::pressureMove, movecreator.cpp:330-550) for
  * ptype 0 (one random edge) and 1 (isotropic); returns accepted=1/0 and the

@@ -520,7 +520,7 @@ class GpuSystem:
         return s2.value, d
 
     def set_block_waves(self, nw):
-        """waves per active cell in the sweep kernels (1 or 4)"""
+        """waves per active cell in the sweep kernels (1, 2, 4; 0 = by surrogate mode)"""
         self._ck(self._L.sc_set_block_waves(self._h, int(nw)), "set_block_waves")
 
     def set_trial_mode(self, mode):

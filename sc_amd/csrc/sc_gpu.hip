@@ -1191,7 +1191,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     }
 }
 
-// NW waves per block work on one active cell (NW = 1 or 4)
+// NW waves per block work on one active cell (NW = 1, 2 or 4)
 template <int DA, int NW>
 __global__ __launch_bounds__(WAVE * NW) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
                                                      uint4* __restrict__ P, float4* __restrict__ O,
@@ -1746,7 +1746,7 @@ struct sc_ctx {
     bool e_valid = false;
     double e_track = 0.0;
     int e_age = 0;                      // volume moves since the last full evaluation
-    int sweep_nw = 0;                   // waves per active cell in the sweep kernels (1, 4; 0 = by mode)
+    int sweep_nw = 0;                   // waves per active cell in the sweep kernels (1, 2, 4; 0 = by mode)
 };
 
 static ChainTab chain_tab(const sc_ctx* ctx) {
@@ -3136,16 +3136,20 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
                        ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,     \
                        ctx->ppmax, ctx->d_nonempty, IMG, MC, chain_tab(ctx), ctx->d_stats, ctx->d_dsum, ctx->d_flag)
             // by mode: the network screen holds its MFMA operands in registers, which
-            // leaves room for one wave per SIMD, so it runs one wave per cell
-            int nw = ctx->sweep_nw ? ctx->sweep_nw : (mode == DA_NET ? 1 : 4);
+            // leaves room for one wave per SIMD, so it runs two waves per cell (two
+            // blocks per CU); the other modes run four
+            int nw = ctx->sweep_nw ? ctx->sweep_nw : (mode == DA_NET ? 2 : 4);
             if (mode == DA_NET) {
                 if (nw == 4) SWEEP_PASS(DA_NET, 4, ctx->mlp_img, mlpc);
+                else if (nw == 2) SWEEP_PASS(DA_NET, 2, ctx->mlp_img, mlpc);
                 else SWEEP_PASS(DA_NET, 1, ctx->mlp_img, mlpc);
             } else if (mode == DA_REP) {
                 if (nw == 4) SWEEP_PASS(DA_REP, 4, nullptr, ctx->mlp_c);
+                else if (nw == 2) SWEEP_PASS(DA_REP, 2, nullptr, ctx->mlp_c);
                 else SWEEP_PASS(DA_REP, 1, nullptr, ctx->mlp_c);
             } else {
                 if (nw == 4) SWEEP_PASS(DA_OFF, 4, nullptr, ctx->mlp_c);
+                else if (nw == 2) SWEEP_PASS(DA_OFF, 2, nullptr, ctx->mlp_c);
                 else SWEEP_PASS(DA_OFF, 1, nullptr, ctx->mlp_c);
             }
 #undef SWEEP_PASS
@@ -3410,9 +3414,11 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             int nw = ctx->sweep_nw ? ctx->sweep_nw : 4;
             if (mode == DA_REP) {
                 if (nw == 4) SWEEP_BATCH(DA_REP, 4);
+                else if (nw == 2) SWEEP_BATCH(DA_REP, 2);
                 else SWEEP_BATCH(DA_REP, 1);
             } else {
                 if (nw == 4) SWEEP_BATCH(DA_OFF, 4);
+                else if (nw == 2) SWEEP_BATCH(DA_OFF, 2);
                 else SWEEP_BATCH(DA_OFF, 1);
             }
 #undef SWEEP_BATCH
@@ -4105,7 +4111,7 @@ int sc_set_trial_mode(sc_ctx* ctx, int mode) {
 }
 
 static int sc_set_block_waves_impl(sc_ctx* ctx, int nw) {
-    if (!ctx || (nw != 0 && nw != 1 && nw != 4)) return SC_EINVAL;
+    if (!ctx || (nw != 0 && nw != 1 && nw != 2 && nw != 4)) return SC_EINVAL;
     ctx->sweep_nw = nw;
     return SC_OK;
 }
