@@ -214,12 +214,30 @@ __global__ void k_cell_scatter(int n, const int* __restrict__ cell_tmp, const in
     src_of_pos[pos] = k;
 }
 
-// order each cell by original index (deterministic, matches a stable CPU sort)
-__global__ void k_cell_sort(int ncell, const int* __restrict__ starts, int* __restrict__ src_of_pos,
-                            const int* __restrict__ ID) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+// order each cell by original index (deterministic, matches a stable CPU
+// sort): one wave per cell; a cell of up to 64 particles is ranked in
+// registers (lane k's rank = how many of the cell's indices are smaller;
+// indices are distinct), a larger one (grids of one or two cells per axis) is
+// insertion-sorted by its first lane
+#define CELL_SORT_CELLS 4   // cells (waves) per 256-thread block
+__global__ __launch_bounds__(WAVE * CELL_SORT_CELLS) void k_cell_sort(int ncell, const int* __restrict__ starts,
+                                                                     int* __restrict__ src_of_pos,
+                                                                     const int* __restrict__ ID) {
+    int c = blockIdx.x * CELL_SORT_CELLS + (int)(threadIdx.x >> 6);
+    int lane = (int)(threadIdx.x & (WAVE - 1));
     if (c >= ncell) return;
-    int b = starts[c], e = starts[c + 1];
+    int b = starts[c], e = starts[c + 1], n = e - b;
+    if (n <= 1) return;
+    if (n <= WAVE) {
+        int v = lane < n ? src_of_pos[b + lane] : 0;
+        int key = lane < n ? ID[v] : 0x7fffffff;
+        int rank = 0;
+        for (int k = 0; k < n; k++) rank += __shfl(key, k, WAVE) < key ? 1 : 0;
+        // every lane's read above precedes its write: the write needs the ranks
+        if (lane < n) src_of_pos[b + rank] = v;
+        return;
+    }
+    if (lane != 0) return;
     for (int i = b + 1; i < e; i++) {
         int v = src_of_pos[i], key = ID[v], j = i - 1;
         while (j >= b && ID[src_of_pos[j]] > key) { src_of_pos[j + 1] = src_of_pos[j]; j--; }
@@ -2006,7 +2024,8 @@ static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const doubl
                        ctx->d_nonempty);
     if (n > 0) {
         hipLaunchKernelGGL(k_cell_scatter, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->cell_tmp, ctx->starts, ctx->fill, ctx->src_of_pos);
-        hipLaunchKernelGGL(k_cell_sort, dim3((ncell + 255) / 256), dim3(256), 0, ctx->stream, ncell, ctx->starts, ctx->src_of_pos, ctx->ID[c]);
+        hipLaunchKernelGGL(k_cell_sort, dim3((ncell + CELL_SORT_CELLS - 1) / CELL_SORT_CELLS), dim3(WAVE * CELL_SORT_CELLS), 0,
+                           ctx->stream, ncell, ctx->starts, ctx->src_of_pos, ctx->ID[c]);
         hipLaunchKernelGGL(k_permute, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->src_of_pos, ctx->P[c], ctx->O[c], ctx->Q[c],
                            ctx->ID[c], ctx->P[c ^ 1], ctx->O[c ^ 1], ctx->Q[c ^ 1], ctx->ID[c ^ 1], ctx->slot);
     }
@@ -3279,8 +3298,8 @@ static int batch_build_grid(sc_ctx* ctx, int r, const std::vector<std::array<uin
               ctx->d_nonempty);
     SC_LAUNCH(k_cell_scatter, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->cell_tmp, ctx->starts, ctx->fill,
               ctx->src_of_pos);
-    SC_LAUNCH(k_cell_sort, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, total, ctx->starts, ctx->src_of_pos,
-              ctx->ID[c]);
+    SC_LAUNCH(k_cell_sort, dim3((total + CELL_SORT_CELLS - 1) / CELL_SORT_CELLS), dim3(WAVE * CELL_SORT_CELLS), 0,
+              ctx->stream, total, ctx->starts, ctx->src_of_pos, ctx->ID[c]);
     SC_LAUNCH(k_permute, dim3(nb), dim3(256), 0, ctx->stream, n, ctx->src_of_pos, ctx->P[c], ctx->O[c], ctx->Q[c],
               ctx->ID[c], ctx->P[c ^ 1], ctx->O[c ^ 1], ctx->Q[c ^ 1], ctx->ID[c ^ 1], ctx->slot);
     SC_LAUNCH(k_nonempty_batch, dim3(R), dim3(256), 0, ctx->stream, ctx->d_rg, R, total, ctx->starts,
