@@ -47,6 +47,9 @@ typedef struct {
     uint64_t stage2_acc, stage2_try;                    /* delayed acceptance: network screen */
     double energy_change;     /* sum of accepted dE (for the drift check, updater.cpp:122-150) */
     uint64_t bond_rejected;   /* moves rejected because a bonded chain partner lay outside the stencil */
+    /* whole-molecule moves (sc_set_chain_moves): Statistics chainm / chainr acc and tries */
+    uint64_t chain_disp_acc, chain_disp_try, chain_rot_acc, chain_rot_try;
+    uint64_t chain_rejected;  /* of those tries: molecule wider than the extent, or anchor leaving its coarse cell */
 } sc_sweep_stats;
 
 /* --- lifetime --------------------------------------------------------- */
@@ -138,6 +141,16 @@ int sc_energy_one_trial(sc_ctx* ctx, int target, const double pos[3], const doub
                         double* e);
 /* commit a particle trial (TotalEMatrix::update(int), :326-328) */
 int sc_commit_particle(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3]);
+/* non-bonded energy of the molecule [first, first + size) with every particle
+ * outside it, part1 = the molecule's particle (TotalEMatrix::mol2others,
+ * totalenergycalculator.h:417-453) */
+int sc_energy_mol(sc_ctx* ctx, int first, int size, double* e);
+/* the same with the molecule's particles at trial states, size x 3 values
+ * each, pos as box fractions (TotalEMatrix::mol2othersTrial, :455-500) */
+int sc_energy_mol_trial(sc_ctx* ctx, int first, int size, const double* pos, const double* dir, const double* pd0,
+                        double* e);
+/* commit a molecule trial (TotalEMatrix::update(Molecule), :330-332) */
+int sc_commit_mol(sc_ctx* ctx, int first, int size, const double* pos, const double* dir, const double* pd0);
 /* every particle's energy with all others, original order */
 int sc_energy_per_particle(sc_ctx* ctx, double* out);
 /* whole-system energy at a trial box (TotalEMatrix::allToAllTrial, :334-336) */
@@ -181,6 +194,26 @@ int sc_set_trial_mode(sc_ctx* ctx, int mode);
  * deterministic for each setting; they differ between settings only by the
  * order of FP64 sums. */
 int sc_set_block_waves(sc_ctx* ctx, int nw);
+/* Whole-molecule moves in sc_sweep (MoveCreator::chainMove, movecreator.cpp:
+ * 305-328): besides its N particle trials, every sweep makes chainprob * N
+ * molecule trials (chainprob is the per-step probability of Updater's step
+ * loop, updater.cpp:206-217), each on a molecule of more than one particle
+ * chosen uniformly, displaced by exactly chain_mx in a random direction
+ * (chainDisplace, :1075-1137; chain_mx is the doubled chainmmx, sim.h:366)
+ * or rotated about its centre by the quaternion half-angle chain_angle * U
+ * (chainRotate / clusterRotate, :1139-1305; sim.h:362), half and half,
+ * accepted by moveTry on mol2others.  The centre is weighted by
+ * Ia_param::volume (clusterCM, :1307-1324) and taken on minimum-image
+ * unwrapped coordinates.  Moves run on a coarse 2x2x2 checkerboard of cells
+ * at least 2 extent + maxcut wide; molecules wider than `extent` (0: 1.2 x
+ * the widest molecule now) do not move, nor does a move leave the anchor's
+ * coarse cell.  Box edges must be at least 2 extent + maxcut and above 4
+ * extent (no molecule reaches its own image, which the reference's
+ * mol2others has no term for, and the minimum image unwraps every molecule).
+ * Molecules of at most 64 particles.  chainprob = 0 turns the moves off. */
+int sc_set_chain_moves(sc_ctx* ctx, double chainprob, double chain_mx, double chain_angle, double extent);
+/* the extent the chain moves use */
+int sc_get_chain_extent(sc_ctx* ctx, double* extent);
 /* Volume move (MoveCreator::pressureMove, movecreator.cpp:330-550) for
  * ptype 0 (one random edge) and 1 (isotropic); returns accepted=1/0 and the
  * new box.  Whole-system energies are FP32 pair terms (FP64 bonds) with FP64

@@ -99,6 +99,13 @@ def lib():
                                      ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.POINTER(ctypes.c_long), D,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_mc_run_npt.restype = ctypes.c_double
+        L.orc_mc_run_chains.argtypes = [T, D, ctypes.c_int, D, D, D, ctypes.c_void_p, I, ctypes.c_long,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                        ctypes.POINTER(ctypes.c_long), D, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double]
+        L.orc_mc_run_chains.restype = ctypes.c_double
+        L.orc_soa_mol2others.argtypes = [T, D, ctypes.c_int, D, D, D, ctypes.c_void_p, I, ctypes.c_int, I, I, D]
         L.orc_nematic_s2.argtypes = [ctypes.c_int, D]
         L.orc_nematic_s2.restype = ctypes.c_double
         _lib = L
@@ -367,6 +374,35 @@ def mc_run_npt(topo, st, nsweeps, temper, transmx, rot_half, seed, ptype, press,
                                  int(seed) & 0xFFFFFFFFFFFFFFFF, stats, ctypes.byref(ef), int(ptype), float(press),
                                  float(edge_mx))
     return s, drift, list(stats), ef.value
+
+
+def mc_run_chains(topo, st, nsweeps, temper, transmx, rot_half, chainprob, chain_mx, chain_angle, seed,
+                  ptype=-1, press=0.0, edge_mx=0.0):
+    """mc_run with whole-molecule moves (each step a chain move with
+    probability chainprob) and, with ptype >= 0, one volume move per sweep;
+    returns (new_state, drift, stats [trans acc, try, rot acc, try, volume
+    acc, try, chain displace acc, try, chain rotate acc, try], e_final)."""
+    s = State(st.box.copy(), st.pos.copy(), st.dir.copy(), st.pd0.copy(), st.types.copy(),
+              None if st.pd1 is None else st.pd1.copy())
+    stats = (ctypes.c_long * 10)()
+    ef = ctypes.c_double(0.0)
+    drift = lib().orc_mc_run_chains(topo.ref, _dp(s.box), s.n, _dp(s.pos), _dp(s.dir), _dp(s.pd0), _opt(s.pd1),
+                                    _ip(s.types), int(nsweeps), float(temper), float(transmx), float(rot_half),
+                                    float(chainprob), float(chain_mx), float(chain_angle),
+                                    int(seed) & 0xFFFFFFFFFFFFFFFF, stats, ctypes.byref(ef), int(ptype),
+                                    float(press), float(edge_mx))
+    return s, drift, list(stats), ef.value
+
+
+def mol2others(topo, st, first, size):
+    """Non-bonded energy of each molecule [first, first + size) with every
+    particle outside it (TotalE mol2others / mol2othersTrial)."""
+    first = np.ascontiguousarray(first, np.int32)
+    size = np.ascontiguousarray(size, np.int32)
+    out = np.zeros(len(first))
+    lib().orc_soa_mol2others(topo.ref, _dp(st.box), st.n, _dp(st.pos), _dp(st.dir), _dp(st.pd0), _opt(st.pd1),
+                             _ip(st.types), len(first), _ip(first), _ip(size), _dp(out))
+    return out
 
 
 def nematic_s2(dirs):

@@ -15,6 +15,9 @@
 //   scref derived <out>             -> dir, patchdir[2], patchsides[4], chdir[2]
 //                                      after Conf::partVecInit
 //   scref mindist <out> <cases.txt> -> EPatch::minDistSegments on raw inputs
+//   scref molecules <out>           -> mol2others / mol2othersTrial of every
+//                                      molecule of more than one particle
+//                                      (totalenergycalculator.h:417-500)
 // Values are printed with %.17g so the fixtures carry full FP64 precision.
 #include <cstdio>
 #include <cstdlib>
@@ -114,6 +117,18 @@ int main(int argc, char** argv) {
         calc.initEM();
         fprintf(out, "%.17g\n", calc.allToAll());
         for (size_t i = 0; i < conf.pvec.size(); i++) fprintf(out, "%.17g\n", calc.oneToAll((int)i));
+    } else if (mode == "molecules") {
+        // every molecule of more than one particle (ParticleVector::getChain):
+        // first index, size, TotalEMatrix::mol2others and mol2othersTrial on
+        // the unchanged state
+        TotalEMatrix<PairE> calc(&sim, &conf);
+        calc.initEM();
+        for (int k = 0; k < conf.pvec.getChainCount(); k++) {
+            Molecule m = conf.pvec.getChain(k);
+            double e0 = calc.mol2others(m);
+            double e1 = calc.mol2othersTrial(m);
+            fprintf(out, "%d %d %.17g %.17g\n", m[0], (int)m.size(), e0, e1);
+        }
     } else {
         fprintf(stderr, "unknown mode\n");
         return 2;

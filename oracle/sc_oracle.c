@@ -982,6 +982,20 @@ double orc_energy_one(const orc_topo* t, const double box[3], const orc_part* p,
     return e;
 }
 
+/* TotalEMatrix::mol2othersTrial without a pair list, totalenergycalculator.h:
+ * 484-497: the molecule [first, first + size) with every particle outside it,
+ * part1 = the molecule's particle, an empty connectivity list (so only the
+ * non-bonded functor, zero beyond sqmaxcut).  mol2others (:438-452) sums the
+ * same pairs from the cached matrix. */
+double orc_mol2others(const orc_topo* t, const double box[3], const orc_part* p, int n, int first, int size) {
+    double e = 0.0;
+    int a, j;
+    for (a = first; a < first + size; a++)
+        for (j = 0; j < n; j++)
+            if (j < first || j >= first + size) e += pair_nb(t, box, &p[a], &p[j]);
+    return e;
+}
+
 /* ------------------------------------------------------------- cell grid */
 typedef struct {
     int nc[3];
@@ -1121,6 +1135,16 @@ int orc_soa_energy_one_each(const orc_topo* t, const double box[3], int n, const
     return 0;
 }
 
+int orc_soa_mol2others(const orc_topo* t, const double box[3], int n, const double* pos, const double* dir,
+                       const double* pd0, const double* pd1, const int* type, int m, const int* first,
+                       const int* size, double* out) {
+    orc_part* p = load_all(t, n, pos, dir, pd0, pd1, type);
+    int k;
+    for (k = 0; k < m; k++) out[k] = orc_mol2others(t, box, p, n, first[k], size[k]);
+    free(p);
+    return 0;
+}
+
 int orc_soa_derived(const orc_topo* t, int n, const double* dir, const double* pd0, const double* pd1,
                     const int* type, double* out) {
     int i, k;
@@ -1229,9 +1253,10 @@ static orc_vec rand_sphere(uint64_t* s) {
     return V(x1 * b, x2 * b, 1.0 - 2.0 * a);
 }
 
-/* energy of p (as part1) with all particles in the 27 cells around it, skipping self */
-static double grid_one(const orc_topo* t, const double box[3], const ogrid* g, const orc_part* p, const orc_part* q,
-                       int self) {
+/* non-bonded energy of q (as part1) with all particles in the 27 cells around
+ * it except those of index lo..hi-1 */
+static double grid_excl(const orc_topo* t, const double box[3], const ogrid* g, const orc_part* p, const orc_part* q,
+                        int lo, int hi) {
     int st[27], m, a, k, cx, cy, cz, c;
     double e = 0.0, s[3] = {frac01(q->pos.x), frac01(q->pos.y), frac01(q->pos.z)};
     cx = (int)(s[0] * g->nc[0]); if (cx >= g->nc[0]) cx = g->nc[0] - 1;
@@ -1242,9 +1267,15 @@ static double grid_one(const orc_topo* t, const double box[3], const ogrid* g, c
     for (a = 0; a < m; a++)
         for (k = g->head[st[a]]; k < g->head[st[a] + 1]; k++) {
             int j = g->idx[k];
-            if (j != self) e += pair_nb(t, box, q, &p[j]);
+            if (j < lo || j >= hi) e += pair_nb(t, box, q, &p[j]);
         }
-    return e + bonds_with_partners(t, box, p, q, 0);
+    return e;
+}
+
+/* energy of p (as part1) with all particles in the 27 cells around it, skipping self */
+static double grid_one(const orc_topo* t, const double box[3], const ogrid* g, const orc_part* p, const orc_part* q,
+                       int self) {
+    return grid_excl(t, box, g, p, q, self, self + 1) + bonds_with_partners(t, box, p, q, 0);
 }
 
 /* move particle i between cells of the CSR grid by re-building only if its cell changed */
@@ -1272,23 +1303,127 @@ static void grid_move(ogrid* g, const orc_part* p, int n, int i) {
     g->cell[i] = nc;
 }
 
+typedef struct {
+    double chainprob, chain_mx, chain_angle;
+} ochainmv;
+
 static double mc_run(const orc_topo* t, double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
                      const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
-                     long* stats, double* e_final, int ptype, double press, double edge_mx);
+                     long* stats, double* e_final, int ptype, double press, double edge_mx, const ochainmv* cm);
 
 double orc_mc_run(const orc_topo* t, const double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
                   const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
                   long* stats, double* e_final) {
     double b[3] = {box[0], box[1], box[2]};
     return mc_run(t, b, n, pos, dir, pd0, pd1, type, nsweeps, temper, transmx, rot_half, seed, stats, e_final, -1,
-                  0.0, 0.0);
+                  0.0, 0.0, NULL);
 }
 
 double orc_mc_run_npt(const orc_topo* t, double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
                       const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
                       long* stats, double* e_final, int ptype, double press, double edge_mx) {
     return mc_run(t, box, n, pos, dir, pd0, pd1, type, nsweeps, temper, transmx, rot_half, seed, stats, e_final,
-                  ptype, press, edge_mx);
+                  ptype, press, edge_mx, NULL);
+}
+
+double orc_mc_run_chains(const orc_topo* t, double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
+                         const int* type, long nsweeps, double temper, double transmx, double rot_half,
+                         double chainprob, double chain_mx, double chain_angle, uint64_t seed, long* stats,
+                         double* e_final, int ptype, double press, double edge_mx) {
+    ochainmv cm;
+    cm.chainprob = chainprob;
+    cm.chain_mx = chain_mx;
+    cm.chain_angle = chain_angle;
+    return mc_run(t, box, n, pos, dir, pd0, pd1, type, nsweeps, temper, transmx, rot_half, seed, stats, e_final,
+                  ptype, press, edge_mx, &cm);
+}
+
+/* non-bonded energy of the molecule [first, first + size) with every other
+ * particle, through the grid (mol2others / mol2othersTrial) */
+static double grid_mol(const orc_topo* t, const double box[3], const ogrid* g, const orc_part* p, int first,
+                       int size) {
+    double e = 0.0;
+    int a;
+    for (a = first; a < first + size; a++) e += grid_excl(t, box, g, p, &p[a], first, first + size);
+    return e;
+}
+
+/* MoveCreator::clusterCM, movecreator.cpp:1307-1324: the centre of the
+ * molecule weighted by Ia_param::volume, in box fractions.  It is taken on
+ * coordinates unwrapped by the minimum image from the first particle (off[a]
+ * = offset of particle a); the reference averages the stored fractions, which
+ * it wraps into the box only on input, so a molecule that straddles the box
+ * edge there rotates about a point far from it. */
+static orc_vec chain_cm(const orc_topo* t, const orc_part* p, int first, int size, orc_vec* off) {
+    orc_vec c = V(0.0, 0.0, 0.0);
+    double vsum = 0.0;
+    int a;
+    for (a = 0; a < size; a++) {
+        const orc_part* q = &p[first + a];
+        double v = t->ia[q->type][q->type].volume;
+        orc_vec d = V(q->pos.x - p[first].pos.x, q->pos.y - p[first].pos.y, q->pos.z - p[first].pos.z);
+        d.x -= nearbyint(d.x);
+        d.y -= nearbyint(d.y);
+        d.z -= nearbyint(d.z);
+        off[a] = d;
+        c.x += d.x * v; c.y += d.y * v; c.z += d.z * v;
+        vsum += v;
+    }
+    return SCL(1.0 / vsum, c);
+}
+
+/* MoveCreator::chainMove, movecreator.cpp:305-328: a molecule of more than one
+ * particle chosen uniformly, displaced (chainDisplace, :1075-1137) or rotated
+ * about its centre (chainRotate + clusterRotate, :1139-1305) with equal
+ * probability; moveTry on mol2others before and mol2othersTrial after.
+ * Returns the accepted energy change (0 on rejection); stats[6..9] = displace
+ * acc, tries, rotate acc, tries. */
+static double chain_move(const orc_topo* t, const double box[3], ogrid* g, orc_part* p, int n, const int* cfirst,
+                         const int* csize, int nch, double temper, const ochainmv* cm, uint64_t* rs, long* stats,
+                         orc_part* save, orc_vec* off) {
+    int c = (int)(urand(rs) * nch), first, size, a, disp;
+    double eo, en;
+    if (c >= nch) c = nch - 1;
+    first = cfirst[c];
+    size = csize[c];
+    disp = urand(rs) < 0.5;
+    eo = grid_mol(t, box, g, p, first, size);
+    for (a = 0; a < size; a++) save[a] = p[first + a];
+    if (disp) {
+        /* |dr| = chain_mx exactly in a uniform direction, :1088-1091 */
+        orc_vec dr = rand_sphere(rs);
+        for (a = 0; a < size; a++) {
+            p[first + a].pos.x += dr.x * cm->chain_mx / box[0];
+            p[first + a].pos.y += dr.y * cm->chain_mx / box[1];
+            p[first + a].pos.z += dr.z * cm->chain_mx / box[2];
+        }
+    } else {
+        /* clusterRotate, :1258-1305: axis, then the quaternion half-angle
+         * chain_angle * U and a random sign; positions rotate about the
+         * centre in real units, with the axes and patch directions */
+        orc_vec ax = rand_sphere(rs), cen, base = p[first].pos;
+        double vc = cos(cm->chain_angle * urand(rs));
+        double vs = (urand(rs) < 0.5) ? sqrt(1.0 - vc * vc) : -sqrt(1.0 - vc * vc);
+        cen = chain_cm(t, p, first, size, off);
+        for (a = 0; a < size; a++) {
+            orc_part* q = &p[first + a];
+            orc_vec o = V((off[a].x - cen.x) * box[0], (off[a].y - cen.y) * box[1], (off[a].z - cen.z) * box[2]);
+            o = ROT(o, ax, vc, vs);
+            q->pos = V(base.x + cen.x + o.x / box[0], base.y + cen.y + o.y / box[1], base.z + cen.z + o.z / box[2]);
+            q->dir = ROT(q->dir, ax, vc, vs);
+            q->patchdir[0] = ROT(q->patchdir[0], ax, vc, vs);
+            orc_part_init(t, q);
+        }
+    }
+    en = grid_mol(t, box, g, p, first, size);
+    stats[disp ? 7 : 9]++;
+    if (!(en <= eo) && !(exp(-1.0 * (en - eo) / temper) > urand(rs))) {   /* moveTry, movecreator.h:175-187 */
+        for (a = 0; a < size; a++) p[first + a] = save[a];
+        return 0.0;
+    }
+    stats[disp ? 6 : 8]++;
+    for (a = 0; a < size; a++) grid_move(g, p, n, first + a);
+    return en - eo;
 }
 
 /* MoveCreator::pressureMove, movecreator.cpp:330-421 (ptype 0: one random
@@ -1331,19 +1466,49 @@ static double pressure_move(const orc_topo* t, double box[3], const orc_part* p,
 
 static double mc_run(const orc_topo* t, double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
                      const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
-                     long* stats, double* e_final, int ptype, double press, double edge_mx) {
+                     long* stats, double* e_final, int ptype, double press, double edge_mx, const ochainmv* cm) {
     orc_part* p = load_all(t, n, pos, dir, pd0, pd1, type);
     ogrid g;
     uint64_t rs = seed * 0x2545F4914F6CDD1Dull + 12345u;
     double e_start, e_end, edrift = 0.0, e_cur = 0.0;
     long sweep, step;
-    int i;
+    int i, nch = 0, maxsize = 1;
+    int* cfirst = NULL;
+    int* csize = NULL;
+    orc_part* save = NULL;
+    orc_vec* off = NULL;
     stats[0] = stats[1] = stats[2] = stats[3] = 0;
     if (ptype >= 0) stats[4] = stats[5] = 0;
+    if (cm) {
+        stats[6] = stats[7] = stats[8] = stats[9] = 0;
+        /* the molecules of more than one particle (ParticleVector::getChain,
+         * Conf.h:150-163); none: no chain moves (Inicializer::testChains) */
+        cfirst = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+        csize = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+        for (i = 0; i < n; i++) {
+            int m;
+            if (p[i].moltype < 0 || p[i].molpos != 0) continue;
+            m = t->mol[p[i].moltype].size;
+            if (m < 2 || i + m > n) continue;
+            cfirst[nch] = i;
+            csize[nch] = m;
+            if (m > maxsize) maxsize = m;
+            nch++;
+        }
+        save = (orc_part*)malloc(sizeof(orc_part) * (size_t)maxsize);
+        off = (orc_vec*)malloc(sizeof(orc_vec) * (size_t)maxsize);
+    }
     grid_build(&g, p, n, box, t->maxcut);
     e_start = (e_final || ptype >= 0) ? orc_energy_all_cells(t, box, p, n) : 0.0;
     for (sweep = 0; sweep < nsweeps; sweep++) {
         for (step = 0; step < n; step++) {
+            /* the step's move kind, Updater's step loop (updater.cpp:206-226):
+             * a chain move with probability chainprob (a per-step probability
+             * there, despite its "per sweep" description) */
+            if (cm && cm->chainprob > 0.0 && nch > 0 && urand(&rs) < cm->chainprob) {
+                edrift += chain_move(t, box, &g, p, n, cfirst, csize, nch, temper, cm, &rs, stats, save, off);
+                continue;
+            }
             /* MoveCreator::particleMove, movecreator.cpp:16-29 */
             int target = (int)(urand(&rs) * n);
             orc_part old = p[target];
@@ -1406,6 +1571,10 @@ static double mc_run(const orc_topo* t, double box[3], int n, double* pos, doubl
     }
     grid_free(&g);
     free(p);
+    free(cfirst);
+    free(csize);
+    free(save);
+    free(off);
     return e_end - e_start - edrift;
 }
 

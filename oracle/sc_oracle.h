@@ -180,6 +180,26 @@ double orc_mc_run_npt(const orc_topo* t, double box[3], int n,
                       long nsweeps, double temper, double transmx, double rot_half,
                       uint64_t seed, long* stats, double* e_final, int ptype, double press, double edge_mx);
 
+/* As orc_mc_run_npt (ptype < 0: no volume moves) with whole-molecule moves
+ * (MoveCreator::chainMove / chainDisplace / chainRotate, movecreator.cpp:
+ * 305-328, 1075-1305): each step is a chain move with probability chainprob;
+ * chain_mx is the displacement length (already doubled, sim.h:366),
+ * chain_angle the maximum quaternion half-angle (sim.h:362).  The rotation
+ * centre is taken on minimum-image unwrapped coordinates.  stats needs 10
+ * entries: [6..9] = chain displace acc, tries, rotate acc, tries. */
+double orc_mc_run_chains(const orc_topo* t, double box[3], int n,
+                         double* pos, double* dir, double* pd0, double* pd1, const int* type,
+                         long nsweeps, double temper, double transmx, double rot_half,
+                         double chainprob, double chain_mx, double chain_angle,
+                         uint64_t seed, long* stats, double* e_final, int ptype, double press, double edge_mx);
+/* mol2others / mol2othersTrial (totalenergycalculator.h:438-452, 484-497):
+ * non-bonded energy of the molecule [first, first + size) with every
+ * particle outside it (part1 = the molecule's particle). */
+double orc_mol2others(const orc_topo* t, const double box[3], const orc_part* p, int n, int first, int size);
+int orc_soa_mol2others(const orc_topo* t, const double box[3], int n, const double* pos, const double* dir,
+                       const double* pd0, const double* pd1, const int* type, int m, const int* first,
+                       const int* size, double* out);
+
 /* ---- observables ---- */
 /* nematic order S2 = -2 * middle eigenvalue of Q (Tests/sc35.c:8563-8603) */
 double orc_nematic_s2(int n, const double* dir);

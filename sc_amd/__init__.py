@@ -25,7 +25,10 @@ class SweepStats(ctypes.Structure):
     _fields_ = [("trans_acc", ctypes.c_uint64), ("trans_try", ctypes.c_uint64), ("rot_acc", ctypes.c_uint64),
                 ("rot_try", ctypes.c_uint64), ("left_cell", ctypes.c_uint64), ("stage1_acc", ctypes.c_uint64),
                 ("stage1_try", ctypes.c_uint64), ("stage2_acc", ctypes.c_uint64), ("stage2_try", ctypes.c_uint64),
-                ("energy_change", ctypes.c_double), ("bond_rejected", ctypes.c_uint64)]
+                ("energy_change", ctypes.c_double), ("bond_rejected", ctypes.c_uint64),
+                ("chain_disp_acc", ctypes.c_uint64), ("chain_disp_try", ctypes.c_uint64),
+                ("chain_rot_acc", ctypes.c_uint64), ("chain_rot_try", ctypes.c_uint64),
+                ("chain_rejected", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -69,6 +72,11 @@ SIGNATURES = {
     "sc_energy_one": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "sc_energy_one_trial": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D, _D]),
     "sc_commit_particle": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D]),
+    "sc_energy_mol": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
+    "sc_energy_mol_trial": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]),
+    "sc_commit_mol": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, _D, _D]),
+    "sc_set_chain_moves": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "sc_get_chain_extent": (ctypes.c_int, [_P, _D]),
     "sc_energy_per_particle": (ctypes.c_int, [_P, _D]),
     "sc_energy_all_trial": (ctypes.c_int, [_P, _D, _D]),
     "sc_commit_box": (ctypes.c_int, [_P, _D]),
@@ -347,6 +355,36 @@ class GpuSystem:
     def update_particle(self, target, pos, dir, pd0):
         self._ck(self._L.sc_commit_particle(self._h, int(target), _d(_vec3(pos)), _d(_vec3(dir)), _d(_vec3(pd0))),
                  "commit_particle")
+
+    def mol2others(self, first, size):
+        """TotalEMatrix::mol2others: molecule [first, first + size) with the rest"""
+        e = ctypes.c_double()
+        self._ck(self._L.sc_energy_mol(self._h, int(first), int(size), ctypes.byref(e)), "energy_mol")
+        return e.value
+
+    def mol2othersTrial(self, first, size, pos, dir, pd0):
+        """the same with the molecule's particles at trial states (size x 3 each)"""
+        pos, dir, pd0 = [np.ascontiguousarray(a, np.float64).reshape(int(size), 3) for a in (pos, dir, pd0)]
+        e = ctypes.c_double()
+        self._ck(self._L.sc_energy_mol_trial(self._h, int(first), int(size), _d(pos), _d(dir), _d(pd0),
+                                             ctypes.byref(e)), "energy_mol_trial")
+        return e.value
+
+    def update_mol(self, first, size, pos, dir, pd0):
+        pos, dir, pd0 = [np.ascontiguousarray(a, np.float64).reshape(int(size), 3) for a in (pos, dir, pd0)]
+        self._ck(self._L.sc_commit_mol(self._h, int(first), int(size), _d(pos), _d(dir), _d(pd0)), "commit_mol")
+
+    def set_chain_moves(self, chainprob, chain_mx, chain_angle, extent=0.0):
+        """whole-molecule moves in sweep(): chainprob per step, displacement
+        length chain_mx (doubled chainmmx), maximum quaternion half-angle
+        chain_angle (radians); extent 0 = 1.2 x the widest molecule now"""
+        self._ck(self._L.sc_set_chain_moves(self._h, float(chainprob), float(chain_mx), float(chain_angle),
+                                            float(extent)), "set_chain_moves")
+
+    def chain_extent(self):
+        e = ctypes.c_double()
+        self._ck(self._L.sc_get_chain_extent(self._h, ctypes.byref(e)), "get_chain_extent")
+        return e.value
 
     def update_box(self, box_new):
         self._ck(self._L.sc_commit_box(self._h, _d(_vec3(box_new))), "commit_box")

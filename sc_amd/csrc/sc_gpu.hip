@@ -621,14 +621,17 @@ __global__ __launch_bounds__(1024) void k_reduce(int n, const double* __restrict
 
 // --------------------------------------------------------- energy (queries)
 // One wave per query particle: its energy with every particle of the stencil
-// around its cell except `exclude` (the particle itself).  The query state is
-// given explicitly (trial states) or taken from a slot.
+// around its cell except those of ID ex_lo..ex_hi (the particle itself, or
+// its whole molecule for mol2others), plus its bonds when it is queried alone
+// (ex_lo == ex_hi).  The query state is given explicitly (trial states) or
+// taken from a slot.
 struct Query {
     uint32_t u[3];
     int t;
     float4 o;
     float2 q;
-    int exclude;
+    int id;            // the query particle's ID
+    int ex_lo, ex_hi;  // IDs left out
 };
 
 __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __restrict__ qs, const int* __restrict__ qslot,
@@ -646,7 +649,7 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
         int s = qslot[qi];
         uint4 p = P[s];
         q.u[0] = p.x; q.u[1] = p.y; q.u[2] = p.z; q.t = p.w;
-        q.o = O[s]; q.q = Q[s]; q.exclude = ID[s];
+        q.o = O[s]; q.q = Q[s]; q.id = q.ex_lo = q.ex_hi = ID[s];
     } else {
         q = qs[qi];
     }
@@ -660,7 +663,8 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
         int cc = stencil_cell(g, cx, cy, cz, si, hm);
         int b = starts[cc], e = starts[cc + 1];
         for (int k = b + lane; k < e; k += WAVE) {
-            if (ID[k] == q.exclude) continue;
+            int idk = ID[k];
+            if (idk >= q.ex_lo && idk <= q.ex_hi) continue;
             uint4 pb4 = P[k];
             V3<float> r = fx_image_c(q.u[0], q.u[1], q.u[2], pb4.x, pb4.y, pb4.z, sc.Lh, sc.Ll);
             if (dot(r, r) > sc.sqmaxcut) continue;
@@ -668,9 +672,9 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
             acc += (double)pair_energy<float>(pp[q.t * sc.nt + pb4.w], pa, pb, r, sc.sqmaxcut);
         }
     }
-    // chain partners, wherever they are (the query's ID is `exclude`)
-    if (ct.cw && lane == 0)
-        acc += bonds_global(ct, sc.nt, sc.Ld, q.exclude, make_uint4(q.u[0], q.u[1], q.u[2], (uint32_t)q.t), q.o, q.q,
+    // chain partners, wherever they are
+    if (ct.cw && lane == 0 && q.ex_lo == q.ex_hi)
+        acc += bonds_global(ct, sc.nt, sc.Ld, q.id, make_uint4(q.u[0], q.u[1], q.u[2], (uint32_t)q.t), q.o, q.q,
                             15, slot, P, O, Q);
     double s = wave_sum(acc);
     if (lane == 0) out[qi] = s;
@@ -1640,6 +1644,391 @@ __global__ void k_nematic_q(int n, const float4* __restrict__ O, double* __restr
     }
 }
 
+// ------------------------------------------------------------- chain moves
+// Whole-molecule moves (MoveCreator::chainMove, movecreator.cpp:305-328;
+// chainDisplace :1075-1137; chainRotate + clusterRotate :1139-1305) on a
+// coarse checkerboard of their own.  A molecule is anchored at its centre
+// (clusterCM, :1307-1324: weighted by Ia_param::volume, taken here on the
+// coordinates unwrapped by the minimum image from its first particle); its
+// extent is the largest centre-to-particle distance.  Coarse cells are at
+// least 2 rho + maxcut wide and coloured 2x2x2: two molecules moved in one
+// pass have their particles within rho of anchors in different active
+// cells, before and after every move, so they stay beyond maxcut of each
+// other and each active cell runs an independent Metropolis chain.  A move
+// is rejected when the molecule is wider than rho or its anchor would leave
+// its coarse cell (the reverse move meets the same conditions).
+#define CHAIN_MAXM 64     // particles per molecule
+#define CHAIN_LCAP 4096   // molecules per coarse cell
+
+struct ChainMoveParams {
+    double temper, chain_mx, chain_angle, rho2;
+    double frac;             // trials per molecule and sweep
+    uint32_t key0, key1;
+    uint32_t pass_lo, pass_hi;
+    uint32_t stamp;          // marks the particles moved in this pass
+    int ox, oy, oz;          // colour of the pass
+    int pa[3];               // colour period per axis (1 on an axis of one cell)
+    int nact[3];             // active coarse cells per axis
+    int nchain;
+};
+
+__device__ __forceinline__ double u01_32(uint32_t x) { return (double)x * 2.3283064365386963e-10; }
+
+__device__ __forceinline__ V3<double> unit_from_d(double a, double b) {
+    double z = 1.0 - 2.0 * a;
+    double s = sqrt(fmax(0.0, 1.0 - z * z));
+    double ph = 6.283185307179586 * b;
+    return mk<double>(s * cos(ph), s * sin(ph), z);
+}
+
+// Centre of a molecule relative to its particle 0 (real units, FP64) and the
+// square of its extent.  Every lane of the wave passes its particle (lanes
+// without one pass valid = false) and gets the same result.
+struct ChainGeom {
+    double c[3];
+    double ext2;
+};
+__device__ ChainGeom chain_geom(uint4 ua, uint4 u0, bool valid, double v, const double* L) {
+    const double k = 2.3283064365386963e-10;
+    double dx = valid ? (double)(int32_t)(ua.x - u0.x) * (L[0] * k) : 0.0;
+    double dy = valid ? (double)(int32_t)(ua.y - u0.y) * (L[1] * k) : 0.0;
+    double dz = valid ? (double)(int32_t)(ua.z - u0.z) * (L[2] * k) : 0.0;
+    double wv = valid ? v : 0.0;
+    double vs = wave_sum(wv);
+    ChainGeom G;
+    G.c[0] = wave_sum(wv * dx) / vs;
+    G.c[1] = wave_sum(wv * dy) / vs;
+    G.c[2] = wave_sum(wv * dz) / vs;
+    double ex = dx - G.c[0], ey = dy - G.c[1], ez = dz - G.c[2];
+    double e2 = valid ? ex * ex + ey * ey + ez * ez : 0.0;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) e2 = fmax(e2, __shfl_xor(e2, o, WAVE));
+    G.ext2 = e2;
+    return G;
+}
+
+// the centre as a fixed-point position
+__device__ __forceinline__ uint4 chain_anchor_u(uint4 u0, const ChainGeom& G, const double* L) {
+    return make_uint4(u0.x + (uint32_t)(long long)rint(G.c[0] * (4294967296.0 / L[0])),
+                      u0.y + (uint32_t)(long long)rint(G.c[1] * (4294967296.0 / L[1])),
+                      u0.z + (uint32_t)(long long)rint(G.c[2] * (4294967296.0 / L[2])), 0u);
+}
+
+__device__ __forceinline__ bool chain_cell_active(const Grid& gc, const ChainMoveParams& cp, int cc) {
+    int x = cc % gc.nc[0], y = (cc / gc.nc[0]) % gc.nc[1], z = cc / (gc.nc[0] * gc.nc[1]);
+    return x % cp.pa[0] == cp.ox && y % cp.pa[1] == cp.oy && z % cp.pa[2] == cp.oz;
+}
+
+// One wave per molecule: its coarse cell, and for each of its particles the
+// cell while the molecule can move (extent <= rho), else -1.
+__global__ __launch_bounds__(256) void k_chain_anchor(int nchain, const int* __restrict__ cfirst,
+                                                      const int* __restrict__ csize, const int* __restrict__ slot,
+                                                      const uint4* __restrict__ P, const double* __restrict__ vol,
+                                                      SysConst sc, Grid gc, double rho2, int* __restrict__ ccell_chain,
+                                                      int* __restrict__ ccell_id) {
+    int c = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & (WAVE - 1));
+    if (c >= nchain) return;
+    int f = cfirst[c], m = csize[c];
+    bool v = lane < m;
+    uint4 u0 = P[slot[f]];
+    uint4 ua = v ? P[slot[f + lane]] : u0;
+    ChainGeom G = chain_geom(ua, u0, v, vol[ua.w], sc.Ld);
+    uint4 an = chain_anchor_u(u0, G, sc.Ld);
+    int cell = cell_of_u(gc, an.x, an.y, an.z);
+    if (lane == 0) ccell_chain[c] = cell;
+    if (v) ccell_id[f + lane] = G.ext2 <= rho2 ? cell : -1;
+}
+
+// Energy of one particle of molecule [f, f + m) at the state (ua, pa) with
+// everything outside its molecule (the wave's lanes hold partial sums): the
+// stencil of its fine cell, without the particles of molecules that may move
+// in another active coarse cell (they lie beyond maxcut) and without those
+// this block moved earlier in the pass (their cell list entries are stale:
+// they are taken from mlist instead).  Candidates inside sqmaxcut are
+// queued and evaluated 64 at a time.
+__device__ double chain_item_energy(uint4 ua, const DPart<float>& pa, int f, int m, int cell, const Grid& g,
+                                    const Grid& gc, const SysConst& sc, const ChainMoveParams& cp,
+                                    const int* __restrict__ starts, const uint4* P, const float4* O, const float2* Q,
+                                    const int* __restrict__ ID, const int* __restrict__ slot,
+                                    const PP32* __restrict__ pp, const TP32* __restrict__ tp,
+                                    const int* __restrict__ cfirst, const int* __restrict__ ccell_id,
+                                    const uint32_t* stamp, const int* mlist, const int* mpre, int nm, int* pre,
+                                    int* cbeg, int* queue) {
+    int lane = lane_id();
+    int c = cell_of_u(g, ua.x, ua.y, ua.z);
+    int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
+    int ns = stencil_size(g);
+    int carry = 0;
+    for (int b0 = 0; b0 < ns; b0 += WAVE) {
+        int i = b0 + lane, cnt = 0, hm = 0;
+        if (i < ns) {
+            int cc = stencil_cell(g, cx, cy, cz, i, hm);
+            int st = starts[cc];
+            cnt = starts[cc + 1] - st;
+            cbeg[i] = st;
+        }
+        int v = cnt;
+        for (int o = 1; o < WAVE; o <<= 1) {
+            int t = __shfl_up(v, o, WAVE);
+            if (lane >= o) v += t;
+        }
+        if (i < ns) pre[i] = carry + v - cnt;
+        carry += __shfl(v, WAVE - 1, WAVE);
+    }
+    wave_lds_sync();
+    int total = carry, nmb = nm > 0 ? mpre[nm] : 0, all = total + nmb;
+    double acc = 0.0;
+    int qn = 0;
+    for (int base = 0; base < all; base += WAVE) {
+        int idx = base + lane, s = -1;
+        if (idx < total) {
+            int lo = 0, hi = ns;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (pre[mid] <= idx) lo = mid; else hi = mid;
+            }
+            int k = cbeg[lo] + idx - pre[lo];
+            int id = ID[k];
+            bool skip = id >= f && id < f + m;
+            if (!skip) {
+                int cc = ccell_id[id];
+                if (cc >= 0 && cc != cell && chain_cell_active(gc, cp, cc)) skip = true;
+                else if (stamp[id] == cp.stamp) skip = true;
+            }
+            if (!skip) s = k;
+        } else if (idx < all) {
+            int j = idx - total, lo = 0, hi = nm;
+            while (hi - lo > 1) {
+                int mid = (lo + hi) >> 1;
+                if (mpre[mid] <= j) lo = mid; else hi = mid;
+            }
+            int id = cfirst[mlist[lo]] + j - mpre[lo];
+            if (!(id >= f && id < f + m)) s = slot[id];
+        }
+        bool need = false;
+        if (s >= 0) {
+            uint4 ub = P[s];
+            V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
+            need = dot(r, r) <= sc.sqmaxcut;
+        }
+        unsigned long long msk = __ballot(need);
+        if (need) queue[qn + wave_prefix(msk)] = s;
+        qn += __popcll(msk);
+        wave_lds_sync();
+        bool last = base + WAVE >= all;
+        if (qn >= WAVE || (last && qn > 0)) {
+            int take = last ? qn : WAVE;
+            for (int e0 = 0; e0 < take; e0 += WAVE) {
+                int e = e0 + lane;
+                if (e < take) {
+                    int sb = queue[e];
+                    uint4 ub = P[sb];
+                    V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
+                    DPart<float> pb = load_part<float>(O[sb], Q[sb], ub.w, tp);
+                    acc += (double)pair_energy<float>(pp[ua.w * sc.nt + ub.w], pa, pb, r, sc.sqmaxcut);
+                }
+            }
+            wave_lds_sync();
+            int rest = qn - take;
+            int cq = lane < rest ? queue[take + lane] : 0;
+            wave_lds_sync();
+            if (lane < rest) queue[lane] = cq;
+            wave_lds_sync();
+            qn = rest;
+        }
+    }
+    return acc;
+}
+
+// LDS of k_chain_pass
+__host__ __device__ constexpr size_t chain_pass_lds(int nw) {
+    return (size_t)CHAIN_MAXM * (2 * 16 + 2 * 16 + 2 * 8) + 2 * nw * 8 +
+           sizeof(int) * ((size_t)3 * CHAIN_LCAP + 1 + (size_t)nw * (256 + QCAP) + nw + 4);
+}
+
+// One block of NW waves per active coarse cell: trials on the cell's
+// molecules, one after another.  Every thread draws the same numbers; wave 0
+// builds the trial state; the energy items (particle, old/new state) are
+// spread over the waves.
+template <int NW>
+__global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysConst sc, ChainMoveParams cp,
+                                                          const int* __restrict__ starts, uint4* P, float4* O,
+                                                          float2* Q, const int* __restrict__ ID,
+                                                          const int* __restrict__ slot, const PP32* __restrict__ pp,
+                                                          const TP32* __restrict__ tp, const double* __restrict__ vol,
+                                                          const int* __restrict__ cfirst, const int* __restrict__ csize,
+                                                          const int* __restrict__ ccell_chain,
+                                                          const int* __restrict__ ccell_id, uint32_t* stamp,
+                                                          unsigned long long* __restrict__ stats,
+                                                          double* __restrict__ de_sum, int* __restrict__ flag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    uint4* Uo = (uint4*)smem;
+    uint4* Un = Uo + CHAIN_MAXM;
+    float4* Oo = (float4*)(Un + CHAIN_MAXM);
+    float4* On = Oo + CHAIN_MAXM;
+    float2* Qo = (float2*)(On + CHAIN_MAXM);
+    float2* Qn = Qo + CHAIN_MAXM;
+    double* red = (double*)(Qn + CHAIN_MAXM);   // 2 NW
+    int* clist = (int*)(red + 2 * NW);          // the cell's molecules
+    int* mlist = clist + CHAIN_LCAP;            // molecules moved in this pass
+    int* mpre = mlist + CHAIN_LCAP;             // particle prefix of mlist (CHAIN_LCAP + 1)
+    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), w = tid >> 6;
+    int* pre = mpre + CHAIN_LCAP + 1 + w * 256; // this wave's stencil prefix and cell starts
+    int* cbeg = pre + 128;
+    int* queue = mpre + CHAIN_LCAP + 1 + NW * 256 + w * QCAP;
+    int* misc = mpre + CHAIN_LCAP + 1 + NW * (256 + QCAP);   // NW counts, reject code
+    int b = blockIdx.x;
+    int bx = b % cp.nact[0], by = (b / cp.nact[0]) % cp.nact[1], bz = b / (cp.nact[0] * cp.nact[1]);
+    int cell = (bx * cp.pa[0] + cp.ox) + gc.nc[0] * ((by * cp.pa[1] + cp.oy) + gc.nc[1] * (bz * cp.pa[2] + cp.oz));
+    // the cell's molecules in index order
+    int total = 0;
+    for (int base = 0; base < cp.nchain; base += WAVE * NW) {
+        int i = base + tid;
+        bool fl = i < cp.nchain && ccell_chain[i] == cell;
+        int pos = block_compact<NW>(fl, total, misc);
+        if (fl && pos < CHAIN_LCAP) clist[pos] = i;
+    }
+    if (total > CHAIN_LCAP) {
+        if (tid == 0) atomicOr(flag, 4);
+        return;
+    }
+    if (total == 0) return;
+    if (tid == 0) mpre[0] = 0;
+    __syncthreads();
+    U4 ctr;
+    ctr.y = (uint32_t)cell;
+    ctr.z = cp.pass_lo;
+    // trials: the cell's share of the sweep's chain moves, randomly rounded
+    ctr.x = 0xffffffffu;
+    ctr.w = (cp.pass_hi << 2) | 3u;
+    U4 rd = philox(ctr, cp.key0, cp.key1);
+    double mean = (double)total * cp.frac;
+    int ntrial = (int)floor(mean) + (u01d(rd.x, rd.y) < mean - floor(mean) ? 1 : 0);
+    unsigned long long n_dacc = 0, n_dtry = 0, n_racc = 0, n_rtry = 0, n_rej = 0;
+    double dsum = 0.0;
+    int nm = 0;
+    for (int t = 0; t < ntrial; t++) {
+        ctr.x = (uint32_t)t;
+        ctr.w = cp.pass_hi << 2;
+        U4 ra = philox(ctr, cp.key0, cp.key1);
+        int k = clist[(int)(((uint64_t)ra.x * (uint64_t)total) >> 32)];
+        bool disp = u01f(ra.y) < 0.5f;
+        int f = cfirst[k], m = csize[k];
+        bool fresh = stamp[f] != cp.stamp;
+        if (disp) n_dtry++; else n_rtry++;
+        if (tid < m) {
+            int s = slot[f + tid];
+            Uo[tid] = P[s];
+            Oo[tid] = O[s];
+            Qo[tid] = Q[s];
+        }
+        __syncthreads();
+        if (w == 0) {
+            bool v = lane < m;
+            int a = v ? lane : 0;
+            uint4 u0 = Uo[0], ua = Uo[a];
+            float4 oa = Oo[a];
+            float2 qa = Qo[a];
+            double va = vol[ua.w];
+            ChainGeom G = chain_geom(ua, u0, v, va, sc.Ld);
+            uint4 un = ua;
+            float4 on = oa;
+            float2 qn = qa;
+            if (disp) {
+                // |dr| = chain_mx exactly, uniform direction (chainDisplace, :1088-1091)
+                V3<double> d = unit_from_d(u01_32(ra.z), u01_32(ra.w));
+                un.x = ua.x + (uint32_t)(long long)rint(d.x * cp.chain_mx * (4294967296.0 / sc.Ld[0]));
+                un.y = ua.y + (uint32_t)(long long)rint(d.y * cp.chain_mx * (4294967296.0 / sc.Ld[1]));
+                un.z = ua.z + (uint32_t)(long long)rint(d.z * cp.chain_mx * (4294967296.0 / sc.Ld[2]));
+            } else {
+                // clusterRotate (:1258-1305): quaternion (cos a, sin a * axis),
+                // a = chain_angle * U, random sign; positions about the centre
+                // in real units, axes and patch directions with them
+                U4 c1 = ctr;
+                c1.w |= 1u;
+                U4 rb = philox(c1, cp.key0, cp.key1);
+                V3<double> axv = unit_from_d(u01_32(rb.x), u01_32(rb.y));
+                double vc = cos(cp.chain_angle * u01d(rb.z, rb.w));
+                double vs = sqrt(fmax(0.0, 1.0 - vc * vc));
+                if (ra.y & 1u) vs = -vs;
+                const double k32 = 2.3283064365386963e-10;
+                V3<double> o = mk<double>((double)(int32_t)(ua.x - u0.x) * (sc.Ld[0] * k32) - G.c[0],
+                                          (double)(int32_t)(ua.y - u0.y) * (sc.Ld[1] * k32) - G.c[1],
+                                          (double)(int32_t)(ua.z - u0.z) * (sc.Ld[2] * k32) - G.c[2]);
+                V3<double> o2 = qrot(o, axv, vc, vs);
+                un.x = u0.x + (uint32_t)(long long)rint((G.c[0] + o2.x) * (4294967296.0 / sc.Ld[0]));
+                un.y = u0.y + (uint32_t)(long long)rint((G.c[1] + o2.y) * (4294967296.0 / sc.Ld[1]));
+                un.z = u0.z + (uint32_t)(long long)rint((G.c[2] + o2.z) * (4294967296.0 / sc.Ld[2]));
+                V3<double> d = normalised(qrot(mk<double>(oa.x, oa.y, oa.z), axv, vc, vs));
+                V3<double> pq = qrot(mk<double>(oa.w, qa.x, qa.y), axv, vc, vs);
+                pq = normalised(perp(pq, d));
+                on = make_float4((float)d.x, (float)d.y, (float)d.z, (float)pq.x);
+                qn = make_float2((float)pq.y, (float)pq.z);
+            }
+            if (v) {
+                Un[lane] = un;
+                On[lane] = on;
+                Qn[lane] = qn;
+            }
+            uint4 un0 = make_uint4(__shfl(un.x, 0, WAVE), __shfl(un.y, 0, WAVE), __shfl(un.z, 0, WAVE), 0u);
+            ChainGeom Gn = chain_geom(un, un0, v, va, sc.Ld);
+            uint4 an = chain_anchor_u(un0, Gn, sc.Ld);
+            int rej = 0;
+            if (G.ext2 > cp.rho2 || Gn.ext2 > cp.rho2) rej = 1;
+            else if (cell_of_u(gc, an.x, an.y, an.z) != cell) rej = 2;
+            if (lane == 0) misc[NW] = rej;
+        }
+        __syncthreads();
+        if (misc[NW] != 0) {
+            n_rej++;
+            __syncthreads();
+            continue;
+        }
+        double acc_o = 0.0, acc_n = 0.0;
+        for (int it = w; it < 2 * m; it += NW) {
+            int a = it >> 1;
+            bool st = (it & 1) != 0;
+            uint4 ua = st ? Un[a] : Uo[a];
+            DPart<float> pa = load_part<float>(st ? On[a] : Oo[a], st ? Qn[a] : Qo[a], ua.w, tp);
+            double e = chain_item_energy(ua, pa, f, m, cell, g, gc, sc, cp, starts, P, O, Q, ID, slot, pp, tp, cfirst,
+                                         ccell_id, stamp, mlist, mpre, nm, pre, cbeg, queue);
+            if (st) acc_n += e; else acc_o += e;
+        }
+        double eo = block_sum<NW>(acc_o, red), en = block_sum<NW>(acc_n, red + NW);
+        ctr.w = (cp.pass_hi << 2) | 2u;
+        U4 rc = philox(ctr, cp.key0, cp.key1);
+        double de = en - eo;
+        // moveTry (movecreator.h:175-187) on mol2others before and after
+        bool acc = de <= 0.0 || exp(-de / cp.temper) > u01d(rc.x, rc.y);
+        if (acc) {
+            if (disp) n_dacc++; else n_racc++;
+            dsum += de;
+            if (tid < m) {
+                int s = slot[f + tid];
+                P[s] = Un[tid];
+                O[s] = On[tid];
+                Q[s] = Qn[tid];
+                stamp[f + tid] = cp.stamp;
+            }
+            if (fresh) {
+                if (tid == 0) {
+                    mlist[nm] = k;
+                    mpre[nm + 1] = mpre[nm] + m;
+                }
+                nm++;
+            }
+        }
+        __syncthreads();
+    }
+    int rep = (int)(blockIdx.x & (STAT_REP - 1));
+    if (tid < 5) {
+        unsigned long long v = tid == 0 ? n_dacc : tid == 1 ? n_dtry : tid == 2 ? n_racc : tid == 3 ? n_rtry : n_rej;
+        if (v) atomicAdd(&stats[rep * STAT_STRIDE + 10 + tid], v);
+    } else if (tid == 5) {
+        atomicAdd(&de_sum[rep * STAT_STRIDE], dsum);
+    }
+}
+
 // plain streaming copy: the calibration kernel for HBM byte counters
 __global__ void k_copy(long n16, const uint4* __restrict__ a, uint4* __restrict__ b) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n16; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
@@ -1765,6 +2154,17 @@ struct sc_ctx {
     double e_track = 0.0;
     int e_age = 0;                      // volume moves since the last full evaluation
     int sweep_nw = 0;                   // waves per active cell in the sweep kernels (1, 2, 4; 0 = by mode)
+    // whole-molecule moves of sc_sweep
+    std::vector<int> chain_first, chain_size;   // molecules of more than one particle
+    int nchain_dev = 0;                 // molecules uploaded below
+    int* d_cfirst = nullptr;
+    int* d_csize = nullptr;
+    int* d_ccell_chain = nullptr;       // per molecule: coarse cell of its anchor
+    int* d_ccell_id = nullptr;          // per ID: that cell while the molecule can move, else -1
+    uint32_t* d_stamp = nullptr;        // per ID: stamp of the pass that last moved it
+    double* d_vol = nullptr;            // Ia_param::volume per dense type
+    uint32_t chain_stamp = 0;
+    double chainprob = 0.0, chain_mx = 0.0, chain_angle = 0.0, chain_rho = 0.0;
 };
 
 static ChainTab chain_tab(const sc_ctx* ctx) {
@@ -1913,6 +2313,9 @@ static int upload_tables(sc_ctx* ctx) {
 // (Conf.h:97-147).  A configuration of another layout cannot carry chains.
 static int upload_chains(sc_ctx* ctx) {
     ctx->bonds = ctx->molecules = false;
+    ctx->chain_first.clear();
+    ctx->chain_size.clear();
+    ctx->nchain_dev = 0;
     const schost::Topology& T = ctx->topo;
     bool multi = false;
     for (const auto& m : T.molecules) multi |= m.types.size() > 1;
@@ -1928,6 +2331,10 @@ static int upload_chains(sc_ctx* ctx) {
             bool second = size > 1 && m.bond2c >= 0.0;
             for (long c = 0; c < sy.second; c++)
                 for (int pos = 0; pos < size; pos++) {
+                    if (pos == 0 && size > 1) {   // ParticleVector::getChain, Conf.h:150-163
+                        ctx->chain_first.push_back((int)cw.size());
+                        ctx->chain_size.push_back(size);
+                    }
                     int w = (int)mi << 8;
                     if (nearb && pos > 0) w |= 1;
                     if (nearb && pos + 1 < size) w |= 2;
@@ -2093,6 +2500,7 @@ static int check_flag(sc_ctx* ctx, const char* what) {
     HIPCK(hipStreamSynchronize(ctx->stream));
     if (f) {
         HIPCK(hipMemset(ctx->d_flag, 0, sizeof(int)));
+        if (f & 4) return fail(ctx, SC_ECAPACITY, std::string(what) + ": a coarse cell held more than 4096 molecules");
         return fail(ctx, SC_ECAPACITY, std::string(what) + ": a stencil held more particles than the stage capacity");
     }
     return SC_OK;
@@ -2239,7 +2647,8 @@ static void sc_destroy_impl(sc_ctx* ctx) {
                     ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, ctx->mlp_img, ctx->tr_x, ctx->tr_y, ctx->tr_e, ctx->act,
                     ctx->bx, ctx->d_loss, ctx->d_pcount, ctx->d_pi, ctx->d_pj, ctx->d_pe, ctx->d_label, ctx->d_q6,
                     ctx->d_changed, ctx->d_rg, ctx->d_rp, ctx->d_rep_nonempty, ctx->d_rep_stats, ctx->d_rep_dsum,
-                    ctx->d_rep_e, ctx->d_cw, ctx->d_mol, ctx->d_bond};
+                    ctx->d_rep_e, ctx->d_cw, ctx->d_mol, ctx->d_bond, ctx->d_cfirst, ctx->d_csize,
+                    ctx->d_ccell_chain, ctx->d_ccell_id, ctx->d_stamp, ctx->d_vol};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
@@ -2959,7 +3368,7 @@ static Query make_query(sc_ctx* ctx, int target, const double pos[3], const doub
     q.t = ctx->ref2dense[ctx->ptypes[target]];
     q.o = make_float4((float)d[0], (float)d[1], (float)d[2], (float)p[0]);
     q.q = make_float2((float)p[1], (float)p[2]);
-    q.exclude = target;
+    q.id = q.ex_lo = q.ex_hi = target;
     return q;
 }
 
@@ -2997,6 +3406,79 @@ static int sc_commit_particle_impl(sc_ctx* ctx, int target, const double pos[3],
 
 int sc_commit_particle(sc_ctx* ctx, int target, const double pos[3], const double dir[3], const double pd0[3]) {
     SC_GUARD(ctx, sc_commit_particle_impl(ctx, target, pos, dir, pd0))
+}
+
+// mol2others / mol2othersTrial: one query per member of [first, first +
+// size), each leaving the whole molecule out; the stored state of the
+// members (pos == nullptr) or the given trial states
+static int sc_energy_mol_impl(sc_ctx* ctx, int first, int size, const double* pos, const double* dir,
+                              const double* pd0, double* e) {
+    if (!ctx || !e || size < 1 || (pos && (!dir || !pd0))) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (first < 0 || first + size > ctx->n) return fail(ctx, SC_EINVAL, "molecule out of range");
+    HIPCK(hipSetDevice(ctx->device));
+    std::vector<Query> q(size);
+    if (pos) {
+        for (int a = 0; a < size; a++) q[a] = make_query(ctx, first + a, pos + 3 * a, dir + 3 * a, pd0 + 3 * a);
+    } else {
+        std::vector<int> sl(size);
+        HIPCK(hipMemcpy(sl.data(), ctx->slot + first, size * sizeof(int), hipMemcpyDeviceToHost));
+        int c = ctx->cur;
+        for (int a = 0; a < size; a++) {
+            uint4 p;
+            HIPCK(hipMemcpy(&p, ctx->P[c] + sl[a], sizeof(uint4), hipMemcpyDeviceToHost));
+            HIPCK(hipMemcpy(&q[a].o, ctx->O[c] + sl[a], sizeof(float4), hipMemcpyDeviceToHost));
+            HIPCK(hipMemcpy(&q[a].q, ctx->Q[c] + sl[a], sizeof(float2), hipMemcpyDeviceToHost));
+            q[a].u[0] = p.x; q[a].u[1] = p.y; q[a].u[2] = p.z;
+            q[a].t = (int)p.w;
+            q[a].id = first + a;
+        }
+    }
+    for (int a = 0; a < size; a++) {
+        q[a].ex_lo = first;
+        q[a].ex_hi = first + size - 1;
+    }
+    std::vector<double> out(size);
+    int rc = run_queries(ctx, size, q.data(), nullptr, out.data());
+    if (rc) return rc;
+    double s = 0.0;
+    for (double v : out) s += v;
+    *e = s;
+    return SC_OK;
+}
+
+int sc_energy_mol(sc_ctx* ctx, int first, int size, double* e) {
+    SC_GUARD(ctx, sc_energy_mol_impl(ctx, first, size, nullptr, nullptr, nullptr, e))
+}
+
+int sc_energy_mol_trial(sc_ctx* ctx, int first, int size, const double* pos, const double* dir, const double* pd0,
+                        double* e) {
+    if (!pos) return SC_EINVAL;
+    SC_GUARD(ctx, sc_energy_mol_impl(ctx, first, size, pos, dir, pd0, e))
+}
+
+static int sc_commit_mol_impl(sc_ctx* ctx, int first, int size, const double* pos, const double* dir,
+                              const double* pd0) {
+    if (!ctx || !pos || !dir || !pd0 || size < 1) return SC_EINVAL;
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (first < 0 || first + size > ctx->n) return fail(ctx, SC_EINVAL, "molecule out of range");
+    HIPCK(hipSetDevice(ctx->device));
+    std::vector<int> sl(size);
+    HIPCK(hipMemcpy(sl.data(), ctx->slot + first, size * sizeof(int), hipMemcpyDeviceToHost));
+    int c = ctx->cur;
+    for (int a = 0; a < size; a++) {
+        Query q = make_query(ctx, first + a, pos + 3 * a, dir + 3 * a, pd0 + 3 * a);
+        uint4 p = make_uint4(q.u[0], q.u[1], q.u[2], (uint32_t)q.t);
+        HIPCK(hipMemcpy(ctx->P[c] + sl[a], &p, sizeof(uint4), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->O[c] + sl[a], &q.o, sizeof(float4), hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(ctx->Q[c] + sl[a], &q.q, sizeof(float2), hipMemcpyHostToDevice));
+    }
+    ctx->e_valid = false;
+    return build_grid(ctx, ctx->grid_valid ? ctx->g.r : 2, nullptr, ctx->box);
+}
+
+int sc_commit_mol(sc_ctx* ctx, int first, int size, const double* pos, const double* dir, const double* pd0) {
+    SC_GUARD(ctx, sc_commit_mol_impl(ctx, first, size, pos, dir, pd0))
 }
 
 static int sc_overlap_flags_impl(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, uint8_t* out, double* dmin) {
@@ -3077,6 +3559,181 @@ static void sweep_plan(uint64_t seed, uint64_t sw, int p, SweepPlan& pl) {
         int j = (int)(hs % (uint64_t)(k + 1));
         std::swap(pl.order[k], pl.order[j]);
     }
+}
+
+// ---- whole-molecule moves of the sweep ---------------------------------------
+static int chain_prepare(sc_ctx* ctx) {
+    int nch = (int)ctx->chain_first.size();
+    if (ctx->nchain_dev == nch && ctx->d_ccell_id) return SC_OK;
+    int rc;
+    if ((rc = dalloc(ctx, &ctx->d_cfirst, nch)) || (rc = dalloc(ctx, &ctx->d_csize, nch)) ||
+        (rc = dalloc(ctx, &ctx->d_ccell_chain, nch)) || (rc = dalloc(ctx, &ctx->d_ccell_id, ctx->n)) ||
+        (rc = dalloc(ctx, &ctx->d_stamp, ctx->n)) || (rc = dalloc(ctx, &ctx->d_vol, ctx->nt)))
+        return rc;
+    HIPCK(hipMemcpy(ctx->d_cfirst, ctx->chain_first.data(), nch * sizeof(int), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(ctx->d_csize, ctx->chain_size.data(), nch * sizeof(int), hipMemcpyHostToDevice));
+    HIPCK(hipMemset(ctx->d_ccell_id, 0xff, ctx->n * sizeof(int)));   // atomic particles: -1 for good
+    HIPCK(hipMemset(ctx->d_stamp, 0, ctx->n * sizeof(uint32_t)));
+    std::vector<double> vol(ctx->nt);
+    for (int d = 0; d < ctx->nt; d++) {
+        int r = ctx->dense2ref[d];
+        vol[d] = ctx->topo.ia[r][r].volume;
+    }
+    HIPCK(hipMemcpy(ctx->d_vol, vol.data(), ctx->nt * sizeof(double), hipMemcpyHostToDevice));
+    ctx->chain_stamp = 0;
+    ctx->nchain_dev = nch;
+    return SC_OK;
+}
+
+// largest centre-to-particle distance over the molecules (the centre of
+// k_chain_anchor, in FP64 on the host)
+static int chain_max_extent(sc_ctx* ctx, double* out) {
+    std::vector<uint4> hp;
+    std::vector<float4> ho;
+    std::vector<float2> hq;
+    int rc = download(ctx, hp, ho, hq);
+    if (rc) return rc;
+    const double k = 2.3283064365386963e-10;
+    double mx = 0.0;
+    for (size_t c = 0; c < ctx->chain_first.size(); c++) {
+        int f = ctx->chain_first[c], m = ctx->chain_size[c];
+        std::vector<double> d(3 * m);
+        double vs = 0.0, cen[3] = {0, 0, 0};
+        for (int a = 0; a < m; a++) {
+            const uint4& u = hp[f + a];
+            const uint4& u0 = hp[f];
+            int r = ctx->dense2ref[u.w];
+            double v = ctx->topo.ia[r][r].volume;
+            d[3 * a] = (double)(int32_t)(u.x - u0.x) * (ctx->box[0] * k);
+            d[3 * a + 1] = (double)(int32_t)(u.y - u0.y) * (ctx->box[1] * k);
+            d[3 * a + 2] = (double)(int32_t)(u.z - u0.z) * (ctx->box[2] * k);
+            for (int j = 0; j < 3; j++) cen[j] += v * d[3 * a + j];
+            vs += v;
+        }
+        for (int a = 0; a < m; a++) {
+            double e2 = 0.0;
+            for (int j = 0; j < 3; j++) {
+                double x = d[3 * a + j] - cen[j] / vs;
+                e2 += x * x;
+            }
+            mx = std::max(mx, sqrt(e2));
+        }
+    }
+    *out = mx;
+    return SC_OK;
+}
+
+static int sc_set_chain_moves_impl(sc_ctx* ctx, double chainprob, double chain_mx, double chain_angle, double extent) {
+    if (!ctx || !(chainprob >= 0.0 && chainprob <= 1.0) || !(chain_mx >= 0.0) || !(chain_angle >= 0.0) ||
+        !(chain_angle <= 1.5707963267948966) || !(extent >= 0.0))
+        return SC_EINVAL;
+    if (chainprob == 0.0) {
+        ctx->chainprob = 0.0;
+        return SC_OK;
+    }
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    if (ctx->nrep > 0) return fail(ctx, SC_ESTATE, "replica batches hold atomic molecules only");
+    for (int m : ctx->chain_size)
+        if (m > CHAIN_MAXM) return fail(ctx, SC_ECAPACITY, "chain moves take molecules of at most 64 particles");
+    HIPCK(hipSetDevice(ctx->device));
+    double rho = extent;
+    if (rho == 0.0) {
+        int rc = chain_max_extent(ctx, &rho);
+        if (rc) return rc;
+        rho = 1.2 * rho + 0.01;
+    }
+    ctx->chainprob = chainprob;
+    ctx->chain_mx = chain_mx;
+    ctx->chain_angle = chain_angle;
+    ctx->chain_rho = rho;
+    return SC_OK;
+}
+
+int sc_set_chain_moves(sc_ctx* ctx, double chainprob, double chain_mx, double chain_angle, double extent) {
+    SC_GUARD(ctx, sc_set_chain_moves_impl(ctx, chainprob, chain_mx, chain_angle, extent))
+}
+
+int sc_get_chain_extent(sc_ctx* ctx, double* extent) {
+    if (!ctx || !extent) return SC_EINVAL;
+    *extent = ctx->chain_rho;
+    return SC_OK;
+}
+
+// The chain-move passes of sweep `sw`: anchors on a coarse grid with its own
+// random shift, then one pass per colour (random order), each followed by a
+// rebuild of the fine cell list (moved particles may have changed cell).
+static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, int r, const uint32_t* shift) {
+    int nch = (int)ctx->chain_first.size();
+    if (nch == 0 || ctx->chainprob == 0.0) return SC_OK;
+    int rc;
+    if ((rc = chain_prepare(ctx))) return rc;
+    // coarse cells: 2 rho + maxcut.  Box edges: that much (no molecule reaches
+    // its own periodic image) and more than 4 rho (the minimum image from a
+    // molecule's first particle unwraps it)
+    double wmin = 2.0 * ctx->chain_rho + ctx->maxcut;
+    for (int k = 0; k < 3; k++)
+        if (!(ctx->box[k] >= wmin) || !(ctx->box[k] > 4.0 * ctx->chain_rho))
+            return fail(ctx, SC_EINVAL,
+                        "chain moves need box edges of at least 2 x molecule extent + maxcut and above 4 x the "
+                        "extent (a molecule must not reach its own periodic image)");
+    uint64_t cseed = mix64(seed ^ 0x6368616E6D6F7665ull);
+    SweepPlan plan;
+    sweep_plan(cseed, sw, 2, plan);
+    Grid gc;
+    gc.r = 1;
+    for (int k = 0; k < 3; k++) {
+        gc.nc[k] = grid_axis_cells(ctx->box[k], wmin, 2);
+        gc.sh[k] = plan.shift[k];
+    }
+    SysConst sc = sysconst(ctx, ctx->box);
+    double rho2 = ctx->chain_rho * ctx->chain_rho;
+    int c = ctx->cur;
+    hipLaunchKernelGGL(k_chain_anchor, dim3((nch + 3) / 4), dim3(256), 0, ctx->stream, nch, ctx->d_cfirst,
+                       ctx->d_csize, ctx->slot, ctx->P[c], ctx->d_vol, sc, gc, rho2, ctx->d_ccell_chain,
+                       ctx->d_ccell_id);
+    HIPCK(hipGetLastError());
+    ChainMoveParams cp;
+    cp.temper = temper;
+    cp.chain_mx = ctx->chain_mx;
+    cp.chain_angle = ctx->chain_angle;
+    cp.rho2 = rho2;
+    // MoveCreator::chainMove is tried with probability chainprob at each of
+    // the N steps of a sweep (updater.cpp:206-217): chainprob N trials
+    cp.frac = ctx->chainprob * (double)ctx->n / (double)nch;
+    cp.key0 = (uint32_t)cseed;
+    cp.key1 = (uint32_t)(cseed >> 32);
+    cp.nchain = nch;
+    int pa[3];
+    for (int k = 0; k < 3; k++) {
+        pa[k] = gc.nc[k] >= 2 ? 2 : 1;
+        cp.pa[k] = pa[k];
+        cp.nact[k] = gc.nc[k] / pa[k];
+    }
+    size_t lds = chain_pass_lds(4);
+    for (int ci = 0; ci < 8; ci++) {
+        int col = plan.order[ci];
+        cp.ox = col % 2;
+        cp.oy = (col / 2) % 2;
+        cp.oz = col / 4;
+        if (cp.ox >= pa[0] || cp.oy >= pa[1] || cp.oz >= pa[2]) continue;
+        uint64_t pass = sw * 32 + (uint64_t)ci;
+        cp.pass_lo = (uint32_t)pass;
+        cp.pass_hi = (uint32_t)(pass >> 32);
+        cp.stamp = ++ctx->chain_stamp;
+        if (cp.stamp == 0) {   // wrapped: clear the stamps once
+            HIPCK(hipMemsetAsync(ctx->d_stamp, 0, ctx->n * sizeof(uint32_t), ctx->stream));
+            cp.stamp = ctx->chain_stamp = 1;
+        }
+        c = ctx->cur;
+        int nblk = cp.nact[0] * cp.nact[1] * cp.nact[2];
+        hipLaunchKernelGGL((k_chain_pass<4>), dim3(nblk), dim3(WAVE * 4), lds, ctx->stream, ctx->g, gc, sc, cp,
+                           ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->slot, ctx->d_pp32,
+                           ctx->d_tp32, ctx->d_vol, ctx->d_cfirst, ctx->d_csize, ctx->d_ccell_chain, ctx->d_ccell_id,
+                           ctx->d_stamp, ctx->d_stats, ctx->d_dsum, ctx->d_flag);
+        HIPCK(hipGetLastError());
+        if ((rc = build_grid(ctx, r, shift, ctx->box))) return rc;
+    }
+    return SC_OK;
 }
 
 static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot_half, int r, uint64_t seed,
@@ -3176,6 +3833,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
             launches++;
         }
         HIPCK(hipGetLastError());
+        if ((rc = chain_passes(ctx, seed, sw, temper, r, shift))) return rc;
     }
     HIPCK(hipEventRecord(ctx->ev1, ctx->stream));
     unsigned long long hs[STAT_STRIDE];
@@ -3219,6 +3877,11 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         stats->stage2_try = hs[8];
         stats->energy_change = ds;
         stats->bond_rejected = hs[9];
+        stats->chain_disp_acc = hs[10];
+        stats->chain_disp_try = hs[11];
+        stats->chain_rot_acc = hs[12];
+        stats->chain_rot_try = hs[13];
+        stats->chain_rejected = hs[14];
     }
     return SC_OK;
 }

@@ -361,6 +361,9 @@ std::string Options::parse(const std::string& text) {
         if (!values.count(k)) return std::string("option '") + k + "' is not set!";
     double rotmx = values["rotmx"];
     if (rotmx < 0.0 || rotmx > 180) return "ERROR: Maximum orientation change must be in range 0 to 180.";
+    double chainrmx = values["chainrmx"];
+    if (chainrmx < 0.0 || chainrmx > 180)
+        return "ERROR: Maximum orientation change for chains must be in range 0 to 180.";
     int ptype = (int)values["ptype"];
     if (ptype < 0 || ptype > 5) return "ERROR: Unknown pressure coupling";
     const double PIH = 1.57079632679489661923132169163975;
@@ -368,6 +371,7 @@ std::string Options::parse(const std::string& text) {
     values["transmx"] *= 2.0;                          // sim.h:365
     values["edge_mx"] *= 2.0;                          // sim.h:364
     values["chainmmx"] *= 2.0;                         // sim.h:366
+    values["chain_angle"] = chainrmx / 180.0 * PIH;    // sim.h:362 (quaternion half-angle)
     return "";
 }
 

@@ -20,7 +20,8 @@ Sets:
   chain_golden.npz    p2p(i,j) over all ordered pairs, allToAll and oneToAll(i)
                       of chain systems covering every bond and angle term
                       (bond1, bond2, bondd, bondh, angle1, angle2; spheres at
-                      either end, PSC and CPSC segments)
+                      either end, PSC and CPSC segments), and mol2others /
+                      mol2othersTrial of each of their molecules
 
 `python tests/golden/make_golden.py chains` rewrites chain_golden.npz only.
 """
@@ -340,9 +341,13 @@ def make_chains(rng):
         e = pairs_query(top, box, pos, d, q, pi, pj)
         txt = po.ref_query("total", top, box, pos, d, q)
         vals = np.array([float(x) for x in txt.split()])
+        mol = np.array([[float(x) for x in l.split()] for l in
+                        po.ref_query("molecules", top, box, pos, d, q).strip().splitlines()])
         kw.update({name + "_top": top, name + "_box": box, name + "_pos": pos, name + "_dir": d, name + "_pd0": q,
                    name + "_types": types, name + "_i": pi, name + "_j": pj, name + "_e": e,
-                   name + "_eall": vals[0], name + "_eone": vals[1:]})
+                   name + "_eall": vals[0], name + "_eone": vals[1:],
+                   name + "_mfirst": mol[:, 0].astype(np.int32), name + "_msize": mol[:, 1].astype(np.int32),
+                   name + "_emol": mol[:, 2], name + "_emol_trial": mol[:, 3]})
         print("chains %s: %d particles, %d pairs, E_all %.6f" % (name, n, len(e), vals[0]))
     np.savez_compressed(os.path.join(HERE, "chain_golden.npz"), **kw)
 

@@ -57,6 +57,12 @@ def test_options_parser_postprocessing():
     h.load_options(W.options_text(transmx=0.15, rotmx=20.0))
     assert h.option("transmx") == pytest.approx(0.3)            # sim.h:365 doubles it
     assert h.option("rot_half") == pytest.approx(20.0 / 180 * np.pi / 2 * 0.5)   # sim.h:360
+    h.load_options(W.options_text().replace("chainmmx = 0.0", "chainmmx = 0.4").replace("chainrmx = 0.0",
+                                                                                          "chainrmx = 30.0"))
+    assert h.option("chainmmx") == pytest.approx(0.8)           # sim.h:366
+    assert h.option("chain_angle") == pytest.approx(30.0 / 180 * np.pi / 2)       # sim.h:362
+    with pytest.raises(sc_amd.SCError, match="chains must be in range"):
+        h.load_options(W.options_text().replace("chainrmx = 0.0", "chainrmx = 200.0"))
     bad = W.options_text().replace("seed = 12345\n", "")
     with pytest.raises(sc_amd.SCError, match="seed"):
         h.load_options(bad)
