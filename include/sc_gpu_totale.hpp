@@ -15,6 +15,11 @@
 //   volume move:   E_old = allToAll(); caller mutates conf->geo.box;
 //                  E_new = allToAllTrial(); accept -> update()
 //                  (MoveCreator::pressureMove, movecreator.cpp:330-421)
+//   chain move:    E_old = mol2others(chain); caller mutates the chain's
+//                  pvec entries; E_new = mol2othersTrial(chain);
+//                  accept -> update(chain)
+//                  (MoveCreator::chainDisplace / chainRotate,
+//                   movecreator.cpp:1075-1256)
 // It only reads the members the reference's Particle / Conf carry (pos as box
 // fractions, dir, patchdir[0], type; geo.box; pvec.size()).  Errors are
 // reported on stderr and return 0.0 / leave the state, as the reference's
@@ -87,6 +92,34 @@ class GpuTotalE {
     void update() {
         double box[3] = {conf_->geo.box.x, conf_->geo.box.y, conf_->geo.box.z};
         check(sc_commit_box(ctx_, box), "sc_commit_box");
+    }
+    // TotalEMatrix::mol2others (:417-453); a Molecule (structures.h:15, a
+    // vector<int>) lists consecutive particle indices
+    template <typename MolT>
+    double mol2others(MolT& mol) {
+        double e = 0;
+        check(sc_energy_mol(ctx_, mol[0], (int)mol.size(), &e), "sc_energy_mol");
+        return e;
+    }
+    // TotalEMatrix::mol2othersTrial (:455-500): the states the caller just
+    // wrote into the molecule's pvec entries (chainDisplace / chainRotate,
+    // movecreator.cpp:1075-1256)
+    template <typename MolT>
+    double mol2othersTrial(MolT& mol) {
+        int m = (int)mol.size();
+        std::vector<double> pos(3 * m), dir(3 * m), pd0(3 * m);
+        for (int a = 0; a < m; a++) read(mol[a], &pos[3 * a], &dir[3 * a], &pd0[3 * a]);
+        double e = 0;
+        check(sc_energy_mol_trial(ctx_, mol[0], m, pos.data(), dir.data(), pd0.data(), &e), "sc_energy_mol_trial");
+        return e;
+    }
+    // TotalEMatrix::update(Molecule) (:330-332)
+    template <typename MolT>
+    void update(const MolT& mol) {
+        int m = (int)mol.size();
+        std::vector<double> pos(3 * m), dir(3 * m), pd0(3 * m);
+        for (int a = 0; a < m; a++) read(mol[a], &pos[3 * a], &dir[3 * a], &pd0[3 * a]);
+        check(sc_commit_mol(ctx_, mol[0], m, pos.data(), dir.data(), pd0.data()), "sc_commit_mol");
     }
     // TotalE::p2p (:200-203)
     double p2p(int part1, int part2) {

@@ -19,5 +19,12 @@ int main() {
     conf.pvec.push_back({{0.1, 0.1, 0.1}, {1, 0, 0}, {{0, 1, 0}, {0, 0, 0}}, 1});
     GpuTotalE<Sim, Conf> e(&sim, &conf, "[Types]\nA 1 PSC 1 1 1 0.2 80 5 3 0\n[Molecules]\nA: {\nparticles: 1\n}\n[System]\nA 1\n", 0);
     printf("context %s\n", e.context() ? "created" : "absent");
+    if (!e.context()) {   // the molecule methods report the missing context and return 0
+        std::vector<int> mol(1, 0);
+        double a = e.mol2others(mol) + e.mol2othersTrial(mol);
+        e.update(mol);
+        e.update(0);
+        printf("molecule calls %g\n", a);
+    }
     return 0;
 }
