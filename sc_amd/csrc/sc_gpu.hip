@@ -1368,96 +1368,246 @@ __global__ void k_mlp_gen(long ns, uint32_t key0, uint32_t key1, const PP32* __r
     efull[s] = erep + ea;
 }
 
-// dense layer forward, one thread per (sample, unit): Z = X W + b, H = SiLU(Z)
-__global__ void k_dense_fwd(int B, int K, const float* __restrict__ X, const float* __restrict__ W,
-                            const float* __restrict__ b, float* __restrict__ Z, float* __restrict__ H) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= B * MLP_NH) return;
-    int s = t / MLP_NH, n = t - s * MLP_NH;
-    float z = b[n];
-    for (int k = 0; k < K; k++) z += X[s * K + k] * W[k * MLP_NH + n];
-    Z[t] = z;
-    H[t] = silu(z);
-}
+// ------------------------------------------------------------- training
+// One training step is two launches.  k_mlp_fwdbwd: each block takes TR_S
+// samples through the forward pass and back, with the weights and every
+// activation in LDS, and writes its share of the gradient (and of the
+// squared error) to its own row of `part`.  k_mlp_reduce_adam sums the rows
+// in a fixed order (the gradient is deterministic) and takes the Adam step.
+// FP32 throughout.  The hidden weight matrices sit in LDS with a row stride
+// of 65 floats, so the backward pass, whose lanes walk down a column, hits
+// 64 different banks.  Eight waves per block (two per SIMD) hide the LDS
+// latency; lane j of wave w owns column j of rows w, w + 8, ...
+#define TR_S 32
+#define TR_NW 8
+#define TR_WS 65
+constexpr int TL_W1 = 0, TL_B1 = TL_W1 + MLP_NF * MLP_NH, TL_W2 = TL_B1 + MLP_NH, TL_B2 = TL_W2 + MLP_NH * TR_WS,
+              TL_W3 = TL_B2 + MLP_NH, TL_B3 = TL_W3 + MLP_NH * TR_WS, TL_W4 = TL_B3 + MLP_NH, TL_B4 = TL_W4 + MLP_NH,
+              TL_X = TL_B4 + 4, TL_Z1 = TL_X + TR_S * MLP_NF, TL_H1 = TL_Z1 + TR_S * MLP_NH,
+              TL_Z2 = TL_H1 + TR_S * MLP_NH, TL_H2 = TL_Z2 + TR_S * MLP_NH, TL_Z3 = TL_H2 + TR_S * MLP_NH,
+              TL_H3 = TL_Z3 + TR_S * MLP_NH, TL_DO = TL_H3 + TR_S * MLP_NH, TL_LS = TL_DO + TR_S,
+              TL_END = TL_LS + TR_NW;
+constexpr size_t TRAIN_LDS_BYTES = sizeof(float) * TL_END;
+// a block's row of `part`: the gradient, then its sum of squared errors (rows 256-byte aligned)
+constexpr int TR_PART = ((MLP_NPARAM + 1 + 63) / 64) * 64;
+constexpr int TR_ROWS = TR_S / TR_NW;   // sample rows per thread
 
-// output layer + squared-error loss: out = H3 w4 + b4, dOut = 2 (out - y) / B
-__global__ void k_out_loss(int B, const float* __restrict__ H3, const float* __restrict__ W, const float* __restrict__ y,
-                           float* __restrict__ out, float* __restrict__ dout, double* __restrict__ loss) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    double l = 0.0;
-    if (s < B) {
-        float o = W[MLP_OFF_B4];
-        for (int k = 0; k < MLP_NH; k++) o += H3[s * MLP_NH + k] * W[MLP_OFF_W4 + k];
-        out[s] = o;
-        float e = o - y[s];
-        dout[s] = 2.0f * e / (float)B;
-        l = (double)e * (double)e;
+// z = b + in W (K inputs, W row stride ws), h = SiLU(z), for the block's TR_S
+// x 64 outputs; the inputs of a row are read four at a time (K a multiple
+// of 4, rows 16-byte aligned)
+template <int K>
+__device__ __forceinline__ void tr_dense(const float* in, const float* w, int ws, const float* b, float* z, float* h) {
+    int tid = (int)threadIdx.x, j = tid & 63, s0 = tid >> 6;
+    float acc[TR_ROWS];
+#pragma unroll
+    for (int r = 0; r < TR_ROWS; r++) acc[r] = b[j];
+    for (int k = 0; k < K; k += 4) {
+        float w0 = w[k * ws + j], w1 = w[(k + 1) * ws + j], w2 = w[(k + 2) * ws + j], w3 = w[(k + 3) * ws + j];
+#pragma unroll
+        for (int r = 0; r < TR_ROWS; r++) {
+            float4 x = *(const float4*)&in[(s0 + TR_NW * r) * K + k];
+            acc[r] += x.x * w0 + x.y * w1 + x.z * w2 + x.w * w3;
+        }
     }
-    l = wave_sum(l);
-    if (lane_id() == 0) atomicAdd(loss, l);
+#pragma unroll
+    for (int r = 0; r < TR_ROWS; r++) {
+        z[(s0 + TR_NW * r) * MLP_NH + j] = acc[r];
+        h[(s0 + TR_NW * r) * MLP_NH + j] = silu(acc[r]);
+    }
 }
 
-// dZ3 = dOut * w4 * SiLU'(Z3)
-__global__ void k_out_bwd(int B, const float* __restrict__ dout, const float* __restrict__ W, const float* __restrict__ Z3,
-                          float* __restrict__ dZ3) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= B * MLP_NH) return;
-    int s = t / MLP_NH, n = t - s * MLP_NH;
-    dZ3[t] = dout[s] * W[MLP_OFF_W4 + n] * silu_grad(Z3[t]);
+// dZprev = (dZ W^T) * SiLU'(Zprev), in place over Zprev; lane k = tid & 63
+__device__ __forceinline__ void tr_back(const float* dz, const float* w, float* zprev) {
+    int tid = (int)threadIdx.x, k = tid & 63, s0 = tid >> 6;
+    float acc[TR_ROWS];
+#pragma unroll
+    for (int r = 0; r < TR_ROWS; r++) acc[r] = 0.0f;
+    for (int j = 0; j < MLP_NH; j += 4) {
+        float w0 = w[k * TR_WS + j], w1 = w[k * TR_WS + j + 1], w2 = w[k * TR_WS + j + 2], w3 = w[k * TR_WS + j + 3];
+#pragma unroll
+        for (int r = 0; r < TR_ROWS; r++) {
+            float4 d = *(const float4*)&dz[(s0 + TR_NW * r) * MLP_NH + j];
+            acc[r] += d.x * w0 + d.y * w1 + d.z * w2 + d.w * w3;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < TR_ROWS; r++) {
+        float& zp = zprev[(s0 + TR_NW * r) * MLP_NH + k];
+        zp = acc[r] * silu_grad(zp);
+    }
 }
 
-// dZprev = (dZ W^T) * SiLU'(Zprev), one thread per (sample, input unit)
-__global__ void k_dense_bwd_x(int B, const float* __restrict__ dZ, const float* __restrict__ W,
-                              const float* __restrict__ Zprev, float* __restrict__ dZprev) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= B * MLP_NH) return;
-    int s = t / MLP_NH, k = t - s * MLP_NH;
-    float g = 0.0f;
-    for (int n = 0; n < MLP_NH; n++) g += dZ[s * MLP_NH + n] * W[k * MLP_NH + n];
-    dZprev[t] = g * silu_grad(Zprev[t]);
+// gradient block dW[k][j] = sum_s in[s][k] dz[s][j] (K rows) and db[j] =
+// sum_s dz[s][j]: lane j; wave w takes rows w R ... w R + R - 1 (read four
+// at a time) when K is a multiple of 32, else rows w, w + 8, ...
+template <int K>
+__device__ __forceinline__ void tr_grad(const float* in, const float* dz, float* out_w, float* out_b) {
+    int tid = (int)threadIdx.x, j = tid & 63, wv = tid >> 6;
+    constexpr bool blocked = K % (4 * TR_NW) == 0;
+    constexpr int R = blocked ? K / TR_NW : (K + TR_NW - 1) / TR_NW;
+    float acc[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) acc[i] = 0.0f;
+    float bs = 0.0f;
+    for (int s = 0; s < TR_S; s++) {
+        float d = dz[s * MLP_NH + j];
+        bs += d;
+        if constexpr (blocked) {
+#pragma unroll
+            for (int i = 0; i < R; i += 4) {
+                float4 x = *(const float4*)&in[s * K + wv * R + i];
+                acc[i] += x.x * d;
+                acc[i + 1] += x.y * d;
+                acc[i + 2] += x.z * d;
+                acc[i + 3] += x.w * d;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < R; i++)
+                if (wv + TR_NW * i < K) acc[i] += in[s * K + wv + TR_NW * i] * d;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        int k = blocked ? wv * R + i : wv + TR_NW * i;
+        if (k < K) out_w[k * MLP_NH + j] = acc[i];
+    }
+    if (wv == 0) out_b[j] = bs;
 }
 
-// weight gradients dW[k][n] = sum_s X[s][k] dZ[s][n], db[n] = sum_s dZ[s][n],
-// batch split over blocks (chunks of 256 samples) with atomic accumulation
-__global__ void k_dense_bwd_w(int B, int K, const float* __restrict__ X, const float* __restrict__ dZ,
-                              float* __restrict__ dW, float* __restrict__ db) {
-    int t = threadIdx.x + blockDim.x * blockIdx.x;   // (k, n) over gridDim.x blocks
-    int chunk = blockIdx.y;
-    if (t >= (K + 1) * MLP_NH) return;
-    int k = t / MLP_NH, n = t - k * MLP_NH;
-    int s0 = chunk * 256, s1 = min(B, s0 + 256);
-    float g = 0.0f;
-    if (k < K)
-        for (int s = s0; s < s1; s++) g += X[s * K + k] * dZ[s * MLP_NH + n];
-    else
-        for (int s = s0; s < s1; s++) g += dZ[s * MLP_NH + n];
-    if (k < K) atomicAdd(&dW[k * MLP_NH + n], g);
-    else atomicAdd(&db[n], g);
+__device__ __forceinline__ long tr_row(long a, long b, long n, int i) {
+    return (long)(((unsigned long long)a * (unsigned long long)i + (unsigned long long)b) % (unsigned long long)n);
 }
 
-// w4 / b4 gradients: dW4[k] = sum_s H3[s][k] dOut[s]; db4 = sum dOut
-__global__ void k_out_bwd_w(int B, const float* __restrict__ H3, const float* __restrict__ dout, float* __restrict__ G) {
-    int k = threadIdx.x;   // 0..64 (64 = bias)
-    int chunk = blockIdx.x;
-    int s0 = chunk * 256, s1 = min(B, s0 + 256);
-    float g = 0.0f;
-    if (k < MLP_NH)
-        for (int s = s0; s < s1; s++) g += H3[s * MLP_NH + k] * dout[s];
-    else if (k == MLP_NH)
-        for (int s = s0; s < s1; s++) g += dout[s];
-    if (k < MLP_NH) atomicAdd(&G[MLP_OFF_W4 + k], g);
-    else if (k == MLP_NH) atomicAdd(&G[MLP_OFF_B4], g);
+// forward and backward of samples rows(i) = (a i + b) mod n of X / Y, i in
+// [blockIdx.x TR_S, ...) below B; dOut = 2 (out - y) / B
+__global__ __launch_bounds__(WAVE * TR_NW) void k_mlp_fwdbwd(int B, long n, long a, long b,
+                                                             const float* __restrict__ X, const float* __restrict__ Y,
+                                                             const float* __restrict__ W, float* __restrict__ part) {
+    extern __shared__ __align__(16) float sm[];
+    constexpr int NT = WAVE * TR_NW;
+    const int tid = (int)threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int s0 = blockIdx.x * TR_S, ns = min(TR_S, B - s0);
+    // weights (hidden matrices to the padded stride)
+    for (int i = tid; i < MLP_NF * MLP_NH + MLP_NH; i += NT) sm[TL_W1 + i] = W[MLP_OFF_W1 + i];
+    for (int i = tid; i < MLP_NH * MLP_NH; i += NT) {
+        int k = i >> 6, j = i & 63;
+        sm[TL_W2 + k * TR_WS + j] = W[MLP_OFF_W2 + i];
+        sm[TL_W3 + k * TR_WS + j] = W[MLP_OFF_W3 + i];
+    }
+    if (tid < MLP_NH) {
+        sm[TL_B2 + tid] = W[MLP_OFF_B2 + tid];
+        sm[TL_B3 + tid] = W[MLP_OFF_B3 + tid];
+        sm[TL_W4 + tid] = W[MLP_OFF_W4 + tid];
+    }
+    if (tid == 0) sm[TL_B4] = W[MLP_OFF_B4];
+    // samples (rows past the batch are zero and carry no error)
+    for (int i = tid; i < TR_S * MLP_NF; i += NT) {
+        int s = i / MLP_NF, q = i - s * MLP_NF;
+        sm[TL_X + i] = s < ns ? X[tr_row(a, b, n, s0 + s) * MLP_NF + q] : 0.0f;
+    }
+    __syncthreads();
+    tr_dense<MLP_NF>(sm + TL_X, sm + TL_W1, MLP_NH, sm + TL_B1, sm + TL_Z1, sm + TL_H1);
+    __syncthreads();
+    tr_dense<MLP_NH>(sm + TL_H1, sm + TL_W2, TR_WS, sm + TL_B2, sm + TL_Z2, sm + TL_H2);
+    __syncthreads();
+    tr_dense<MLP_NH>(sm + TL_H2, sm + TL_W3, TR_WS, sm + TL_B3, sm + TL_Z3, sm + TL_H3);
+    __syncthreads();
+    // output layer and error: wave w takes rows w, w + 8, ..., its lanes
+    // split the dot product over the 64 units
+    float l = 0.0f;
+    for (int s = wv; s < TR_S; s += TR_NW) {
+        float o = sm[TL_H3 + s * MLP_NH + lane] * sm[TL_W4 + lane];
+#pragma unroll
+        for (int o2 = 32; o2 >= 1; o2 >>= 1) o += __shfl_xor(o, o2, WAVE);
+        if (lane == 0) {
+            float e = s < ns ? o + sm[TL_B4] - Y[tr_row(a, b, n, s0 + s)] : 0.0f;
+            sm[TL_DO + s] = 2.0f * e / (float)B;
+            l += e * e;
+        }
+    }
+    if (lane == 0) sm[TL_LS + wv] = l;
+    __syncthreads();
+    float* prow = part + (size_t)blockIdx.x * TR_PART;
+    if (tid == 0) {
+        float ls = 0.0f;
+#pragma unroll
+        for (int k = 0; k < TR_NW; k++) ls += sm[TL_LS + k];
+        prow[MLP_NPARAM] = ls;
+    }
+    {   // dZ3 = dOut w4 SiLU'(Z3), in place
+        int j = lane;
+        for (int s = wv; s < TR_S; s += TR_NW) {
+            float& z = sm[TL_Z3 + s * MLP_NH + j];
+            z = sm[TL_DO + s] * sm[TL_W4 + j] * silu_grad(z);
+        }
+    }
+    // gradient of the output layer (wave 0): four independent sums per lane
+    if (wv == 0) {
+        float g[4] = {0.0f, 0.0f, 0.0f, 0.0f}, gb = 0.0f;
+        for (int s = 0; s < TR_S; s += 4) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                float d = sm[TL_DO + s + q];
+                g[q] += sm[TL_H3 + (s + q) * MLP_NH + lane] * d;
+                gb += d;
+            }
+        }
+        prow[MLP_OFF_W4 + lane] = (g[0] + g[1]) + (g[2] + g[3]);
+        if (lane == 0) prow[MLP_OFF_B4] = gb;
+    }
+    __syncthreads();
+    tr_grad<MLP_NH>(sm + TL_H2, sm + TL_Z3, prow + MLP_OFF_W3, prow + MLP_OFF_B3);
+    tr_back(sm + TL_Z3, sm + TL_W3, sm + TL_Z2);   // Z2 <- dZ2
+    __syncthreads();
+    tr_grad<MLP_NH>(sm + TL_H1, sm + TL_Z2, prow + MLP_OFF_W2, prow + MLP_OFF_B2);
+    tr_back(sm + TL_Z2, sm + TL_W2, sm + TL_Z1);   // Z1 <- dZ1
+    __syncthreads();
+    tr_grad<MLP_NF>(sm + TL_X, sm + TL_Z1, prow + MLP_OFF_W1, prow + MLP_OFF_B1);
 }
 
-__global__ void k_adam(int n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float gi = g[i];
-    float mi = b1 * m[i] + (1.0f - b1) * gi, vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+// g[p] = sum of the blocks' rows; with lr > 0 also the Adam step; the squared
+// errors, divided by B, are added to *loss.  One block of 16 waves per 64
+// entries: lane = entry, wave w sums rows w, w + 16, ... (all its loads in
+// flight), the waves combine in a fixed order (the sum is deterministic).
+#define TR_RNW 16
+__global__ __launch_bounds__(WAVE * TR_RNW) void k_mlp_reduce_adam(int nblk, int B, const float* __restrict__ part,
+                                                                   float* __restrict__ g, float* __restrict__ w,
+                                                                   float* __restrict__ m, float* __restrict__ v,
+                                                                   float lr, float b1, float b2, float eps, float c1,
+                                                                   float c2, double* __restrict__ loss) {
+    __shared__ double red[TR_RNW][64];
+    int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+    int p = blockIdx.x * 64 + lane;
+    bool ok = p <= MLP_NPARAM;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    int k = wv;
+    for (; k + 3 * TR_RNW < nblk; k += 4 * TR_RNW) {
+        float x[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) x[q] = ok ? part[(size_t)(k + TR_RNW * q) * TR_PART + p] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q] += (double)x[q];
+    }
+    for (; k < nblk; k += TR_RNW) acc[0] += ok ? (double)part[(size_t)k * TR_PART + p] : 0.0;
+    red[wv][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    __syncthreads();
+    if (wv != 0 || !ok) return;
+    double sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < TR_RNW; q++) sum += red[q][lane];
+    if (p == MLP_NPARAM) {
+        if (loss) *loss += sum / (double)B;
+        return;
+    }
+    float gi = (float)sum;
+    g[p] = gi;
+    if (lr > 0.0f) {
+        float mi = b1 * m[p] + (1.0f - b1) * gi, vi = b2 * v[p] + (1.0f - b2) * gi * gi;
+        m[p] = mi;
+        v[p] = vi;
+        w[p] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
 }
 
 // FP32 master weights -> packed BF16 MFMA image (operand order of mlp_tile)
@@ -2107,9 +2257,10 @@ struct sc_ctx {
     float* tr_y = nullptr;
     float* tr_e = nullptr;
     long tr_n = 0;
-    float* act = nullptr;               // Z1 H1 Z2 H2 Z3 H3 dZa dZb (batch x 64 each)
-    float* bx = nullptr;                // batch features / targets / out / dout
+    float* bx = nullptr;                // a batch handed in by the caller: features, then targets
     int act_batch = 0;
+    float* d_part = nullptr;            // per block of a training step: gradient + squared error
+    int part_blocks = 0;
     double* d_loss = nullptr;
     int da_mode = DA_OFF;              // delayed-acceptance mode of sc_sweep
     // pair lists, clusters, order parameter
@@ -2644,7 +2795,7 @@ static void sc_destroy_impl(sc_ctx* ctx) {
                     ctx->Q[0], ctx->Q[1], ctx->ID[0], ctx->ID[1], ctx->slot, ctx->cell_tmp, ctx->src_of_pos,
                     ctx->starts, ctx->counts, ctx->fill, ctx->partial, ctx->d_out, ctx->d_flag, ctx->d_maxocc,
                     ctx->d_nonempty, ctx->d_stats, ctx->d_dsum, ctx->d_q, ctx->d_i, ctx->d_j, ctx->mlp_w,
-                    ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, ctx->mlp_img, ctx->tr_x, ctx->tr_y, ctx->tr_e, ctx->act,
+                    ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, ctx->mlp_img, ctx->tr_x, ctx->tr_y, ctx->tr_e, ctx->d_part,
                     ctx->bx, ctx->d_loss, ctx->d_pcount, ctx->d_pi, ctx->d_pj, ctx->d_pe, ctx->d_label, ctx->d_q6,
                     ctx->d_changed, ctx->d_rg, ctx->d_rp, ctx->d_rep_nonempty, ctx->d_rep_stats, ctx->d_rep_dsum,
                     ctx->d_rep_e, ctx->d_cw, ctx->d_mol, ctx->d_bond, ctx->d_cfirst, ctx->d_csize,
@@ -4533,48 +4684,30 @@ extern "C" int sc_mlp_training_data(sc_ctx* ctx, long first, long count, float* 
 }
 
 static int ensure_act(sc_ctx* ctx, int B) {
-    if (B <= ctx->act_batch) return SC_OK;
-    int rc;
-    if ((rc = dalloc(ctx, &ctx->act, (size_t)8 * B * MLP_NH)) || (rc = dalloc(ctx, &ctx->bx, (size_t)B * (MLP_NF + 3))))
-        return rc;
-    ctx->act_batch = B;
+    int nblk = (B + TR_S - 1) / TR_S, rc;
+    if (B > ctx->act_batch) {
+        if ((rc = dalloc(ctx, &ctx->bx, (size_t)B * (MLP_NF + 1)))) return rc;
+        ctx->act_batch = B;
+    }
+    if (nblk > ctx->part_blocks) {
+        if ((rc = dalloc(ctx, &ctx->d_part, (size_t)nblk * TR_PART))) return rc;
+        ctx->part_blocks = nblk;
+    }
     return SC_OK;
 }
 
-// forward + backward of one batch held in ctx->bx (x: B*12, then y: B), gradients into mlp_g
-static int mlp_fwd_bwd(sc_ctx* ctx, int B, double* loss) {
-    float* X = ctx->bx;
-    float* Y = X + (size_t)B * MLP_NF;
-    float* OUT = Y + B;
-    float* DOUT = OUT + B;
-    size_t L = (size_t)B * MLP_NH;
-    float *Z1 = ctx->act, *H1 = Z1 + L, *Z2 = H1 + L, *H2 = Z2 + L, *Z3 = H2 + L, *H3 = Z3 + L, *DA = H3 + L, *DB = DA + L;
-    const float* W = ctx->mlp_w;
-    float* G = ctx->mlp_g;
-    int nb = (int)((L + 255) / 256);
-    HIPCK(hipMemsetAsync(G, 0, MLP_NPARAM * sizeof(float), ctx->stream));
-    HIPCK(hipMemsetAsync(ctx->d_loss, 0, sizeof(double), ctx->stream));
-    SC_LAUNCH(k_dense_fwd, dim3(nb), dim3(256), 0, ctx->stream, B, MLP_NF, X, W + MLP_OFF_W1, W + MLP_OFF_B1, Z1, H1);
-    SC_LAUNCH(k_dense_fwd, dim3(nb), dim3(256), 0, ctx->stream, B, MLP_NH, H1, W + MLP_OFF_W2, W + MLP_OFF_B2, Z2, H2);
-    SC_LAUNCH(k_dense_fwd, dim3(nb), dim3(256), 0, ctx->stream, B, MLP_NH, H2, W + MLP_OFF_W3, W + MLP_OFF_B3, Z3, H3);
-    SC_LAUNCH(k_out_loss, dim3((B + 255) / 256), dim3(256), 0, ctx->stream, B, H3, W, Y, OUT, DOUT, ctx->d_loss);
-    int nchunk = (B + 255) / 256;
-    SC_LAUNCH(k_out_bwd_w, dim3(nchunk), dim3(128), 0, ctx->stream, B, H3, DOUT, G);
-    SC_LAUNCH(k_out_bwd, dim3(nb), dim3(256), 0, ctx->stream, B, DOUT, W, Z3, DA);
-    SC_LAUNCH(k_dense_bwd_w, dim3((65 * MLP_NH + 255) / 256, nchunk), dim3(256), 0, ctx->stream, B, MLP_NH, H2, DA,
-                       G + MLP_OFF_W3, G + MLP_OFF_B3);
-    SC_LAUNCH(k_dense_bwd_x, dim3(nb), dim3(256), 0, ctx->stream, B, DA, W + MLP_OFF_W3, Z2, DB);
-    SC_LAUNCH(k_dense_bwd_w, dim3((65 * MLP_NH + 255) / 256, nchunk), dim3(256), 0, ctx->stream, B, MLP_NH, H1, DB,
-                       G + MLP_OFF_W2, G + MLP_OFF_B2);
-    SC_LAUNCH(k_dense_bwd_x, dim3(nb), dim3(256), 0, ctx->stream, B, DB, W + MLP_OFF_W2, Z1, DA);
-    SC_LAUNCH(k_dense_bwd_w, dim3((13 * MLP_NH + 255) / 256, nchunk), dim3(256), 0, ctx->stream, B, MLP_NF, X, DA,
-                       G + MLP_OFF_W1, G + MLP_OFF_B1);
+// One step on B samples, rows (a i + b) mod n of X / Y: gradient into mlp_g;
+// with lr > 0 also the Adam step on mlp_w; the batch's mean squared error is
+// added to *loss (device, may be null)
+static int mlp_step(sc_ctx* ctx, int B, long n, long a, long b, const float* X, const float* Y, float lr, float c1,
+                    float c2, double* loss) {
+    int nblk = (B + TR_S - 1) / TR_S;
+    SC_LAUNCH(k_mlp_fwdbwd, dim3(nblk), dim3(WAVE * TR_NW), TRAIN_LDS_BYTES, ctx->stream, B, n, a, b, X, Y,
+              ctx->mlp_w, ctx->d_part);
+    SC_LAUNCH(k_mlp_reduce_adam, dim3((MLP_NPARAM + 1 + 63) / 64), dim3(WAVE * TR_RNW), 0, ctx->stream, nblk, B,
+              ctx->d_part,
+              ctx->mlp_g, ctx->mlp_w, ctx->mlp_m, ctx->mlp_v, lr, 0.9f, 0.999f, 1e-8f, c1, c2, loss);
     HIPCK(hipGetLastError());
-    if (loss) {
-        HIPCK(hipMemcpyAsync(loss, ctx->d_loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCK(hipStreamSynchronize(ctx->stream));
-        *loss /= B;
-    }
     return SC_OK;
 }
 
@@ -4587,9 +4720,13 @@ static int sc_mlp_gradients_impl(sc_ctx* ctx, int B, const float* x, const float
     if (rc) return rc;
     HIPCK(hipMemcpy(ctx->bx, x, (size_t)B * MLP_NF * sizeof(float), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(ctx->bx + (size_t)B * MLP_NF, y, (size_t)B * sizeof(float), hipMemcpyHostToDevice));
+    HIPCK(hipMemsetAsync(ctx->d_loss, 0, sizeof(double), ctx->stream));
+    if ((rc = mlp_step(ctx, B, B, 1, 0, ctx->bx, ctx->bx + (size_t)B * MLP_NF, 0.0f, 1.0f, 1.0f, ctx->d_loss)))
+        return rc;
     double l = 0;
-    if ((rc = mlp_fwd_bwd(ctx, B, &l))) return rc;
-    HIPCK(hipMemcpy(grads, ctx->mlp_g, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpyAsync(&l, ctx->d_loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipMemcpyAsync(grads, ctx->mlp_g, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
     if (loss) *loss = l;
     return SC_OK;
 }
@@ -4600,16 +4737,6 @@ extern "C" int sc_mlp_gradients(sc_ctx* ctx, int B, const float* x, const float*
 }
 
 extern "C" int sc_mlp_nparam(void) { return MLP_NPARAM; }
-
-// gather a batch: rows (a*i + b) mod nrows of the training set
-__global__ void k_gather(int B, long n, long a, long b, const float* __restrict__ X, const float* __restrict__ Y,
-                         float* __restrict__ bx) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= B) return;
-    long r = (long)(((unsigned long long)a * (unsigned long long)i + (unsigned long long)b) % (unsigned long long)n);
-    for (int q = 0; q < MLP_NF; q++) bx[(size_t)i * MLP_NF + q] = X[r * MLP_NF + q];
-    bx[(size_t)B * MLP_NF + i] = Y[r];
-}
 
 static long gcdl(long a, long b) { while (b) { long t = a % b; a = b; b = t; } return a; }
 
@@ -4624,8 +4751,7 @@ static int sc_mlp_train_impl(sc_ctx* ctx, int epochs, int batch, double lr, doub
     if (rc) return rc;
     long steps_per_epoch = ntr / batch, total = steps_per_epoch * epochs, step = 0;
     uint64_t h = mix64(seed);
-    double lsum = 0.0;
-    long lcount = 0;
+    HIPCK(hipMemsetAsync(ctx->d_loss, 0, sizeof(double), ctx->stream));
     for (int ep = 0; ep < epochs; ep++) {
         h = mix64(h);
         long a = (long)(h % (uint64_t)ntr) | 1;
@@ -4633,24 +4759,23 @@ static int sc_mlp_train_impl(sc_ctx* ctx, int epochs, int batch, double lr, doub
         h = mix64(h);
         long b0 = (long)(h % (uint64_t)ntr);
         for (long st = 0; st < steps_per_epoch; st++, step++) {
-            SC_LAUNCH(k_gather, dim3((batch + 255) / 256), dim3(256), 0, ctx->stream, batch, ntr, a,
-                               (long)((b0 + (unsigned long long)a * (unsigned long long)(st * batch)) % (unsigned long long)ntr),
-                               ctx->tr_x, ctx->tr_y, ctx->bx);
-            bool last_epoch = ep == epochs - 1;
-            double l = 0;
-            if ((rc = mlp_fwd_bwd(ctx, batch, last_epoch ? &l : nullptr))) return rc;
-            if (last_epoch) { lsum += l; lcount++; }
+            // batch rows (a i + b) mod ntr: a permutation of the training rows per epoch
+            long bs = (long)((b0 + (unsigned long long)a * (unsigned long long)(st * batch)) % (unsigned long long)ntr);
             // cosine decay from lr to lr_final
             double lrs = lr_final + 0.5 * (lr - lr_final) * (1.0 + cos(3.141592653589793 * (double)step / (double)total));
             ctx->adam_t++;
             float c1 = (float)(1.0 - pow(0.9, (double)ctx->adam_t)), c2 = (float)(1.0 - pow(0.999, (double)ctx->adam_t));
-            SC_LAUNCH(k_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, ctx->stream, MLP_NPARAM, ctx->mlp_w,
-                               ctx->mlp_g, ctx->mlp_m, ctx->mlp_v, (float)lrs, 0.9f, 0.999f, 1e-8f, c1, c2);
+            if ((rc = mlp_step(ctx, batch, ntr, a, bs, ctx->tr_x, ctx->tr_y, (float)lrs, c1, c2,
+                               ep == epochs - 1 ? ctx->d_loss : nullptr)))
+                return rc;
         }
     }
     HIPCK(hipGetLastError());
     if ((rc = mlp_repack(ctx))) return rc;
-    if (train_loss) *train_loss = lcount ? lsum / lcount : 0.0;
+    double lsum = 0.0;
+    HIPCK(hipMemcpyAsync(&lsum, ctx->d_loss, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    if (train_loss) *train_loss = steps_per_epoch ? lsum / (double)steps_per_epoch : 0.0;
     if (val_rmse) {
         *val_rmse = 0.0;
         if (nval > 0) {
