@@ -111,6 +111,34 @@ def npt_leg(a, transmx, rot_half):
         log("npt chains %s: %.2f steps/s (sweep + volume move), volume acc %.3f, eta %.4f -> %.4f" % (
             label, npt[label]["sweeps_per_s"], npt[label]["volume_acceptance"], npt["eta_start"],
             npt[label]["eta_end"]))
+    if a.npt_chainprob > 0:
+        # the exact sweep plus whole-molecule moves (chainMove) at the reference's usual option values
+        gn.set_config(bx0, p0, d0_, q0_, tc_)
+        mx_, ang_ = 2.0 * a.npt_chainmmx, a.npt_chainrmx / 180.0 * np.pi / 2.0   # sim.h:362, 366
+        gn.set_chain_moves(a.npt_chainprob, mx_, ang_)
+        npt_steps(5, 0, 10 ** 5)
+        gn.set_config(bx0, p0, d0_, q0_, tc_)
+        gn.set_chain_moves(a.npt_chainprob, mx_, ang_)
+        t1 = time.perf_counter()
+        acc, tot, bx = npt_steps(a.npt_steps, 0, 10 ** 6)
+        dt = time.perf_counter() - t1
+        lab = "exact_with_chain_moves"
+        npt[lab] = {"chainprob": a.npt_chainprob, "chainmmx": a.npt_chainmmx, "chainrmx_deg": a.npt_chainrmx,
+                    "extent": gn.chain_extent(), "sweeps_per_s": a.npt_steps / dt,
+                    "ms_per_step": 1e3 * dt / a.npt_steps, "ms_sweep": 1e3 * tot["sweep_s"] / a.npt_steps,
+                    "ms_volume_move": 1e3 * tot["volume_s"] / a.npt_steps,
+                    "volume_acceptance": acc / float(a.npt_steps),
+                    "chain_moves_per_sweep": (tot["chain_disp_try"] + tot["chain_rot_try"]) / float(a.npt_steps),
+                    "chain_displace_acceptance": tot["chain_disp_acc"] / max(1, tot["chain_disp_try"]),
+                    "chain_rotate_acceptance": tot["chain_rot_acc"] / max(1, tot["chain_rot_try"]),
+                    "chain_rejected": int(tot["chain_rejected"]),
+                    "eta_end": nch * W.chain_volume() / float(np.prod(bx)), "energy_per_particle": gn.allToAll() / nb_}
+        if "exact" in npt:
+            npt[lab]["ms_chain_moves_per_sweep"] = npt[lab]["ms_sweep"] - npt["exact"]["ms_sweep"]
+        log("npt chains %s: %.2f steps/s, %.0f chain moves per sweep (displace acc %.3f, rotate acc %.3f)" % (
+            lab, npt[lab]["sweeps_per_s"], npt[lab]["chain_moves_per_sweep"], npt[lab]["chain_displace_acceptance"],
+            npt[lab]["chain_rotate_acceptance"]))
+        gn.set_chain_moves(0.0, 0.0, 0.0)
     gn.close()
     return npt
 
@@ -139,6 +167,10 @@ def main():
     ap.add_argument("--pt-sweeps", type=int, default=100)
     ap.add_argument("--equil", type=int, default=4000, help="untimed exact sweeps before the legs")
     ap.add_argument("--no-npt", action="store_true", help="skip the NPT chain leg")
+    ap.add_argument("--npt-chainprob", type=float, default=0.1,
+                    help="chain-move probability per step of the NPT leg's chain-move run (0: no such run)")
+    ap.add_argument("--npt-chainmmx", type=float, default=0.1, help="options' chainmmx (displacement, doubled)")
+    ap.add_argument("--npt-chainrmx", type=float, default=5.0, help="options' chainrmx in degrees")
     ap.add_argument("--npt-chains", type=int, default=43690, help="3-bead chains of the NPT leg (131,070 beads)")
     ap.add_argument("--npt-steps", type=int, default=200, help="timed NPT steps (one sweep + one volume move)")
     ap.add_argument("--npt-warmup", type=int, default=50)

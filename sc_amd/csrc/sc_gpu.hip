@@ -1808,6 +1808,7 @@ __global__ void k_nematic_q(int n, const float4* __restrict__ O, double* __restr
 // is rejected when the molecule is wider than rho or its anchor would leave
 // its coarse cell (the reverse move meets the same conditions).
 #define CHAIN_MAXM 64     // particles per molecule
+#define CHAIN_NW 8        // waves per block of the pass (each old/new item of a 3-particle molecule its own)
 #define CHAIN_LCAP 4096   // molecules per coarse cell
 
 struct ChainMoveParams {
@@ -1869,13 +1870,13 @@ __device__ __forceinline__ bool chain_cell_active(const Grid& gc, const ChainMov
     return x % cp.pa[0] == cp.ox && y % cp.pa[1] == cp.oy && z % cp.pa[2] == cp.oz;
 }
 
-// One wave per molecule: its coarse cell, and for each of its particles the
-// cell while the molecule can move (extent <= rho), else -1.
+// One wave per molecule: its coarse cell (counted per cell), and for each of
+// its particles the cell while the molecule can move (extent <= rho), else -1.
 __global__ __launch_bounds__(256) void k_chain_anchor(int nchain, const int* __restrict__ cfirst,
                                                       const int* __restrict__ csize, const int* __restrict__ slot,
                                                       const uint4* __restrict__ P, const double* __restrict__ vol,
                                                       SysConst sc, Grid gc, double rho2, int* __restrict__ ccell_chain,
-                                                      int* __restrict__ ccell_id) {
+                                                      int* __restrict__ ccell_id, int* __restrict__ ccount) {
     int c = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & (WAVE - 1));
     if (c >= nchain) return;
     int f = cfirst[c], m = csize[c];
@@ -1885,7 +1886,10 @@ __global__ __launch_bounds__(256) void k_chain_anchor(int nchain, const int* __r
     ChainGeom G = chain_geom(ua, u0, v, vol[ua.w], sc.Ld);
     uint4 an = chain_anchor_u(u0, G, sc.Ld);
     int cell = cell_of_u(gc, an.x, an.y, an.z);
-    if (lane == 0) ccell_chain[c] = cell;
+    if (lane == 0) {
+        ccell_chain[c] = cell;
+        atomicAdd(&ccount[cell], 1);
+    }
     if (v) ccell_id[f + lane] = G.ext2 <= rho2 ? cell : -1;
 }
 
@@ -1894,8 +1898,11 @@ __global__ __launch_bounds__(256) void k_chain_anchor(int nchain, const int* __r
 // stencil of its fine cell, without the particles of molecules that may move
 // in another active coarse cell (they lie beyond maxcut) and without those
 // this block moved earlier in the pass (their cell list entries are stale:
-// they are taken from mlist instead).  Candidates inside sqmaxcut are
-// queued and evaluated 64 at a time.
+// they are taken from mlist instead).  Each lane takes four candidates per
+// step: positions and IDs are loaded together, the molecule checks only for
+// candidates inside sqmaxcut.  Candidates that pass are queued and evaluated
+// 64 at a time.
+#define CHAIN_QCAP (WAVE + 4 * WAVE)
 __device__ double chain_item_energy(uint4 ua, const DPart<float>& pa, int f, int m, int cell, const Grid& g,
                                     const Grid& gc, const SysConst& sc, const ChainMoveParams& cp,
                                     const int* __restrict__ starts, const uint4* P, const float4* O, const float2* Q,
@@ -1929,53 +1936,66 @@ __device__ double chain_item_energy(uint4 ua, const DPart<float>& pa, int f, int
     int total = carry, nmb = nm > 0 ? mpre[nm] : 0, all = total + nmb;
     double acc = 0.0;
     int qn = 0;
-    for (int base = 0; base < all; base += WAVE) {
-        int idx = base + lane, s = -1;
-        if (idx < total) {
-            int lo = 0, hi = ns;
-            while (hi - lo > 1) {
-                int mid = (lo + hi) >> 1;
-                if (pre[mid] <= idx) lo = mid; else hi = mid;
+    for (int base = 0; base < all; base += 4 * WAVE) {
+        int s[4], id[4];
+        uint4 ub[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int idx = base + u * WAVE + lane;
+            s[u] = -1;
+            id[u] = -1;
+            if (idx < total) {
+                int lo = 0, hi = ns;
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (pre[mid] <= idx) lo = mid; else hi = mid;
+                }
+                s[u] = cbeg[lo] + idx - pre[lo];
+                id[u] = ID[s[u]];
+            } else if (idx < all) {
+                int j = idx - total, lo = 0, hi = nm;
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (mpre[mid] <= j) lo = mid; else hi = mid;
+                }
+                id[u] = cfirst[mlist[lo]] + j - mpre[lo];
+                s[u] = slot[id[u]];
             }
-            int k = cbeg[lo] + idx - pre[lo];
-            int id = ID[k];
-            bool skip = id >= f && id < f + m;
-            if (!skip) {
-                int cc = ccell_id[id];
-                if (cc >= 0 && cc != cell && chain_cell_active(gc, cp, cc)) skip = true;
-                else if (stamp[id] == cp.stamp) skip = true;
-            }
-            if (!skip) s = k;
-        } else if (idx < all) {
-            int j = idx - total, lo = 0, hi = nm;
-            while (hi - lo > 1) {
-                int mid = (lo + hi) >> 1;
-                if (mpre[mid] <= j) lo = mid; else hi = mid;
-            }
-            int id = cfirst[mlist[lo]] + j - mpre[lo];
-            if (!(id >= f && id < f + m)) s = slot[id];
+            if (s[u] >= 0) ub[u] = P[s[u]];
         }
-        bool need = false;
-        if (s >= 0) {
-            uint4 ub = P[s];
-            V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
-            need = dot(r, r) <= sc.sqmaxcut;
+        bool need[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int idx = base + u * WAVE + lane;
+            need[u] = false;
+            if (s[u] >= 0 && !(id[u] >= f && id[u] < f + m)) {
+                V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub[u].x, ub[u].y, ub[u].z, sc.Lh, sc.Ll);
+                need[u] = dot(r, r) <= sc.sqmaxcut;
+                if (need[u] && idx < total) {   // a cell-list entry: is it one that moves elsewhere, or moved here?
+                    int cc = ccell_id[id[u]];
+                    if (cc >= 0 && cc != cell && chain_cell_active(gc, cp, cc)) need[u] = false;
+                    else if (stamp[id[u]] == cp.stamp) need[u] = false;
+                }
+            }
         }
-        unsigned long long msk = __ballot(need);
-        if (need) queue[qn + wave_prefix(msk)] = s;
-        qn += __popcll(msk);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            unsigned long long msk = __ballot(need[u]);
+            if (need[u]) queue[qn + wave_prefix(msk)] = s[u];
+            qn += __popcll(msk);
+        }
         wave_lds_sync();
-        bool last = base + WAVE >= all;
+        bool last = base + 4 * WAVE >= all;
         if (qn >= WAVE || (last && qn > 0)) {
-            int take = last ? qn : WAVE;
+            int take = last ? qn : (qn / WAVE) * WAVE;
             for (int e0 = 0; e0 < take; e0 += WAVE) {
                 int e = e0 + lane;
                 if (e < take) {
                     int sb = queue[e];
-                    uint4 ub = P[sb];
-                    V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
-                    DPart<float> pb = load_part<float>(O[sb], Q[sb], ub.w, tp);
-                    acc += (double)pair_energy<float>(pp[ua.w * sc.nt + ub.w], pa, pb, r, sc.sqmaxcut);
+                    uint4 ubb = P[sb];
+                    V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ubb.x, ubb.y, ubb.z, sc.Lh, sc.Ll);
+                    DPart<float> pb = load_part<float>(O[sb], Q[sb], ubb.w, tp);
+                    acc += (double)pair_energy<float>(pp[ua.w * sc.nt + ubb.w], pa, pb, r, sc.sqmaxcut);
                 }
             }
             wave_lds_sync();
@@ -1990,10 +2010,27 @@ __device__ double chain_item_energy(uint4 ua, const DPart<float>& pa, int f, int
     return acc;
 }
 
+// molecules of each coarse cell into index order (the scatter's atomics
+// leave them in arrival order): one block per cell, a rank sort in LDS
+__global__ __launch_bounds__(256) void k_chain_sort(int ncc, const int* __restrict__ cstart, int* __restrict__ clist) {
+    __shared__ int v[CHAIN_LCAP];
+    int c = blockIdx.x, tid = (int)threadIdx.x;
+    if (c >= ncc) return;
+    int b = cstart[c], n = cstart[c + 1] - b;
+    if (n <= 1 || n > CHAIN_LCAP) return;   // too many: k_chain_pass reports it
+    for (int i = tid; i < n; i += 256) v[i] = clist[b + i];
+    __syncthreads();
+    for (int i = tid; i < n; i += 256) {
+        int key = v[i], rank = 0;
+        for (int j = 0; j < n; j++) rank += v[j] < key ? 1 : 0;
+        clist[b + rank] = key;   // keys are distinct molecule indices
+    }
+}
+
 // LDS of k_chain_pass
 __host__ __device__ constexpr size_t chain_pass_lds(int nw) {
     return (size_t)CHAIN_MAXM * (2 * 16 + 2 * 16 + 2 * 8) + 2 * nw * 8 +
-           sizeof(int) * ((size_t)3 * CHAIN_LCAP + 1 + (size_t)nw * (256 + QCAP) + nw + 4);
+           sizeof(int) * ((size_t)3 * CHAIN_LCAP + 1 + (size_t)nw * (256 + CHAIN_QCAP) + nw + 4);
 }
 
 // One block of NW waves per active coarse cell: trials on the cell's
@@ -2007,7 +2044,8 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
                                                           const int* __restrict__ slot, const PP32* __restrict__ pp,
                                                           const TP32* __restrict__ tp, const double* __restrict__ vol,
                                                           const int* __restrict__ cfirst, const int* __restrict__ csize,
-                                                          const int* __restrict__ ccell_chain,
+                                                          const int* __restrict__ cstart,
+                                                          const int* __restrict__ clist_g,
                                                           const int* __restrict__ ccell_id, uint32_t* stamp,
                                                           unsigned long long* __restrict__ stats,
                                                           double* __restrict__ de_sum, int* __restrict__ flag) {
@@ -2025,24 +2063,19 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), w = tid >> 6;
     int* pre = mpre + CHAIN_LCAP + 1 + w * 256; // this wave's stencil prefix and cell starts
     int* cbeg = pre + 128;
-    int* queue = mpre + CHAIN_LCAP + 1 + NW * 256 + w * QCAP;
-    int* misc = mpre + CHAIN_LCAP + 1 + NW * (256 + QCAP);   // NW counts, reject code
+    int* queue = mpre + CHAIN_LCAP + 1 + NW * 256 + w * CHAIN_QCAP;
+    int* misc = mpre + CHAIN_LCAP + 1 + NW * (256 + CHAIN_QCAP);   // NW counts, reject code
     int b = blockIdx.x;
     int bx = b % cp.nact[0], by = (b / cp.nact[0]) % cp.nact[1], bz = b / (cp.nact[0] * cp.nact[1]);
     int cell = (bx * cp.pa[0] + cp.ox) + gc.nc[0] * ((by * cp.pa[1] + cp.oy) + gc.nc[1] * (bz * cp.pa[2] + cp.oz));
-    // the cell's molecules in index order
-    int total = 0;
-    for (int base = 0; base < cp.nchain; base += WAVE * NW) {
-        int i = base + tid;
-        bool fl = i < cp.nchain && ccell_chain[i] == cell;
-        int pos = block_compact<NW>(fl, total, misc);
-        if (fl && pos < CHAIN_LCAP) clist[pos] = i;
-    }
+    // the cell's molecules in index order; the sign bit marks those moved in this pass
+    int c0 = cstart[cell], total = cstart[cell + 1] - c0;
     if (total > CHAIN_LCAP) {
         if (tid == 0) atomicOr(flag, 4);
         return;
     }
     if (total == 0) return;
+    for (int i = tid; i < total; i += WAVE * NW) clist[i] = clist_g[c0 + i];
     if (tid == 0) mpre[0] = 0;
     __syncthreads();
     U4 ctr;
@@ -2061,10 +2094,11 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
         ctr.x = (uint32_t)t;
         ctr.w = cp.pass_hi << 2;
         U4 ra = philox(ctr, cp.key0, cp.key1);
-        int k = clist[(int)(((uint64_t)ra.x * (uint64_t)total) >> 32)];
+        int pick = (int)(((uint64_t)ra.x * (uint64_t)total) >> 32);
+        int k = clist[pick] & 0x7fffffff;
+        bool fresh = clist[pick] >= 0;
         bool disp = u01f(ra.y) < 0.5f;
         int f = cfirst[k], m = csize[k];
-        bool fresh = stamp[f] != cp.stamp;
         if (disp) n_dtry++; else n_rtry++;
         if (tid < m) {
             int s = slot[f + tid];
@@ -2164,6 +2198,7 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
                 if (tid == 0) {
                     mlist[nm] = k;
                     mpre[nm + 1] = mpre[nm] + m;
+                    clist[pick] = k | (int)0x80000000;
                 }
                 nm++;
             }
@@ -2311,6 +2346,11 @@ struct sc_ctx {
     int* d_cfirst = nullptr;
     int* d_csize = nullptr;
     int* d_ccell_chain = nullptr;       // per molecule: coarse cell of its anchor
+    int* d_ccount = nullptr;            // coarse cells: count, fill, CSR start, molecules by cell
+    int* d_cfill = nullptr;
+    int* d_cstart = nullptr;
+    int* d_clist = nullptr;
+    int ccell_alloc = 0;
     int* d_ccell_id = nullptr;          // per ID: that cell while the molecule can move, else -1
     uint32_t* d_stamp = nullptr;        // per ID: stamp of the pass that last moved it
     double* d_vol = nullptr;            // Ia_param::volume per dense type
@@ -2799,7 +2839,8 @@ static void sc_destroy_impl(sc_ctx* ctx) {
                     ctx->bx, ctx->d_loss, ctx->d_pcount, ctx->d_pi, ctx->d_pj, ctx->d_pe, ctx->d_label, ctx->d_q6,
                     ctx->d_changed, ctx->d_rg, ctx->d_rp, ctx->d_rep_nonempty, ctx->d_rep_stats, ctx->d_rep_dsum,
                     ctx->d_rep_e, ctx->d_cw, ctx->d_mol, ctx->d_bond, ctx->d_cfirst, ctx->d_csize,
-                    ctx->d_ccell_chain, ctx->d_ccell_id, ctx->d_stamp, ctx->d_vol};
+                    ctx->d_ccell_chain, ctx->d_ccell_id, ctx->d_stamp, ctx->d_vol, ctx->d_ccount, ctx->d_cfill,
+                    ctx->d_cstart, ctx->d_clist};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
@@ -3719,7 +3760,8 @@ static int chain_prepare(sc_ctx* ctx) {
     int rc;
     if ((rc = dalloc(ctx, &ctx->d_cfirst, nch)) || (rc = dalloc(ctx, &ctx->d_csize, nch)) ||
         (rc = dalloc(ctx, &ctx->d_ccell_chain, nch)) || (rc = dalloc(ctx, &ctx->d_ccell_id, ctx->n)) ||
-        (rc = dalloc(ctx, &ctx->d_stamp, ctx->n)) || (rc = dalloc(ctx, &ctx->d_vol, ctx->nt)))
+        (rc = dalloc(ctx, &ctx->d_stamp, ctx->n)) || (rc = dalloc(ctx, &ctx->d_vol, ctx->nt)) ||
+        (rc = dalloc(ctx, &ctx->d_clist, nch)))
         return rc;
     HIPCK(hipMemcpy(ctx->d_cfirst, ctx->chain_first.data(), nch * sizeof(int), hipMemcpyHostToDevice));
     HIPCK(hipMemcpy(ctx->d_csize, ctx->chain_size.data(), nch * sizeof(int), hipMemcpyHostToDevice));
@@ -3827,21 +3869,49 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
             return fail(ctx, SC_EINVAL,
                         "chain moves need box edges of at least 2 x molecule extent + maxcut and above 4 x the "
                         "extent (a molecule must not reach its own periodic image)");
+    // colour period p: same-colour cells are p - 1 cells apart, so cells of
+    // wmin / (p - 1); the period with the most active cells per pass (the
+    // passes run their cells' trials in sequence)
+    int p = 2;
+    long best = 0;
+    for (int q = 2; q <= 3; q++) {
+        long act = 1;
+        for (int k = 0; k < 3; k++) {
+            int nc = grid_axis_cells(ctx->box[k], wmin / (q - 1), q);
+            act *= nc >= q ? nc / q : 1;
+        }
+        if (act > best) { best = act; p = q; }
+    }
     uint64_t cseed = mix64(seed ^ 0x6368616E6D6F7665ull);
     SweepPlan plan;
-    sweep_plan(cseed, sw, 2, plan);
+    sweep_plan(cseed, sw, p, plan);
     Grid gc;
     gc.r = 1;
     for (int k = 0; k < 3; k++) {
-        gc.nc[k] = grid_axis_cells(ctx->box[k], wmin, 2);
+        gc.nc[k] = grid_axis_cells(ctx->box[k], wmin / (p - 1), p);
         gc.sh[k] = plan.shift[k];
     }
+    int ncc = gc.nc[0] * gc.nc[1] * gc.nc[2];
+    if (ncc > ctx->ccell_alloc) {
+        if ((rc = dalloc(ctx, &ctx->d_ccount, ncc)) || (rc = dalloc(ctx, &ctx->d_cfill, ncc)) ||
+            (rc = dalloc(ctx, &ctx->d_cstart, ncc + 3)))   // + the scan's occupancy outputs
+            return rc;
+        ctx->ccell_alloc = ncc;
+    }
+    HIPCK(hipMemsetAsync(ctx->d_ccount, 0, ncc * sizeof(int), ctx->stream));
+    HIPCK(hipMemsetAsync(ctx->d_cfill, 0, ncc * sizeof(int), ctx->stream));
     SysConst sc = sysconst(ctx, ctx->box);
     double rho2 = ctx->chain_rho * ctx->chain_rho;
     int c = ctx->cur;
     hipLaunchKernelGGL(k_chain_anchor, dim3((nch + 3) / 4), dim3(256), 0, ctx->stream, nch, ctx->d_cfirst,
                        ctx->d_csize, ctx->slot, ctx->P[c], ctx->d_vol, sc, gc, rho2, ctx->d_ccell_chain,
-                       ctx->d_ccell_id);
+                       ctx->d_ccell_id, ctx->d_ccount);
+    // molecules by coarse cell, in index order
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, ctx->stream, ncc, ctx->d_ccount, ctx->d_cstart,
+                       ctx->d_cstart + ncc + 1, ctx->d_cstart + ncc + 2);
+    hipLaunchKernelGGL(k_cell_scatter, dim3((nch + 255) / 256), dim3(256), 0, ctx->stream, nch, ctx->d_ccell_chain,
+                       ctx->d_cstart, ctx->d_cfill, ctx->d_clist);
+    hipLaunchKernelGGL(k_chain_sort, dim3(ncc), dim3(256), 0, ctx->stream, ncc, ctx->d_cstart, ctx->d_clist);
     HIPCK(hipGetLastError());
     ChainMoveParams cp;
     cp.temper = temper;
@@ -3856,16 +3926,16 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
     cp.nchain = nch;
     int pa[3];
     for (int k = 0; k < 3; k++) {
-        pa[k] = gc.nc[k] >= 2 ? 2 : 1;
+        pa[k] = gc.nc[k] >= p ? p : 1;
         cp.pa[k] = pa[k];
         cp.nact[k] = gc.nc[k] / pa[k];
     }
-    size_t lds = chain_pass_lds(4);
-    for (int ci = 0; ci < 8; ci++) {
+    size_t lds = chain_pass_lds(CHAIN_NW);
+    for (int ci = 0; ci < p * p * p; ci++) {
         int col = plan.order[ci];
-        cp.ox = col % 2;
-        cp.oy = (col / 2) % 2;
-        cp.oz = col / 4;
+        cp.ox = col % p;
+        cp.oy = (col / p) % p;
+        cp.oz = col / (p * p);
         if (cp.ox >= pa[0] || cp.oy >= pa[1] || cp.oz >= pa[2]) continue;
         uint64_t pass = sw * 32 + (uint64_t)ci;
         cp.pass_lo = (uint32_t)pass;
@@ -3877,10 +3947,10 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
         }
         c = ctx->cur;
         int nblk = cp.nact[0] * cp.nact[1] * cp.nact[2];
-        hipLaunchKernelGGL((k_chain_pass<4>), dim3(nblk), dim3(WAVE * 4), lds, ctx->stream, ctx->g, gc, sc, cp,
+        hipLaunchKernelGGL((k_chain_pass<CHAIN_NW>), dim3(nblk), dim3(WAVE * CHAIN_NW), lds, ctx->stream, ctx->g, gc, sc, cp,
                            ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->slot, ctx->d_pp32,
-                           ctx->d_tp32, ctx->d_vol, ctx->d_cfirst, ctx->d_csize, ctx->d_ccell_chain, ctx->d_ccell_id,
-                           ctx->d_stamp, ctx->d_stats, ctx->d_dsum, ctx->d_flag);
+                           ctx->d_tp32, ctx->d_vol, ctx->d_cfirst, ctx->d_csize, ctx->d_cstart, ctx->d_clist,
+                           ctx->d_ccell_id, ctx->d_stamp, ctx->d_stats, ctx->d_dsum, ctx->d_flag);
         HIPCK(hipGetLastError());
         if ((rc = build_grid(ctx, r, shift, ctx->box))) return rc;
     }
