@@ -124,7 +124,8 @@ def npt_leg(a, transmx, rot_half):
         dt = time.perf_counter() - t1
         lab = "exact_with_chain_moves"
         npt[lab] = {"chainprob": a.npt_chainprob, "chainmmx": a.npt_chainmmx, "chainrmx_deg": a.npt_chainrmx,
-                    "extent": gn.chain_extent(), "sweeps_per_s": a.npt_steps / dt,
+                    "extent": gn.chain_extent(), "colour_period": gn.chain_pass_info()[0],
+                    "lds_stage": gn.chain_pass_info()[1], "sweeps_per_s": a.npt_steps / dt,
                     "ms_per_step": 1e3 * dt / a.npt_steps, "ms_sweep": 1e3 * tot["sweep_s"] / a.npt_steps,
                     "ms_volume_move": 1e3 * tot["volume_s"] / a.npt_steps,
                     "volume_acceptance": acc / float(a.npt_steps),
@@ -135,9 +136,12 @@ def npt_leg(a, transmx, rot_half):
                     "eta_end": nch * W.chain_volume() / float(np.prod(bx)), "energy_per_particle": gn.allToAll() / nb_}
         if "exact" in npt:
             npt[lab]["ms_chain_moves_per_sweep"] = npt[lab]["ms_sweep"] - npt["exact"]["ms_sweep"]
-        log("npt chains %s: %.2f steps/s, %.0f chain moves per sweep (displace acc %.3f, rotate acc %.3f)" % (
-            lab, npt[lab]["sweeps_per_s"], npt[lab]["chain_moves_per_sweep"], npt[lab]["chain_displace_acceptance"],
-            npt[lab]["chain_rotate_acceptance"]))
+        log("npt chains %s: %.2f steps/s, %.0f chain moves per sweep (displace acc %.3f, rotate acc %.3f; "
+            "period %d, LDS stage %s, cells skipped %d)" % (
+                lab, npt[lab]["sweeps_per_s"], npt[lab]["chain_moves_per_sweep"],
+                npt[lab]["chain_displace_acceptance"], npt[lab]["chain_rotate_acceptance"], npt[lab]["colour_period"],
+                npt[lab]["lds_stage"], tot["chain_cells_skipped"]))
+        npt[lab]["chain_cells_skipped"] = int(tot["chain_cells_skipped"])
         gn.set_chain_moves(0.0, 0.0, 0.0)
     gn.close()
     return npt

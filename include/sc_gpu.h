@@ -50,6 +50,7 @@ typedef struct {
     /* whole-molecule moves (sc_set_chain_moves): Statistics chainm / chainr acc and tries */
     uint64_t chain_disp_acc, chain_disp_try, chain_rot_acc, chain_rot_try;
     uint64_t chain_rejected;  /* of those tries: molecule wider than the extent, or anchor leaving its coarse cell */
+    uint64_t chain_cells_skipped;   /* coarse cells whose neighbourhood outgrew the LDS stage: no move that pass */
 } sc_sweep_stats;
 
 /* --- lifetime --------------------------------------------------------- */
@@ -214,6 +215,10 @@ int sc_set_block_waves(sc_ctx* ctx, int nw);
 int sc_set_chain_moves(sc_ctx* ctx, double chainprob, double chain_mx, double chain_angle, double extent);
 /* the extent the chain moves use */
 int sc_get_chain_extent(sc_ctx* ctx, double* extent);
+/* how the last sweep ran its chain moves: the coarse colour period (2 or 3,
+ * 0 before any) and whether the trials read an LDS copy of each cell's
+ * neighbourhood (1) or global memory (0) */
+int sc_chain_pass_info(const sc_ctx* ctx, int* period, int* staged);
 /* Volume move (MoveCreator::pressureMove, movecreator.cpp:330-550) for
  * ptype 0 (one random edge) and 1 (isotropic); returns accepted=1/0 and the
  * new box.  Whole-system energies are FP32 pair terms (FP64 bonds) with FP64

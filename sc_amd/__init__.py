@@ -28,7 +28,7 @@ class SweepStats(ctypes.Structure):
                 ("energy_change", ctypes.c_double), ("bond_rejected", ctypes.c_uint64),
                 ("chain_disp_acc", ctypes.c_uint64), ("chain_disp_try", ctypes.c_uint64),
                 ("chain_rot_acc", ctypes.c_uint64), ("chain_rot_try", ctypes.c_uint64),
-                ("chain_rejected", ctypes.c_uint64)]
+                ("chain_rejected", ctypes.c_uint64), ("chain_cells_skipped", ctypes.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -77,6 +77,7 @@ SIGNATURES = {
     "sc_commit_mol": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, _D, _D]),
     "sc_set_chain_moves": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "sc_get_chain_extent": (ctypes.c_int, [_P, _D]),
+    "sc_chain_pass_info": (ctypes.c_int, [_P, _I, _I]),
     "sc_energy_per_particle": (ctypes.c_int, [_P, _D]),
     "sc_energy_all_trial": (ctypes.c_int, [_P, _D, _D]),
     "sc_commit_box": (ctypes.c_int, [_P, _D]),
@@ -380,6 +381,12 @@ class GpuSystem:
         chain_angle (radians); extent 0 = 1.2 x the widest molecule now"""
         self._ck(self._L.sc_set_chain_moves(self._h, float(chainprob), float(chain_mx), float(chain_angle),
                                             float(extent)), "set_chain_moves")
+
+    def chain_pass_info(self):
+        """(colour period, staged) of the last sweep's chain moves"""
+        p, st = ctypes.c_int32(), ctypes.c_int32()
+        self._ck(self._L.sc_chain_pass_info(self._h, ctypes.byref(p), ctypes.byref(st)), "chain_pass_info")
+        return p.value, bool(st.value)
 
     def chain_extent(self):
         e = ctypes.c_double()

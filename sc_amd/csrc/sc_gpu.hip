@@ -1821,6 +1821,7 @@ struct ChainMoveParams {
     int pa[3];               // colour period per axis (1 on an axis of one cell)
     int nact[3];             // active coarse cells per axis
     int nchain;
+    uint32_t reach[3];       // rho + maxcut as box fractions (fixed point), the staged region's margin
 };
 
 __device__ __forceinline__ double u01_32(uint32_t x) { return (double)x * 2.3283064365386963e-10; }
@@ -1893,6 +1894,16 @@ __global__ __launch_bounds__(256) void k_chain_anchor(int nchain, const int* __r
     if (v) ccell_id[f + lane] = G.ext2 <= rho2 ? cell : -1;
 }
 
+// the pair functor of a queued pair as a call of its own (its registers are
+// not added to those of the surrounding scan)
+__device__ __attribute__((noinline)) float chain_pair(const PP32* __restrict__ pp, const TP32* __restrict__ tp, int nt,
+                                                      float sqmaxcut, uint4 ua, float4 oa, float2 qa, uint4 ub,
+                                                      float4 ob, float2 qb, V3<float> r) {
+    DPart<float> pa = load_part<float>(oa, qa, ua.w, tp);
+    DPart<float> pb = load_part<float>(ob, qb, ub.w, tp);
+    return pair_energy<float>(pp[ua.w * nt + ub.w], pa, pb, r, sqmaxcut);
+}
+
 // Energy of one particle of molecule [f, f + m) at the state (ua, pa) with
 // everything outside its molecule (the wave's lanes hold partial sums): the
 // stencil of its fine cell, without the particles of molecules that may move
@@ -1903,7 +1914,7 @@ __global__ __launch_bounds__(256) void k_chain_anchor(int nchain, const int* __r
 // candidates inside sqmaxcut.  Candidates that pass are queued and evaluated
 // 64 at a time.
 #define CHAIN_QCAP (WAVE + 4 * WAVE)
-__device__ double chain_item_energy(uint4 ua, const DPart<float>& pa, int f, int m, int cell, const Grid& g,
+__device__ double chain_item_energy(uint4 ua, float4 oa, float2 qa, int f, int m, int cell, const Grid& g,
                                     const Grid& gc, const SysConst& sc, const ChainMoveParams& cp,
                                     const int* __restrict__ starts, const uint4* P, const float4* O, const float2* Q,
                                     const int* __restrict__ ID, const int* __restrict__ slot,
@@ -1994,8 +2005,7 @@ __device__ double chain_item_energy(uint4 ua, const DPart<float>& pa, int f, int
                     int sb = queue[e];
                     uint4 ubb = P[sb];
                     V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ubb.x, ubb.y, ubb.z, sc.Lh, sc.Ll);
-                    DPart<float> pb = load_part<float>(O[sb], Q[sb], ubb.w, tp);
-                    acc += (double)pair_energy<float>(pp[ua.w * sc.nt + ubb.w], pa, pb, r, sc.sqmaxcut);
+                    acc += (double)chain_pair(pp, tp, sc.nt, sc.sqmaxcut, ua, oa, qa, ubb, O[sb], Q[sb], r);
                 }
             }
             wave_lds_sync();
@@ -2008,6 +2018,69 @@ __device__ double chain_item_energy(uint4 ua, const DPart<float>& pa, int f, int
         }
     }
     return acc;
+}
+
+// Wave 0 of a chain-move block: the trial state (Un, On, Qn) of the molecule's
+// m particles (Uo, Oo, Qo) under the move drawn from ra (and the stream ctr),
+// and the reject code (0: none, 1: wider than rho, 2: anchor leaves the
+// coarse cell); every lane returns the same code.
+__device__ int chain_trial_state(bool disp, U4 ra, U4 ctr, const ChainMoveParams& cp, const SysConst& sc,
+                                 const Grid& gc, int cell, int m, const uint4* Uo, const float4* Oo,
+                                 const float2* Qo, uint4* Un, float4* On, float2* Qn,
+                                 const double* __restrict__ vol) {
+    int lane = lane_id();
+    bool v = lane < m;
+    int a = v ? lane : 0;
+    uint4 u0 = Uo[0], ua = Uo[a];
+    float4 oa = Oo[a];
+    float2 qa = Qo[a];
+    double va = vol[ua.w];
+    ChainGeom G = chain_geom(ua, u0, v, va, sc.Ld);
+    uint4 un = ua;
+    float4 on = oa;
+    float2 qn = qa;
+    if (disp) {
+        // |dr| = chain_mx exactly, uniform direction (chainDisplace, :1088-1091)
+        V3<double> d = unit_from_d(u01_32(ra.z), u01_32(ra.w));
+        un.x = ua.x + (uint32_t)(long long)rint(d.x * cp.chain_mx * (4294967296.0 / sc.Ld[0]));
+        un.y = ua.y + (uint32_t)(long long)rint(d.y * cp.chain_mx * (4294967296.0 / sc.Ld[1]));
+        un.z = ua.z + (uint32_t)(long long)rint(d.z * cp.chain_mx * (4294967296.0 / sc.Ld[2]));
+    } else {
+        // clusterRotate (:1258-1305): quaternion (cos a, sin a * axis),
+        // a = chain_angle * U, random sign; positions about the centre
+        // in real units, axes and patch directions with them
+        U4 c1 = ctr;
+        c1.w |= 1u;
+        U4 rb = philox(c1, cp.key0, cp.key1);
+        V3<double> axv = unit_from_d(u01_32(rb.x), u01_32(rb.y));
+        double vc = cos(cp.chain_angle * u01d(rb.z, rb.w));
+        double vs = sqrt(fmax(0.0, 1.0 - vc * vc));
+        if (ra.y & 1u) vs = -vs;
+        const double k32 = 2.3283064365386963e-10;
+        V3<double> o = mk<double>((double)(int32_t)(ua.x - u0.x) * (sc.Ld[0] * k32) - G.c[0],
+                                  (double)(int32_t)(ua.y - u0.y) * (sc.Ld[1] * k32) - G.c[1],
+                                  (double)(int32_t)(ua.z - u0.z) * (sc.Ld[2] * k32) - G.c[2]);
+        V3<double> o2 = qrot(o, axv, vc, vs);
+        un.x = u0.x + (uint32_t)(long long)rint((G.c[0] + o2.x) * (4294967296.0 / sc.Ld[0]));
+        un.y = u0.y + (uint32_t)(long long)rint((G.c[1] + o2.y) * (4294967296.0 / sc.Ld[1]));
+        un.z = u0.z + (uint32_t)(long long)rint((G.c[2] + o2.z) * (4294967296.0 / sc.Ld[2]));
+        V3<double> d = normalised(qrot(mk<double>(oa.x, oa.y, oa.z), axv, vc, vs));
+        V3<double> pq = qrot(mk<double>(oa.w, qa.x, qa.y), axv, vc, vs);
+        pq = normalised(perp(pq, d));
+        on = make_float4((float)d.x, (float)d.y, (float)d.z, (float)pq.x);
+        qn = make_float2((float)pq.y, (float)pq.z);
+    }
+    if (v) {
+        Un[lane] = un;
+        On[lane] = on;
+        Qn[lane] = qn;
+    }
+    uint4 un0 = make_uint4(__shfl(un.x, 0, WAVE), __shfl(un.y, 0, WAVE), __shfl(un.z, 0, WAVE), 0u);
+    ChainGeom Gn = chain_geom(un, un0, v, va, sc.Ld);
+    uint4 an = chain_anchor_u(un0, Gn, sc.Ld);
+    if (G.ext2 > cp.rho2 || Gn.ext2 > cp.rho2) return 1;
+    if (cell_of_u(gc, an.x, an.y, an.z) != cell) return 2;
+    return 0;
 }
 
 // molecules of each coarse cell into index order (the scatter's atomics
@@ -2108,58 +2181,7 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
         }
         __syncthreads();
         if (w == 0) {
-            bool v = lane < m;
-            int a = v ? lane : 0;
-            uint4 u0 = Uo[0], ua = Uo[a];
-            float4 oa = Oo[a];
-            float2 qa = Qo[a];
-            double va = vol[ua.w];
-            ChainGeom G = chain_geom(ua, u0, v, va, sc.Ld);
-            uint4 un = ua;
-            float4 on = oa;
-            float2 qn = qa;
-            if (disp) {
-                // |dr| = chain_mx exactly, uniform direction (chainDisplace, :1088-1091)
-                V3<double> d = unit_from_d(u01_32(ra.z), u01_32(ra.w));
-                un.x = ua.x + (uint32_t)(long long)rint(d.x * cp.chain_mx * (4294967296.0 / sc.Ld[0]));
-                un.y = ua.y + (uint32_t)(long long)rint(d.y * cp.chain_mx * (4294967296.0 / sc.Ld[1]));
-                un.z = ua.z + (uint32_t)(long long)rint(d.z * cp.chain_mx * (4294967296.0 / sc.Ld[2]));
-            } else {
-                // clusterRotate (:1258-1305): quaternion (cos a, sin a * axis),
-                // a = chain_angle * U, random sign; positions about the centre
-                // in real units, axes and patch directions with them
-                U4 c1 = ctr;
-                c1.w |= 1u;
-                U4 rb = philox(c1, cp.key0, cp.key1);
-                V3<double> axv = unit_from_d(u01_32(rb.x), u01_32(rb.y));
-                double vc = cos(cp.chain_angle * u01d(rb.z, rb.w));
-                double vs = sqrt(fmax(0.0, 1.0 - vc * vc));
-                if (ra.y & 1u) vs = -vs;
-                const double k32 = 2.3283064365386963e-10;
-                V3<double> o = mk<double>((double)(int32_t)(ua.x - u0.x) * (sc.Ld[0] * k32) - G.c[0],
-                                          (double)(int32_t)(ua.y - u0.y) * (sc.Ld[1] * k32) - G.c[1],
-                                          (double)(int32_t)(ua.z - u0.z) * (sc.Ld[2] * k32) - G.c[2]);
-                V3<double> o2 = qrot(o, axv, vc, vs);
-                un.x = u0.x + (uint32_t)(long long)rint((G.c[0] + o2.x) * (4294967296.0 / sc.Ld[0]));
-                un.y = u0.y + (uint32_t)(long long)rint((G.c[1] + o2.y) * (4294967296.0 / sc.Ld[1]));
-                un.z = u0.z + (uint32_t)(long long)rint((G.c[2] + o2.z) * (4294967296.0 / sc.Ld[2]));
-                V3<double> d = normalised(qrot(mk<double>(oa.x, oa.y, oa.z), axv, vc, vs));
-                V3<double> pq = qrot(mk<double>(oa.w, qa.x, qa.y), axv, vc, vs);
-                pq = normalised(perp(pq, d));
-                on = make_float4((float)d.x, (float)d.y, (float)d.z, (float)pq.x);
-                qn = make_float2((float)pq.y, (float)pq.z);
-            }
-            if (v) {
-                Un[lane] = un;
-                On[lane] = on;
-                Qn[lane] = qn;
-            }
-            uint4 un0 = make_uint4(__shfl(un.x, 0, WAVE), __shfl(un.y, 0, WAVE), __shfl(un.z, 0, WAVE), 0u);
-            ChainGeom Gn = chain_geom(un, un0, v, va, sc.Ld);
-            uint4 an = chain_anchor_u(un0, Gn, sc.Ld);
-            int rej = 0;
-            if (G.ext2 > cp.rho2 || Gn.ext2 > cp.rho2) rej = 1;
-            else if (cell_of_u(gc, an.x, an.y, an.z) != cell) rej = 2;
+            int rej = chain_trial_state(disp, ra, ctr, cp, sc, gc, cell, m, Uo, Oo, Qo, Un, On, Qn, vol);
             if (lane == 0) misc[NW] = rej;
         }
         __syncthreads();
@@ -2173,9 +2195,9 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
             int a = it >> 1;
             bool st = (it & 1) != 0;
             uint4 ua = st ? Un[a] : Uo[a];
-            DPart<float> pa = load_part<float>(st ? On[a] : Oo[a], st ? Qn[a] : Qo[a], ua.w, tp);
-            double e = chain_item_energy(ua, pa, f, m, cell, g, gc, sc, cp, starts, P, O, Q, ID, slot, pp, tp, cfirst,
-                                         ccell_id, stamp, mlist, mpre, nm, pre, cbeg, queue);
+            double e = chain_item_energy(ua, st ? On[a] : Oo[a], st ? Qn[a] : Qo[a], f, m, cell, g, gc, sc, cp, starts,
+                                         P, O, Q, ID, slot, pp, tp, cfirst, ccell_id, stamp, mlist, mpre, nm, pre, cbeg,
+                                         queue);
             if (st) acc_n += e; else acc_o += e;
         }
         double eo = block_sum<NW>(acc_o, red), en = block_sum<NW>(acc_n, red + NW);
@@ -2211,6 +2233,434 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
         if (v) atomicAdd(&stats[rep * STAT_STRIDE + 10 + tid], v);
     } else if (tid == 5) {
         atomicAdd(&de_sum[rep * STAT_STRIDE], dsum);
+    }
+}
+
+// ---- staged chain pass
+// Each block first copies into LDS every particle within rho + maxcut of its
+// coarse cell (the region its molecules' particles and their neighbours can
+// occupy during the pass), leaving out the particles of molecules that may
+// move in another active cell (they stay beyond maxcut).  The trials then
+// read nothing but LDS; an accepted move updates the staged copy and global
+// memory.  The region is found through the fine cell list (whole cells
+// overlapping it, filtered by position).  A block whose region holds more
+// than `cap` particles makes no move in this pass (its trials are counted as
+// rejected, and the block in chain_skipped).
+#define CHAIN_ST_LCAP 512    // molecules per coarse cell
+#define CHAIN_ST_OCAP 1024   // particles of the cell's movable molecules
+#define CHAIN_ST_RCAP 1024   // fine cells overlapping a region
+#define CHAIN_LDS_BUDGET 163840
+#define CHAIN_OWN 0x40000000      // stage ID flag: a particle that may move in this pass
+#define CHAIN_IDMASK 0x00ffffff
+
+struct ChainStLayout {
+    size_t su, so, sq, red, ints, sid, total;
+};
+__host__ __device__ inline ChainStLayout chain_st_layout(int nw, int cap) {
+    ChainStLayout L;
+    size_t o = 4 * CHAIN_MAXM * 16;   // Uo Un Oo On
+    L.su = o; o += (size_t)cap * 16;
+    L.so = o; o += (size_t)cap * 16;
+    o += 2 * CHAIN_MAXM * 8;          // Qo Qn
+    L.sq = o; o += (size_t)cap * 8;
+    L.red = o; o += (size_t)2 * nw * 8;
+    // bidx, clist, spre (stage start of each region cell), then the region
+    // prefix (staging) sharing its space with the queues and stencil ranges (trials)
+    L.ints = o;
+    size_t shared = std::max((size_t)2 * CHAIN_ST_RCAP + 1, (size_t)nw * (CHAIN_QCAP + 64));
+    o += sizeof(int) * ((size_t)CHAIN_MAXM + CHAIN_ST_LCAP + CHAIN_ST_OCAP + CHAIN_ST_RCAP + 1 + shared + nw + 4);
+    L.sid = o; o += (size_t)cap * 4;
+    L.total = o;
+    return L;
+}
+
+// energy of one molecule particle at (ua, pa) with the staged particles
+// outside its molecule (wave-level; lanes hold partial sums): the staged
+// particles of the fine cells around it (every partner within maxcut lies
+// in a region cell; stencil cells outside the region hold none), except the
+// particles that may move in this pass, which may have left the cell they
+// are staged under: those are taken from `own`, wherever they are now.
+__device__ double chain_item_energy_st(uint4 ua, float4 oa, float2 qa, int f, int m, const Grid& g, const int* f0,
+                                       const int* fn, const int* spre, const int* own, int nown, const SysConst& sc,
+                                       const PP32* __restrict__ pp, const TP32* __restrict__ tp, const uint4* su,
+                                       const float4* so, const float2* sq, const int* sid, int* queue, int* spr) {
+    int lane = lane_id();
+    int c = cell_of_u(g, ua.x, ua.y, ua.z);
+    int cx = c % g.nc[0], cy = (c / g.nc[0]) % g.nc[1], cz = c / (g.nc[0] * g.nc[1]);
+    int ns = stencil_size(g);   // 27 at most: r = 1 grids only reach this kernel
+    int beg = 0, cnt = 0;
+    if (lane < ns) {
+        int hm;
+        int sc_ = stencil_cell(g, cx, cy, cz, lane, hm);
+        int x = sc_ % g.nc[0], y = (sc_ / g.nc[0]) % g.nc[1], z = sc_ / (g.nc[0] * g.nc[1]);
+        int rx = (x - f0[0] + g.nc[0]) % g.nc[0], ry = (y - f0[1] + g.nc[1]) % g.nc[1],
+            rz = (z - f0[2] + g.nc[2]) % g.nc[2];
+        if (rx < fn[0] && ry < fn[1] && rz < fn[2]) {
+            int rc = rx + fn[0] * (ry + fn[1] * rz);
+            beg = spre[rc];
+            cnt = spre[rc + 1] - beg;
+        }
+    }
+    int v = cnt;
+    for (int o = 1; o < WAVE; o <<= 1) {
+        int t = __shfl_up(v, o, WAVE);
+        if (lane >= o) v += t;
+    }
+    int total = __shfl(v, WAVE - 1, WAVE), all = total + nown;
+    if (lane < 32) {
+        spr[lane] = v - cnt;
+        spr[32 + lane] = beg;
+    }
+    wave_lds_sync();
+    double acc = 0.0;
+    int qn = 0;
+    for (int base = 0; base < all; base += 4 * WAVE) {
+        bool need[4];
+        int si[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            int idx = base + u * WAVE + lane;
+            need[u] = false;
+            si[u] = 0;
+            bool ok = false;
+            if (idx < total) {
+                int lo = 0, hi = ns;
+                while (hi - lo > 1) {
+                    int mid = (lo + hi) >> 1;
+                    if (spr[mid] <= idx) lo = mid; else hi = mid;
+                }
+                si[u] = spr[32 + lo] + idx - spr[lo];
+                ok = (sid[si[u]] & CHAIN_OWN) == 0;
+            } else if (idx < all) {
+                si[u] = own[idx - total];
+                int id = sid[si[u]] & CHAIN_IDMASK;
+                ok = !(id >= f && id < f + m);
+            }
+            if (ok) {
+                uint4 ub = su[si[u]];
+                V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
+                need[u] = dot(r, r) <= sc.sqmaxcut;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            unsigned long long msk = __ballot(need[u]);
+            if (need[u]) queue[qn + wave_prefix(msk)] = si[u];
+            qn += __popcll(msk);
+        }
+        wave_lds_sync();
+        bool last = base + 4 * WAVE >= all;
+        if (qn >= WAVE || (last && qn > 0)) {
+            int take = last ? qn : (qn / WAVE) * WAVE;
+            for (int e0 = 0; e0 < take; e0 += WAVE) {
+                int e = e0 + lane;
+                if (e < take) {
+                    int sb = queue[e];
+                    uint4 ub = su[sb];
+                    V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
+                    acc += (double)chain_pair(pp, tp, sc.nt, sc.sqmaxcut, ua, oa, qa, ub, so[sb], sq[sb], r);
+                }
+            }
+            wave_lds_sync();
+            int rest = qn - take;
+            int cq = lane < rest ? queue[take + lane] : 0;
+            wave_lds_sync();
+            if (lane < rest) queue[lane] = cq;
+            wave_lds_sync();
+            qn = rest;
+        }
+    }
+    return acc;
+}
+
+template <int NW>
+__global__ __launch_bounds__(WAVE * NW) void k_chain_pass_st(Grid g, Grid gc, SysConst sc, ChainMoveParams cp,
+                                                             int cap, const int* __restrict__ starts, uint4* P,
+                                                             float4* O, float2* Q, const int* __restrict__ ID,
+                                                             const int* __restrict__ slot,
+                                                             const PP32* __restrict__ pp, const TP32* __restrict__ tp,
+                                                             const double* __restrict__ vol,
+                                                             const int* __restrict__ cfirst,
+                                                             const int* __restrict__ csize,
+                                                             const int* __restrict__ cstart,
+                                                             const int* __restrict__ clist_g,
+                                                             const int* __restrict__ ccell_id,
+                                                             unsigned long long* __restrict__ stats,
+                                                             double* __restrict__ de_sum) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const ChainStLayout L = chain_st_layout(NW, cap);
+    uint4* Uo = (uint4*)smem;
+    uint4* Un = Uo + CHAIN_MAXM;
+    float4* Oo = (float4*)(Un + CHAIN_MAXM);
+    float4* On = Oo + CHAIN_MAXM;
+    uint4* su = (uint4*)(smem + L.su);
+    float4* so = (float4*)(smem + L.so);
+    float2* Qo = (float2*)(smem + L.so + (size_t)cap * 16);
+    float2* Qn = Qo + CHAIN_MAXM;
+    float2* sq = (float2*)(smem + L.sq);
+    double* red = (double*)(smem + L.red);
+    int* bidx = (int*)(smem + L.ints);   // stage index of the trial molecule's particles
+    int* clist = bidx + CHAIN_MAXM;
+    int* own = clist + CHAIN_ST_LCAP;    // stage indices of the particles of the cell's movable molecules
+    int* spre = own + CHAIN_ST_OCAP;     // stage index of each region cell's first particle (+ the end)
+    int* rpre = spre + CHAIN_ST_RCAP + 1; // region cells: prefix of counts, cell starts (staging only)
+    int* rbeg = rpre + CHAIN_ST_RCAP + 1;
+    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), w = tid >> 6;
+    int* queue = rpre + w * (CHAIN_QCAP + 64);   // the same space, once the stage is built
+    int* spr = queue + CHAIN_QCAP;               // this wave's stencil: 32 prefix + 32 starts
+    int* misc = rpre + (2 * CHAIN_ST_RCAP + 1 > NW * (CHAIN_QCAP + 64) ? 2 * CHAIN_ST_RCAP + 1 : NW * (CHAIN_QCAP + 64));
+    int* sid = (int*)(smem + L.sid);
+    int b = blockIdx.x;
+    int bx = b % cp.nact[0], by = (b / cp.nact[0]) % cp.nact[1], bz = b / (cp.nact[0] * cp.nact[1]);
+    int gxyz[3] = {bx * cp.pa[0] + cp.ox, by * cp.pa[1] + cp.oy, bz * cp.pa[2] + cp.oz};
+    int cell = gxyz[0] + gc.nc[0] * (gxyz[1] + gc.nc[1] * gxyz[2]);
+    int c0 = cstart[cell], total = cstart[cell + 1] - c0;
+    if (total == 0) return;
+    // the region: the coarse cell widened by cp.reach on each side; the fine
+    // cells that overlap it
+    uint32_t rlo[3];
+    uint64_t rwid[3];
+    int f0[3], fn[3];
+    for (int k = 0; k < 3; k++) {
+        uint64_t nc = (uint64_t)gc.nc[k];
+        uint32_t lo = gc.sh[k] + (uint32_t)(((uint64_t)gxyz[k] * 4294967296ull + nc - 1) / nc);
+        uint32_t hi = gc.sh[k] + (uint32_t)(((uint64_t)(gxyz[k] + 1) * 4294967296ull + nc - 1) / nc);
+        uint64_t wid = (uint64_t)(uint32_t)(hi - lo);
+        if (gc.nc[k] == 1) wid = 4294967296ull;
+        rwid[k] = wid + 2ull * cp.reach[k];
+        rlo[k] = lo - cp.reach[k];
+        uint64_t cw = 4294967296ull / (uint64_t)g.nc[k];
+        if (rwid[k] + 2 * cw >= 4294967296ull) {
+            rwid[k] = 4294967296ull;
+            f0[k] = 0;
+            fn[k] = g.nc[k];
+        } else {
+            f0[k] = axis_cell(rlo[k], g.sh[k], g.nc[k]);
+            int f1 = axis_cell(rlo[k] + (uint32_t)(rwid[k] - 1), g.sh[k], g.nc[k]);
+            fn[k] = (f1 - f0[k] + g.nc[k]) % g.nc[k] + 1;
+        }
+    }
+    int nrc = fn[0] * fn[1] * fn[2];
+    bool over = total > CHAIN_ST_LCAP || nrc > CHAIN_ST_RCAP;
+    int nst = 0, nown_s = 0;
+    if (!over) {
+        for (int i = tid; i < total; i += WAVE * NW) clist[i] = clist_g[c0 + i];
+        for (int rc = tid; rc < nrc; rc += WAVE * NW) {
+            spre[rc] = -1;
+            int ix = (f0[0] + rc % fn[0]) % g.nc[0], iy = (f0[1] + (rc / fn[0]) % fn[1]) % g.nc[1],
+                iz = (f0[2] + rc / (fn[0] * fn[1])) % g.nc[2];
+            int c = ix + g.nc[0] * (iy + g.nc[1] * iz);
+            int st = starts[c];
+            rbeg[rc] = st;
+            rpre[rc] = starts[c + 1] - st;
+        }
+        __syncthreads();
+        if (w == 0) {   // counts -> exclusive prefix
+            int carry = 0;
+            for (int b0 = 0; b0 < nrc; b0 += WAVE) {
+                int i = b0 + lane, cnt = i < nrc ? rpre[i] : 0, v = cnt;
+                for (int o = 1; o < WAVE; o <<= 1) {
+                    int t = __shfl_up(v, o, WAVE);
+                    if (lane >= o) v += t;
+                }
+                if (i < nrc) rpre[i] = carry + v - cnt;
+                carry += __shfl(v, WAVE - 1, WAVE);
+            }
+            if (lane == 0) rpre[nrc] = carry;
+        }
+        __syncthreads();
+        // four candidates per thread per step (their loads in flight
+        // together), one block scan of the per-thread counts per step; the
+        // stage keeps the candidates' order
+        int T = rpre[nrc];
+        constexpr int NT = WAVE * NW;
+        for (int base = 0; base < T; base += 4 * NT) {
+            int k[4], id[4], rcq[4];
+            uint4 u[4];
+            bool in[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                int idx = base + 4 * tid + q;
+                in[q] = false;
+                k[q] = 0;
+                rcq[q] = -1;
+                if (idx < T) {
+                    int lo = 0, hi = nrc;
+                    while (hi - lo > 1) {
+                        int mid = (lo + hi) >> 1;
+                        if (rpre[mid] <= idx) lo = mid; else hi = mid;
+                    }
+                    if (rpre[lo] == idx) rcq[q] = lo;   // the first candidate of its region cell
+                    k[q] = rbeg[lo] + idx - rpre[lo];
+                    u[q] = P[k[q]];
+                    id[q] = ID[k[q]];
+                    in[q] = (uint64_t)(uint32_t)(u[q].x - rlo[0]) < rwid[0] &&
+                            (uint64_t)(uint32_t)(u[q].y - rlo[1]) < rwid[1] &&
+                            (uint64_t)(uint32_t)(u[q].z - rlo[2]) < rwid[2];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (in[q]) {
+                    int cc = ccell_id[id[q]];
+                    if (cc >= 0 && cc != cell && chain_cell_active(gc, cp, cc)) in[q] = false;
+                    if (cc == cell) id[q] |= CHAIN_OWN;   // may move in this pass
+                }
+            }
+            int cnt = (int)in[0] + (int)in[1] + (int)in[2] + (int)in[3];
+            // exclusive block scan of cnt: lanes, then the waves in order
+            int incl = cnt;
+            for (int o = 1; o < WAVE; o <<= 1) {
+                int t = __shfl_up(incl, o, WAVE);
+                if (lane >= o) incl += t;
+            }
+            if (lane == WAVE - 1) misc[w] = incl;
+            __syncthreads();
+            int wbase = 0, all = 0;
+#pragma unroll
+            for (int q = 0; q < NW; q++) {
+                int c = misc[q];
+                wbase += q < w ? c : 0;
+                all += c;
+            }
+            __syncthreads();
+            int pos = nst + wbase + incl - cnt;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if (rcq[q] >= 0) spre[rcq[q]] = pos;
+                if (in[q]) {
+                    if (pos < cap) {
+                        su[pos] = u[q];
+                        so[pos] = O[k[q]];
+                        sq[pos] = Q[k[q]];
+                        sid[pos] = id[q];
+                    }
+                    pos++;
+                }
+            }
+            nst += all;
+        }
+        over = nst > cap;
+        __syncthreads();
+        if (w == 0) {   // region cells without candidates start where the next one does
+            int carry = nst;
+            for (int b0 = ((nrc - 1) / WAVE) * WAVE; b0 >= 0; b0 -= WAVE) {
+                int i = b0 + lane;
+                int v = (i < nrc && spre[i] >= 0) ? spre[i] : 0x7fffffff;
+                for (int o = 1; o < WAVE; o <<= 1) {
+                    int t = __shfl_down(v, o, WAVE);
+                    if (lane + o < WAVE) v = min(v, t);
+                }
+                v = min(v, carry);
+                if (i < nrc) spre[i] = v;
+                carry = __shfl(v, 0, WAVE);
+            }
+            if (lane == 0) spre[nrc] = nst;
+        }
+        __syncthreads();
+        // the particles that may move in this pass, in stage order
+        int nown = 0;
+        if (!over) {
+            for (int b0 = 0; b0 < nst; b0 += WAVE * NW) {
+                int i = b0 + tid;
+                bool fl = i < nst && (sid[i] & CHAIN_OWN) != 0;
+                int pos = block_compact<NW>(fl, nown, misc);
+                if (fl && pos < CHAIN_ST_OCAP) own[pos] = i;
+            }
+        }
+        over = over || nown > CHAIN_ST_OCAP;
+        nown_s = nown;
+    }
+    U4 ctr;
+    ctr.y = (uint32_t)cell;
+    ctr.z = cp.pass_lo;
+    ctr.x = 0xffffffffu;
+    ctr.w = (cp.pass_hi << 2) | 3u;
+    U4 rd = philox(ctr, cp.key0, cp.key1);
+    double mean = (double)total * cp.frac;
+    int ntrial = (int)floor(mean) + (u01d(rd.x, rd.y) < mean - floor(mean) ? 1 : 0);
+    unsigned long long n_dacc = 0, n_dtry = 0, n_racc = 0, n_rtry = 0, n_rej = 0;
+    double dsum = 0.0;
+    __syncthreads();
+    for (int t = 0; t < ntrial; t++) {
+        ctr.x = (uint32_t)t;
+        ctr.w = cp.pass_hi << 2;
+        U4 ra = philox(ctr, cp.key0, cp.key1);
+        bool disp = u01f(ra.y) < 0.5f;
+        if (disp) n_dtry++; else n_rtry++;
+        if (over) {
+            n_rej++;
+            continue;
+        }
+        int k = clist[(int)(((uint64_t)ra.x * (uint64_t)total) >> 32)];
+        int f = cfirst[k], m = csize[k];
+        if (tid < m) bidx[tid] = -1;
+        __syncthreads();
+        for (int i = tid; i < nown_s; i += WAVE * NW) {   // a movable molecule's particles are all in `own`
+            int id = sid[own[i]] & CHAIN_IDMASK;
+            if (id >= f && id < f + m) bidx[id - f] = own[i];
+        }
+        __syncthreads();
+        if (w == 0) {
+            // a particle outside the region means the molecule is wider than rho
+            bool miss = __ballot(lane < m && bidx[lane] < 0) != 0ull;
+            int rej = 1;
+            if (!miss) {
+                if (lane < m) {
+                    int i = bidx[lane];
+                    Uo[lane] = su[i];
+                    Oo[lane] = so[i];
+                    Qo[lane] = sq[i];
+                }
+                wave_lds_sync();
+                rej = chain_trial_state(disp, ra, ctr, cp, sc, gc, cell, m, Uo, Oo, Qo, Un, On, Qn, vol);
+            }
+            if (lane == 0) misc[NW] = rej;
+        }
+        __syncthreads();
+        if (misc[NW] != 0) {
+            n_rej++;
+            __syncthreads();
+            continue;
+        }
+        double acc_o = 0.0, acc_n = 0.0;
+        for (int it = w; it < 2 * m; it += NW) {
+            int a = it >> 1;
+            bool st = (it & 1) != 0;
+            uint4 ua = st ? Un[a] : Uo[a];
+            double e = chain_item_energy_st(ua, st ? On[a] : Oo[a], st ? Qn[a] : Qo[a], f, m, g, f0, fn, spre, own,
+                                            nown_s, sc, pp, tp, su, so, sq, sid, queue, spr);
+            if (st) acc_n += e; else acc_o += e;
+        }
+        double eo = block_sum<NW>(acc_o, red), en = block_sum<NW>(acc_n, red + NW);
+        ctr.w = (cp.pass_hi << 2) | 2u;
+        U4 rc = philox(ctr, cp.key0, cp.key1);
+        double de = en - eo;
+        // moveTry (movecreator.h:175-187) on mol2others before and after
+        bool acc = de <= 0.0 || exp(-de / cp.temper) > u01d(rc.x, rc.y);
+        if (acc) {
+            if (disp) n_dacc++; else n_racc++;
+            dsum += de;
+            if (tid < m) {
+                int i = bidx[tid], s = slot[f + tid];
+                su[i] = Un[tid];
+                so[i] = On[tid];
+                sq[i] = Qn[tid];
+                P[s] = Un[tid];
+                O[s] = On[tid];
+                Q[s] = Qn[tid];
+            }
+        }
+        __syncthreads();
+    }
+    int rep = (int)(blockIdx.x & (STAT_REP - 1));
+    if (tid < 5) {
+        unsigned long long v = tid == 0 ? n_dacc : tid == 1 ? n_dtry : tid == 2 ? n_racc : tid == 3 ? n_rtry : n_rej;
+        if (v) atomicAdd(&stats[rep * STAT_STRIDE + 10 + tid], v);
+    } else if (tid == 5) {
+        atomicAdd(&de_sum[rep * STAT_STRIDE], dsum);
+    } else if (tid == 6 && over) {
+        atomicAdd(&stats[rep * STAT_STRIDE + 15], 1ull);
     }
 }
 
@@ -2356,6 +2806,8 @@ struct sc_ctx {
     double* d_vol = nullptr;            // Ia_param::volume per dense type
     uint32_t chain_stamp = 0;
     double chainprob = 0.0, chain_mx = 0.0, chain_angle = 0.0, chain_rho = 0.0;
+    bool chain_staged = false;          // the last chain passes ran with the LDS stage
+    int chain_period = 0;               // and this colour period
 };
 
 static ChainTab chain_tab(const sc_ctx* ctx) {
@@ -3852,6 +4304,13 @@ int sc_get_chain_extent(sc_ctx* ctx, double* extent) {
     return SC_OK;
 }
 
+int sc_chain_pass_info(const sc_ctx* ctx, int* period, int* staged) {
+    if (!ctx || !period || !staged) return SC_EINVAL;
+    *period = ctx->chain_period;
+    *staged = ctx->chain_staged ? 1 : 0;
+    return SC_OK;
+}
+
 // The chain-move passes of sweep `sw`: anchors on a coarse grid with its own
 // random shift, then one pass per colour (random order), each followed by a
 // rebuild of the fine cell list (moved particles may have changed cell).
@@ -3931,6 +4390,23 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
         cp.nact[k] = gc.nc[k] / pa[k];
     }
     size_t lds = chain_pass_lds(CHAIN_NW);
+    // the staged pass when a region (the coarse cell widened by rho + maxcut)
+    // holds at most `cap` particles with a margin of 1.25 over the mean density
+    int cap = 64;
+    while (chain_st_layout(CHAIN_NW, cap + 64).total <= CHAIN_LDS_BUDGET) cap += 64;
+    double reach = ctx->chain_rho + ctx->maxcut, vreg = 1.0;
+    long nrc = 1;
+    for (int k = 0; k < 3; k++) {
+        double wk = ctx->box[k] / gc.nc[k], ext = std::min(ctx->box[k], wk + 2.0 * reach);
+        vreg *= ext;
+        nrc *= std::min((long)ctx->g.nc[k], (long)ceil(ext / (ctx->box[k] / ctx->g.nc[k])) + 1);
+        cp.reach[k] = (uint32_t)std::min(4294967295.0, ceil(reach / ctx->box[k] * 4294967296.0) + 1.0);
+    }
+    double boxv = ctx->box[0] * ctx->box[1] * ctx->box[2];
+    bool staged = 1.25 * (double)ctx->n * vreg / boxv <= (double)cap && nrc <= CHAIN_ST_RCAP && ctx->g.r == 1;
+    size_t lds_st = chain_st_layout(CHAIN_NW, cap).total;
+    ctx->chain_staged = staged;
+    ctx->chain_period = p;
     for (int ci = 0; ci < p * p * p; ci++) {
         int col = plan.order[ci];
         cp.ox = col % p;
@@ -3947,10 +4423,16 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
         }
         c = ctx->cur;
         int nblk = cp.nact[0] * cp.nact[1] * cp.nact[2];
-        hipLaunchKernelGGL((k_chain_pass<CHAIN_NW>), dim3(nblk), dim3(WAVE * CHAIN_NW), lds, ctx->stream, ctx->g, gc, sc, cp,
-                           ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->slot, ctx->d_pp32,
-                           ctx->d_tp32, ctx->d_vol, ctx->d_cfirst, ctx->d_csize, ctx->d_cstart, ctx->d_clist,
-                           ctx->d_ccell_id, ctx->d_stamp, ctx->d_stats, ctx->d_dsum, ctx->d_flag);
+        if (staged)
+            hipLaunchKernelGGL((k_chain_pass_st<CHAIN_NW>), dim3(nblk), dim3(WAVE * CHAIN_NW), lds_st, ctx->stream,
+                               ctx->g, gc, sc, cp, cap, ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c],
+                               ctx->slot, ctx->d_pp32, ctx->d_tp32, ctx->d_vol, ctx->d_cfirst, ctx->d_csize,
+                               ctx->d_cstart, ctx->d_clist, ctx->d_ccell_id, ctx->d_stats, ctx->d_dsum);
+        else
+            hipLaunchKernelGGL((k_chain_pass<CHAIN_NW>), dim3(nblk), dim3(WAVE * CHAIN_NW), lds, ctx->stream, ctx->g,
+                               gc, sc, cp, ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->slot,
+                               ctx->d_pp32, ctx->d_tp32, ctx->d_vol, ctx->d_cfirst, ctx->d_csize, ctx->d_cstart,
+                               ctx->d_clist, ctx->d_ccell_id, ctx->d_stamp, ctx->d_stats, ctx->d_dsum, ctx->d_flag);
         HIPCK(hipGetLastError());
         if ((rc = build_grid(ctx, r, shift, ctx->box))) return rc;
     }
@@ -4103,6 +4585,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         stats->chain_rot_acc = hs[12];
         stats->chain_rot_try = hs[13];
         stats->chain_rejected = hs[14];
+        stats->chain_cells_skipped = hs[15];
     }
     return SC_OK;
 }

@@ -120,22 +120,29 @@ def kabsch(a, b):
     return vt.T @ np.diag([1.0, 1.0, dd]) @ u.T
 
 
-def test_chain_move_passes_are_rigid_reproducible_and_keep_the_energy():
+@pytest.mark.parametrize("extent", [0.0, 7.0])
+def test_chain_move_passes_are_rigid_reproducible_and_keep_the_energy(extent):
     """Particle moves of length and angle 0 (they change nothing), chain moves
     at chainprob 0.25: every molecule keeps its shape, the energy changes by
     the sum of the accepted dE, a second run with the same seed gives the
-    same state bit for bit."""
-    say("case: chain-move passes, 8000 chains")
+    same state bit for bit.  With the automatic extent each coarse cell's
+    neighbourhood fits the LDS stage; an extent of 7 sigma makes the regions
+    too large for it, so the passes read global memory."""
+    say("case: chain-move passes, 8000 chains, extent", extent)
     g = sc_amd.GpuSystem(0)
     nch = 8000
     tp, (box, pos, d, q, t) = lattice(g, nch, 0.2, 6)
-    g.set_chain_moves(0.25, 0.6, 0.25)
+    g.set_chain_moves(0.25, 0.6, 0.25, extent)
     rho = g.chain_extent()
     say("box %.3f, extent %.3f, maxcut %.3f" % (box[0], rho, tp.maxcut))
     assert box[0] >= 2 * (2 * rho + tp.maxcut)   # at least two coarse cells per axis
     e0 = g.allToAll()
     b0, p0, d0, q0 = g.get_config()
     s = g.sweep(3, 1.1, 0.0, 0.0, r=1, seed=9, counter=0)
+    period, staged = g.chain_pass_info()
+    say("colour period %d, LDS stage %s, cells skipped %d" % (period, staged, s["chain_cells_skipped"]))
+    assert staged == (extent == 0.0)
+    assert s["chain_cells_skipped"] == 0
     e1 = g.allToAll()
     n = len(t)
     tries = s["chain_disp_try"] + s["chain_rot_try"]
@@ -229,6 +236,7 @@ def test_chain_move_ensemble_matches_cpu_oracle():
     g.set_chain_moves(0.1, 0.5, 0.2)
     n = len(t)
     g.sweep(300, 1.1, 0.3, ROT_HALF, r=1, seed=5, counter=0)
+    assert g.chain_pass_info()[1]   # the LDS-staged passes
     obs = []
     tot = {}
     for blk in range(32):
