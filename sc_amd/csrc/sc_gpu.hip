@@ -4391,7 +4391,7 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
     }
     size_t lds = chain_pass_lds(CHAIN_NW);
     // the staged pass when a region (the coarse cell widened by rho + maxcut)
-    // holds at most `cap` particles with a margin of 1.25 over the mean density
+    // holds at most `cap` particles with a margin of 1.12 over the mean density
     int cap = 64;
     while (chain_st_layout(CHAIN_NW, cap + 64).total <= CHAIN_LDS_BUDGET) cap += 64;
     double reach = ctx->chain_rho + ctx->maxcut, vreg = 1.0;
@@ -4403,7 +4403,7 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
         cp.reach[k] = (uint32_t)std::min(4294967295.0, ceil(reach / ctx->box[k] * 4294967296.0) + 1.0);
     }
     double boxv = ctx->box[0] * ctx->box[1] * ctx->box[2];
-    bool staged = 1.25 * (double)ctx->n * vreg / boxv <= (double)cap && nrc <= CHAIN_ST_RCAP && ctx->g.r == 1;
+    bool staged = 1.12 * (double)ctx->n * vreg / boxv <= (double)cap && nrc <= CHAIN_ST_RCAP && ctx->g.r == 1;
     size_t lds_st = chain_st_layout(CHAIN_NW, cap).total;
     ctx->chain_staged = staged;
     ctx->chain_period = p;
