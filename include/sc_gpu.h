@@ -208,7 +208,8 @@ int sc_set_block_waves(sc_ctx* ctx, int nw);
  * unwrapped coordinates.  Moves run on a coarse 2x2x2 checkerboard of cells
  * at least 2 extent + maxcut wide; molecules wider than `extent` (0: 1.2 x
  * the widest molecule now) do not move, nor does a move leave the anchor's
- * coarse cell.  Box edges must be at least 2 extent + maxcut and above 4
+ * coarse cell (the extent is fixed at this call; call again after loading
+ * another configuration).  Box edges must be at least 2 extent + maxcut and above 4
  * extent (no molecule reaches its own image, which the reference's
  * mol2others has no term for, and the minimum image unwraps every molecule).
  * Molecules of at most 64 particles.  chainprob = 0 turns the moves off. */

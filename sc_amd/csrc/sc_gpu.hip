@@ -622,9 +622,9 @@ __global__ __launch_bounds__(1024) void k_reduce(int n, const double* __restrict
 // --------------------------------------------------------- energy (queries)
 // One wave per query particle: its energy with every particle of the stencil
 // around its cell except those of ID ex_lo..ex_hi (the particle itself, or
-// its whole molecule for mol2others), plus its bonds when it is queried alone
-// (ex_lo == ex_hi).  The query state is given explicitly (trial states) or
-// taken from a slot.
+// its whole molecule for mol2others), plus its bonds for oneToAll (not for
+// mol2others, whose connectivity list is empty).  The query state is given
+// explicitly (trial states) or taken from a slot.
 struct Query {
     uint32_t u[3];
     int t;
@@ -632,6 +632,7 @@ struct Query {
     float2 q;
     int id;            // the query particle's ID
     int ex_lo, ex_hi;  // IDs left out
+    int bonds;         // add the bonds with its chain partners
 };
 
 __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __restrict__ qs, const int* __restrict__ qslot,
@@ -650,6 +651,7 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
         uint4 p = P[s];
         q.u[0] = p.x; q.u[1] = p.y; q.u[2] = p.z; q.t = p.w;
         q.o = O[s]; q.q = Q[s]; q.id = q.ex_lo = q.ex_hi = ID[s];
+        q.bonds = 1;
     } else {
         q = qs[qi];
     }
@@ -673,7 +675,7 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
         }
     }
     // chain partners, wherever they are
-    if (ct.cw && lane == 0 && q.ex_lo == q.ex_hi)
+    if (ct.cw && lane == 0 && q.bonds)
         acc += bonds_global(ct, sc.nt, sc.Ld, q.id, make_uint4(q.u[0], q.u[1], q.u[2], (uint32_t)q.t), q.o, q.q,
                             15, slot, P, O, Q);
     double s = wave_sum(acc);
@@ -4013,6 +4015,7 @@ static Query make_query(sc_ctx* ctx, int target, const double pos[3], const doub
     q.o = make_float4((float)d[0], (float)d[1], (float)d[2], (float)p[0]);
     q.q = make_float2((float)p[1], (float)p[2]);
     q.id = q.ex_lo = q.ex_hi = target;
+    q.bonds = 1;
     return q;
 }
 
@@ -4081,6 +4084,7 @@ static int sc_energy_mol_impl(sc_ctx* ctx, int first, int size, const double* po
     for (int a = 0; a < size; a++) {
         q[a].ex_lo = first;
         q[a].ex_hi = first + size - 1;
+        q[a].bonds = 0;
     }
     std::vector<double> out(size);
     int rc = run_queries(ctx, size, q.data(), nullptr, out.data());

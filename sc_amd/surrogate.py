@@ -34,6 +34,14 @@ def train(g, ptype, nsamples=10_000_000, epochs=10, batch=4096, lr=2e-3, lr_fina
            "target_std": float(np.std(yv)),
            "target_std_inside_rcut": float(np.std(yv[inside])) if inside.any() else 0.0,
            "generate_s": t1 - t0, "train_s": t2 - t1}
+    # arithmetic of the training steps: forward 2 (12*64 + 64*64 + 64*64 + 64) = 18,048 flop per sample,
+    # backward twice that (input and weight gradients); FP32 on the vector units (peak 157.3 TFLOP/s)
+    steps = ((nsamples - nval) // batch) * epochs
+    flop = 3.0 * 18048.0 * batch * steps
+    res["train_steps"] = int(steps)
+    res["train_tflops"] = flop / max(t2 - t1, 1e-9) / 1e12
+    res["train_roofline"] = {"bound": "fp32", "achieved": res["train_tflops"], "peak": 157.3, "unit": "TFLOP/s",
+                             "frac": res["train_tflops"] / 157.3}
     if log:
         log("surrogate: %d samples (%.2fs), %d epochs on the GPU (%.2fs): train mse %.3e, held-out rmse %.4f "
             "(target std %.4f)" % (nsamples, t1 - t0, epochs, t2 - t1, loss, rmse, res["target_std"]))

@@ -73,6 +73,11 @@ def test_mol2others_trial_and_commit(name):
         len(first), np.abs(eg - eo).max(), np.abs(eg - ref).max()))
     np.testing.assert_allclose(eg, eo, rtol=1e-4, atol=1e-5)
     np.testing.assert_allclose(eg, ref, rtol=1e-3, atol=1e-4)
+    # a one-particle "molecule" taken out of a chain: mol2others's empty
+    # connectivity list leaves the bonds out (oneToAll has them)
+    f = int(first[0])
+    want1 = po.mol2others(tp, st, [f], [1])[0]
+    assert g.mol2others(f, 1) == pytest.approx(want1, rel=1e-4, abs=1e-5)
     rng = np.random.default_rng(4)
     vol = np.array([tp.ia(t, t).volume for t in st.types])
     for k in range(min(4, len(first))):
