@@ -4407,7 +4407,12 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
         cp.reach[k] = (uint32_t)std::min(4294967295.0, ceil(reach / ctx->box[k] * 4294967296.0) + 1.0);
     }
     double boxv = ctx->box[0] * ctx->box[1] * ctx->box[2];
-    bool staged = 1.12 * (double)ctx->n * vreg / boxv <= (double)cap && nrc <= CHAIN_ST_RCAP && ctx->g.r == 1;
+    // and when a cell's molecules and their particles fit their lists (1.25 x the mean)
+    double mol_cell = (double)nch / (double)ncc, part_mol = 0.0;
+    for (int m : ctx->chain_size) part_mol += m;
+    part_mol /= (double)nch;
+    bool staged = 1.12 * (double)ctx->n * vreg / boxv <= (double)cap && nrc <= CHAIN_ST_RCAP && ctx->g.r == 1 &&
+                  1.25 * mol_cell <= CHAIN_ST_LCAP && 1.25 * mol_cell * part_mol <= CHAIN_ST_OCAP;
     size_t lds_st = chain_st_layout(CHAIN_NW, cap).total;
     ctx->chain_staged = staged;
     ctx->chain_period = p;

@@ -261,3 +261,60 @@ def test_chain_move_ensemble_matches_cpu_oracle():
         say("%s: gpu %.5f +- %.5f, cpu %.5f +- %.5f" % (name, mg, sg, mc, scc))
         assert abs(mg - mc) <= 3.0 * np.hypot(sg, scc) + 0.01 * abs(mc), name
     g.close()
+
+
+def test_chain_move_npt_ensemble_matches_cpu_oracle():
+    """NPT (P* = 0.5, T* = 1.1, one isotropic volume move per sweep) of 400
+    chains with particle and chain moves (chainprob 0.1): packing fraction and
+    E/N of the GPU against the oracle's sequential Metropolis with the same
+    moves, both continuing from one oracle-compressed state: equal within 3
+    combined standard errors + 1 %.  The chain moves run on boxes that change
+    every sweep."""
+    say("case: chain-move NPT ensemble GPU vs CPU oracle")
+    nch, press, temper, edge = 400, 0.5, 1.1, 0.3
+    cp_, cmx, cang = 0.1, 0.5, 0.2
+    top = W.chain_topology(nch)
+    tp = po.Topo(top)
+    box, pos, d, q, t = W.chain_lattice(nch, 0.15, 23)
+    st, _, _, _ = po.mc_run_chains(tp, po.State(box, pos, d, q, t), 800, temper, 0.3, ROT_HALF, cp_, cmx, cang,
+                                   seed=5, ptype=1, press=press, edge_mx=edge)
+    vchain = nch * W.chain_volume()
+    n = st.n
+    cur, drift, _, _ = po.mc_run_chains(tp, st, 200, temper, 0.3, ROT_HALF, cp_, cmx, cang, seed=7, ptype=1,
+                                        press=press, edge_mx=edge)
+    assert abs(drift) < 1e-7 * max(1.0, n)
+    ec = []
+    for blk in range(24):
+        cur, _, s, ef = po.mc_run_chains(tp, cur, 25, temper, 0.3, ROT_HALF, cp_, cmx, cang, seed=300 + blk, ptype=1,
+                                         press=press, edge_mx=edge)
+        ec.append((vchain / np.prod(cur.box), ef / n))
+    ec = np.array(ec)
+    g = sc_amd.GpuSystem(0)
+    g.load_topology(top)
+    g.set_config(st.box, st.pos, st.dir, st.pd0, st.types)
+    # an extent limit of 4.8 sigma keeps 4 x the extent below the compressed
+    # box (~21 sigma); wider molecules do not move (same ensemble, slower)
+    g.set_chain_moves(cp_, cmx, cang, 4.8)
+    say("box %.3f, extent %.3f" % (st.box[0], g.chain_extent()))
+    obs = []
+    tot = {}
+    for sw in range(800):
+        s = g.sweep(1, temper, 0.3, ROT_HALF, r=1, seed=11, counter=sw)
+        g.volume_move(1, press, temper, edge, seed=12, counter=sw)
+        if sw >= 200:
+            for k, v in s.items():
+                tot[k] = tot.get(k, 0) + v
+            if (sw - 200) % 25 == 24:
+                b = g.get_config()[0]
+                obs.append((vchain / np.prod(b), g.allToAll() / n))
+    obs = np.array(obs)
+    say("gpu chain moves: displace %d/%d, rotate %d/%d, rejected %d, cells skipped %d; period %d, LDS stage %s" % (
+        tot["chain_disp_acc"], tot["chain_disp_try"], tot["chain_rot_acc"], tot["chain_rot_try"],
+        tot["chain_rejected"], tot["chain_cells_skipped"], *g.chain_pass_info()))
+    assert tot["chain_disp_acc"] > 0 and tot["chain_rot_acc"] > 0 and tot["chain_cells_skipped"] == 0
+    for c, name in enumerate(("packing fraction", "E/N")):
+        mg, sg = _blocked(obs[:, c])
+        mc, scc = _blocked(ec[:, c])
+        say("%s: gpu %.5f +- %.5f, cpu %.5f +- %.5f" % (name, mg, sg, mc, scc))
+        assert abs(mg - mc) <= 3.0 * np.hypot(sg, scc) + 0.01 * abs(mc), name
+    g.close()
