@@ -71,6 +71,81 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
         return s;
     }
 }
+// Wave sum in a fixed tree through DPP lane moves instead of LDS shuffles:
+// pairs of lanes (xor 1, xor 2), quads (half-row mirror) and halves (row
+// mirror) inside each row of 16 lanes, then the four row sums read back in
+// order.  The same value on every lane.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, true);
+    int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double row_lane(double v, int l) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                            __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_move<0xb1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_move<0x4e>(v);    // quad_perm [2,3,0,1]
+    v += dpp_move<0x141>(v);   // row_half_mirror
+    v += dpp_move<0x140>(v);   // row_mirror
+    return (row_lane(v, 0) + row_lane(v, 16)) + (row_lane(v, 32) + row_lane(v, 48));
+}
+// Two block sums of a sequence of block-uniform calls behind one barrier:
+// consecutive calls alternate between two halves of red (par flips), so a
+// call's writes never meet the previous call's reads.  red: 4 NW doubles.
+template <int NW>
+__device__ __forceinline__ void block_sum2(double a, double b, double* red, int& par, double& sa, double& sb) {
+    a = wave_sum_dpp(a);
+    b = wave_sum_dpp(b);
+    if constexpr (NW == 1) {
+        sa = a;
+        sb = b;
+    } else {
+        double* r = red + 2 * NW * par;
+        par ^= 1;
+        if (lane_id() == 0) {
+            r[2 * (threadIdx.x >> 6)] = a;
+            r[2 * (threadIdx.x >> 6) + 1] = b;
+        }
+        __syncthreads();
+        double x = 0.0, y = 0.0;
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            x += r[2 * k];
+            y += r[2 * k + 1];
+        }
+        sa = x;
+        sb = y;
+    }
+}
+// block_compact with one barrier, alternating halves of cnt as block_sum2
+// does.  cnt: 2 NW ints.
+template <int NW>
+__device__ __forceinline__ int block_compact1(bool flag, int& total, int* cnt, int& par) {
+    unsigned long long m = __ballot(flag);
+    if constexpr (NW == 1) {
+        int pos = total + wave_prefix(m);
+        total += __popcll(m);
+        return pos;
+    } else {
+        int* c = cnt + NW * par;
+        par ^= 1;
+        int w = threadIdx.x >> 6;
+        if (lane_id() == 0) c[w] = __popcll(m);
+        __syncthreads();
+        int base = total, all = 0;
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            int ck = c[k];
+            base += k < w ? ck : 0;
+            all += ck;
+        }
+        total += all;
+        return base + wave_prefix(m);
+    }
+}
 // Stable compaction step over a chunk of NW*64 items: the output index of
 // this thread's item (when flag), in thread order; total advances by the
 // chunk's count.  cnt: NW ints of LDS.
@@ -814,9 +889,29 @@ struct TrialScratch {
     float* xd;   // segment distance of each expensive entry, 2*cap
     float* mv;   // 2 * MV_STRIDE floats
     int* bp;     // staged index of each chain partner (4 ints)
-    int* cnt;    // block compaction counts (4 ints)
-    double* red; // block sums (4 doubles)
+    int* cnt;    // block compaction counts (8 ints)
+    double* red; // block sums (16 doubles)
 };
+
+// A trial's proposal and its Metropolis thresholds, drawn for TRIAL_CHUNK
+// trials at a time by the lanes of the first wave, so that the chain of
+// trials does not wait on the RNG.  The thresholds are -T ln u: the test
+// exp(-dE/T) > u of moveTry (movecreator.h:175-187) is dE < -T ln u.
+struct __align__(16) TrialRec {
+    int k;          // staged index of the particle
+    int translate;  // 1: translation by (dx, dy, dz); 0: rotation
+    int w[5];       // dx, dy, dz (fixed point) | axis x, y, z, cos, sin of the half angle
+    int pad;
+    double thr;     // final stage
+    double thr1;    // first screen of the delayed acceptance
+    double thr2;    // second screen
+    double pad2;
+};
+static_assert(sizeof(TrialRec) == 64, "trial record size");
+#define TRIAL_CHUNK 64
+// LDS after the trial scratch: chain partners, compaction counts, block sums
+// (8-aligned), trial records (16-aligned)
+#define SWEEP_SMALL_LDS (16 + 32 + 8 + 128 + 8 + TRIAL_CHUNK * 64)
 
 // sweep counters: STAT_REP replicas of 8 counters, STAT_STRIDE apart (128 B)
 #define STAT_REP 64
@@ -847,8 +942,8 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                                                const DPart<float>& po, const DPart<float>& pn, const SysConst& sc,
                                                const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
                                                const TrialScratch& T, const float* ml, const MlpRegs& mr,
-                                               const MlpConst& mc, const ChainTab& ct, double temper, double u1,
-                                               double u2, double& eo, double& en, double& des, int& stage) {
+                                               const MlpConst& mc, const ChainTab& ct, double thr1, double thr2,
+                                               double& eo, double& en, double& des, int& stage) {
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
     int t = uo.w;
     bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
@@ -858,6 +953,9 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         cwk = ct.cw[idk];
     }
     int bmask = cwk & 15;
+    // parities of the one-barrier helpers; a trial starts after a barrier
+    // that follows every read of the previous trial's helpers
+    int cpar = 0, rpar = 0;
     if (tid == 0) {
         mv_store(T.mv, po);
         mv_store(T.mv + MV_STRIDE, pn);
@@ -882,7 +980,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                 if (bit >= 0 && ((bmask >> bit) & 1)) T.bp[bit] = j;
             }
         }
-        int pos = block_compact<NW>(fl != 0, nn, T.cnt);
+        int pos = block_compact1<NW>(fl != 0, nn, T.cnt, cpar);
         if (fl) T.nl[pos] = j | (fl << 28);
     }
     __syncthreads();
@@ -938,7 +1036,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                 if (which) an += (double)ev; else ao += (double)ev;
             }
         }
-        int pos = block_compact<NW>(need, nx, T.cnt);
+        int pos = block_compact1<NW>(need, nx, T.cnt, cpar);
         if (need) {
             T.xl[pos] = jj | (which << 30);
             if constexpr (DA == DA_NET) T.xd[pos] = sqrtf(dseg2);
@@ -951,10 +1049,11 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     des = 0.0;
     stage = 2;
     if constexpr (DA != DA_OFF) {
-        double rn = block_sum<NW>(an, T.red);
-        double rep = rn - block_sum<NW>(ao, T.red);
+        double rn, ro;
+        block_sum2<NW>(an, ao, T.red, rpar, rn, ro);
+        double rep = rn - ro;
         stage = 0;
-        if (!(rep <= 0.0 || exp(-rep / temper) > u1)) return false;
+        if (!(rep <= 0.0 || rep < thr1)) return false;
         double ds = 0.0;
         if constexpr (DA == DA_NET) {
             stage = 1;
@@ -988,9 +1087,10 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                     if (which) sn += (double)out; else so += (double)out;
                 }
             }
-            double dn = block_sum<NW>(sn, T.red);
-            ds = dn - block_sum<NW>(so, T.red);
-            if (!(ds <= 0.0 || exp(-ds / temper) > u2)) return false;
+            double dn, d0;
+            block_sum2<NW>(sn, so, T.red, rpar, dn, d0);
+            ds = dn - d0;
+            if (!(ds <= 0.0 || ds < thr2)) return false;
         }
         des = rep + ds;
         stage = 2;
@@ -1023,8 +1123,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             if (which) an += (double)ev; else ao += (double)ev;
         }
     }
-    eo = block_sum<NW>(ao, T.red);
-    en = block_sum<NW>(an, T.red);
+    block_sum2<NW>(ao, an, T.red, rpar, eo, en);
     return true;
 }
 
@@ -1058,11 +1157,12 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     int* scratch = (int*)(T.mv + 2 * MV_STRIDE);
     T.bp = scratch + 2 * 125 + 2;
     T.cnt = T.bp + 4;
-    T.red = (double*)(((uintptr_t)(T.cnt + 4) + 7) & ~(uintptr_t)7);
+    T.red = (double*)(((uintptr_t)(T.cnt + 8) + 7) & ~(uintptr_t)7);
+    TrialRec* recs = (TrialRec*)(((uintptr_t)(T.red + 16) + 15) & ~(uintptr_t)15);
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1);
     float* ml = nullptr;
     MlpRegs mr;
-    uintptr_t tail = (uintptr_t)(T.red + 4);
+    uintptr_t tail = (uintptr_t)(recs + TRIAL_CHUNK);
     if constexpr (DA == DA_NET) {
         ml = (float*)((tail + 15) & ~(uintptr_t)15);
         mlp_setup(mlp_img, ml, lane, mr);
@@ -1101,43 +1201,81 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         int fl = (int)floorf(mean);
         ntrial = fl + ((u01f(rd.x) < mean - (float)fl) ? 1 : 0);
     }
-    unsigned long long n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0,
-                       n_s2try = 0, n_s2acc = 0, n_bond = 0;
+    unsigned n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0, n_s2try = 0,
+             n_s2acc = 0, n_bond = 0;   // per block: far below 2^32
     double dsum = 0.0;
     for (int t = 0; t < ntrial; t++) {
-        ctr.x = (uint32_t)t;
-        ctr.w = mp.pass_hi << 2;
-        U4 ra = philox(ctr, mp.key0, mp.key1);
-        int k = S.h0 + (int)(((uint64_t)ra.x * (uint64_t)nh) >> 32);
+        if ((t & (TRIAL_CHUNK - 1)) == 0) {
+            // the next TRIAL_CHUNK trials' proposals and thresholds, one per lane
+            // of the first wave (the same draws as trial by trial: counter
+            // (trial, cell, pass, stream))
+            __syncthreads();
+            int tt = t + tid;
+            if (tid < TRIAL_CHUNK && tt < ntrial) {
+                U4 c = ctr;
+                c.x = (uint32_t)tt;
+                c.w = mp.pass_hi << 2;
+                U4 ra = philox(c, mp.key0, mp.key1);
+                TrialRec R;
+                R.k = S.h0 + (int)(((uint64_t)ra.x * (uint64_t)nh) >> 32);
+                int g0 = tp[S.u[R.k].w].geotype;
+                R.translate = (u01f(ra.y) < 0.5f) || (g0 >= 30);
+                if (R.translate) {
+                    V3<float> d = unit_from(u01f(ra.z), u01f(ra.w));
+                    R.w[0] = (int32_t)rintf(d.x * mp.transmx * (4294967296.0f / Lx));
+                    R.w[1] = (int32_t)rintf(d.y * mp.transmx * (4294967296.0f / Ly));
+                    R.w[2] = (int32_t)rintf(d.z * mp.transmx * (4294967296.0f / Lz));
+                    R.w[3] = R.w[4] = 0;
+                } else {
+                    c.w = (mp.pass_hi << 2) | 1u;
+                    U4 rb = philox(c, mp.key0, mp.key1);
+                    V3<float> axv = unit_from(u01f(rb.x), u01f(rb.y));
+                    float ang = mp.rot_half * u01f(rb.z);
+                    float vc = cosf(ang), vs = sqrtf(fmaxf(0.0f, 1.0f - vc * vc));
+                    if (u01f(rb.w) >= 0.5f) vs = -vs;   // Particle::pscRotate direction choice (particle.h:194-196)
+                    R.w[0] = __float_as_int(axv.x);
+                    R.w[1] = __float_as_int(axv.y);
+                    R.w[2] = __float_as_int(axv.z);
+                    R.w[3] = __float_as_int(vc);
+                    R.w[4] = __float_as_int(vs);
+                }
+                R.pad = 0;
+                c.w = (mp.pass_hi << 2) | 2u;
+                U4 rc = philox(c, mp.key0, mp.key1);
+                R.thr = -mp.temper * log(u01d(rc.x, rc.y));
+                R.thr1 = R.thr2 = R.pad2 = 0.0;
+                if constexpr (DA != DA_OFF) {   // the screens' uniforms
+                    R.thr1 = -mp.temper * log(u01d(rc.z, rc.w));
+                    c.x |= 0x80000000u;
+                    U4 rt = philox(c, mp.key0, mp.key1);
+                    R.thr2 = -mp.temper * log(u01d(rt.x, rt.y));
+                }
+                recs[tid] = R;
+            }
+            __syncthreads();
+        }
+        const TrialRec& R = recs[t & (TRIAL_CHUNK - 1)];
+        int k = R.k;
         uint4 uo = S.u[k];
         float4 oo = S.o[k];
         float2 qo = S.q[k];
-        int g0 = tp[uo.w].geotype;
-        bool translate = (u01f(ra.y) < 0.5f) || (g0 >= 30);
+        bool translate = R.translate != 0;
         uint4 un = uo;
         float4 on = oo;
         float2 qn = qo;
         if (translate) {
             n_ttry++;
-            V3<float> d = unit_from(u01f(ra.z), u01f(ra.w));
-            int32_t dx = (int32_t)rintf(d.x * mp.transmx * (4294967296.0f / Lx));
-            int32_t dy = (int32_t)rintf(d.y * mp.transmx * (4294967296.0f / Ly));
-            int32_t dz = (int32_t)rintf(d.z * mp.transmx * (4294967296.0f / Lz));
-            un.x = uo.x + (uint32_t)dx;
-            un.y = uo.y + (uint32_t)dy;
-            un.z = uo.z + (uint32_t)dz;
+            un.x = uo.x + (uint32_t)R.w[0];
+            un.y = uo.y + (uint32_t)R.w[1];
+            un.z = uo.z + (uint32_t)R.w[2];
             if (cell_of_u(g, un.x, un.y, un.z) != cell) {   // leaving the active cell: rejected
                 n_left++;
                 continue;
             }
         } else {
             n_rtry++;
-            ctr.w = (mp.pass_hi << 2) | 1u;
-            U4 rb = philox(ctr, mp.key0, mp.key1);
-            V3<float> axv = unit_from(u01f(rb.x), u01f(rb.y));
-            float ang = mp.rot_half * u01f(rb.z);
-            float vc = cosf(ang), vs = sqrtf(fmaxf(0.0f, 1.0f - vc * vc));
-            if (u01f(rb.w) >= 0.5f) vs = -vs;   // Particle::pscRotate direction choice (particle.h:194-196)
+            V3<float> axv = mk<float>(__int_as_float(R.w[0]), __int_as_float(R.w[1]), __int_as_float(R.w[2]));
+            float vc = __int_as_float(R.w[3]), vs = __int_as_float(R.w[4]);
             V3<float> d = qrot(mk<float>(oo.x, oo.y, oo.z), axv, vc, vs);
             V3<float> pq = qrot(mk<float>(oo.w, qo.x, qo.y), axv, vc, vs);
             d = normalised(d);
@@ -1146,23 +1284,13 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
             qn = make_float2(pq.y, pq.z);
         }
         double eo, en, des;
-        ctr.w = (mp.pass_hi << 2) | 2u;
-        double u1 = 0.0, u2 = 0.0;   // the screens' uniforms (drawn before the trial only with DA)
-        if constexpr (DA != DA_OFF) {
-            U4 rs = philox(ctr, mp.key0, mp.key1);
-            u1 = u01d(rs.z, rs.w);
-            U4 c2 = ctr;
-            c2.x |= 0x80000000u;
-            U4 rt = philox(c2, mp.key0, mp.key1);
-            u2 = u01d(rt.x, rt.y);
-        }
         bool pass;
         int stage;
         {
             DPart<float> po = load_part<float>(oo, qo, uo.w, tp);
             DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
-            pass = trial_energies<DA, NW>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, mp.temper, u1,
-                                          u2, eo, en, des, stage);
+            pass = trial_energies<DA, NW>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, R.thr1, R.thr2,
+                                          eo, en, des, stage);
         }
         if (stage < 0) {   // a chain partner outside the stencil
             n_bond++;
@@ -1178,12 +1306,11 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
             }
             if (!pass) { __syncthreads(); continue; }
         }
-        U4 rc = philox(ctr, mp.key0, mp.key1);
         double de = en - eo;
         // MoveCreator::moveTry (movecreator.h:175-187); with the surrogate the
         // second stage accepts with min(1, exp(-(dE - dE_s)/T)) (Christen & Fox 2005)
         double de2 = de - des;
-        bool acc = de2 <= 0.0 || exp(-de2 / mp.temper) > u01d(rc.x, rc.y);
+        bool acc = de2 <= 0.0 || de2 < R.thr;
         if (acc) {
             if (translate) n_tacc++; else n_racc++;
             dsum += de;
@@ -1206,10 +1333,10 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     // line each), so a thousand waves ending together do not queue on one line
     int rep = (int)(blockIdx.x & (STAT_REP - 1));
     if (tid < 10) {
-        unsigned long long v = tid == 0 ? n_tacc : tid == 1 ? n_ttry : tid == 2 ? n_racc : tid == 3 ? n_rtry
-                             : tid == 4 ? n_left : tid == 5 ? n_s1acc : tid == 6 ? n_s1try : tid == 7 ? n_s2acc
-                             : tid == 8 ? n_s2try : n_bond;
-        if (v) atomicAdd(&stats[rep * STAT_STRIDE + tid], v);
+        unsigned v = tid == 0 ? n_tacc : tid == 1 ? n_ttry : tid == 2 ? n_racc : tid == 3 ? n_rtry
+                   : tid == 4 ? n_left : tid == 5 ? n_s1acc : tid == 6 ? n_s1try : tid == 7 ? n_s2acc
+                   : tid == 8 ? n_s2try : n_bond;
+        if (v) atomicAdd(&stats[rep * STAT_STRIDE + tid], (unsigned long long)v);
     } else if (tid == 10) {
         atomicAdd(&de_sum[rep * STAT_STRIDE], dsum);
     }
@@ -4479,8 +4606,9 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         mlpc.tnet = ctx->mlp_type >= 0 ? ctx->ref2dense[ctx->mlp_type] : -1;
         // LDS: stage + near/expensive lists + segment distances + moving particle
         // + stencil scratch (+ the network and its activation tile)
-        // (+ 80 B: chain partners, block compaction counts and block sums)
-        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + 80 + (da ? MLP_LDS_BYTES + 16 : 0) +
+        // (+ chain partners, block compaction counts, block sums, trial records)
+        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + SWEEP_SMALL_LDS +
+                       (da ? MLP_LDS_BYTES + 16 : 0) +
                        sweep_table_bytes(ctx->nt);
         size_t per = stage_bytes(1) + sizeof(int) * 3 + (da ? sizeof(float) * 2 : 0);
         int capmax = (int)((SWEEP_LDS_BUDGET - fixed) / per);
@@ -4758,7 +4886,8 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             double b[3] = {ctx->rep_box[q][0], ctx->rep_box[q][1], ctx->rep_box[q][2]};
             cap = std::max(cap, stage_cap_n(ctx->rep_n[q], ctx->minvol, ctx->rep_grid[q].g, b));
         }
-        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + 80 + sweep_table_bytes(ctx->nt);
+        size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + SWEEP_SMALL_LDS +
+                       sweep_table_bytes(ctx->nt);
         size_t per = stage_bytes(1) + sizeof(int) * 3;
         int capmax = (int)((SWEEP_LDS_BUDGET - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
