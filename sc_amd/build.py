@@ -16,8 +16,12 @@ SOURCES = [os.path.join(HERE, "csrc", "sc_gpu.hip"), os.path.join(HERE, "csrc", 
 HEADERS = [os.path.join(HERE, "csrc", h) for h in ("sc_pair.h", "sc_rng.h", "sc_host.h", "sc_mlp.h")] + [
     os.path.join(ROOT, "include", "sc_gpu.h")]
 
+# FP32 division and square root as the hardware's reciprocal and square root
+# (about 1 ulp) instead of the correctly rounded sequences: the pair terms are
+# FP32 anyway, and the sweep's chain of trials waits on ~35 of them per patch
+# pair.
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-               "-Wno-unused-result", "-Wno-unused-value",
+               "-Wno-unused-result", "-Wno-unused-value", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 

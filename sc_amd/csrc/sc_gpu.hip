@@ -120,8 +120,41 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* red, int&
         sb = y;
     }
 }
-// block_compact with one barrier, alternating halves of cnt as block_sum2
-// does.  cnt: 2 NW ints.
+// Two stable compaction steps at once (items of the first set before those
+// of the second), one barrier, alternating halves of cnt.  cnt: 4 NW ints.
+template <int NW>
+__device__ __forceinline__ void block_compact2(bool f0, bool f1, int& total, int* cnt, int& par, int& p0, int& p1) {
+    unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
+    if constexpr (NW == 1) {
+        p0 = total + wave_prefix(m0);
+        total += __popcll(m0);
+        p1 = total + wave_prefix(m1);
+        total += __popcll(m1);
+    } else {
+        int* c = cnt + 2 * NW * par;
+        par ^= 1;
+        int w = threadIdx.x >> 6;
+        if (lane_id() == 0) {
+            c[w] = __popcll(m0);
+            c[NW + w] = __popcll(m1);
+        }
+        __syncthreads();
+        int b0 = total, b1 = 0, a0 = 0, a1 = 0;
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            int c0 = c[k], c1 = c[NW + k];
+            b0 += k < w ? c0 : 0;
+            b1 += k < w ? c1 : 0;
+            a0 += c0;
+            a1 += c1;
+        }
+        p0 = b0 + wave_prefix(m0);
+        p1 = total + a0 + b1 + wave_prefix(m1);
+        total += a0 + a1;
+    }
+}
+// block_compact with one barrier, alternating halves of cnt (2 NW ints
+// each, as block_compact2, with which it shares cnt and par).  cnt: 4 NW ints.
 template <int NW>
 __device__ __forceinline__ int block_compact1(bool flag, int& total, int* cnt, int& par) {
     unsigned long long m = __ballot(flag);
@@ -130,7 +163,7 @@ __device__ __forceinline__ int block_compact1(bool flag, int& total, int* cnt, i
         total += __popcll(m);
         return pos;
     } else {
-        int* c = cnt + NW * par;
+        int* c = cnt + 2 * NW * par;
         par ^= 1;
         int w = threadIdx.x >> 6;
         if (lane_id() == 0) c[w] = __popcll(m);
@@ -889,7 +922,7 @@ struct TrialScratch {
     float* xd;   // segment distance of each expensive entry, 2*cap
     float* mv;   // 2 * MV_STRIDE floats
     int* bp;     // staged index of each chain partner (4 ints)
-    int* cnt;    // block compaction counts (8 ints)
+    int* cnt;    // block compaction counts (16 ints)
     double* red; // block sums (16 doubles)
 };
 
@@ -911,7 +944,7 @@ static_assert(sizeof(TrialRec) == 64, "trial record size");
 #define TRIAL_CHUNK 64
 // LDS after the trial scratch: chain partners, compaction counts, block sums
 // (8-aligned), trial records (16-aligned)
-#define SWEEP_SMALL_LDS (16 + 32 + 8 + 128 + 8 + TRIAL_CHUNK * 64)
+#define SWEEP_SMALL_LDS (16 + 64 + 8 + 128 + 8 + TRIAL_CHUNK * 64)
 
 // sweep counters: STAT_REP replicas of 8 counters, STAT_STRIDE apart (128 B)
 #define STAT_REP 64
@@ -962,26 +995,35 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     }
     if (bmask && tid < 4) T.bp[tid] = -1;
     if (bmask) __syncthreads();
-    // phase A
+    // phase A, two staged particles per thread and round (one barrier per
+    // round; the near list keeps stencil order)
     int nn = 0;
-    for (int base = 0; base < S.n; base += BS) {
-        int j = base + tid, fl = 0;
-        if (j < S.n && j != k) {
-            uint4 b = S.u[j];
-            V3<float> ro = fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-            bool io = dot(ro, ro) <= sc.sqmaxcut, in_ = io;
-            if (moved) {
-                V3<float> rn = fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-                in_ = dot(rn, rn) <= sc.sqmaxcut;
+    for (int base = 0; base < S.n; base += 2 * BS) {
+        int fl[2], jv[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            int j = base + h * BS + tid, f = 0;
+            if (j < S.n && j != k) {
+                uint4 b = S.u[j];
+                V3<float> ro = fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+                bool io = dot(ro, ro) <= sc.sqmaxcut, in_ = io;
+                if (moved) {
+                    V3<float> rn = fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+                    in_ = dot(rn, rn) <= sc.sqmaxcut;
+                }
+                f = (io ? 1 : 0) | (in_ ? 2 : 0);
+                if (bmask) {
+                    int bit = cw_bit(S.id[j] - idk);
+                    if (bit >= 0 && ((bmask >> bit) & 1)) T.bp[bit] = j;
+                }
             }
-            fl = (io ? 1 : 0) | (in_ ? 2 : 0);
-            if (bmask) {
-                int bit = cw_bit(S.id[j] - idk);
-                if (bit >= 0 && ((bmask >> bit) & 1)) T.bp[bit] = j;
-            }
+            fl[h] = f;
+            jv[h] = j;
         }
-        int pos = block_compact1<NW>(fl != 0, nn, T.cnt, cpar);
-        if (fl) T.nl[pos] = j | (fl << 28);
+        int p0, p1;
+        block_compact2<NW>(fl[0] != 0, fl[1] != 0, nn, T.cnt, cpar, p0, p1);
+        if (fl[0]) T.nl[p0] = jv[0] | (fl[0] << 28);
+        if (fl[1]) T.nl[p1] = jv[1] | (fl[1] << 28);
     }
     __syncthreads();
     double ao = 0.0, an = 0.0;
@@ -1095,33 +1137,46 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         des = rep + ds;
         stage = 2;
     }
-    // phase C
-    int items = nx * ppmax;
+    // phase C: (expensive pair, patch pair, side) items.  The two wedge
+    // intersections of a patch pair run on neighbouring lanes (side = lane
+    // parity) and meet through a lane swap; the even lane adds the attraction.
+    int items = nx * ppmax * 2;
     for (int base = 0; base < items; base += BS) {
-        int it = base + tid;
+        int it = base + tid, which = 0, k1 = 0, k2 = 0, side = it & 1;
+        bool hit = false, pair_item = false;
+        float x1 = 0.0f, x2 = 0.0f, ev = 0.0f;
+        const PP32* P2 = pp;
+        V3<float> r = mk<float>(0.0f, 0.0f, 0.0f);
+        PatchView<float> va, vb;
         if (it < items) {
-            int e = it / ppmax, mpp = it - e * ppmax;
+            int e = it / (2 * ppmax), mpp = (it - e * 2 * ppmax) >> 1;
             int w = T.xl[e];
-            int jj = w & 0x3fffffff, which = (w >> 30) & 1;
+            int jj = w & 0x3fffffff;
+            which = (w >> 30) & 1;
             uint4 b = S.u[jj];
-            const PP32& P2 = pp[t * sc.nt + b.w];
-            V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
-                                : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+            P2 = pp + t * sc.nt + b.w;
+            r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
+                      : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
             const float* mv = T.mv + which * MV_STRIDE;
-            float ev = 0.0f;
-            if (P2.kind <= K_SCA) {
-                if (mpp < npatch_pairs(P2)) {
-                    int k1, k2;
-                    patch_pair_index(P2, mpp, k1, k2);
+            if (P2->kind <= K_SCA) {
+                if (mpp < npatch_pairs(*P2)) {
+                    patch_pair_index(*P2, mpp, k1, k2);
                     DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
-                    ev = patch_pair_v<float>(P2, mv_view(mv, k1), view(pb, k2), r, k1, k2);
+                    va = mv_view(mv, k1);
+                    vb = view(pb, k2);
+                    hit = patch_side<float>(*P2, va, vb, r, k1, k2, side, x1, x2);
+                    pair_item = true;
                 }
-            } else if (mpp == 0) {
+            } else if (mpp == 0 && side == 0) {
                 DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
-                ev = spsc(P2, mv_part(mv, t, tp[t].geotype), pb, r);
+                ev = spsc(*P2, mv_part(mv, t, tp[t].geotype), pb, r);
             }
-            if (which) an += (double)ev; else ao += (double)ev;
         }
+        // the other side's result (lane ^ 1 holds item it ^ 1: BS is even)
+        float y1 = __shfl_xor(x1, 1), y2 = __shfl_xor(x2, 1);
+        int oh = __shfl_xor(hit ? 1 : 0, 1);
+        if (pair_item && side == 0 && hit && oh) ev = patch_atr<float>(*P2, va, vb, r, k1, k2, x1, x2, y1, y2);
+        if (which) an += (double)ev; else ao += (double)ev;
     }
     block_sum2<NW>(ao, an, T.red, rpar, eo, en);
     return true;
@@ -1157,7 +1212,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     int* scratch = (int*)(T.mv + 2 * MV_STRIDE);
     T.bp = scratch + 2 * 125 + 2;
     T.cnt = T.bp + 4;
-    T.red = (double*)(((uintptr_t)(T.cnt + 8) + 7) & ~(uintptr_t)7);
+    T.red = (double*)(((uintptr_t)(T.cnt + 16) + 7) & ~(uintptr_t)7);
     TrialRec* recs = (TrialRec*)(((uintptr_t)(T.red + 16) + 15) & ~(uintptr_t)15);
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1);
     float* ml = nullptr;

@@ -363,23 +363,38 @@ SC_HD PatchView<R> view(const DPart<R>& p, int k) {
 // (paire.h:469-484, :658-672, :864-889) and EPatch::atrE (paire.cpp:244-301).
 // k1/k2 select the patch of each particle; they only pick parameter indices,
 // so one code path serves all four patch pairs (no divergence between lanes
-// evaluating different patch pairs).
+// evaluating different patch pairs).  The pair term is two independent
+// wedge intersections (patch_side 0: particle 2's axis against particle 1's
+// patch region, giving T1, T2; side 1: the reverse, S1, S2), zero unless both
+// hit, then the attraction of the overlap (patch_atr).
 template <typename R>
-SC_HD R patch_pair_v(const PairParamsT<R>& P, const PatchView<R>& A, const PatchView<R>& Bv, const V3<R>& r, int k1,
-                     int k2) {
+SC_HD bool patch_side(const PairParamsT<R>& P, const PatchView<R>& A, const PatchView<R>& Bv, const V3<R>& r, int k1,
+                      int k2, int side, R& x1, R& x2) {
     bool c1 = (P.kind == K_CPSC) || (P.kind == K_MIXED && !is_psc(P.g0));
     bool c2 = (P.kind == K_CPSC) || (P.kind == K_MIXED && is_psc(P.g0));
+    R sw1 = k1 ? P.pcanglsw[2] : P.pcanglsw[0];
+    R sw2 = k2 ? P.pcanglsw[3] : P.pcanglsw[1];
+    // one intersect call for either side (lanes of both sides share its path)
+    bool s1 = side != 0;
+    IQ<R> q;
+    q.rcSq = P.rcutSq;
+    q.p1 = s1 ? Bv.ax : A.ax;
+    q.p2 = s1 ? A.ax : Bv.ax;
+    q.pdir = s1 ? Bv.pd : A.pd;
+    q.s0 = s1 ? Bv.s0 : A.s0;
+    q.s1 = s1 ? Bv.s1 : A.s1;
+    q.r = s1 ? mk<R>(r.x * R(-1), r.y * R(-1), r.z * R(-1)) : r;
+    q.cossw = s1 ? sw2 : sw1;
+    q.h1 = s1 ? P.hl1 : P.hl0;
+    q.h2 = s1 ? P.hl0 : P.hl1;
+    return intersect(q, s1 ? c2 : c1, x1, x2);
+}
+
+template <typename R>
+SC_HD R patch_atr(const PairParamsT<R>& P, const PatchView<R>& A, const PatchView<R>& Bv, const V3<R>& r, int k1,
+                  int k2, R T1, R T2, R S1, R S2) {
     R sw1 = k1 ? P.pcanglsw[2] : P.pcanglsw[0], pc1 = k1 ? P.pcangl[2] : P.pcangl[0];
     R sw2 = k2 ? P.pcanglsw[3] : P.pcanglsw[1], pc2 = k2 ? P.pcangl[3] : P.pcangl[1];
-    IQ<R> q;
-    q.p1 = A.ax; q.p2 = Bv.ax; q.pdir = A.pd; q.s0 = A.s0; q.s1 = A.s1;
-    q.r = r; q.cossw = sw1; q.rcSq = P.rcutSq; q.h1 = P.hl0; q.h2 = P.hl1;
-    R T1, T2, S1, S2;
-    if (!intersect(q, c1, T1, T2)) return R(0);
-    q.p1 = Bv.ax; q.p2 = A.ax; q.pdir = Bv.pd; q.s0 = Bv.s0; q.s1 = Bv.s1;
-    q.r = mk<R>(r.x * R(-1), r.y * R(-1), r.z * R(-1)); q.cossw = sw2; q.h1 = P.hl1; q.h2 = P.hl0;
-    if (!intersect(q, c2, S1, S2)) return R(0);
-    // atrE
     const V3<R>& d1 = A.ax;
     const V3<R>& d2 = Bv.ax;
     R v1 = fabs(S1 - S2), v2 = fabs(T1 - T2), f0 = R(0.5) * (v1 + v2);
@@ -397,6 +412,15 @@ SC_HD R patch_pair_v(const PairParamsT<R>& P, const PatchView<R>& A, const Patch
         if ((P.parallel > R(0) && ca > R(0)) || (P.parallel < R(0) && ca < R(0))) paral = R(1) + P.parallel * ca;
     }
     return e * (f0 * f1 * f2 * paral);
+}
+
+template <typename R>
+SC_HD R patch_pair_v(const PairParamsT<R>& P, const PatchView<R>& A, const PatchView<R>& Bv, const V3<R>& r, int k1,
+                     int k2) {
+    R T1, T2, S1, S2;
+    if (!patch_side(P, A, Bv, r, k1, k2, 0, T1, T2)) return R(0);
+    if (!patch_side(P, A, Bv, r, k1, k2, 1, S1, S2)) return R(0);
+    return patch_atr(P, A, Bv, r, k1, k2, T1, T2, S1, S2);
 }
 
 template <typename R, int k1, int k2>
