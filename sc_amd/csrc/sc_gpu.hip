@@ -120,41 +120,50 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* red, int&
         sb = y;
     }
 }
-// Two stable compaction steps at once (items of the first set before those
-// of the second), one barrier, alternating halves of cnt.  cnt: 4 NW ints.
-template <int NW>
-__device__ __forceinline__ void block_compact2(bool f0, bool f1, int& total, int* cnt, int& par, int& p0, int& p1) {
-    unsigned long long m0 = __ballot(f0), m1 = __ballot(f1);
+// K stable compaction steps at once (the items of set 0, then those of set 1,
+// ...), one barrier.  Like block_sum2, consecutive calls alternate between two
+// halves of cnt (CNT_HALF ints each, K NW <= CNT_HALF; block_compact1 shares
+// cnt and par).  cnt: 2 CNT_HALF ints.
+#define CNT_HALF 8
+template <int NW, int K>
+__device__ __forceinline__ void block_compactK(const bool* f, int& total, int* cnt, int& par, int* p) {
+    static_assert(NW == 1 || K * NW <= CNT_HALF, "compaction counts exceed their half");
+    unsigned long long m[K];
+#pragma unroll
+    for (int h = 0; h < K; h++) m[h] = __ballot(f[h]);
     if constexpr (NW == 1) {
-        p0 = total + wave_prefix(m0);
-        total += __popcll(m0);
-        p1 = total + wave_prefix(m1);
-        total += __popcll(m1);
+#pragma unroll
+        for (int h = 0; h < K; h++) {
+            p[h] = total + wave_prefix(m[h]);
+            total += __popcll(m[h]);
+        }
     } else {
-        int* c = cnt + 2 * NW * par;
+        int* c = cnt + CNT_HALF * par;
         par ^= 1;
         int w = threadIdx.x >> 6;
         if (lane_id() == 0) {
-            c[w] = __popcll(m0);
-            c[NW + w] = __popcll(m1);
+#pragma unroll
+            for (int h = 0; h < K; h++) c[h * NW + w] = __popcll(m[h]);
         }
         __syncthreads();
-        int b0 = total, b1 = 0, a0 = 0, a1 = 0;
+        int base = total;
 #pragma unroll
-        for (int k = 0; k < NW; k++) {
-            int c0 = c[k], c1 = c[NW + k];
-            b0 += k < w ? c0 : 0;
-            b1 += k < w ? c1 : 0;
-            a0 += c0;
-            a1 += c1;
+        for (int h = 0; h < K; h++) {
+            int before = 0, all = 0;
+#pragma unroll
+            for (int k = 0; k < NW; k++) {
+                int ck = c[h * NW + k];
+                before += k < w ? ck : 0;
+                all += ck;
+            }
+            p[h] = base + before + wave_prefix(m[h]);
+            base += all;
         }
-        p0 = b0 + wave_prefix(m0);
-        p1 = total + a0 + b1 + wave_prefix(m1);
-        total += a0 + a1;
+        total = base;
     }
 }
-// block_compact with one barrier, alternating halves of cnt (2 NW ints
-// each, as block_compact2, with which it shares cnt and par).  cnt: 4 NW ints.
+// block_compact with one barrier, alternating halves of cnt as
+// block_compactK, with which it shares cnt and par.
 template <int NW>
 __device__ __forceinline__ int block_compact1(bool flag, int& total, int* cnt, int& par) {
     unsigned long long m = __ballot(flag);
@@ -163,7 +172,7 @@ __device__ __forceinline__ int block_compact1(bool flag, int& total, int* cnt, i
         total += __popcll(m);
         return pos;
     } else {
-        int* c = cnt + 2 * NW * par;
+        int* c = cnt + CNT_HALF * par;
         par ^= 1;
         int w = threadIdx.x >> 6;
         if (lane_id() == 0) c[w] = __popcll(m);
@@ -995,13 +1004,14 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     }
     if (bmask && tid < 4) T.bp[tid] = -1;
     if (bmask) __syncthreads();
-    // phase A, two staged particles per thread and round (one barrier per
-    // round; the near list keeps stencil order)
+    // phase A, PA staged particles per thread and round (one barrier per
+    // round, 256-512 particles; the near list keeps stencil order)
+    constexpr int PA = NW >= 4 ? 2 : 4;
     int nn = 0;
-    for (int base = 0; base < S.n; base += 2 * BS) {
-        int fl[2], jv[2];
+    for (int base = 0; base < S.n; base += PA * BS) {
+        int fl[PA], jv[PA];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < PA; h++) {
             int j = base + h * BS + tid, f = 0;
             if (j < S.n && j != k) {
                 uint4 b = S.u[j];
@@ -1020,10 +1030,14 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             fl[h] = f;
             jv[h] = j;
         }
-        int p0, p1;
-        block_compact2<NW>(fl[0] != 0, fl[1] != 0, nn, T.cnt, cpar, p0, p1);
-        if (fl[0]) T.nl[p0] = jv[0] | (fl[0] << 28);
-        if (fl[1]) T.nl[p1] = jv[1] | (fl[1] << 28);
+        bool fb[PA];
+        int pv[PA];
+#pragma unroll
+        for (int h = 0; h < PA; h++) fb[h] = fl[h] != 0;
+        block_compactK<NW, PA>(fb, nn, T.cnt, cpar, pv);
+#pragma unroll
+        for (int h = 0; h < PA; h++)
+            if (fl[h]) T.nl[pv[h]] = jv[h] | (fl[h] << 28);
     }
     __syncthreads();
     double ao = 0.0, an = 0.0;
