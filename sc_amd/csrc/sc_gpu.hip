@@ -2388,6 +2388,8 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
             __syncthreads();
             continue;
         }
+        ctr.w = (cp.pass_hi << 2) | 2u;
+        U4 rc = philox(ctr, cp.key0, cp.key1);   // the acceptance uniform, drawn ahead of the energies
         double acc_o = 0.0, acc_n = 0.0;
         for (int it = w; it < 2 * m; it += NW) {
             int a = it >> 1;
@@ -2398,9 +2400,9 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
                                          queue);
             if (st) acc_n += e; else acc_o += e;
         }
-        double eo = block_sum<NW>(acc_o, red), en = block_sum<NW>(acc_n, red + NW);
-        ctr.w = (cp.pass_hi << 2) | 2u;
-        U4 rc = philox(ctr, cp.key0, cp.key1);
+        double eo, en;
+        int rpar = 0;   // one paired sum per trial, after the previous trial's barrier
+        block_sum2<NW>(acc_o, acc_n, red, rpar, eo, en);
         double de = en - eo;
         // moveTry (movecreator.h:175-187) on mol2others before and after
         bool acc = de <= 0.0 || exp(-de / cp.temper) > u01d(rc.x, rc.y);
@@ -2821,6 +2823,8 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass_st(Grid g, Grid gc, Sy
             __syncthreads();
             continue;
         }
+        ctr.w = (cp.pass_hi << 2) | 2u;
+        U4 rc = philox(ctr, cp.key0, cp.key1);   // the acceptance uniform, drawn ahead of the energies
         double acc_o = 0.0, acc_n = 0.0;
         for (int it = w; it < 2 * m; it += NW) {
             int a = it >> 1;
@@ -2830,9 +2834,9 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass_st(Grid g, Grid gc, Sy
                                             nown_s, sc, pp, tp, su, so, sq, sid, queue, spr);
             if (st) acc_n += e; else acc_o += e;
         }
-        double eo = block_sum<NW>(acc_o, red), en = block_sum<NW>(acc_n, red + NW);
-        ctr.w = (cp.pass_hi << 2) | 2u;
-        U4 rc = philox(ctr, cp.key0, cp.key1);
+        double eo, en;
+        int rpar = 0;   // one paired sum per trial, after the previous trial's barrier
+        block_sum2<NW>(acc_o, acc_n, red, rpar, eo, en);
         double de = en - eo;
         // moveTry (movecreator.h:175-187) on mol2others before and after
         bool acc = de <= 0.0 || exp(-de / cp.temper) > u01d(rc.x, rc.y);
