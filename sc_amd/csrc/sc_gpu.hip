@@ -953,7 +953,10 @@ static_assert(sizeof(TrialRec) == 64, "trial record size");
 #define TRIAL_CHUNK 64
 // LDS after the trial scratch: chain partners, compaction counts, block sums
 // (8-aligned), trial records (16-aligned)
-#define SWEEP_SMALL_LDS (16 + 64 + 8 + 128 + 8 + TRIAL_CHUNK * 64)
+// derived vectors of the home cell's particles in their current state (the
+// old state of every trial), MV_STRIDE floats each, for cells of up to HCAP
+#define HCAP 64
+#define SWEEP_SMALL_LDS (16 + 64 + 8 + 128 + 8 + TRIAL_CHUNK * 64 + HCAP * MV_STRIDE * 4)
 
 // sweep counters: STAT_REP replicas of 8 counters, STAT_STRIDE apart (128 B)
 #define STAT_REP 64
@@ -985,7 +988,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                                                const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
                                                const TrialScratch& T, const float* ml, const MlpRegs& mr,
                                                const MlpConst& mc, const ChainTab& ct, double thr1, double thr2,
-                                               double& eo, double& en, double& des, int& stage) {
+                                               const float* hvk, double& eo, double& en, double& des, int& stage) {
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
     int t = uo.w;
     bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
@@ -998,7 +1001,10 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     // parities of the one-barrier helpers; a trial starts after a barrier
     // that follows every read of the previous trial's helpers
     int cpar = 0, rpar = 0;
-    if (tid == 0) {
+    if (hvk) {   // the old state's derived vectors from the home cell's cache
+        if (tid < 27) T.mv[tid] = hvk[tid];
+        if (tid == 0) mv_store(T.mv + MV_STRIDE, pn);
+    } else if (tid == 0) {
         mv_store(T.mv, po);
         mv_store(T.mv + MV_STRIDE, pn);
     }
@@ -1064,7 +1070,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     }
     // phase B
     int nx = 0;
-    V3<float> dold = po.dir, dnew = pn.dir;
+    V3<float> dold = hvk ? mk<float>(hvk[0], hvk[1], hvk[2]) : po.dir, dnew = pn.dir;
     for (int base = 0; base < 2 * nn; base += BS) {
         int it = base + tid, jj = 0, which = 0;
         bool need = false;
@@ -1228,10 +1234,11 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     T.cnt = T.bp + 4;
     T.red = (double*)(((uintptr_t)(T.cnt + 16) + 7) & ~(uintptr_t)7);
     TrialRec* recs = (TrialRec*)(((uintptr_t)(T.red + 16) + 15) & ~(uintptr_t)15);
+    float* hv = (float*)(recs + TRIAL_CHUNK);
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1);
     float* ml = nullptr;
     MlpRegs mr;
-    uintptr_t tail = (uintptr_t)(recs + TRIAL_CHUNK);
+    uintptr_t tail = (uintptr_t)(hv + HCAP * MV_STRIDE);
     if constexpr (DA == DA_NET) {
         ml = (float*)((tail + 15) & ~(uintptr_t)15);
         mlp_setup(mlp_img, ml, lane, mr);
@@ -1272,6 +1279,11 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     }
     unsigned n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0, n_s2try = 0,
              n_s2acc = 0, n_bond = 0;   // per block: far below 2^32
+    // derived vectors of the home particles (read after the first trial's barrier)
+    const bool hcache = nh <= HCAP;
+    if (hcache)
+        for (int i = tid; i < nh; i += WAVE * NW)
+            mv_store(hv + i * MV_STRIDE, load_part<float>(S.o[S.h0 + i], S.q[S.h0 + i], S.u[S.h0 + i].w, tp));
     double dsum = 0.0;
     for (int t = 0; t < ntrial; t++) {
         if ((t & (TRIAL_CHUNK - 1)) == 0) {
@@ -1355,11 +1367,13 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         double eo, en, des;
         bool pass;
         int stage;
+        const float* hvk = hcache ? hv + (k - S.h0) * MV_STRIDE : nullptr;
         {
-            DPart<float> po = load_part<float>(oo, qo, uo.w, tp);
+            DPart<float> po;
+            if (!hcache) po = load_part<float>(oo, qo, uo.w, tp);
             DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
             pass = trial_energies<DA, NW>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, R.thr1, R.thr2,
-                                          eo, en, des, stage);
+                                          hvk, eo, en, des, stage);
         }
         if (stage < 0) {   // a chain partner outside the stencil
             n_bond++;
@@ -1388,6 +1402,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
                 S.o[k] = on;
                 S.q[k] = qn;
             }
+            if (hcache && tid < 27) hv[(k - S.h0) * MV_STRIDE + tid] = T.mv[MV_STRIDE + tid];
         }
         __syncthreads();
     }
