@@ -984,11 +984,13 @@ struct MlpConst {
 // not depend on the move (the particle stays in its cell).
 template <int DA, int NW>
 __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un,
-                                               const DPart<float>& po, const DPart<float>& pn, const SysConst& sc,
+                                               const float4 oo, const float2 qo, const float4 on, const float2 qn,
+                                               const SysConst& sc,
                                                const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
                                                const TrialScratch& T, const float* ml, const MlpRegs& mr,
                                                const MlpConst& mc, const ChainTab& ct, double thr1, double thr2,
-                                               const float* hvk, double& eo, double& en, double& des, int& stage) {
+                                               const float* hvk, bool same_or, double& eo, double& en, double& des,
+                                               int& stage) {
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
     int t = uo.w;
     bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
@@ -1001,12 +1003,30 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     // parities of the one-barrier helpers; a trial starts after a barrier
     // that follows every read of the previous trial's helpers
     int cpar = 0, rpar = 0;
-    if (hvk) {   // the old state's derived vectors from the home cell's cache
-        if (tid < 27) T.mv[tid] = hvk[tid];
-        if (tid == 0) mv_store(T.mv + MV_STRIDE, pn);
-    } else if (tid == 0) {
-        mv_store(T.mv, po);
-        mv_store(T.mv + MV_STRIDE, pn);
+    // derived vectors of both states into T.mv: the old state's from the home
+    // cell's cache (a translation keeps them for the new state too)
+    V3<float> dold, dnew;
+    if (hvk) {
+        if (tid < 27) {
+            T.mv[tid] = hvk[tid];
+            if (same_or) T.mv[MV_STRIDE + tid] = hvk[tid];
+        }
+        dold = mk<float>(hvk[0], hvk[1], hvk[2]);
+        if (same_or) {
+            dnew = dold;
+        } else {
+            DPart<float> pn = load_part<float>(on, qn, t, tp);
+            if (tid == 0) mv_store(T.mv + MV_STRIDE, pn);
+            dnew = pn.dir;
+        }
+    } else {
+        DPart<float> po = load_part<float>(oo, qo, t, tp), pn = load_part<float>(on, qn, t, tp);
+        if (tid == 0) {
+            mv_store(T.mv, po);
+            mv_store(T.mv + MV_STRIDE, pn);
+        }
+        dold = po.dir;
+        dnew = pn.dir;
     }
     if (bmask && tid < 4) T.bp[tid] = -1;
     if (bmask) __syncthreads();
@@ -1070,7 +1090,6 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     }
     // phase B
     int nx = 0;
-    V3<float> dold = hvk ? mk<float>(hvk[0], hvk[1], hvk[2]) : po.dir, dnew = pn.dir;
     for (int base = 0; base < 2 * nn; base += BS) {
         int it = base + tid, jj = 0, which = 0;
         bool need = false;
@@ -1368,13 +1387,9 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         bool pass;
         int stage;
         const float* hvk = hcache ? hv + (k - S.h0) * MV_STRIDE : nullptr;
-        {
-            DPart<float> po;
-            if (!hcache) po = load_part<float>(oo, qo, uo.w, tp);
-            DPart<float> pn = load_part<float>(on, qn, uo.w, tp);
-            pass = trial_energies<DA, NW>(S, k, uo, un, po, pn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, R.thr1, R.thr2,
-                                          hvk, eo, en, des, stage);
-        }
+        const bool same_or = hcache && translate;   // new state's derived vectors = the cached old ones
+        pass = trial_energies<DA, NW>(S, k, uo, un, oo, qo, on, qn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, R.thr1,
+                                      R.thr2, hvk, same_or, eo, en, des, stage);
         if (stage < 0) {   // a chain partner outside the stencil
             n_bond++;
             __syncthreads();
