@@ -108,6 +108,8 @@ def lib():
         L.orc_soa_mol2others.argtypes = [T, D, ctypes.c_int, D, D, D, ctypes.c_void_p, I, ctypes.c_int, I, I, D]
         L.orc_nematic_s2.argtypes = [ctypes.c_int, D]
         L.orc_nematic_s2.restype = ctypes.c_double
+        L.orc_set_sampler_fault.argtypes = [ctypes.c_int]
+        L.orc_set_sampler_fault.restype = None
         _lib = L
     return _lib
 
@@ -360,6 +362,13 @@ def mc_run(topo, st, nsweeps, temper, transmx, rot_half, seed, energies=True):
                              _ip(s.types), int(nsweeps), float(temper), float(transmx), float(rot_half),
                              int(seed) & 0xFFFFFFFFFFFFFFFF, stats, ctypes.byref(ef) if energies else None)
     return s, drift, list(stats), ef.value
+
+
+def set_sampler_fault(kind):
+    """Plant a mistake in the sequential sampler (tests of the sampler's own
+    checks only): 0 none, 1 Metropolis test at 2T, 2 rotation leaving the
+    patch direction unrotated, 3 translations along +dir only."""
+    lib().orc_set_sampler_fault(int(kind))
 
 
 def mc_run_npt(topo, st, nsweeps, temper, transmx, rot_half, seed, ptype, press, edge_mx):

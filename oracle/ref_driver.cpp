@@ -18,9 +18,15 @@
 //   scref molecules <out>           -> mol2others / mol2othersTrial of every
 //                                      molecule of more than one particle
 //                                      (totalenergycalculator.h:417-500)
+//   scref run <out>                 -> the reference's whole program (its own
+//                                      main(), scOOP/main.cpp:33-330: options,
+//                                      sequential MC sweeps, energy.dat every
+//                                      `report` sweeps, config.last); <out>
+//                                      gets the wall-clock seconds of the run
 // Values are printed with %.17g so the fixtures carry full FP64 precision.
 #include <cstdio>
 #include <cstdlib>
+#include <ctime>
 #include <vector>
 
 #include "mc/inicializer.h"
@@ -28,6 +34,7 @@
 #include "mc/totalenergycalculator.h"
 
 extern Topo topo;
+int scoop_reference_main(int argc, char** argv);   // scOOP/main.cpp, renamed at compile time
 
 static void dumpVec(FILE* f, const Vector& v) { fprintf(f, " %.17g %.17g %.17g", v.x, v.y, v.z); }
 
@@ -43,6 +50,17 @@ int main(int argc, char** argv) {
     std::string mode(argv[1]);
     FILE* out = fopen(argv[2], "w");
     if (!out) { perror("out"); return 2; }
+
+    if (mode == "run") {
+        char* args[] = {argv[0], nullptr};
+        timespec t0, t1;
+        clock_gettime(CLOCK_MONOTONIC, &t0);
+        int rc = scoop_reference_main(1, args);
+        clock_gettime(CLOCK_MONOTONIC, &t1);
+        fprintf(out, "%.6f\n", (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec));
+        fclose(out);
+        return rc;
+    }
 
     if (mode == "mindist") {
         // each line: ax ay az bx by bz h1 h2 rx ry rz

@@ -1311,6 +1311,15 @@ static double mc_run(const orc_topo* t, double box[3], int n, double* pos, doubl
                      const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
                      long* stats, double* e_final, int ptype, double press, double edge_mx, const ochainmv* cm);
 
+/* Planted mistakes for the sampler's own tests (0 = none, the default): they
+ * show that the two-particle Boltzmann check fails for plausible errors.
+ *   1: the Metropolis test at twice the temperature;
+ *   2: rotations about the particle's own axis dir x patchdir[0], always in
+ *      the same sense (turning dir towards the patch), so no move has a reverse;
+ *   3: translations always along the particle's own +dir axis (no reverse move). */
+static int orc_fault = 0;
+void orc_set_sampler_fault(int f) { orc_fault = f; }
+
 double orc_mc_run(const orc_topo* t, const double box[3], int n, double* pos, double* dir, double* pd0, double* pd1,
                   const int* type, long nsweeps, double temper, double transmx, double rot_half, uint64_t seed,
                   long* stats, double* e_final) {
@@ -1520,6 +1529,10 @@ static double mc_run(const orc_topo* t, double box[3], int n, double* pos, doubl
             if (translate) {
                 /* partDisplace, movecreator.cpp:956-964: |dr| = transmx exactly */
                 orc_vec dr = rand_sphere(&rs);
+                if (orc_fault == 3) {
+                    double dn = sqrt(DOT(p[target].dir, p[target].dir));
+                    dr.x = p[target].dir.x / dn; dr.y = p[target].dir.y / dn; dr.z = p[target].dir.z / dn;
+                }
                 p[target].pos.x += dr.x * transmx / box[0];
                 p[target].pos.y += dr.y * transmx / box[1];
                 p[target].pos.z += dr.z * transmx / box[2];
@@ -1532,6 +1545,13 @@ static double mc_run(const orc_topo* t, double box[3], int n, double* pos, doubl
                 orc_vec ax = rand_sphere(&rs);
                 vc = cos(ang);
                 vs = (urand(&rs) < 0.5) ? sqrt(1.0 - vc * vc) : -sqrt(1.0 - vc * vc);
+                if (orc_fault == 2) {
+                    orc_vec d0 = p[target].dir, q0 = p[target].patchdir[0];
+                    orc_vec nx = {d0.y * q0.z - d0.z * q0.y, d0.z * q0.x - d0.x * q0.z, d0.x * q0.y - d0.y * q0.x};
+                    double nn = sqrt(DOT(nx, nx));
+                    ax.x = nx.x / nn; ax.y = nx.y / nn; ax.z = nx.z / nn;
+                    vs = fabs(vs);
+                }
                 p[target].dir = ROT(p[target].dir, ax, vc, vs);
                 p[target].patchdir[0] = ROT(p[target].patchdir[0], ax, vc, vs);
                 p[target].patchdir[0] = ORTHO(p[target].patchdir[0], p[target].dir);  /* partRotate:1007 */
@@ -1540,7 +1560,7 @@ static double mc_run(const orc_topo* t, double box[3], int n, double* pos, doubl
             en = grid_one(t, box, &g, p, &p[target], target);
             /* moveTry, movecreator.h:175-187 */
             reject = 0;
-            if (!(en <= eo)) reject = !(exp(-1.0 * (en - eo) / temper) > urand(&rs));
+            if (!(en <= eo)) reject = !(exp(-1.0 * (en - eo) / (orc_fault == 1 ? 2.0 * temper : temper)) > urand(&rs));
             stats[translate ? 1 : 3]++;
             if (reject) {
                 p[target] = old;

@@ -106,3 +106,19 @@ def test_cpp_dropin_adapter_compiles_and_links(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0
     assert "context" in r.stdout
+
+
+def test_philox_known_answers(tmp_path):
+    """The device generator (sc_rng.h, built for the host) reproduces the
+    published Philox4x32-10 known-answer vectors of Random123 and its uniform
+    conversions map the ends of their input ranges into [0, 1)."""
+    import subprocess
+    exe = tmp_path / "philox_kat"
+    src = os.path.join(ROOT, "tests", "native", "philox_kat.cpp")
+    r = subprocess.run(["g++", "-O2", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", src, "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
+    assert r.stdout.count("ok ") == 5
