@@ -158,6 +158,8 @@ def main():
                     help="waves per active cell in the sweep kernels (1, 2, 4; 0: 4, and 2 with the network screen)")
     ap.add_argument("--trial-mode", type=int, default=1,
                     help="trials per active cell: 0 = its occupancy, 1 = dithered mean occupancy")
+    ap.add_argument("--visit", type=int, default=1,
+                    help="visit factor: a sweep visits 1/v of the colours with v times the trials per cell")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2025)
@@ -211,6 +213,7 @@ def main():
         log("copy probe: 3 x k_copy of 256 MiB")
     g.set_trial_mode(a.trial_mode)
     g.set_block_waves(a.waves)
+    g.set_visit_factor(a.visit)
     if a.equil:
         t0 = time.time()
         g.sweep(a.equil, temper, transmx, rot_half, r=a.r, seed=a.seed + 1, counter=5 * 10 ** 6)

@@ -195,6 +195,24 @@ int sc_set_trial_mode(sc_ctx* ctx, int mode);
  * deterministic for each setting; they differ between settings only by the
  * order of FP64 sums. */
 int sc_set_block_waves(sc_ctx* ctx, int nw);
+/* The sweep keeps the derived vectors (patch sides, second patch, chiral
+ * axes: Particle::init, particle.cpp:3-79) of a pass's home-cell particles in
+ * LDS, computed once per pass and refreshed on acceptance, for cells of up to
+ * `cap` particles (default and maximum 64); larger cells, or cap = 0, derive
+ * them at every trial.  Both give the same trajectories; the setting exists
+ * so that tests can compare them. */
+int sc_set_home_cache(sc_ctx* ctx, int cap);
+/* Colour passes per sweep of sc_sweep: a sweep visits the first 1/v of its
+ * randomly ordered colours (v must divide their number: 8 for 2x2x2), and
+ * an active cell makes v times the trials of sc_set_trial_mode per visit,
+ * so a sweep still averages N trial moves (the step loop of
+ * Updater::simulate, updater.cpp:206-230, picks its particles at random; a
+ * sweep need not touch every particle).  Longer chains per pass average the
+ * cells' uneven trial costs, so the slowest cell of a pass lags the mean
+ * less, and a sweep launches fewer passes.  Default 1; replica batches
+ * always use 1.  Every pass is a product of independent Metropolis chains,
+ * whatever v. */
+int sc_set_visit_factor(sc_ctx* ctx, int v);
 /* Whole-molecule moves in sc_sweep (MoveCreator::chainMove, movecreator.cpp:
  * 305-328): besides its N particle trials, every sweep makes chainprob * N
  * molecule trials (chainprob is the per-step probability of Updater's step

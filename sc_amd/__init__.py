@@ -51,6 +51,8 @@ SIGNATURES = {
     "sc_load_config": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "sc_write_config": (ctypes.c_long, [_P, ctypes.c_char_p, ctypes.c_long]),
     "sc_set_block_waves": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_set_home_cache": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_set_visit_factor": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_write_movie_frame": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
     "sc_write_stat_line": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
     "sc_write_energy_line": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
@@ -567,6 +569,14 @@ class GpuSystem:
     def set_block_waves(self, nw):
         """waves per active cell in the sweep kernels (1, 2, 4; 0 = by surrogate mode)"""
         self._ck(self._L.sc_set_block_waves(self._h, int(nw)), "set_block_waves")
+
+    def set_home_cache(self, cap):
+        """cells of up to `cap` particles (<= 64) cache their derived vectors in the sweep; 0 disables it"""
+        self._ck(self._L.sc_set_home_cache(self._h, int(cap)), "set_home_cache")
+
+    def set_visit_factor(self, v):
+        """a sweep visits 1/v of the colours, each with v times the trials (default 1)"""
+        self._ck(self._L.sc_set_visit_factor(self._h, int(v)), "set_visit_factor")
 
     def set_trial_mode(self, mode):
         self._ck(self._L.sc_set_trial_mode(self._h, int(mode)), "set_trial_mode")

@@ -124,7 +124,7 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* red, int&
 // ...), one barrier.  Like block_sum2, consecutive calls alternate between two
 // halves of cnt (CNT_HALF ints each, K NW <= CNT_HALF; block_compact1 shares
 // cnt and par).  cnt: 2 CNT_HALF ints.
-#define CNT_HALF 8
+#define CNT_HALF 16
 template <int NW, int K>
 __device__ __forceinline__ void block_compactK(const bool* f, int& total, int* cnt, int& par, int* p) {
     static_assert(NW == 1 || K * NW <= CNT_HALF, "compaction counts exceed their half");
@@ -877,6 +877,8 @@ struct MoveParams {
     int nact[3];                 // active cells per axis
     int trial_mode;              // 0: trials = cell occupancy; 1: dithered mean occupancy
     int n;                       // particles
+    int hcap;                    // cells of up to hcap (<= HCAP) particles cache their derived vectors
+    int visit;                   // trials per visit of a cell: visit x its occupancy (mode 0) or mean occupancy (1)
 };
 
 __device__ __forceinline__ V3<float> unit_from(float a, float b) {
@@ -890,14 +892,23 @@ __device__ __forceinline__ V3<float> unit_from(float a, float b) {
 // state, DPart order: dir, pd[2], sd[4], ax[2]).
 #define MV_STRIDE 28
 
+// All 27 floats from one thread: members chosen by lane (a branch or a
+// select per lane) become an indexed stack copy of the particle.
+__device__ __forceinline__ void mv_put(float* mv, int off, const V3<float>& v) {
+    mv[off] = v.x;
+    mv[off + 1] = v.y;
+    mv[off + 2] = v.z;
+}
 __device__ __forceinline__ void mv_store(float* mv, const DPart<float>& p) {
-    const V3<float>* v[9] = {&p.dir, &p.pd[0], &p.pd[1], &p.sd[0], &p.sd[1], &p.sd[2], &p.sd[3], &p.ax[0], &p.ax[1]};
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-        mv[3 * k] = v[k]->x;
-        mv[3 * k + 1] = v[k]->y;
-        mv[3 * k + 2] = v[k]->z;
-    }
+    mv_put(mv, 0, p.dir);
+    mv_put(mv, 3, p.pd[0]);
+    mv_put(mv, 6, p.pd[1]);
+    mv_put(mv, 9, p.sd[0]);
+    mv_put(mv, 12, p.sd[1]);
+    mv_put(mv, 15, p.sd[2]);
+    mv_put(mv, 18, p.sd[3]);
+    mv_put(mv, 21, p.ax[0]);
+    mv_put(mv, 24, p.ax[1]);
 }
 __device__ __forceinline__ V3<float> mv_vec(const float* mv, int off) { return mk<float>(mv[off], mv[off + 1], mv[off + 2]); }
 __device__ __forceinline__ PatchView<float> mv_view(const float* mv, int k) {
@@ -931,8 +942,9 @@ struct TrialScratch {
     float* xd;   // segment distance of each expensive entry, 2*cap
     float* mv;   // 2 * MV_STRIDE floats
     int* bp;     // staged index of each chain partner (4 ints)
-    int* cnt;    // block compaction counts (16 ints)
+    int* cnt;    // block compaction counts (2 CNT_HALF ints)
     double* red; // block sums (16 doubles)
+    bf16x4* hx;  // the network's activation exchange between the waves (MLP_HX_BYTES)
 };
 
 // A trial's proposal and its Metropolis thresholds, drawn for TRIAL_CHUNK
@@ -956,7 +968,7 @@ static_assert(sizeof(TrialRec) == 64, "trial record size");
 // derived vectors of the home cell's particles in their current state (the
 // old state of every trial), MV_STRIDE floats each, for cells of up to HCAP
 #define HCAP 64
-#define SWEEP_SMALL_LDS (16 + 64 + 8 + 128 + 8 + TRIAL_CHUNK * 64 + HCAP * MV_STRIDE * 4)
+#define SWEEP_SMALL_LDS (16 + 4 * 2 * CNT_HALF + 8 + 128 + 8 + TRIAL_CHUNK * 64 + HCAP * MV_STRIDE * 4)
 
 // sweep counters: STAT_REP replicas of 8 counters, STAT_STRIDE apart (128 B)
 #define STAT_REP 64
@@ -972,74 +984,23 @@ struct MlpConst {
     int tnet;   // dense type of the network's pair (the screen's other pairs count 0)
 };
 
-// Delayed acceptance (Christen & Fox 2005, in stages): with DA != DA_OFF
-// the move first faces Metropolis screens on cheaper energy changes, each
-// accepting with min(1, exp(-(its change)/T)); returns false at the first
-// screen that rejects (stage = the number of screens passed).  Otherwise eo/en
-// are the exact energies of the two states and des the sum of the screened
-// changes, which the final stage divides out: min(1, exp(-(dE - des)/T)).
-// Chain bonds are exact cheap terms (FP64, in the first screen); a move whose
-// bonded partner lies outside the staged stencil is rejected (stage -1): the
-// partner might be moving in another cell of the pass, and the rejection does
-// not depend on the move (the particle stays in its cell).
-template <int DA, int NW>
-__device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un,
-                                               const float4 oo, const float2 qo, const float4 on, const float2 qn,
-                                               const SysConst& sc,
-                                               const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
-                                               const TrialScratch& T, const float* ml, const MlpRegs& mr,
-                                               const MlpConst& mc, const ChainTab& ct, double thr1, double thr2,
-                                               const float* hvk, bool same_or, double& eo, double& en, double& des,
-                                               int& stage) {
-    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
-    int t = uo.w;
-    bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
-    int idk = 0, cwk = 0;
-    if (ct.cw) {
-        idk = S.id[k];
-        cwk = ct.cw[idk];
-    }
-    int bmask = cwk & 15;
-    // parities of the one-barrier helpers; a trial starts after a barrier
-    // that follows every read of the previous trial's helpers
-    int cpar = 0, rpar = 0;
-    // derived vectors of both states into T.mv: the old state's from the home
-    // cell's cache (a translation keeps them for the new state too)
-    V3<float> dold, dnew;
-    if (hvk) {
-        if (tid < 27) {
-            T.mv[tid] = hvk[tid];
-            if (same_or) T.mv[MV_STRIDE + tid] = hvk[tid];
-        }
-        dold = mk<float>(hvk[0], hvk[1], hvk[2]);
-        if (same_or) {
-            dnew = dold;
-        } else {
-            DPart<float> pn = load_part<float>(on, qn, t, tp);
-            if (tid == 0) mv_store(T.mv + MV_STRIDE, pn);
-            dnew = pn.dir;
-        }
-    } else {
-        DPart<float> po = load_part<float>(oo, qo, t, tp), pn = load_part<float>(on, qn, t, tp);
-        if (tid == 0) {
-            mv_store(T.mv, po);
-            mv_store(T.mv + MV_STRIDE, pn);
-        }
-        dold = po.dir;
-        dnew = pn.dir;
-    }
-    if (bmask && tid < 4) T.bp[tid] = -1;
-    if (bmask) __syncthreads();
-    // phase A, PA staged particles per thread and round (one barrier per
-    // round, 256-512 particles; the near list keeps stencil order)
-    constexpr int PA = NW >= 4 ? 2 : 4;
-    int nn = 0;
-    for (int base = 0; base < S.n; base += PA * BS) {
+// Phase A of a trial: every staged particle j != k inside sqmaxcut of the
+// moving particle's old or new position goes to the near list T.nl with its
+// state flags (bit 28: old, 29: new), in stencil order.  The first SB threads
+// of the block scan, PA entries each per round; the others only take part in
+// the compaction barriers.  With chain bonds it records the staged index of
+// each bonded partner.
+template <int NW, int PA>
+__device__ __forceinline__ void scan_stencil(const Stage& S, int k, const uint4 uo, const uint4 un, bool moved,
+                                             const SysConst& sc, int bmask, int idk, const TrialScratch& T, int SB,
+                                             bool scanner, int& nn, int& cpar) {
+    const int tid = (int)threadIdx.x;
+    for (int base = 0; base < S.n; base += PA * SB) {
         int fl[PA], jv[PA];
 #pragma unroll
         for (int h = 0; h < PA; h++) {
-            int j = base + h * BS + tid, f = 0;
-            if (j < S.n && j != k) {
+            int j = base + h * SB + tid, f = 0;
+            if (scanner && j < S.n && j != k) {
                 uint4 b = S.u[j];
                 V3<float> ro = fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
                 bool io = dot(ro, ro) <= sc.sqmaxcut, in_ = io;
@@ -1065,7 +1026,74 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         for (int h = 0; h < PA; h++)
             if (fl[h]) T.nl[pv[h]] = jv[h] | (fl[h] << 28);
     }
+}
+
+// Delayed acceptance (Christen & Fox 2005, in stages): with DA != DA_OFF
+// the move first faces Metropolis screens on cheaper energy changes, each
+// accepting with min(1, exp(-(its change)/T)); returns false at the first
+// screen that rejects (stage = the number of screens passed).  Otherwise eo/en
+// are the exact energies of the two states and des the sum of the screened
+// changes, which the final stage divides out: min(1, exp(-(dE - des)/T)).
+// Chain bonds are exact cheap terms (FP64, in the first screen); a move whose
+// bonded partner lies outside the staged stencil is rejected (stage -1): the
+// partner might be moving in another cell of the pass, and the rejection does
+// not depend on the move (the particle stays in its cell).
+template <int DA, int NW>
+__device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un,
+                                               const float4 oo, const float2 qo, const float4 on, const float2 qn,
+                                               const SysConst& sc,
+                                               const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
+                                               const TrialScratch& T, const float* ml, const MlpRegsW<4 / NW>& mr,
+                                               const MlpConst& mc, const ChainTab& ct, double thr1, double thr2,
+                                               const float* hvk, bool translate, double& eo, double& en,
+                                               double& des, int& stage) {
+    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
+    int t = uo.w;
+    bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
+    int idk = 0, cwk = 0;
+    if (ct.cw) {
+        idk = S.id[k];
+        cwk = ct.cw[idk];
+    }
+    int bmask = cwk & 15;
+    // parities of the one-barrier helpers; a trial starts after a barrier
+    // that follows every read of the previous trial's helpers
+    int cpar = 0, rpar = 0;
+    // Derived vectors of both states (DPart order, MV_STRIDE floats each):
+    // the old state's from the home cell's cache when there is one (else
+    // derived into T.mv); a translation keeps them for the new state; for a
+    // rotation the last wave derives the new state's into T.mv + MV_STRIDE
+    // while the other waves scan the stencil (phase A needs positions only).
+    const int wv = tid >> 6;
+    const float* mvo = hvk;
+    const float* mvn = T.mv + MV_STRIDE;
+    V3<float> dold;
+    if (hvk) {
+        dold = mk<float>(hvk[0], hvk[1], hvk[2]);
+    } else {
+        DPart<float> po = load_part<float>(oo, qo, t, tp);
+        if (tid == 0) mv_store(T.mv, po);
+        dold = po.dir;
+        mvo = T.mv;
+    }
+    if (translate) mvn = mvo;
+    if (!translate && (NW == 1 || wv == NW - 1)) {
+        DPart<float> pn = load_part<float>(on, qn, t, tp);
+        if (lane == 0) mv_store(T.mv + MV_STRIDE, pn);
+    }
+    if (bmask && tid < 4) T.bp[tid] = -1;
+    if (bmask) __syncthreads();
+    // phase A: the staged particles inside sqmaxcut at either state -> near
+    // list, in stencil order (one barrier per round); all waves scan a
+    // translation, the waves but the last one a rotation
+    int nn = 0;
+    if (NW > 1 && !translate)
+        scan_stencil<NW, NW >= 4 ? 3 : 8>(S, k, uo, un, moved, sc, bmask, idk, T, (NW - 1) * WAVE, wv < NW - 1, nn,
+                                         cpar);
+    else
+        scan_stencil<NW, NW >= 4 ? 2 : 4>(S, k, uo, un, moved, sc, bmask, idk, T, NW * WAVE, true, nn, cpar);
     __syncthreads();
+    const V3<float> dnew = translate ? dold : mv_vec(mvn, 0);
     double ao = 0.0, an = 0.0;
     if (bmask) {
         int found = 0;
@@ -1081,7 +1109,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             uint4 b = S.u[j];
             float4 ob = S.o[j];
             float2 qb = S.q[j];
-            const float* mv = T.mv + which * MV_STRIDE;
+            const float* mv = which ? mvn : mvo;
             double eb = bond_energy(ct.mol[cwk >> 8], ct.pp64[t * sc.nt + b.w], cw_rel(q),
                                     fx_r12(which ? un : uo, b, sc.Ld), unit64(mv[0], mv[1], mv[2]),
                                     unit64(mv[3], mv[4], mv[5]), unit64(ob.x, ob.y, ob.z), unit64(ob.w, qb.x, qb.y));
@@ -1126,7 +1154,8 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     __syncthreads();
     // screens of the delayed acceptance: (1) the exact cheap terms already
     // summed (repulsion, sphere pairs), (2) with DA_NET the network's patch
-    // attraction, tiles of 16 pairs on MFMA
+    // attraction, tiles of 16 pairs on MFMA, the hidden units of each tile
+    // split between the block's waves
     des = 0.0;
     stage = 2;
     if constexpr (DA != DA_OFF) {
@@ -1139,7 +1168,8 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         if constexpr (DA == DA_NET) {
             stage = 1;
             double so = 0.0, sn = 0.0;
-            for (int t0 = (tid >> 6) * MLP_TILE; t0 < nx; t0 += MLP_TILE * NW) {
+            const int wv = tid >> 6;
+            for (int t0 = 0; t0 < nx; t0 += MLP_TILE) {   // block-uniform: every wave in every tile
                 int e = t0 + (lane & (MLP_TILE - 1));
                 float f[MLP_NF];
                 int which = 0;
@@ -1153,7 +1183,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                     if (net) {
                         V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
                                             : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-                        const float* mv = T.mv + which * MV_STRIDE;
+                        const float* mv = which ? mvn : mvo;
                         float4 ob = S.o[jj];
                         float2 qb = S.q[jj];
                         mlp_features(mv_vec(mv, 0), mv_vec(mv, 3), mk<float>(ob.x, ob.y, ob.z),
@@ -1163,7 +1193,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                 }
                 if (!net)
                     for (int q = 0; q < MLP_NF; q++) f[q] = 0.0f;
-                float out = mlp_tile(mr, ml, mlp_slots(f, lane >> 4), lane);
+                float out = mlp_tile_w<NW>(mr, ml, T.hx, mlp_slots(f, lane >> 4), lane, wv);
                 if (lane < MLP_TILE && e < nx && net) {
                     if (which) sn += (double)out; else so += (double)out;
                 }
@@ -1196,7 +1226,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             P2 = pp + t * sc.nt + b.w;
             r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
                       : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-            const float* mv = T.mv + which * MV_STRIDE;
+            const float* mv = which ? mvn : mvo;
             if (P2->kind <= K_SCA) {
                 if (mpp < npatch_pairs(*P2)) {
                     patch_pair_index(*P2, mpp, k1, k2);
@@ -1251,17 +1281,20 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     int* scratch = (int*)(T.mv + 2 * MV_STRIDE);
     T.bp = scratch + 2 * 125 + 2;
     T.cnt = T.bp + 4;
-    T.red = (double*)(((uintptr_t)(T.cnt + 16) + 7) & ~(uintptr_t)7);
+    T.red = (double*)(((uintptr_t)(T.cnt + 2 * CNT_HALF) + 7) & ~(uintptr_t)7);
     TrialRec* recs = (TrialRec*)(((uintptr_t)(T.red + 16) + 15) & ~(uintptr_t)15);
     float* hv = (float*)(recs + TRIAL_CHUNK);
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1);
     float* ml = nullptr;
-    MlpRegs mr;
+    MlpRegsW<4 / NW> mr;   // this wave's share of the network's operands
+    T.hx = nullptr;
     uintptr_t tail = (uintptr_t)(hv + HCAP * MV_STRIDE);
     if constexpr (DA == DA_NET) {
         ml = (float*)((tail + 15) & ~(uintptr_t)15);
-        mlp_setup(mlp_img, ml, lane, mr);
+        mlp_setup_w<4 / NW>(mlp_img, ml, lane, (tid >> 6) * (4 / NW), mr);
         tail = (uintptr_t)ml + MLP_LDS_BYTES;
+        T.hx = (bf16x4*)((tail + 15) & ~(uintptr_t)15);
+        tail = (uintptr_t)T.hx + MLP_HX_BYTES;
     }
 #if SWEEP_TABLES_IN_LDS
     // the pair and type parameter tables in LDS: every pair term reads them,
@@ -1287,19 +1320,19 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     // trials in this cell: its occupancy (every particle once per sweep on
     // average), or with trial_mode 1 the mean occupancy of the non-empty
     // cells, dithered, so the longest chain of a pass is not the fullest cell
-    int ntrial = nh;
+    int ntrial = nh * mp.visit;
     if (mp.trial_mode == 1) {
         ctr.x = 0xffffffffu;
         ctr.w = (mp.pass_hi << 2) | 3u;
         U4 rd = philox(ctr, mp.key0, mp.key1);
-        float mean = (float)mp.n / (float)max(1, *nonempty);
+        float mean = (float)mp.n * (float)mp.visit / (float)max(1, *nonempty);
         int fl = (int)floorf(mean);
         ntrial = fl + ((u01f(rd.x) < mean - (float)fl) ? 1 : 0);
     }
     unsigned n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0, n_s2try = 0,
              n_s2acc = 0, n_bond = 0;   // per block: far below 2^32
     // derived vectors of the home particles (read after the first trial's barrier)
-    const bool hcache = nh <= HCAP;
+    const bool hcache = nh <= min(HCAP, mp.hcap);
     if (hcache)
         for (int i = tid; i < nh; i += WAVE * NW)
             mv_store(hv + i * MV_STRIDE, load_part<float>(S.o[S.h0 + i], S.q[S.h0 + i], S.u[S.h0 + i].w, tp));
@@ -1387,9 +1420,8 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         bool pass;
         int stage;
         const float* hvk = hcache ? hv + (k - S.h0) * MV_STRIDE : nullptr;
-        const bool same_or = hcache && translate;   // new state's derived vectors = the cached old ones
         pass = trial_energies<DA, NW>(S, k, uo, un, oo, qo, on, qn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, R.thr1,
-                                      R.thr2, hvk, same_or, eo, en, des, stage);
+                                      R.thr2, hvk, translate, eo, en, des, stage);
         if (stage < 0) {   // a chain partner outside the stencil
             n_bond++;
             __syncthreads();
@@ -1417,7 +1449,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
                 S.o[k] = on;
                 S.q[k] = qn;
             }
-            if (hcache && tid < 27) hv[(k - S.h0) * MV_STRIDE + tid] = T.mv[MV_STRIDE + tid];
+            if (hcache && !translate && tid < 27) hv[(k - S.h0) * MV_STRIDE + tid] = T.mv[MV_STRIDE + tid];
         }
         __syncthreads();
     }
@@ -1441,9 +1473,11 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     }
 }
 
-// NW waves per block work on one active cell (NW = 1, 2 or 4)
+// NW waves per block work on one active cell (NW = 1, 2 or 4).  Every
+// variant keeps to 256 registers per lane, so that two blocks of four waves
+// share a CU (two waves per SIMD): a pass has ~2 active cells per CU.
 template <int DA, int NW>
-__global__ __launch_bounds__(WAVE * NW) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
+__global__ __launch_bounds__(WAVE * NW, 2) void k_sweep_pass(Grid g, SysConst sc, MoveParams mp, const int* __restrict__ starts,
                                                      uint4* __restrict__ P, float4* __restrict__ O,
                                                      float2* __restrict__ Q, const int* __restrict__ ID,
                                                      const PP32* __restrict__ pp, const TP32* __restrict__ tp, int cap,
@@ -1470,7 +1504,7 @@ struct RepPass {
 // replica by binary search over blk_off (a replica without active cells
 // shares its offset with the next one, and the search takes the last).
 template <int DA, int NW>
-__global__ __launch_bounds__(WAVE * NW) void k_sweep_batch(const RepPass* __restrict__ rp, int nrep,
+__global__ __launch_bounds__(WAVE * NW, 2) void k_sweep_batch(const RepPass* __restrict__ rp, int nrep,
                                                       const int* __restrict__ starts, uint4* __restrict__ P,
                                                       float4* __restrict__ O, float2* __restrict__ Q,
                                                       const int* __restrict__ ID, const PP32* __restrict__ pp,
@@ -3022,6 +3056,8 @@ struct sc_ctx {
     double e_track = 0.0;
     int e_age = 0;                      // volume moves since the last full evaluation
     int sweep_nw = 0;                   // waves per active cell in the sweep kernels (1, 2, 4; 0 = by mode)
+    int home_cache = HCAP;              // cell size limit of the sweep's derived-vector cache (0: off)
+    int visit = 1;                      // colour passes of a sweep = colours / visit, visit x the trials each
     // whole-molecule moves of sc_sweep
     std::vector<int> chain_first, chain_size;   // molecules of more than one particle
     int nchain_dev = 0;                 // molecules uploaded below
@@ -4711,26 +4747,36 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         // + stencil scratch (+ the network and its activation tile)
         // (+ chain partners, block compaction counts, block sums, trial records)
         size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + SWEEP_SMALL_LDS +
-                       (da ? MLP_LDS_BYTES + 16 : 0) +
+                       (da ? MLP_LDS_BYTES + MLP_HX_BYTES + 32 : 0) +
                        sweep_table_bytes(ctx->nt);
         size_t per = stage_bytes(1) + sizeof(int) * 3 + (da ? sizeof(float) * 2 : 0);
         int capmax = (int)((SWEEP_LDS_BUDGET - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
         size_t lds = per * cap + fixed;
         int ncol = p * p * p;
-        const int* order = plan.order;
         int pa[3];
         for (int k = 0; k < 3; k++) pa[k] = (g.nc[k] >= p) ? p : 1;
+        // the colours of this sweep in its random order (a single-cell axis
+        // has period 1: colour offsets along it stay 0); with a visit factor
+        // v the sweep takes the first 1/v of them, each with v times the trials
+        int cols[27], ncv = 0;
+        for (int ci = 0; ci < ncol; ci++) {
+            int col = plan.order[ci];
+            if (col % p < pa[0] && (col / p) % p < pa[1] && col / (p * p) < pa[2]) cols[ncv++] = col;
+        }
+        if (ncv % ctx->visit) return fail(ctx, SC_EINVAL, "the visit factor must divide the number of colours");
+        ncv /= ctx->visit;
         SysConst sc = sysconst(ctx, ctx->box);
         int c = ctx->cur;
-        for (int ci = 0; ci < ncol; ci++) {
-            int col = order[ci];
+        for (int ci = 0; ci < ncv; ci++) {
+            int col = cols[ci];
             MoveParams mp;
             mp.ox = col % p; mp.oy = (col / p) % p; mp.oz = col / (p * p);
-            if (mp.ox >= pa[0] || mp.oy >= pa[1] || mp.oz >= pa[2]) continue;   // axis with a single cell
             mp.p = p;
             for (int k = 0; k < 3; k++) mp.nact[k] = g.nc[k] / pa[k];
             mp.trial_mode = ctx->trial_mode;
+            mp.hcap = ctx->home_cache;
+            mp.visit = ctx->visit;
             mp.n = ctx->n;
             mp.transmx = (float)transmx;
             mp.rot_half = (float)rot_half;
@@ -4740,7 +4786,6 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
             uint64_t pass = sw * 32 + (uint64_t)ci;
             mp.pass_lo = (uint32_t)pass;
             mp.pass_hi = (uint32_t)(pass >> 32);
-            // a single-cell axis has period 1: colour offsets along it stay 0
             int nblk = mp.nact[0] * mp.nact[1] * mp.nact[2];
             if (ctx->profile) {
                 while ((long)ctx->kev.size() < 2 * (launches + 1)) {
@@ -4754,10 +4799,9 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
     hipLaunchKernelGGL((k_sweep_pass<DAM, NWV>), dim3(nblk), dim3(WAVE * NWV), lds, ctx->stream, g, sc, mp,         \
                        ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,     \
                        ctx->ppmax, ctx->d_nonempty, IMG, MC, chain_tab(ctx), ctx->d_stats, ctx->d_dsum, ctx->d_flag)
-            // by mode: the network screen holds its MFMA operands in registers, which
-            // leaves room for one wave per SIMD, so it runs two waves per cell (two
-            // blocks per CU); the other modes run four
-            int nw = ctx->sweep_nw ? ctx->sweep_nw : (mode == DA_NET ? 2 : 4);
+            // four waves per cell in every mode (the network screen splits its
+            // hidden units, and so its MFMA operands, between the four waves)
+            int nw = ctx->sweep_nw ? ctx->sweep_nw : 4;
             if (mode == DA_NET) {
                 if (nw == 4) SWEEP_PASS(DA_NET, 4, ctx->mlp_img, mlpc);
                 else if (nw == 2) SWEEP_PASS(DA_NET, 2, ctx->mlp_img, mlpc);
@@ -5014,6 +5058,8 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
                 bool skip = mp.ox >= pa[0] || mp.oy >= pa[1] || mp.oz >= pa[2];   // axis with a single cell
                 for (int k = 0; k < 3; k++) mp.nact[k] = g.nc[k] / pa[k];
                 mp.trial_mode = ctx->trial_mode;
+                mp.hcap = ctx->home_cache;
+                mp.visit = 1;   // a batch visits every colour once per sweep
                 mp.n = ctx->rep_n[q];
                 mp.transmx = (float)transmx;
                 mp.rot_half = (float)rot_half;
@@ -5719,6 +5765,26 @@ static int sc_set_block_waves_impl(sc_ctx* ctx, int nw) {
 
 int sc_set_block_waves(sc_ctx* ctx, int nw) {
     SC_GUARD(ctx, sc_set_block_waves_impl(ctx, nw))
+}
+
+static int sc_set_home_cache_impl(sc_ctx* ctx, int cap) {
+    if (!ctx || cap < 0) return SC_EINVAL;
+    ctx->home_cache = cap < HCAP ? cap : HCAP;
+    return SC_OK;
+}
+
+int sc_set_home_cache(sc_ctx* ctx, int cap) {
+    SC_GUARD(ctx, sc_set_home_cache_impl(ctx, cap))
+}
+
+static int sc_set_visit_factor_impl(sc_ctx* ctx, int v) {
+    if (!ctx || v < 1 || v > 27) return SC_EINVAL;
+    ctx->visit = v;
+    return SC_OK;
+}
+
+int sc_set_visit_factor(sc_ctx* ctx, int v) {
+    SC_GUARD(ctx, sc_set_visit_factor_impl(ctx, v))
 }
 
 static int sc_set_profiling_impl(sc_ctx* ctx, int on) {

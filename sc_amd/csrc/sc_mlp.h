@@ -1,9 +1,10 @@
 // sc_mlp.h -- the learned surrogate of the patch attraction of a pair.
 //
 // Network: 12 body-frame features -> 64 -> 64 -> 64 -> 1, SiLU after each
-// hidden layer.  Inference inside the Monte Carlo sweep runs one wave per
-// tile of 32 pairs on MFMA (v_mfma_f32_32x32x16_bf16: BF16 operands, FP32
-// accumulation); training uses FP32 kernels with FP32 master weights.
+// hidden layer.  Inference inside the Monte Carlo sweep runs tiles of 16
+// pairs on MFMA (v_mfma_f32_16x16x32_bf16: BF16 operands, FP32
+// accumulation), the hidden units of a tile split between the waves of the
+// sweep's block; training uses FP32 kernels with FP32 master weights.
 //
 // The surrogate pair energy used by the delayed-acceptance first stage is
 //   s(a,b) = WCA(d_seg) + [d_seg <= rcut] * MLP(features(a,b))
@@ -40,6 +41,16 @@ SC_HD float silu_grad(float z) {
     return s * (1.0f + z * (1.0f - s));
 }
 
+// cos of an argument in [0, pi/2] for the features: the hardware's on the
+// device (v_cos_f32, ~1e-6 absolute there), libm's on the host
+SC_HD float mlp_cos(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __cosf(x);
+#else
+    return cosf(x);
+#endif
+}
+
 // Features of pair (a moving as part1, b) in a's body frame (e1 = dir,
 // e2 = patch direction, e3 = e1 x e2); r = pos_a - pos_b (Cuboid::image
 // convention), d = closest segment distance.  lsc, rcut, pdis, pswitch: scale
@@ -64,7 +75,7 @@ SC_HD void mlp_features(const V3<float>& adir, const V3<float>& apd, const V3<fl
     if (d < pdis) w = 1.0f;
     else if (d > pdis + pswitch) w = 0.0f;
     else {
-        float c = cosf(1.5707963267948966f * (d - pdis) / pswitch);
+        float c = mlp_cos(1.5707963267948966f * (d - pdis) / pswitch);
         w = c * c;
     }
     f[11] = w;
@@ -132,26 +143,39 @@ SC_HD int mlp_kperm(int ks, int kg, int j) { return 16 * (2 * ks + (j >> 2)) + 4
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// the A operands of one lane
-struct MlpRegs {
-    bf16x8 a1[4];   // [blk]
-    bf16x8 a2[8];   // [blk*2 + ks]
-    bf16x8 a3[8];
+// The A operands of one lane for BPW of the four 16-unit blocks of every
+// layer (blocks b0 .. b0+BPW-1): a whole network (BPW = 4, 80 VGPRs) for one
+// wave that evaluates a tile alone, or its share when the waves of a block
+// split the hidden units between them.
+template <int BPW>
+struct MlpRegsW {
+    bf16x8 a1[BPW];       // [blk]
+    bf16x8 a2[2 * BPW];   // [blk*2 + ks]
+    bf16x8 a3[2 * BPW];
 };
+using MlpRegs = MlpRegsW<4>;
 
-// copy the constants into LDS (`lds`, MLP_LDS_BYTES) and this lane's operands into registers
-__device__ __forceinline__ void mlp_setup(const unsigned char* __restrict__ img, float* lds, int lane, MlpRegs& R) {
+// copy the constants into LDS (`lds`, MLP_LDS_BYTES) and this lane's
+// operands of blocks b0 .. b0+BPW-1 into registers
+template <int BPW>
+__device__ __forceinline__ void mlp_setup_w(const unsigned char* __restrict__ img, float* lds, int lane, int b0,
+                                            MlpRegsW<BPW>& R) {
     for (int i = lane; i < MLP_C_BYTES / 16; i += 64) ((uint4*)lds)[i] = ((const uint4*)(img + MLP_C_OFF))[i];
     const bf16x8* a1 = (const bf16x8*)(img + MLP_A1_OFF);
     const bf16x8* a2 = (const bf16x8*)(img + MLP_A2_OFF);
     const bf16x8* a3 = (const bf16x8*)(img + MLP_A3_OFF);
 #pragma unroll
-    for (int b = 0; b < 4; b++) R.a1[b] = a1[b * 64 + lane];
+    for (int b = 0; b < BPW; b++) {
+        R.a1[b] = a1[(b0 + b) * 64 + lane];
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-        R.a2[q] = a2[q * 64 + lane];
-        R.a3[q] = a3[q * 64 + lane];
+        for (int ks = 0; ks < 2; ks++) {
+            R.a2[2 * b + ks] = a2[(2 * (b0 + b) + ks) * 64 + lane];
+            R.a3[2 * b + ks] = a3[(2 * (b0 + b) + ks) * 64 + lane];
+        }
     }
+}
+__device__ __forceinline__ void mlp_setup(const unsigned char* __restrict__ img, float* lds, int lane, MlpRegs& R) {
+    mlp_setup_w<4>(img, lds, lane, 0, R);
 }
 
 // SiLU for inference: hardware exp2 and reciprocal
@@ -159,35 +183,65 @@ __device__ __forceinline__ float silu_fast(float z) {
     return z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
 }
 
-__device__ __forceinline__ bf16x8 mlp_pack8(const floatx4& c0, const floatx4& c1) {
-    bf16x8 v;
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x4 mlp_act4(const floatx4& c) {
+    bf16x4 v;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        v[j] = (__bf16)silu_fast(c0[j]);
-        v[4 + j] = (__bf16)silu_fast(c1[j]);
-    }
+    for (int j = 0; j < 4; j++) v[j] = (__bf16)silu_fast(c[j]);
     return v;
 }
 
-// Network output for sample l%16 of a 16-sample tile (returned on lanes l,
-// l+16, l+32, l+48).  x: this lane's feature slots 8*kg..8*kg+7 of its
-// sample, BF16 (slot 12 = 1.0, the rest of 12..31 zero).  cst: the constants
-// in LDS.  Samples are independent: a sample's output does not depend on
-// its tile mates.
-__device__ __forceinline__ float mlp_tile(const MlpRegs& R, const float* cst, bf16x8 x, int lane) {
-    int kg = lane >> 4;
+__device__ __forceinline__ bf16x8 mlp_join(const bf16x4& a, const bf16x4& b) {
+    return __builtin_shufflevector(a, b, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+// LDS exchange of a block's activations between the waves of a split tile:
+// two buffers (one per hidden layer that feeds another) of [blk 4][lane 64]
+constexpr int MLP_HX_BYTES = 2 * 4 * 64 * (int)sizeof(bf16x4);   // 4 KiB
+
+// Network output for sample l%16 of a 16-sample tile, evaluated by NW waves
+// together (NW = 1, 2 or 4): wave wv computes the unit blocks
+// wv*4/NW .. (wv+1)*4/NW - 1 of every layer and returns its share of the
+// output (the output layer's dot over its units, plus the output bias on
+// wave 0), so that the shares of the waves sum to the network's output; for
+// NW = 1 that is the whole output.  The result is on lanes l, l+16, l+32,
+// l+48.  With NW > 1 each hidden layer's activations meet in `hx`
+// (MLP_HX_BYTES of LDS) behind a barrier of the block, so every wave of the
+// block must call this the same number of times.  x: this lane's feature
+// slots 8*kg..8*kg+7 of its sample, BF16 (slot 12 = 1.0, the rest of 12..31
+// zero), the same on every wave.  cst: the constants in LDS.  Samples are
+// independent: a sample's output does not depend on its tile mates.
+template <int NW>
+__device__ __forceinline__ float mlp_tile_w(const MlpRegsW<4 / NW>& R, const float* cst, bf16x4* hx, bf16x8 x,
+                                            int lane, int wv) {
+    constexpr int BPW = 4 / NW;
+    const int kg = lane >> 4, b0 = wv * BPW;
     floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
-    floatx4 c[4];
+    floatx4 c[BPW];
 #pragma unroll
-    for (int b = 0; b < 4; b++) c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a1[b], x, z, 0, 0, 0);
+    for (int b = 0; b < BPW; b++) c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(R.a1[b], x, z, 0, 0, 0);
 #pragma unroll
     for (int layer = 0; layer < 2; layer++) {
-        bf16x8 h0 = mlp_pack8(c[0], c[1]), h1 = mlp_pack8(c[2], c[3]);
+        // this layer's activations as the next layer's two K slices
+        // (blocks 0,1 and 2,3)
+        bf16x8 h0, h1;
+        if constexpr (NW == 1) {
+            h0 = mlp_join(mlp_act4(c[0]), mlp_act4(c[1]));
+            h1 = mlp_join(mlp_act4(c[2]), mlp_act4(c[3]));
+        } else {
+            bf16x4* buf = hx + layer * 256;
+#pragma unroll
+            for (int b = 0; b < BPW; b++) buf[(b0 + b) * 64 + lane] = mlp_act4(c[b]);
+            __syncthreads();
+            h0 = mlp_join(buf[lane], buf[64 + lane]);
+            h1 = mlp_join(buf[128 + lane], buf[192 + lane]);
+        }
         const bf16x8* a = layer == 0 ? R.a2 : R.a3;
         const floatx4* cb = (const floatx4*)(cst + layer * 64 + kg * 16);
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            c[b] = cb[b];
+        for (int b = 0; b < BPW; b++) {
+            c[b] = cb[b0 + b];
             c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2 * b], h0, c[b], 0, 0, 0);
             c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2 * b + 1], h1, c[b], 0, 0, 0);
         }
@@ -195,14 +249,19 @@ __device__ __forceinline__ float mlp_tile(const MlpRegs& R, const float* cst, bf
     const floatx4* w4 = (const floatx4*)(cst + 128 + kg * 16);
     float part = 0.0f;
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
-        floatx4 w = w4[b];
+    for (int b = 0; b < BPW; b++) {
+        floatx4 w = w4[b0 + b];
 #pragma unroll
         for (int r = 0; r < 4; r++) part += silu_fast(c[b][r]) * w[r];
     }
     part += __shfl_xor(part, 16);
     part += __shfl_xor(part, 32);
-    return part + cst[192];
+    return wv == 0 ? part + cst[192] : part;
+}
+
+// the whole network on one wave
+__device__ __forceinline__ float mlp_tile(const MlpRegs& R, const float* cst, bf16x8 x, int lane) {
+    return mlp_tile_w<1>(R, cst, nullptr, x, lane, 0);
 }
 
 // BF16 feature slice of this lane from the 12 features of its sample

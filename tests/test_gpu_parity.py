@@ -4,7 +4,11 @@ Tolerances:
   * pair energies, FP64 device mode: identical arithmetic to the oracle up to
     libm rounding -> 1e-9 relative for >= 99.9% of pairs (the rest are
     degenerate-branch pairs, counted and bounded);
-  * pair energies, FP32: 1e-4 relative (1e-5 absolute) for >= 99% of pairs;
+  * pair energies, FP32: 1e-5 relative for >= 99.5% of the pairs with
+    0.1 <= |E| < 1e3 kT (all within 2e-5), 2e-6 kT absolute below, 1e-3
+    relative above;
+  * per-particle energies: 1e-5 relative (1e-5 kT where terms cancel);
+  * sweep energy bookkeeping: 1e-6 |E|;
   * whole-system energy (FP32 pair terms, FP64 reduction): 1e-5 relative;
   * cell lists, neighbour lists: bit-exact;
   * overlap flags: identical except pairs within 1e-9 sigma of contact.
@@ -64,6 +68,17 @@ def test_pair_energies_fp64_mode(dev):
 
 
 def test_pair_energies_fp32(dev):
+    """FP32 pair terms against the FP64 oracle on the 59,918 mixed-species
+    pairs, class by class.  The FP32 error of a pair term is that of its
+    separation vector (~2e-7 sigma at the stencil's distances) times the
+    term's slope, so the relative error is bounded only where |E'/E| is:
+      * 0.1 <= |E| < 1e3 kT: >= 99.5% within 1e-5 relative, every one within
+        2e-5 (measured: 4 of 2,096 beyond 1e-5, the largest 1.3e-5);
+      * |E| < 0.1 kT (the WCA cutoff and the end of the attraction's cos^2
+        switch, where E -> 0 with a finite or zero slope): 2e-6 kT absolute;
+      * |E| >= 1e3 kT (segments closer than 0.56 sigma; Boltzmann weight
+        below e^-1000): 1e-3 relative (WCA's d^-12 multiplies the distance's
+        relative error by 12)."""
     say("case: pair energies fp32, mixed 12-type system")
     g = sc_amd.GpuSystem(dev)
     z, tp, st = mixed(g)
@@ -71,9 +86,21 @@ def test_pair_energies_fp32(dev):
     eg = g.p2p(i, j)
     eo = po.pair_energies(tp, st, i, j)
     err = np.abs(eg - eo)
-    ok = (err <= 1e-4 * np.abs(eo)) | (err <= 1e-5)
-    say("fp32: %d pairs, %d outside 1e-4 rel/1e-5 abs" % (len(i), (~ok).sum()))
-    assert ok.mean() > 0.99
+    a = np.abs(eo)
+    mid, small, huge = (a >= 0.1) & (a < 1e3), a < 0.1, a >= 1e3
+    rel = err / np.maximum(a, 1e-300)
+    out = rel[mid] > 1e-5
+    say("fp32: %d pairs; %d with 0.1 <= |E| < 1e3: %d beyond 1e-5 relative (max %.2e, quantiles 0.5/0.99/0.999 "
+        "%.2e/%.2e/%.2e); %d with |E| < 0.1: max abs error %.2e; %d with |E| >= 1e3: max rel %.2e" % (
+            len(i), mid.sum(), out.sum(), rel[mid].max(), np.quantile(rel[mid], 0.5), np.quantile(rel[mid], 0.99),
+            np.quantile(rel[mid], 0.999), small.sum(), err[small].max(), huge.sum(), rel[huge].max()))
+    for k in np.nonzero(mid)[0][out][:20]:
+        say("  pair %d-%d types %d-%d: gpu %.9f oracle %.9f" % (i[k], j[k], st.types[i[k]], st.types[j[k]], eg[k],
+                                                               eo[k]))
+    assert out.mean() <= 5e-3
+    assert rel[mid].max() <= 2e-5
+    assert err[small].max() <= 2e-6
+    assert rel[huge].max() <= 1e-3
 
 
 def test_pair_energies_grid_all_type_pairs(dev):
@@ -114,9 +141,12 @@ def test_energy_all_and_per_particle_golden_systems(dev, key):
     assert abs(eg - eo) <= 1e-5 * max(1.0, abs(eo))
     pp = g.energy_per_particle()
     po_ = po.energy_one_each(tp, st, np.arange(st.n))
-    np.testing.assert_allclose(pp, po_, rtol=1e-4, atol=1e-4)
+    rel = np.abs(pp - po_) / np.maximum(np.abs(po_), 1e-3)
+    say("per particle: max rel %.2e" % rel.max())
+    # 1e-5 relative, or 1e-5 kT where a particle's terms cancel
+    np.testing.assert_allclose(pp, po_, rtol=1e-5, atol=1e-5)
     for t in (0, 7, st.n - 1):
-        assert g.oneToAll(t) == pytest.approx(po_[t], rel=1e-4, abs=1e-4)
+        assert g.oneToAll(t) == pytest.approx(po_[t], rel=1e-5, abs=1e-5)
 
 
 def test_energy_all_config2_cpsc_32768(dev):
@@ -206,11 +236,11 @@ def test_trial_and_commit_protocol(dev):
         st2 = po.State(st.box, st.pos.copy(), st.dir, st.pd0, st.types)
         st2.pos[t] = newpos
         want = po.energy_one_each(tp, st2, [t])[0]
-        assert e_new == pytest.approx(want, rel=1e-4, abs=1e-4)
+        assert e_new == pytest.approx(want, rel=1e-5, abs=1e-5)
         g.update_particle(t, newpos, st.dir[t], st.pd0[t])
-        assert g.oneToAll(t) == pytest.approx(e_new, rel=1e-5, abs=1e-5)
+        assert g.oneToAll(t) == pytest.approx(e_new, rel=1e-6, abs=1e-6)
         e_all1 = g.allToAll()
-        assert e_all1 - e_all0 == pytest.approx(e_new - e_old, rel=1e-4, abs=1e-4)
+        assert abs((e_all1 - e_all0) - (e_new - e_old)) <= 1e-6 * max(1.0, abs(e_all1))
         e_all0 = e_all1
         _, p, _, _ = g.get_config()
         st.pos[:] = p
@@ -234,7 +264,7 @@ def test_sweep_energy_drift_and_determinism(dev):
             e0, e1, s["energy_change"], drift, tries, s["trans_acc"], s["trans_try"], s["rot_acc"], s["rot_try"],
             s["left_cell"]))
         assert tries == 20 * 1000
-        assert abs(drift) < 1e-4 * max(1.0, abs(e1)) + 1e-3
+        assert abs(drift) <= 1e-6 * max(1.0, abs(e1))
         _, p, dd, qq = g.get_config()
         out.append((p, dd, qq))
         g.close()
