@@ -130,6 +130,15 @@ int sc_neighbor_pairs(sc_ctx* ctx, double rc, int32_t* pairs, long cap, long* co
 /* pair energies E(i,j) with i as part1 (TotalE::p2p, totalenergycalculator.h:200);
  * fp64 != 0 evaluates the same device code in FP64 (cross-check mode). */
 int sc_pair_energies(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64);
+/* The same pair energies through an entry point that allocates nothing (no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops).  Its staging
+ * (indices and energies of up to SC_NOALLOC_MAX_PAIRS pairs, 1 MiB) is one
+ * fixed device array per GPU that lives until the process ends; more pairs
+ * return SC_ECAPACITY.  Results equal sc_pair_energies' bit for bit.  Calls
+ * that overlap on one GPU share that array and race. */
+#define SC_NOALLOC_MAX_PAIRS 65536
+int sc_pair_energies_noalloc(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64);
 /* whole-system energy: FP32 pair terms, FP64 reduction
  * (TotalEMatrix::allToAll / initEM, totalenergycalculator.h:314-353,502-521) */
 int sc_energy_all(sc_ctx* ctx, double* e);

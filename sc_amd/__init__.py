@@ -70,6 +70,7 @@ SIGNATURES = {
     "sc_build_cells": (ctypes.c_int, [_P, ctypes.c_int, _U, _I, _I, _I, _I]),
     "sc_neighbor_pairs": (ctypes.c_int, [_P, ctypes.c_double, _I, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]),
     "sc_pair_energies": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _D, ctypes.c_int]),
+    "sc_pair_energies_noalloc": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, _D, ctypes.c_int]),
     "sc_energy_all": (ctypes.c_int, [_P, _D]),
     "sc_energy_one": (ctypes.c_int, [_P, ctypes.c_int, _D]),
     "sc_energy_one_trial": (ctypes.c_int, [_P, ctypes.c_int, _D, _D, _D, _D]),
@@ -120,6 +121,7 @@ SIGNATURES = {
 }
 
 MLP_NPARAM = 9217   # W1[12][64] b1 W2[64][64] b2 W3[64][64] b3 W4[64] b4
+NOALLOC_MAX_PAIRS = 65536   # SC_NOALLOC_MAX_PAIRS of include/sc_gpu.h
 MLP_NF = 12
 
 _lib = None
@@ -332,6 +334,15 @@ class GpuSystem:
         j = np.ascontiguousarray(np.atleast_1d(j), np.int32)
         out = np.zeros(len(i))
         self._ck(self._L.sc_pair_energies(self._h, len(i), _i(i), _i(j), _d(out), int(bool(fp64))), "pair_energies")
+        return out
+
+    def p2p_noalloc(self, i, j, fp64=False):
+        """p2p through the entry point that allocates nothing (at most SC_NOALLOC_MAX_PAIRS pairs)"""
+        i = np.ascontiguousarray(np.atleast_1d(i), np.int32)
+        j = np.ascontiguousarray(np.atleast_1d(j), np.int32)
+        out = np.zeros(len(i))
+        self._ck(self._L.sc_pair_energies_noalloc(self._h, len(i), _i(i), _i(j), _d(out), int(bool(fp64))),
+                 "pair_energies_noalloc")
         return out
 
     def allToAll(self):

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -406,14 +407,14 @@ __device__ __forceinline__ int stencil_size(const Grid& g) {
 }
 
 // cooperative copy by the block (NW waves; the stencil prefix by wave 0);
-// scratch holds 2*ns+2 ints.  Returns false on overflow.
+// sten holds 2*ns+2 ints.  Returns false on overflow.
 template <int NW>
 __device__ bool stage_load(Stage& S, int cap, const Grid& g, int cx, int cy, int cz, const int* __restrict__ starts,
                            const uint4* __restrict__ P, const float4* __restrict__ O, const float2* __restrict__ Q,
-                           const int* __restrict__ ID, int* scratch) {
+                           const int* __restrict__ ID, int* sten) {
     int ns = stencil_size(g);
-    int* pre = scratch;          // ns+1 prefix of counts
-    int* cst = scratch + ns + 1; // cell start per stencil entry
+    int* pre = sten;          // ns+1 prefix of counts
+    int* cst = sten + ns + 1; // cell start per stencil entry
     int tid = (int)threadIdx.x;
     if (tid < WAVE) {
         int lane = tid;
@@ -441,11 +442,11 @@ __device__ bool stage_load(Stage& S, int cap, const Grid& g, int cx, int cy, int
         }
         if (lane == 0) {
             pre[ns] = carry;
-            scratch[2 * ns + 1] = home_idx;
+            sten[2 * ns + 1] = home_idx;
         }
     }
     __syncthreads();
-    int total = pre[ns], home_idx = scratch[2 * ns + 1];
+    int total = pre[ns], home_idx = sten[2 * ns + 1];
     S.n = total;
     S.h0 = pre[home_idx];
     S.h1 = pre[home_idx + 1];
@@ -636,14 +637,14 @@ __global__ __launch_bounds__(WAVE * NW) void k_energy_all(Grid g, SysConst sc, c
     S.q = (float2*)(S.o + cap);
     S.id = (int*)(S.q + cap);
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), w = tid >> 6;
-    int* scratch = S.id + cap;                           // 2*125+2 ints
-    double* red = (double*)(scratch + 2 * 125 + 2);      // NW doubles (8-byte aligned: cap is a multiple of 64)
+    int* sten = S.id + cap;                              // 2*125+2 ints
+    double* red = (double*)(sten + 2 * 125 + 2);         // NW doubles (8-byte aligned: cap is a multiple of 64)
     int* queue = (int*)(red + NW) + w * 2 * QCAP;        // this wave's QCAP (hi, j) entries
     if (starts[cell + 1] == starts[cell]) {
         if (tid == 0) partial[cell] = 0.0;
         return;
     }
-    if (!stage_load<NW>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
+    if (!stage_load<NW>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, sten)) {
         if (tid == 0) { atomicOr(flag, 1); partial[cell] = 0.0; }
         return;
     }
@@ -1278,8 +1279,8 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     T.xl = T.nl + cap;
     T.xd = (float*)(T.xl + 2 * cap);
     T.mv = DA == DA_NET ? T.xd + 2 * cap : (float*)(T.xl + 2 * cap);
-    int* scratch = (int*)(T.mv + 2 * MV_STRIDE);
-    T.bp = scratch + 2 * 125 + 2;
+    int* sten = (int*)(T.mv + 2 * MV_STRIDE);
+    T.bp = sten + 2 * 125 + 2;
     T.cnt = T.bp + 4;
     T.red = (double*)(((uintptr_t)(T.cnt + 2 * CNT_HALF) + 7) & ~(uintptr_t)7);
     TrialRec* recs = (TrialRec*)(((uintptr_t)(T.red + 16) + 15) & ~(uintptr_t)15);
@@ -1309,7 +1310,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         tp = tpl;
     }
 #endif
-    if (!stage_load<NW>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, scratch)) {
+    if (!stage_load<NW>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, sten)) {
         if (tid == 0) atomicOr(flag, 1);
         return;
     }
@@ -3058,6 +3059,7 @@ struct sc_ctx {
     int sweep_nw = 0;                   // waves per active cell in the sweep kernels (1, 2, 4; 0 = by mode)
     int home_cache = HCAP;              // cell size limit of the sweep's derived-vector cache (0: off)
     int visit = 1;                      // colour passes of a sweep = colours / visit, visit x the trials each
+    unsigned char* noalloc = nullptr;   // this device's address of `scratch` (sc_pair_energies_noalloc)
     // whole-molecule moves of sc_sweep
     std::vector<int> chain_first, chain_size;   // molecules of more than one particle
     int nchain_dev = 0;                 // molecules uploaded below
@@ -3508,6 +3510,33 @@ static int guard_fail(sc_ctx* ctx, const char* fn, const char* what) {
 
 extern "C" {
 
+// Staging of sc_pair_energies_noalloc, the entry point that allocates
+// nothing: the pair indices i and j (int32) and the energies (double) of up to
+// SC_NOALLOC_MAX_PAIRS pairs, 1 MiB in all, in one file-scope device array
+// that every GPU the library runs on holds until the process ends.  Host code
+// never passes the array's name to a kernel or a copy (on the host it is a
+// stand-in, not a device address): sc_create looks up each device's own
+// address once, with that device current, and contexts keep it.
+static __device__ __align__(256) unsigned char scratch[(size_t)SC_NOALLOC_MAX_PAIRS * 16];
+#define SC_SCRATCH_DEVICES 64
+static std::mutex scratch_mu;
+static unsigned char* scratch_addr[SC_SCRATCH_DEVICES];
+
+// the device address of `scratch` on the current device `device`
+static hipError_t scratch_lookup(int device, unsigned char** out) {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (device < 0 || device >= SC_SCRATCH_DEVICES) return hipErrorInvalidDevice;
+    if (!scratch_addr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_addr[device] = (unsigned char*)p;
+    }
+    *out = scratch_addr[device];
+    return hipSuccess;
+}
+
+static void sc_destroy_impl(sc_ctx* ctx);
 static int sc_create_impl(sc_ctx** out, int device) {
     if (!out) return SC_EINVAL;
     *out = nullptr;
@@ -3538,6 +3567,10 @@ static int sc_create_impl(sc_ctx** out, int device) {
         return SC_EHIP;
     }
     hipMemset(ctx->d_flag, 0, sizeof(int));
+    if (scratch_lookup(device, &ctx->noalloc) != hipSuccess) {
+        sc_destroy_impl(ctx);
+        return SC_EHIP;
+    }
     *out = ctx;
     return SC_OK;
 }
@@ -4165,6 +4198,49 @@ static int sc_pair_energies_impl(sc_ctx* ctx, int m, const int32_t* i, const int
 
 int sc_pair_energies(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64) {
     SC_GUARD(ctx, sc_pair_energies_impl(ctx, m, i, j, out, fp64))
+}
+
+// sc_pair_energies through this device's `scratch` instead of the context's
+// staging buffers: no allocation of any kind.  The kernel and its results
+// are those of sc_pair_energies.  Calls that overlap on one GPU share the
+// staging and race; nothing in it needs clearing (a call reads back only
+// what its kernel wrote).
+static int sc_pair_energies_noalloc_impl(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out,
+                                         int fp64) {
+    if (!ctx || m < 0 || (m > 0 && (!i || !j || !out))) return SC_EINVAL;
+    if (ctx->device < 0) return SC_ENODEV;
+    if (m > SC_NOALLOC_MAX_PAIRS)
+        return fail(ctx, SC_ECAPACITY, "sc_pair_energies_noalloc: " + std::to_string(m) + " pairs exceed " +
+                                           std::to_string(SC_NOALLOC_MAX_PAIRS));
+    if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
+    for (int k = 0; k < m; k++)
+        if (i[k] < 0 || i[k] >= ctx->n || j[k] < 0 || j[k] >= ctx->n) return fail(ctx, SC_EINVAL, "particle index out of range");
+    if (m == 0) return SC_OK;
+    if (!ctx->noalloc) return fail(ctx, SC_ESTATE, "no staging address for this device");
+    HIPCK(hipSetDevice(ctx->device));
+    int* di = (int*)ctx->noalloc;
+    int* dj = di + SC_NOALLOC_MAX_PAIRS;
+    double* dout = (double*)(ctx->noalloc + (size_t)SC_NOALLOC_MAX_PAIRS * 8);
+    HIPCK(hipMemcpyAsync(di, i, m * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipMemcpyAsync(dj, j, m * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    SysConst sc = sysconst(ctx, ctx->box);
+    int c = ctx->cur;
+    if (fp64)
+        hipLaunchKernelGGL(k_pairs<double>, dim3((m + 127) / 128), dim3(128), 0, ctx->stream, m, di, dj, ctx->slot, sc,
+                           ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp64, ctx->d_tp64, ctx->sqmaxcut, chain_tab(ctx),
+                           dout);
+    else
+        hipLaunchKernelGGL(k_pairs<float>, dim3((m + 127) / 128), dim3(128), 0, ctx->stream, m, di, dj, ctx->slot, sc,
+                           ctx->P[c], ctx->O[c], ctx->Q[c], ctx->d_pp32, ctx->d_tp32, (float)ctx->sqmaxcut,
+                           chain_tab(ctx), dout);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync(out, dout, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCK(hipStreamSynchronize(ctx->stream));
+    return SC_OK;
+}
+
+int sc_pair_energies_noalloc(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j, double* out, int fp64) {
+    SC_GUARD(ctx, sc_pair_energies_noalloc_impl(ctx, m, i, j, out, fp64))
 }
 
 static int sc_energy_all_impl(sc_ctx* ctx, double* e) {

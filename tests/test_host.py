@@ -30,6 +30,12 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) <= set(sc_amd.SIGNATURES)
 
 
+def test_binding_constants_match_the_header():
+    txt = open(os.path.join(ROOT, "include", "sc_gpu.h")).read()
+    m = re.search(r"#define SC_NOALLOC_MAX_PAIRS (\d+)", txt)
+    assert m and int(m.group(1)) == sc_amd.NOALLOC_MAX_PAIRS
+
+
 def test_topology_parser_matches_reference_params():
     g = json.load(open(os.path.join(GOLD, "topo_golden.json")))
     h = sc_amd.GpuSystem(-1)
