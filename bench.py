@@ -48,6 +48,63 @@ def read_traffic(name):
     return None
 
 
+def host_cpu():
+    """model name and CPU counts of this host (the CPU baseline's cores)"""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count()
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "affinity": aff}
+
+
+def reference_calibration(nsweeps=300, binary=None):
+    """The compiled reference's own sequential program (oracle/_ref/scref,
+    built from its unmodified sources with -O2 -ffp-contract=off) against the
+    oracle's sampler on the same 1,000-PSC state (the plain NVT system: eta
+    0.2, T* 1, 0.3 sigma moves, <= 10 deg rotations), one core each, on this
+    host: the speed of the port relative to the reference.  None when the
+    binary is absent."""
+    import subprocess
+    import tempfile
+    import numpy as np
+    import sc_amd
+    from sc_amd import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    binary = os.path.abspath(binary or po.REF_BIN)
+    if not os.path.exists(binary):
+        return None
+    w = W.workload("c1_psc_1000")
+    h = sc_amd.GpuSystem(-1)
+    h.load_topology(w["top"])
+    pos, d, q = h.generate_rsa(w["types"], w["box"], seed=31)
+    tp, st = po.Topo(w["top"]), po.State(w["box"], pos, d, q, w["types"])
+    t0 = time.perf_counter()
+    po.mc_run(tp, st, nsweeps, 1.0, 0.3, 20.0 / 180 * np.pi / 4, seed=17, energies=False)
+    port = nsweeps / (time.perf_counter() - t0)
+    with tempfile.TemporaryDirectory() as dd:
+        open(os.path.join(dd, "options"), "w").write(
+            W.options_text(nsweeps=nsweeps, transmx=0.15, rotmx=20.0, temper=1.0, seed=4321))
+        open(os.path.join(dd, "top.init"), "w").write(w["top"])
+        open(os.path.join(dd, "config.init"), "w").write(W.config_text(w["box"], pos, d, q))
+        r = subprocess.run([binary, "run", os.path.join(dd, "wall.txt")], cwd=dd, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, timeout=600)
+        if r.returncode != 0:
+            return None
+        ref = nsweeps / float(open(os.path.join(dd, "wall.txt")).read().split()[0])
+    return {"workload": "1,000 PSC, eta 0.2, T* 1 (the plain NVT system), same seeded state", "sweeps": nsweeps,
+            "cores": 1, "reference_sweeps_per_s": ref, "port_sweeps_per_s": port, "reference_over_port": ref / port,
+            "reference_build": "scOOP sources, g++ -O2 -ffp-contract=off -fno-fast-math (TotalEMatrix, pair list)"}
+
+
 def npt_leg(a, transmx, rot_half):
     """NPT self-assembly of bonded chains: SPA head + two PSC tails, harmonic
     bonds, P* = 0.5, T* = 1.1, one isotropic volume move (two whole-system
@@ -60,6 +117,7 @@ def npt_leg(a, transmx, rot_half):
     gn = sc_amd.GpuSystem(0)
     gn.set_block_waves(a.waves)
     gn.set_trial_mode(a.trial_mode)
+    gn.set_visit_factor(a.visit)
     P_, T_ = 0.5, 1.1
     nch = a.npt_chains
     boxc, posc, dc, qc, tc_ = W.chain_lattice(nch, 0.25, a.seed)
@@ -158,8 +216,9 @@ def main():
                     help="waves per active cell in the sweep kernels (1, 2, 4; 0: 4, and 2 with the network screen)")
     ap.add_argument("--trial-mode", type=int, default=1,
                     help="trials per active cell: 0 = its occupancy, 1 = dithered mean occupancy")
-    ap.add_argument("--visit", type=int, default=1,
-                    help="visit factor: a sweep visits 1/v of the colours with v times the trials per cell")
+    ap.add_argument("--visit", type=int, default=8,
+                    help="visit factor: a sweep visits 1/v of the colours with v times the trials per cell "
+                         "(8: one colour per sweep, 8 x the mean occupancy per active cell)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2025)
@@ -173,6 +232,8 @@ def main():
     ap.add_argument("--pt-sweeps", type=int, default=100)
     ap.add_argument("--equil", type=int, default=4000, help="untimed exact sweeps before the legs")
     ap.add_argument("--no-npt", action="store_true", help="skip the NPT chain leg")
+    ap.add_argument("--no-psc", action="store_true", help="skip the 65,536-PSC leg")
+    ap.add_argument("--psc-equil", type=int, default=1000, help="untimed exact sweeps of the PSC leg")
     ap.add_argument("--npt-chainprob", type=float, default=0.1,
                     help="chain-move probability per step of the NPT leg's chain-move run (0: no such run)")
     ap.add_argument("--npt-chainmmx", type=float, default=0.1, help="options' chainmmx (displacement, doubled)")
@@ -292,6 +353,41 @@ def main():
         da = leg("delayed-acceptance, network screen", 1)
     main_leg = da if da is not None else exact
 
+    # the same two sweeps on 65,536 one-patch PSC (L = 3, eta 0.25), the
+    # system the metric's name quotes
+    pscres = None
+    if not a.no_psc:
+        from sc_amd import surrogate
+        wp = W.workload("psc_65536")
+        gq = sc_amd.GpuSystem(0)
+        gq.set_trial_mode(a.trial_mode)
+        gq.set_block_waves(a.waves)
+        prepare.initial_state(gq, wp, seed=a.seed + 7)
+        gq.set_visit_factor(a.visit)
+        gq.sweep(a.psc_equil, temper, transmx, rot_half, r=a.r, seed=a.seed + 8, counter=5 * 10 ** 6)
+        bp = gq.get_config()
+        pscres = {"workload": "psc_65536", "n": len(wp["types"]), "eta": wp["eta"], "box": float(wp["box"][0]),
+                  "equilibration_sweeps": a.psc_equil}
+        for label, mode in (("exact", 0), ("delayed_acceptance", 1)):
+            gq.set_config(*bp, wp["types"])
+            if mode == 1:
+                pscres["surrogate_training"] = surrogate.train(gq, int(wp["types"][0]), nsamples=a.samples,
+                                                               epochs=a.epochs, seed=a.seed + 9, log=log)
+                gq.set_config(*bp, wp["types"])
+            gq.set_surrogate(mode)
+            gq.sweep(a.warmup, temper, transmx, rot_half, r=a.r, seed=a.seed, counter=1)
+            t1 = time.perf_counter()
+            st = gq.sweep(a.steps, temper, transmx, rot_half, r=a.r, seed=a.seed, counter=10 ** 6)
+            dt = time.perf_counter() - t1
+            tries = st["trans_try"] + st["rot_try"]
+            pscres[label] = {"sweeps_per_s": tries / float(len(wp["types"])) / dt,
+                             "acceptance": (st["trans_acc"] + st["rot_acc"]) / max(1, tries),
+                             "energy_per_particle": gq.allToAll() / len(wp["types"])}
+            log("psc 65536 %s: %.2f sweeps/s, acceptance %.4f" % (label, pscres[label]["sweeps_per_s"],
+                                                                  pscres[label]["acceptance"]))
+        pscres["da_over_exact"] = pscres["delayed_acceptance"]["sweeps_per_s"] / pscres["exact"]["sweeps_per_s"]
+        gq.close()
+
     # batched parallel tempering: 64 replicas of uneven size in one context,
     # autotuned launch shape, exchanges every 10 sweeps
     ptres = None
@@ -337,7 +433,12 @@ def main():
         # one independent replica per host core (the oracle's C sampler
         # releases the GIL), each running k sequential Metropolis sweeps
         from concurrent.futures import ThreadPoolExecutor
-        cores = max(1, min(16, os.cpu_count() or 1))
+        # one replica per core of this process's CPU share: the host's
+        # affinity set, capped by the share a one-GPU job gets on these
+        # boxes (OMP_NUM_THREADS, 16)
+        hc = host_cpu()
+        share = int(os.environ.get("OMP_NUM_THREADS") or 16)
+        cores = max(1, min(share, hc["affinity"] or 1))
 
         def replica(i):
             t = time.perf_counter()
@@ -351,8 +452,16 @@ def main():
         agg = cores * k / el
         cpu = {"value": agg, "unit": "sweeps/s", "cores": cores, "kind": "port",
                "per_core": k / float(np.mean(per)),
-               "sample": "%d independent replicas (one per host thread) x %d sequential Metropolis sweeps "
-                         "(FP64 oracle) of the same %d-particle state; value = aggregate" % (cores, k, n)}
+               "sample": "%d independent replicas (one per core of the CPU share) x %d sequential Metropolis sweeps "
+                         "(FP64 oracle, cell list) of the same %d-particle state; value = aggregate" % (cores, k, n),
+               "host": dict(hc, cpu_share=share),
+               "note": "the compiled reference cannot run this system: its TotalEMatrix holds two N(N-1)/2 FP64 "
+                       "matrices (69 GB at N=65,536 per process)"}
+        cal = reference_calibration()
+        if cal:
+            cpu["reference_calibration"] = cal
+            log("reference calibration: reference %.1f, port %.1f sweeps/s (1 core, 1,000 PSC)" % (
+                cal["reference_sweeps_per_s"], cal["port_sweeps_per_s"]))
         log("cpu baseline: %.4f sweeps/s aggregate over %d threads (%.4f per core; %d sweeps each in %.1fs)" % (
             agg, cores, cpu["per_core"], k, el))
 
@@ -372,7 +481,8 @@ def main():
         "config": {"workload": a.workload, "n": n, "eta": w["eta"], "box": float(w["box"][0]), "temper": temper,
                    "translation": transmx, "max_rotation_deg": 10.0,
                    "checkerboard": "%dx%dx%d" % ((a.r + 1,) * 3), "cells": list(nc),
-                   "sweep": "delayed-acceptance" if da else "exact", "trial_mode": a.trial_mode},
+                   "sweep": "delayed-acceptance" if da else "exact", "trial_mode": a.trial_mode,
+                   "visit_factor": a.visit},
         "exact_sweeps_per_s": exact["sweeps_per_s"] if exact else None,
         "da_sweeps_per_s": da["sweeps_per_s"] if da else None,
         "da_over_exact": (da["sweeps_per_s"] / exact["sweeps_per_s"]) if (da and exact) else None,
@@ -386,6 +496,7 @@ def main():
         "surrogate_training": train,
         "batched_tempering": ptres,
         "npt_chains": npt,
+        "psc_65536": pscres,
         "roofline": main_leg["roofline"],
         "cpu_baseline": cpu,
     }

@@ -169,8 +169,10 @@ int sc_energy_all_trial(sc_ctx* ctx, const double box_new[3], double* e);
 int sc_commit_box(sc_ctx* ctx, const double box_new[3]);
 
 /* --- hard-core overlap ---------------------------------------------------- */
-/* overlap(i,j) <=> closest segment (or sphere) distance < sigma_ij.  FP32 with
- * an FP64 recheck when the distance is within 1e-6 sigma of contact.
+/* overlap(i,j) <=> closest segment (or sphere) distance < sigma_ij.  FP32,
+ * with the distance recomputed in FP64 when the FP32 one lies within 1e-5
+ * sigma of contact (ten times the FP32 error at stencil distances), so flags
+ * match an FP64 evaluation except within FP64 rounding of contact.
  * out[k] = 1 on overlap; dmin[k] (optional) = the distance.  Build-defined:
  * the reference's Conf::overlap (structures/Conf.cpp:104-239) is not called
  * anywhere in scOOP. */
@@ -222,6 +224,13 @@ int sc_set_home_cache(sc_ctx* ctx, int cap);
  * always use 1.  Every pass is a product of independent Metropolis chains,
  * whatever v. */
 int sc_set_visit_factor(sc_ctx* ctx, int v);
+/* Largest number of active cells of one colour in sc_sweep's grid: when the
+ * grid of cells >= maxcut/r would give more, it takes coarser cells (fewer
+ * along its longest axis) until a colour has at most n, so that a pass runs
+ * all its cells at once (the sweep kernel runs two blocks per CU).  n = 0: no
+ * limit; n < 0: the default, two per CU of the device.  Coarser cells keep
+ * every property of the checkerboard (cells are still >= maxcut/r wide). */
+int sc_set_max_active_cells(sc_ctx* ctx, int n);
 /* Whole-molecule moves in sc_sweep (MoveCreator::chainMove, movecreator.cpp:
  * 305-328): besides its N particle trials, every sweep makes chainprob * N
  * molecule trials (chainprob is the per-step probability of Updater's step

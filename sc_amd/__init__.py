@@ -53,6 +53,7 @@ SIGNATURES = {
     "sc_set_block_waves": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_set_home_cache": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_set_visit_factor": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_set_max_active_cells": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_write_movie_frame": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
     "sc_write_stat_line": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
     "sc_write_energy_line": (ctypes.c_long, [_P, ctypes.c_long, ctypes.c_char_p, ctypes.c_long]),
@@ -588,6 +589,10 @@ class GpuSystem:
     def set_visit_factor(self, v):
         """a sweep visits 1/v of the colours, each with v times the trials (default 1)"""
         self._ck(self._L.sc_set_visit_factor(self._h, int(v)), "set_visit_factor")
+
+    def set_max_active_cells(self, n):
+        """at most n active cells per colour in sweep grids (0: no limit, < 0: two per CU)"""
+        self._ck(self._L.sc_set_max_active_cells(self._h, int(n)), "set_max_active_cells")
 
     def set_trial_mode(self, mode):
         self._ck(self._L.sc_set_trial_mode(self._h, int(mode)), "set_trial_mode")

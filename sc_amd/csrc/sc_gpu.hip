@@ -835,6 +835,7 @@ __global__ void k_pairs(int m, const int* __restrict__ pi, const int* __restrict
 // ---------------------------------------------------------------- overlaps
 // overlap <=> closest distance < sigma_ij; FP32 first, FP64 when within
 // 1e-6 sigma of contact
+#define OVERLAP_RECHECK 1e-5
 __global__ void k_overlap(int m, const int* __restrict__ pi, const int* __restrict__ pj, const int* __restrict__ slot,
                           SysConst sc, const uint4* __restrict__ P, const float4* __restrict__ O,
                           const float2* __restrict__ Q, const PP64* __restrict__ pp64, const TP64* __restrict__ tp64,
@@ -861,8 +862,11 @@ __global__ void k_overlap(int m, const int* __restrict__ pi, const int* __restri
         V3<R> dv = min_dist(da, db, (R)Pd.hl0, (R)Pd.hl1, r);
         return sqrt((double)dot(dv, dv));
     };
+    // FP32 distance; within OVERLAP_RECHECK sigma of contact, where the FP32
+    // error (~1e-7 of the separation, below 1e-6 sigma at stencil distances)
+    // could put it on the wrong side, the FP64 distance decides
     double d = dist_of(0.0f);
-    if (fabs(d - (double)sig) < 1e-6 * sigd + 1e-5 * sigd) d = dist_of(0.0);
+    if (fabs(d - (double)sig) < OVERLAP_RECHECK * sigd) d = dist_of(0.0);
     out[k] = (d < sigd) ? 1 : 0;
     if (dmin) dmin[k] = d;
 }
@@ -946,6 +950,7 @@ struct TrialScratch {
     int* cnt;    // block compaction counts (2 CNT_HALF ints)
     double* red; // block sums (16 doubles)
     bf16x4* hx;  // the network's activation exchange between the waves (MLP_HX_BYTES)
+    double* netd; // the network screen's energy change, from the wave that computed it
 };
 
 // A trial's proposal and its Metropolis thresholds, drawn for TRIAL_CHUNK
@@ -1029,6 +1034,36 @@ __device__ __forceinline__ void scan_stencil(const Stage& S, int k, const uint4 
     }
 }
 
+// The network's 12 features of expensive-list entry e (moving particle as
+// part1 at the entry's state, the neighbour as part2); false, with zero
+// features, for entries past nx and for pairs of other types (they screen
+// with 0).
+__device__ __forceinline__ bool pair_features(const Stage& S, const TrialScratch& T, int nx, int e, int t,
+                                              const uint4 uo, const uint4 un, const float* mvo, const float* mvn,
+                                              const SysConst& sc, const MlpConst& mc, float* f, int& which) {
+    bool net = false;
+    which = 0;
+    if (e < nx) {
+        int w = T.xl[e];
+        int jj = w & 0x3fffffff;
+        which = (w >> 30) & 1;
+        uint4 b = S.u[jj];
+        net = t == mc.tnet && (int)b.w == mc.tnet;
+        if (net) {
+            V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
+                                : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+            const float* mv = which ? mvn : mvo;
+            float4 ob = S.o[jj];
+            float2 qb = S.q[jj];
+            mlp_features(mv_vec(mv, 0), mv_vec(mv, 3), mk<float>(ob.x, ob.y, ob.z), mk<float>(ob.w, qb.x, qb.y), r,
+                         T.xd[e], mc.lsc, mc.rcut, mc.pdis, mc.pswitch, f);
+        }
+    }
+    if (!net)
+        for (int q = 0; q < MLP_NF; q++) f[q] = 0.0f;
+    return net;
+}
+
 // Delayed acceptance (Christen & Fox 2005, in stages): with DA != DA_OFF
 // the move first faces Metropolis screens on cheaper energy changes, each
 // accepting with min(1, exp(-(its change)/T)); returns false at the first
@@ -1045,6 +1080,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                                                const SysConst& sc,
                                                const PP32* __restrict__ pp, const TP32* __restrict__ tp, int ppmax,
                                                const TrialScratch& T, const float* ml, const MlpRegsW<4 / NW>& mr,
+                                               const unsigned char* __restrict__ mlp_img,
                                                const MlpConst& mc, const ChainTab& ct, double thr1, double thr2,
                                                const float* hvk, bool translate, double& eo, double& en,
                                                double& des, int& stage) {
@@ -1155,63 +1191,68 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     __syncthreads();
     // screens of the delayed acceptance: (1) the exact cheap terms already
     // summed (repulsion, sphere pairs), (2) with DA_NET the network's patch
-    // attraction, tiles of 16 pairs on MFMA, the hidden units of each tile
-    // split between the block's waves
+    // attraction, tiles of 16 pairs on MFMA.  With four waves the network runs
+    // on the last wave while the other three compute phase C, and both meet
+    // at the final sum: a move the network rejects is rejected there (its
+    // phase C result is dropped), so the screen's decisions, and the chain of
+    // states, are those of evaluating it first.  With fewer waves the
+    // network's hidden units are split between the waves and it runs before
+    // phase C.
+    constexpr bool NET_PAR = DA == DA_NET && NW == 4;
     des = 0.0;
     stage = 2;
+    double rep = 0.0;
     if constexpr (DA != DA_OFF) {
         double rn, ro;
         block_sum2<NW>(an, ao, T.red, rpar, rn, ro);
-        double rep = rn - ro;
+        rep = rn - ro;
         stage = 0;
         if (!(rep <= 0.0 || rep < thr1)) return false;
-        double ds = 0.0;
-        if constexpr (DA == DA_NET) {
+        des = rep;
+        if constexpr (DA == DA_NET && !NET_PAR) {
             stage = 1;
             double so = 0.0, sn = 0.0;
             const int wv = tid >> 6;
             for (int t0 = 0; t0 < nx; t0 += MLP_TILE) {   // block-uniform: every wave in every tile
-                int e = t0 + (lane & (MLP_TILE - 1));
                 float f[MLP_NF];
                 int which = 0;
-                bool net = false;   // a pair of the network's type; other pairs screen with 0
-                if (e < nx) {
-                    int w = T.xl[e];
-                    int jj = w & 0x3fffffff;
-                    which = (w >> 30) & 1;
-                    uint4 b = S.u[jj];
-                    net = t == mc.tnet && (int)b.w == mc.tnet;
-                    if (net) {
-                        V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
-                                            : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
-                        const float* mv = which ? mvn : mvo;
-                        float4 ob = S.o[jj];
-                        float2 qb = S.q[jj];
-                        mlp_features(mv_vec(mv, 0), mv_vec(mv, 3), mk<float>(ob.x, ob.y, ob.z),
-                                     mk<float>(ob.w, qb.x, qb.y), r, T.xd[e], mc.lsc, mc.rcut, mc.pdis, mc.pswitch,
-                                     f);
-                    }
-                }
-                if (!net)
-                    for (int q = 0; q < MLP_NF; q++) f[q] = 0.0f;
+                bool net = pair_features(S, T, nx, t0 + (lane & (MLP_TILE - 1)), t, uo, un, mvo, mvn, sc, mc, f,
+                                         which);
                 float out = mlp_tile_w<NW>(mr, ml, T.hx, mlp_slots(f, lane >> 4), lane, wv);
-                if (lane < MLP_TILE && e < nx && net) {
+                if (lane < MLP_TILE && net) {
                     if (which) sn += (double)out; else so += (double)out;
                 }
             }
             double dn, d0;
             block_sum2<NW>(sn, so, T.red, rpar, dn, d0);
-            ds = dn - d0;
+            double ds = dn - d0;
             if (!(ds <= 0.0 || ds < thr2)) return false;
+            des = rep + ds;
         }
-        des = rep + ds;
         stage = 2;
     }
     // phase C: (expensive pair, patch pair, side) items.  The two wedge
     // intersections of a patch pair run on neighbouring lanes (side = lane
     // parity) and meet through a lane swap; the even lane adds the attraction.
+    const int CB = NET_PAR ? 3 * WAVE : BS;   // threads of phase C
+    if (NET_PAR && (tid >> 6) == 3) {
+        // the network screen on the last wave: the whole network per tile,
+        // its operands read from global memory (L1/L2) layer by layer
+        double so = 0.0, sn = 0.0;
+        for (int t0 = 0; t0 < nx; t0 += MLP_TILE) {
+            float f[MLP_NF];
+            int which = 0;
+            bool net = pair_features(S, T, nx, t0 + (lane & (MLP_TILE - 1)), t, uo, un, mvo, mvn, sc, mc, f, which);
+            float out = mlp_tile_g(mlp_img, ml, mlp_slots(f, lane >> 4), lane);
+            if (lane < MLP_TILE && net) {
+                if (which) sn += (double)out; else so += (double)out;
+            }
+        }
+        double ds = wave_sum_dpp(sn) - wave_sum_dpp(so);
+        if (lane == 0) *T.netd = ds;
+    } else {
     int items = nx * ppmax * 2;
-    for (int base = 0; base < items; base += BS) {
+    for (int base = 0; base < items; base += CB) {
         int it = base + tid, which = 0, k1 = 0, k2 = 0, side = it & 1;
         bool hit = false, pair_item = false;
         float x1 = 0.0f, x2 = 0.0f, ev = 0.0f;
@@ -1242,13 +1283,21 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                 ev = spsc(*P2, mv_part(mv, t, tp[t].geotype), pb, r);
             }
         }
-        // the other side's result (lane ^ 1 holds item it ^ 1: BS is even)
+        // the other side's result (lane ^ 1 holds item it ^ 1: CB is even)
         float y1 = __shfl_xor(x1, 1), y2 = __shfl_xor(x2, 1);
         int oh = __shfl_xor(hit ? 1 : 0, 1);
         if (pair_item && side == 0 && hit && oh) ev = patch_atr<float>(*P2, va, vb, r, k1, k2, x1, x2, y1, y2);
         if (which) an += (double)ev; else ao += (double)ev;
     }
+    }
     block_sum2<NW>(ao, an, T.red, rpar, eo, en);
+    if constexpr (NET_PAR) {
+        stage = 1;
+        double ds = *T.netd;   // written before the sum's barrier
+        if (!(ds <= 0.0 || ds < thr2)) return false;
+        des = rep + ds;
+        stage = 2;
+    }
     return true;
 }
 
@@ -1289,13 +1338,19 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
     float* ml = nullptr;
     MlpRegsW<4 / NW> mr;   // this wave's share of the network's operands
     T.hx = nullptr;
+    T.netd = nullptr;
     uintptr_t tail = (uintptr_t)(hv + HCAP * MV_STRIDE);
     if constexpr (DA == DA_NET) {
         ml = (float*)((tail + 15) & ~(uintptr_t)15);
-        mlp_setup_w<4 / NW>(mlp_img, ml, lane, (tid >> 6) * (4 / NW), mr);
+        if constexpr (NW == 4)   // the last wave evaluates whole tiles, operands from global memory
+            for (int i = lane; i < MLP_C_BYTES / 16; i += 64) ((uint4*)ml)[i] = ((const uint4*)(mlp_img + MLP_C_OFF))[i];
+        else
+            mlp_setup_w<4 / NW>(mlp_img, ml, lane, (tid >> 6) * (4 / NW), mr);
         tail = (uintptr_t)ml + MLP_LDS_BYTES;
         T.hx = (bf16x4*)((tail + 15) & ~(uintptr_t)15);
         tail = (uintptr_t)T.hx + MLP_HX_BYTES;
+        T.netd = (double*)tail;
+        tail += 16;
     }
 #if SWEEP_TABLES_IN_LDS
     // the pair and type parameter tables in LDS: every pair term reads them,
@@ -1421,7 +1476,8 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         bool pass;
         int stage;
         const float* hvk = hcache ? hv + (k - S.h0) * MV_STRIDE : nullptr;
-        pass = trial_energies<DA, NW>(S, k, uo, un, oo, qo, on, qn, sc, pp, tp, ppmax, T, ml, mr, mc, ct, R.thr1,
+        pass = trial_energies<DA, NW>(S, k, uo, un, oo, qo, on, qn, sc, pp, tp, ppmax, T, ml, mr, mlp_img, mc, ct,
+                                      R.thr1,
                                       R.thr2, hvk, translate, eo, en, des, stage);
         if (stage < 0) {   // a chain partner outside the stencil
             n_bond++;
@@ -3060,6 +3116,8 @@ struct sc_ctx {
     int home_cache = HCAP;              // cell size limit of the sweep's derived-vector cache (0: off)
     int visit = 1;                      // colour passes of a sweep = colours / visit, visit x the trials each
     unsigned char* noalloc = nullptr;   // this device's address of `scratch` (sc_pair_energies_noalloc)
+    int max_active = 0;                 // sweep grids: at most this many active cells per colour (0: no limit)
+    int max_active_default = 0;         // two per CU of the device
     // whole-molecule moves of sc_sweep
     std::vector<int> chain_first, chain_size;   // molecules of more than one particle
     int nchain_dev = 0;                 // molecules uploaded below
@@ -3299,12 +3357,13 @@ static SysConst sysconst(const sc_ctx* ctx, const double* box) {
 // --- cell list --------------------------------------------------------------
 // Cell list with cells at least wmin wide; with period p > 1 the cell count
 // per axis is a multiple of p (the checkerboard's colour period) or 1.
-static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box, double wmin, int p);
+static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box, double wmin, int p,
+                        int cap_active = 0);
 
-static int build_grid(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box) {
+static int build_grid(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box, int cap_active = 0) {
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     if (r < 1 || r > 2) return fail(ctx, SC_EINVAL, "stencil radius must be 1 or 2");
-    return build_grid_w(ctx, r, shift, box, ctx->maxcut / r, r + 1);
+    return build_grid_w(ctx, r, shift, box, ctx->maxcut / r, r + 1, cap_active);
 }
 
 // cells per axis of a grid with cells >= wmin wide and colour period p
@@ -3317,7 +3376,8 @@ static int grid_axis_cells(double edge, double wmin, int p) {
     return nc;
 }
 
-static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box, double wmin, int p) {
+static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const double* box, double wmin, int p,
+                        int cap_active) {
     if (!ctx->have_state) return fail(ctx, SC_ESTATE, "no configuration loaded");
     Grid g;
     g.r = r;
@@ -3325,6 +3385,20 @@ static int build_grid_w(sc_ctx* ctx, int r, const uint32_t shift[3], const doubl
         g.nc[k] = grid_axis_cells(box[k], wmin, p);
         g.sh[k] = shift ? shift[k] : 0u;
     }
+    // with cap_active > 0, coarser cells (fewer along the axis with the most)
+    // until a colour has at most cap_active cells: a pass whose active cells
+    // all fit the GPU at once beats one that runs them in two rounds
+    if (cap_active > 0)
+        for (;;) {
+            long act = 1;
+            int kmax = -1;
+            for (int k = 0; k < 3; k++) {
+                act *= g.nc[k] >= p ? g.nc[k] / p : 1;
+                if (g.nc[k] >= 2 * p && (kmax < 0 || g.nc[k] > g.nc[kmax])) kmax = k;
+            }
+            if (act <= cap_active || kmax < 0) break;
+            g.nc[kmax] -= p;
+        }
     int ncell = g.nc[0] * g.nc[1] * g.nc[2];
     if (ncell > ctx->ncell_alloc) {
         int rc;
@@ -3567,6 +3641,12 @@ static int sc_create_impl(sc_ctx** out, int device) {
         return SC_EHIP;
     }
     hipMemset(ctx->d_flag, 0, sizeof(int));
+    // two blocks of the sweep per CU (their register budget): a colour of at
+    // most two active cells per CU runs in one round
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->max_active = 2 * cus;
+    ctx->max_active_default = ctx->max_active;
     if (scratch_lookup(device, &ctx->noalloc) != hipSuccess) {
         sc_destroy_impl(ctx);
         return SC_EHIP;
@@ -4785,7 +4865,7 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
                                ctx->d_pp32, ctx->d_tp32, ctx->d_vol, ctx->d_cfirst, ctx->d_csize, ctx->d_cstart,
                                ctx->d_clist, ctx->d_ccell_id, ctx->d_stamp, ctx->d_stats, ctx->d_dsum, ctx->d_flag);
         HIPCK(hipGetLastError());
-        if ((rc = build_grid(ctx, r, shift, ctx->box))) return rc;
+        if ((rc = build_grid(ctx, r, shift, ctx->box, ctx->max_active))) return rc;
     }
     return SC_OK;
 }
@@ -4809,7 +4889,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         SweepPlan plan;
         sweep_plan(seed, sw, p, plan);
         const uint32_t* shift = plan.shift;
-        int rc = build_grid(ctx, r, shift, ctx->box);
+        int rc = build_grid(ctx, r, shift, ctx->box, ctx->max_active);
         if (rc) return rc;
         const Grid& g = ctx->g;
         int cap = stage_cap(ctx, g, ctx->box);
@@ -4823,7 +4903,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         // + stencil scratch (+ the network and its activation tile)
         // (+ chain partners, block compaction counts, block sums, trial records)
         size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + SWEEP_SMALL_LDS +
-                       (da ? MLP_LDS_BYTES + MLP_HX_BYTES + 32 : 0) +
+                       (da ? MLP_LDS_BYTES + MLP_HX_BYTES + 48 : 0) +
                        sweep_table_bytes(ctx->nt);
         size_t per = stage_bytes(1) + sizeof(int) * 3 + (da ? sizeof(float) * 2 : 0);
         int capmax = (int)((SWEEP_LDS_BUDGET - fixed) / per);
@@ -5851,6 +5931,16 @@ static int sc_set_home_cache_impl(sc_ctx* ctx, int cap) {
 
 int sc_set_home_cache(sc_ctx* ctx, int cap) {
     SC_GUARD(ctx, sc_set_home_cache_impl(ctx, cap))
+}
+
+static int sc_set_max_active_cells_impl(sc_ctx* ctx, int n) {
+    if (!ctx) return SC_EINVAL;
+    ctx->max_active = n < 0 ? ctx->max_active_default : n;
+    return SC_OK;
+}
+
+int sc_set_max_active_cells(sc_ctx* ctx, int n) {
+    SC_GUARD(ctx, sc_set_max_active_cells_impl(ctx, n))
 }
 
 static int sc_set_visit_factor_impl(sc_ctx* ctx, int v) {

@@ -177,6 +177,20 @@ __device__ __forceinline__ void mlp_setup_w(const unsigned char* __restrict__ im
 __device__ __forceinline__ void mlp_setup(const unsigned char* __restrict__ img, float* lds, int lane, MlpRegs& R) {
     mlp_setup_w<4>(img, lds, lane, 0, R);
 }
+// this lane's operands of all four unit blocks from the packed image in
+// global memory (the constants stay where mlp_setup put them)
+__device__ __forceinline__ void mlp_load_ops(const unsigned char* __restrict__ img, int lane, MlpRegs& R) {
+    const bf16x8* a1 = (const bf16x8*)(img + MLP_A1_OFF);
+    const bf16x8* a2 = (const bf16x8*)(img + MLP_A2_OFF);
+    const bf16x8* a3 = (const bf16x8*)(img + MLP_A3_OFF);
+#pragma unroll
+    for (int b = 0; b < 4; b++) R.a1[b] = a1[b * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        R.a2[q] = a2[q * 64 + lane];
+        R.a3[q] = a3[q * 64 + lane];
+    }
+}
 
 // SiLU for inference: hardware exp2 and reciprocal
 __device__ __forceinline__ float silu_fast(float z) {
@@ -262,6 +276,47 @@ __device__ __forceinline__ float mlp_tile_w(const MlpRegsW<4 / NW>& R, const flo
 // the whole network on one wave
 __device__ __forceinline__ float mlp_tile(const MlpRegs& R, const float* cst, bf16x8 x, int lane) {
     return mlp_tile_w<1>(R, cst, nullptr, x, lane, 0);
+}
+
+// The whole network on one wave with the operands read from the packed
+// image in global memory layer by layer (at most the 32 VGPRs of one hidden
+// layer live at a time, not all 80): for a wave that evaluates tiles while
+// its register budget is shared with other work.
+__device__ __forceinline__ float mlp_tile_g(const unsigned char* __restrict__ img, const float* cst, bf16x8 x,
+                                            int lane) {
+    const int kg = lane >> 4;
+    const bf16x8* a1 = (const bf16x8*)(img + MLP_A1_OFF);
+    floatx4 z = {0.0f, 0.0f, 0.0f, 0.0f};
+    floatx4 c[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1[b * 64 + lane], x, z, 0, 0, 0);
+#pragma unroll
+    for (int layer = 0; layer < 2; layer++) {
+        // nothing moves across: a layer's operand loads stay after the
+        // previous layer (the scheduler would otherwise hoist all 80 VGPRs)
+        __builtin_amdgcn_sched_barrier(0);
+        bf16x8 h0 = mlp_join(mlp_act4(c[0]), mlp_act4(c[1]));
+        bf16x8 h1 = mlp_join(mlp_act4(c[2]), mlp_act4(c[3]));
+        const bf16x8* a = (const bf16x8*)(img + (layer == 0 ? MLP_A2_OFF : MLP_A3_OFF));
+        const floatx4* cb = (const floatx4*)(cst + layer * 64 + kg * 16);
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            c[b] = cb[b];
+            c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[(2 * b) * 64 + lane], h0, c[b], 0, 0, 0);
+            c[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[(2 * b + 1) * 64 + lane], h1, c[b], 0, 0, 0);
+        }
+    }
+    const floatx4* w4 = (const floatx4*)(cst + 128 + kg * 16);
+    float part = 0.0f;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        floatx4 w = w4[b];
+#pragma unroll
+        for (int r = 0; r < 4; r++) part += silu_fast(c[b][r]) * w[r];
+    }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    return part + cst[192];
 }
 
 // BF16 feature slice of this lane from the 12 features of its sample

@@ -12,8 +12,10 @@ import numpy as np
 
 def initial_state(g, w, seed=1, eta_rsa=0.18, steps=12, sweeps_per_step=4, relax_sweeps=20, temper=1.0,
                   transmx=0.3, rot_half=20.0 / 180 * np.pi / 4, log=None):
-    """Load workload `w` (sc_amd.workloads.workload) into GpuSystem `g`."""
+    """Load workload `w` (sc_amd.workloads.workload) into GpuSystem `g`.
+    The compression sweeps visit every colour (visit factor 1, left set)."""
     g.load_topology(w["top"])
+    g.set_visit_factor(1)
     box_t = np.asarray(w["box"], np.float64)
     eta = w["eta"]
     if eta <= eta_rsa:
