@@ -24,10 +24,25 @@
 // fractions, dir, patchdir[0], type; geo.box; pvec.size()).  Errors are
 // reported on stderr and return 0.0 / leave the state, as the reference's
 // functors do for unprogrammed pairs (paire.h:207-212).
+//
+// The rest of TotalE's interface (totalenergycalculator.h:135-296) is there
+// so that the whole program compiles with this class at the typedef: the
+// (Sim*, Conf*) constructor Updater uses (updater.h:17; it reads top.init
+// from the working directory, as the Inicializer does), update(EMResize)
+// after grand-canonical insertions and deletions (the configuration is
+// uploaded again), and pairE.pbc / pairE.verbose, which chainRotate reads
+// (movecreator.cpp:1155-1222).  The energies of particles outside pvec
+// (p2p(Particle*, int), mol2others(vector<Particle>&), chainInner) belong to
+// the cluster and grand-canonical moves, which this backend does not cover:
+// they report that on stderr and return 0.
 #pragma once
 
 #include <cstdio>
+#include <fstream>
+#include <sstream>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "sc_gpu.h"
@@ -39,12 +54,18 @@ class GpuTotalE {
     // the same pair table, Topo::genParamPairs / genTopoParams)
     GpuTotalE(SimT* sim, ConfT* conf, const std::string& top_init_text, int device = 0)
         : sim_(sim), conf_(conf) {
-        if (sc_create(&ctx_, device) != SC_OK) {
-            fprintf(stderr, "GpuTotalE: no usable gfx950 device %d\n", device);
-            ctx_ = nullptr;
-            return;
-        }
-        check(sc_load_topology(ctx_, top_init_text.c_str()), "sc_load_topology");
+        pairE.pbc = &conf->geo;
+        open(top_init_text, device);
+    }
+    // the constructor of the typedef's users (Updater, updater.h:17): the
+    // topology from top.init in the working directory (FileNames, files.h)
+    GpuTotalE(SimT* sim, ConfT* conf) : sim_(sim), conf_(conf) {
+        pairE.pbc = &conf->geo;
+        std::ifstream in("top.init");
+        std::stringstream text;
+        text << in.rdbuf();
+        if (!in) fprintf(stderr, "GpuTotalE: cannot read top.init\n");
+        open(text.str(), 0);
     }
     ~GpuTotalE() { sc_destroy(ctx_); }
     GpuTotalE(const GpuTotalE&) = delete;
@@ -93,10 +114,17 @@ class GpuTotalE {
         double box[3] = {conf_->geo.box.x, conf_->geo.box.y, conf_->geo.box.z};
         check(sc_commit_box(ctx_, box), "sc_commit_box");
     }
+    // TotalEMatrix::update(EMResize) (:322-324): the particle count changed
+    // (grand-canonical moves): upload the configuration again
+    template <typename R>
+    typename std::enable_if<std::is_empty<R>::value>::type update(const R&) {
+        initEM();
+    }
     // TotalEMatrix::mol2others (:417-453); a Molecule (structures.h:15, a
     // vector<int>) lists consecutive particle indices
     template <typename MolT>
-    double mol2others(MolT& mol) {
+    typename std::enable_if<std::is_convertible<decltype(std::declval<MolT&>()[0]), int>::value, double>::type
+    mol2others(MolT& mol) {
         double e = 0;
         check(sc_energy_mol(ctx_, mol[0], (int)mol.size(), &e), "sc_energy_mol");
         return e;
@@ -115,7 +143,7 @@ class GpuTotalE {
     }
     // TotalEMatrix::update(Molecule) (:330-332)
     template <typename MolT>
-    void update(const MolT& mol) {
+    auto update(const MolT& mol) -> decltype((void)mol.size(), void()) {
         int m = (int)mol.size();
         std::vector<double> pos(3 * m), dir(3 * m), pd0(3 * m);
         for (int a = 0; a < m; a++) read(mol[a], &pos[3 * a], &dir[3 * a], &pd0[3 * a]);
@@ -128,9 +156,46 @@ class GpuTotalE {
         check(sc_pair_energies(ctx_, 1, &i, &j, &e, 0), "sc_pair_energies");
         return e;
     }
+    // TotalE::p2p(Particle*, int), mol2others(vector<Particle>&) and
+    // chainInner (:204-296): particles outside pvec (cluster and
+    // grand-canonical moves), not covered
+    template <typename P>
+    double p2p(P* part1, int part2) {
+        (void)part1; (void)part2;
+        return unsupported("p2p(Particle*, int)");
+    }
+    template <typename PV>
+    typename std::enable_if<!std::is_convertible<decltype(std::declval<PV&>()[0]), int>::value, double>::type
+    mol2others(PV& particles) {
+        (void)particles;
+        return unsupported("mol2others(vector<Particle>&)");
+    }
+    template <typename C>
+    double chainInner(C& chain) {
+        (void)chain;
+        return unsupported("chainInner");
+    }
     sc_ctx* context() { return ctx_; }
 
+    // what chainRotate reads of TotalE::pairE (movecreator.cpp:1155-1222)
+    struct PairView {
+        bool verbose = false;
+        decltype(&std::declval<ConfT&>().geo) pbc = nullptr;
+    } pairE;
+
   private:
+    void open(const std::string& top_init_text, int device) {
+        if (sc_create(&ctx_, device) != SC_OK) {
+            fprintf(stderr, "GpuTotalE: no usable gfx950 device %d\n", device);
+            ctx_ = nullptr;
+            return;
+        }
+        check(sc_load_topology(ctx_, top_init_text.c_str()), "sc_load_topology");
+    }
+    double unsupported(const char* what) const {
+        fprintf(stderr, "GpuTotalE: %s is not covered by the GPU backend (cluster / grand-canonical moves)\n", what);
+        return 0.0;
+    }
     void read(int t, double* pos, double* dir, double* pd0) const {
         const auto& p = conf_->pvec[t];
         pos[0] = p.pos.x; pos[1] = p.pos.y; pos[2] = p.pos.z;

@@ -128,3 +128,25 @@ def test_philox_known_answers(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout
     assert r.stdout.count("ok ") == 5
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/scOOP/mc/totalenergycalculator.h"),
+                    reason="reference sources absent")
+def test_reference_program_builds_with_the_gpu_backend_at_its_typedef():
+    """The reference's whole program (scOOP 4.0, unmodified sources) compiles
+    and links with GpuTotalE<Sim, Conf> as its TotalEnergyCalculator
+    (mc/totalenergycalculator.h:1227), i.e. against the real Particle, Conf,
+    Molecule, EMResize and the calls of Updater, MoveCreator and
+    ClusterSampler (oracle/Makefile, target dropin)."""
+    import subprocess
+    r = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-j8", "dropin"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    hdr = open(os.path.join(ROOT, "oracle", "_ref", "dropin", "src", "mc", "totalenergycalculator.h")).read()
+    assert "typedef GpuTotalE<Sim, Conf> TotalEnergyCalculator;" in hdr
+    assert "typedef TotalEMatrix<PairE> TotalEnergyCalculator;" not in hdr
+    ref = open("/root/reference/scOOP/mc/totalenergycalculator.h").read()
+    # everything else is the reference's text, line for line
+    a = [ln for ln in ref.split("\n") if "typedef TotalEMatrix<PairE> TotalEnergyCalculator;" not in ln]
+    b = [ln for ln in hdr.split("\n") if "GpuTotalE" not in ln and "sc_gpu_totale.hpp" not in ln]
+    assert a == b
+    assert os.access(os.path.join(ROOT, "oracle", "_ref", "scdropin"), os.X_OK)
