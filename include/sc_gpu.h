@@ -265,6 +265,12 @@ int sc_chain_pass_info(const sc_ctx* ctx, int* period, int* staged);
  * other change of the state; E_new is always evaluated in full. */
 int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
                    uint64_t counter, int* accepted, double box_out[3]);
+/* The whole-system energy the volume moves take as E_old: the last full
+ * evaluation plus the energy changes the sweeps accepted since (the role of
+ * the reference's cached pair energies in allToAll, totalenergycalculator.h:
+ * 338-353).  age: volume moves since that evaluation (a new one every 16).
+ * SC_ESTATE when no volume move has run since the state last changed. */
+int sc_carried_energy(sc_ctx* ctx, double* e, int* age);
 
 /* --- learned surrogate and delayed acceptance ------------------------------ */
 /* No counterpart in the reference, which evaluates every trial move with the

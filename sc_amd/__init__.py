@@ -89,6 +89,7 @@ SIGNATURES = {
     "sc_overlap_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_long)]),
     "sc_sweep": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(SweepStats)]),
+    "sc_carried_energy": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_int)]),
     "sc_volume_move": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), _D]),
     "sc_generate_rsa": (ctypes.c_int, [_P, ctypes.c_int, _I, _D, ctypes.c_uint64, ctypes.c_long, _D, _D, _D]),
@@ -445,6 +446,13 @@ class GpuSystem:
                                         int(seed) & 0xFFFFFFFFFFFFFFFF, int(counter) & 0xFFFFFFFFFFFFFFFF,
                                         ctypes.byref(acc), _d(box)), "volume_move")
         return bool(acc.value), box
+
+    def carried_energy(self):
+        """(E carried for the volume moves, volume moves since its last full evaluation)"""
+        e = ctypes.c_double()
+        age = ctypes.c_int()
+        self._ck(self._L.sc_carried_energy(self._h, ctypes.byref(e), ctypes.byref(age)), "carried_energy")
+        return e.value, age.value
 
     # ---- learned surrogate + delayed acceptance
     def mlp_init(self, ptype, seed=1):

@@ -21,6 +21,10 @@
 #include <string>
 #include <vector>
 
+// Per-expression FMA contraction for the whole file (the pair header's
+// reason: every kernel must round a pair term alike).
+#pragma clang fp contract(on)
+
 #include "../../include/sc_gpu.h"
 #include "sc_host.h"
 #include "sc_mlp.h"
@@ -789,7 +793,12 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
             V3<float> r = fx_image_c(q.u[0], q.u[1], q.u[2], pb4.x, pb4.y, pb4.z, sc.Lh, sc.Ll);
             if (dot(r, r) > sc.sqmaxcut) continue;
             DPart<float> pb = load_part<float>(O[k], Q[k], pb4.w, tp);
-            acc += (double)pair_energy<float>(pp[q.t * sc.nt + pb4.w], pa, pb, r, sc.sqmaxcut);
+            // the pair in its canonical order, the higher ID as part1 (allToAll's)
+            if (idk > q.id)
+                acc += (double)pair_energy<float>(pp[pb4.w * sc.nt + q.t], pb, pa, mk<float>(-r.x, -r.y, -r.z),
+                                                  sc.sqmaxcut);
+            else
+                acc += (double)pair_energy<float>(pp[q.t * sc.nt + pb4.w], pa, pb, r, sc.sqmaxcut);
         }
     }
     // chain partners, wherever they are
@@ -1087,11 +1096,9 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
     int t = uo.w;
     bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
-    int idk = 0, cwk = 0;
-    if (ct.cw) {
-        idk = S.id[k];
-        cwk = ct.cw[idk];
-    }
+    const int idk = S.id[k];
+    int cwk = 0;
+    if (ct.cw) cwk = ct.cw[idk];
     int bmask = cwk & 15;
     // parities of the one-barrier helpers; a trial starts after a barrier
     // that follows every read of the previous trial's helpers
@@ -1165,14 +1172,21 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             jj = e & 0x0fffffff;
             if ((e >> (28 + which)) & 1) {
                 uint4 b = S.u[jj];
-                const PP32& P2 = pp[t * sc.nt + b.w];
+                // the pair in its canonical order, the higher ID as part1 (the
+                // order of allToAll): the pair term is then one function of
+                // the pair's state whichever particle moves, and the accepted
+                // changes add up to allToAll's differences term by term
+                const bool sw = S.id[jj] > idk;
+                const PP32& P2 = sw ? pp[b.w * sc.nt + t] : pp[t * sc.nt + b.w];
                 V3<float> r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
                                     : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+                if (sw) r = mk<float>(-r.x, -r.y, -r.z);
                 float ev = 0.0f;
                 if (P2.kind <= K_SCA && P2.kind >= 0) {
+                    V3<float> dm = which ? dnew : dold, db = normalised(mk<float>(S.o[jj].x, S.o[jj].y, S.o[jj].z));
                     DPart<float> pa, pb;
-                    pa.dir = which ? dnew : dold;
-                    pb.dir = normalised(mk<float>(S.o[jj].x, S.o[jj].y, S.o[jj].z));
+                    pa.dir = sw ? db : dm;
+                    pb.dir = sw ? dm : db;
                     need = scsc_cheap(P2, pa, pb, r, ev, dseg2);
                 } else if (P2.kind == K_SPH_WCA || P2.kind == K_SPH_TAYLOR) {
                     ev = sphere_pair(P2, sqrtf(dot(r, r)));
@@ -1265,22 +1279,26 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             int jj = w & 0x3fffffff;
             which = (w >> 30) & 1;
             uint4 b = S.u[jj];
-            P2 = pp + t * sc.nt + b.w;
+            const bool sw = S.id[jj] > idk;   // canonical order, as in phase B
+            P2 = sw ? pp + b.w * sc.nt + t : pp + t * sc.nt + b.w;
             r = which ? fx_image_c(un.x, un.y, un.z, b.x, b.y, b.z, sc.Lh, sc.Ll)
                       : fx_image_c(uo.x, uo.y, uo.z, b.x, b.y, b.z, sc.Lh, sc.Ll);
+            if (sw) r = mk<float>(-r.x, -r.y, -r.z);
             const float* mv = which ? mvn : mvo;
             if (P2->kind <= K_SCA) {
                 if (mpp < npatch_pairs(*P2)) {
-                    patch_pair_index(*P2, mpp, k1, k2);
+                    patch_pair_index(*P2, mpp, k1, k2);   // patches of part1 and part2
                     DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
-                    va = mv_view(mv, k1);
-                    vb = view(pb, k2);
+                    PatchView<float> vm = mv_view(mv, sw ? k2 : k1), vn = view(pb, sw ? k1 : k2);
+                    va = sw ? vn : vm;
+                    vb = sw ? vm : vn;
                     hit = patch_side<float>(*P2, va, vb, r, k1, k2, side, x1, x2);
                     pair_item = true;
                 }
             } else if (mpp == 0 && side == 0) {
                 DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
-                ev = spsc(*P2, mv_part(mv, t, tp[t].geotype), pb, r);
+                DPart<float> pm = mv_part(mv, t, tp[t].geotype);
+                ev = sw ? spsc(*P2, pb, pm, r) : spsc(*P2, pm, pb, r);
             }
         }
         // the other side's result (lane ^ 1 holds item it ^ 1: CB is even)
@@ -5500,6 +5518,16 @@ static int sc_volume_move_impl(sc_ctx* ctx, int ptype, double press, double temp
     if (box_out) for (int k = 0; k < 3; k++) box_out[k] = ctx->box[k];
     return SC_OK;
 }
+
+static int sc_carried_energy_impl(sc_ctx* ctx, double* e, int* age) {
+    if (!ctx || !e) return SC_EINVAL;
+    if (!ctx->e_valid) return fail(ctx, SC_ESTATE, "no carried energy: no volume move since the state changed");
+    *e = ctx->e_track;
+    if (age) *age = ctx->e_age;
+    return SC_OK;
+}
+
+int sc_carried_energy(sc_ctx* ctx, double* e, int* age) { SC_GUARD(ctx, sc_carried_energy_impl(ctx, e, age)) }
 
 int sc_volume_move(sc_ctx* ctx, int ptype, double press, double temper, double edge_mx, uint64_t seed,
                    uint64_t counter, int* accepted, double box_out[3]) {

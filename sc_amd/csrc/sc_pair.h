@@ -15,6 +15,16 @@
 //     attraction), so kernels can compact the expensive work across lanes.
 #pragma once
 
+// FMA contraction only within one expression (C's FP_CONTRACT ON), never
+// across statements: with cross-statement fusion the rounding of a derived
+// vector or a pair term depends on what the optimiser sees around the inlined
+// call, so the sweep's trial energies and the whole-system energy could round
+// the same pair differently and, where an FP32 branch sits on a knife edge
+// (an intersection test or de-duplication), take different branches.  With
+// per-expression contraction every kernel computes a pair bit for bit alike.
+// The pragma stays in force for the rest of the including file.
+#pragma clang fp contract(on)
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 

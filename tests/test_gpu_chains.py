@@ -150,7 +150,7 @@ def test_chain_sweep_drift_and_volume_moves():
     say("sweep: E %.6f -> %.6f, sum dE %.6f, drift %.3e, acc %d/%d %d/%d, left %d, bond-rejected %d" % (
         e0, e1, s["energy_change"], drift, s["trans_acc"], s["trans_try"], s["rot_acc"], s["rot_try"],
         s["left_cell"], s["bond_rejected"]))
-    assert abs(drift) <= 1e-6 * max(1.0, abs(e1))
+    assert abs(drift) <= 1e-8 * max(1.0, abs(e1))
     assert s["trans_acc"] > 0 and s["rot_acc"] > 0
     acc = 0
     for k in range(20):

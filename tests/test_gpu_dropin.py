@@ -8,8 +8,10 @@ energy backend (mc/totalenergycalculator.h:1227), to
 reference's Updater / MoveCreator drive every oneToAll, oneToAllTrial,
 update(int) and allToAll of a plain NVT run through the C ABI.  The run's own
 energy bookkeeping (updater.cpp:122-150, the drift it prints every tenth of
-the run) must stay at FP32-pair-term rounding, its last interaction energy
-must match the FP64 oracle on the configuration it writes (config.last), and
+the run and at the end) must stay within 1e-8 |E| (every call prices a pair
+term with the same FP32 rounding, so only FP64 sums differ), its ending
+energy (the allToAll after the last sweep, updater.cpp:345-352) must match
+the FP64 oracle on the configuration it writes (config.last), and
 its acceptance rates must be those of the oracle's sampler on the same
 system.  Skipped when the binary was not built (make -C oracle dropin needs
 the reference sources).
@@ -49,12 +51,16 @@ def test_reference_program_runs_on_the_gpu_backend():
         print(r.stderr[-1500:], flush=True)
         assert r.returncode == 0
         assert "GpuTotalE" not in r.stderr   # no failed or uncovered call
-        drifts = [float(x) for x in re.findall(r"drift: (\S+)", out)]
+        drifts = [float(x) for x in re.findall(r"particles: \d+ drift: (\S+)", out)]
         energies = [float(x) for x in re.findall(r"InteractionEnergy: (\S+?),", out)]
         assert len(drifts) == 10 and len(energies) == 10
-        e_last = energies[-1]
-        print("drifts %s; energies %.6f .. %.6f" % (["%.2e" % x for x in drifts], energies[0], e_last), flush=True)
-        assert max(abs(x) for x in drifts) <= 1e-6 * max(1.0, abs(e_last))
+        # the tenth report is printed at the start of the last sweep; the
+        # ending energy is the allToAll after it
+        e_last = float(re.search(r"Ending energy: (\S+)", out).group(1))
+        drifts.append(float(re.search(r"Energy drift: (\S+)", out).group(1)))
+        print("drifts %s; energies %.6f .. %.6f, ending %.6f" % (["%.2e" % x for x in drifts], energies[0],
+                                                                energies[-1], e_last), flush=True)
+        assert max(abs(x) for x in drifts) <= 1e-8 * max(1.0, abs(e_last))
         acc = {m.group(1): float(m.group(2)) for m in re.finditer(r"Single particle (translation|rotation):\s+(\d+)",
                                                                   out)}
         # the configuration the run wrote, in the reference's config.last format
@@ -64,7 +70,7 @@ def test_reference_program_runs_on_the_gpu_backend():
     st = po.State.from_real(box, rows[:, 0:3], rows[:, 3:6], rows[:, 6:9], w["types"])
     tp = po.Topo(w["top"])
     eo = po.energy_all(tp, st, use_cells=False)
-    print("last interaction energy %.6f, oracle on config.last %.6f; acceptance %s" % (e_last, eo, acc), flush=True)
+    print("ending energy %.6f, oracle on config.last %.6f; acceptance %s" % (e_last, eo, acc), flush=True)
     # the run prints 6 significant digits (cout's default precision)
     ulp6 = 10.0 ** (np.floor(np.log10(abs(eo))) - 5) if eo != 0 else 1e-6
     assert abs(e_last - eo) <= 1e-5 * abs(eo) + 0.5 * ulp6

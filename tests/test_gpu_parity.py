@@ -5,10 +5,11 @@ Tolerances:
     libm rounding -> 1e-9 relative for >= 99.9% of pairs (the rest are
     degenerate-branch pairs, counted and bounded);
   * pair energies, FP32: 1e-5 relative for >= 99.5% of the pairs with
-    0.1 <= |E| < 1e3 kT (all within 2e-5), 2e-6 kT absolute below, 1e-3
+    0.1 <= |E| < 1e3 kT (all within 5e-5), 2e-6 kT absolute below, 1e-3
     relative above;
   * per-particle energies: 1e-5 relative (1e-5 kT where terms cancel);
-  * sweep energy bookkeeping: 1e-6 |E|;
+  * sweep energy bookkeeping: 1e-8 |E| (every kernel rounds a pair term
+    alike, so only the order of the FP64 sums differs);
   * whole-system energy (FP32 pair terms, FP64 reduction): 1e-5 relative;
   * cell lists, neighbour lists: bit-exact;
   * overlap flags: identical except pairs within 1e-9 sigma of contact.
@@ -73,7 +74,9 @@ def test_pair_energies_fp32(dev):
     separation vector (~2e-7 sigma at the stencil's distances) times the
     term's slope, so the relative error is bounded only where |E'/E| is:
       * 0.1 <= |E| < 1e3 kT: >= 99.5% within 1e-5 relative, every one within
-        2e-5 (measured: 4 of 2,096 beyond 1e-5, the largest 1.3e-5);
+        5e-5 (measured with per-expression FMA contraction: 7 of 2,096
+        beyond 1e-5, the largest 2.0e-5; the steepest terms of this class,
+        |E'/E| ~ 100 per sigma, reach 2e-5 from the separation alone);
       * |E| < 0.1 kT (the WCA cutoff and the end of the attraction's cos^2
         switch, where E -> 0 with a finite or zero slope): 2e-6 kT absolute;
       * |E| >= 1e3 kT (segments closer than 0.56 sigma; Boltzmann weight
@@ -98,7 +101,7 @@ def test_pair_energies_fp32(dev):
         say("  pair %d-%d types %d-%d: gpu %.9f oracle %.9f" % (i[k], j[k], st.types[i[k]], st.types[j[k]], eg[k],
                                                                eo[k]))
     assert out.mean() <= 5e-3
-    assert rel[mid].max() <= 2e-5
+    assert rel[mid].max() <= 5e-5
     assert err[small].max() <= 2e-6
     assert rel[huge].max() <= 1e-3
 
@@ -264,12 +267,46 @@ def test_sweep_energy_drift_and_determinism(dev):
             e0, e1, s["energy_change"], drift, tries, s["trans_acc"], s["trans_try"], s["rot_acc"], s["rot_try"],
             s["left_cell"]))
         assert tries == 20 * 1000
-        assert abs(drift) <= 1e-6 * max(1.0, abs(e1))
+        assert abs(drift) <= 1e-8 * max(1.0, abs(e1))
         _, p, dd, qq = g.get_config()
         out.append((p, dd, qq))
         g.close()
     for a, b in zip(out[0], out[1]):
         np.testing.assert_array_equal(a, b)
+
+
+def test_knife_edge_pair_is_rounded_alike_everywhere(dev):
+    """A pair whose FP32 term sits on a knife edge (tests/golden/
+    knife_edge_pair.json: a last-bit change of a derived vector flips an
+    intersection branch between 0.0437 and -0.6049 kT) has one FP32 value
+    in every kernel: p2p, oneToAll and allToAll agree exactly, and the
+    sweep's trials price it alike (the energy bookkeeping of sweeps that move
+    the pair stays at FP64-sum rounding; with kernels that rounded it
+    differently the first accepted move booked the 0.65 kT jump)."""
+    import json
+    say("case: knife-edge pair (2 TCHPSC of the 65,536 benchmark system)")
+    z = json.load(open(os.path.join(GOLD, "knife_edge_pair.json")))
+    w = W.workload("c4_tchpsc_65536")
+    g = sc_amd.GpuSystem(dev)
+    g.load_topology(w["top"].replace("C 65536", "C 2"))
+    g.set_config(z["box"], z["pos"], z["dir"], z["pd0"], [3, 3])
+    e32 = g.p2p([1], [0])[0]
+    e64 = g.p2p([1], [0], fp64=True)[0]
+    ea, e1, e0 = g.allToAll(), g.oneToAll(1), g.oneToAll(0)
+    say("p2p FP32 %.6f FP64 %.6f allToAll %.6f oneToAll %.6f %.6f" % (e32, e64, ea, e1, e0))
+    assert ea == e32 and e1 == e32 and e0 == e32
+    g.set_visit_factor(1)
+    worst = 0.0
+    moved = 0
+    for c in range(40):
+        e0 = g.allToAll()
+        s = g.sweep(1, 1.0, 0.02, 0.5 / 180 * np.pi, r=1, seed=3, counter=c)
+        e1 = g.allToAll()
+        moved += s["trans_acc"] + s["rot_acc"]
+        worst = max(worst, abs(e1 - e0 - s["energy_change"]))
+    say("40 sweeps, %d accepted moves, worst drift %.3e" % (moved, worst))
+    g.close()
+    assert moved > 0 and worst <= 1e-8
 
 
 def _blocked(x, nb=8):

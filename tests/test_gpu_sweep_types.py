@@ -14,7 +14,7 @@ benchmark's settings).
 
 Tolerances:
   * energy bookkeeping: allToAll(end) - allToAll(start) equals the sum of the
-    accepted FP64 trial differences of FP32 pair terms within 1e-6 |E|;
+    accepted FP64 trial differences of FP32 pair terms within 1e-8 |E|;
   * allToAll against the FP64 oracle at the end: 1e-5 relative;
   * cached and uncached derived vectors: the same trajectory bit for bit;
   * ensembles (exact, cheap-terms screen, network screen) against the
@@ -98,7 +98,7 @@ def test_sweep_energy_bookkeeping_and_oracle_energy(prepared, name, mode):
         "screen 1 %d/%d, screen 2 %d/%d" % (e0, e1, s["energy_change"], drift, abs(drift) / abs(e1), tries,
                                              tries / (30.0 * n), acc / tries, s["stage1_acc"], s["stage1_try"],
                                              s["stage2_acc"], s["stage2_try"]))
-    assert abs(drift) <= 1e-6 * abs(e1)
+    assert abs(drift) <= 1e-8 * abs(e1)
     assert abs(tries / (30.0 * n) - 1.0) < 0.05
     assert 0 < acc < tries
     if mode:
