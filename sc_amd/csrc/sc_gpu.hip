@@ -708,7 +708,7 @@ __global__ __launch_bounds__(WAVE * NW) void k_energy_all(Grid g, SysConst sc, c
                     const PP32& P2 = pp[a.w * sc.nt + b.w];
                     DPart<float> pa = load_part<float>(S.o[i], S.q[i], a.w, tp);
                     DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
-                    if (P2.kind <= K_SCA) acc += (double)scsc_attr(P2, pa, pb, r);
+                    if (P2.kind <= K_SCA) acc += scsc_attr_d(P2, pa, pb, r);
                     else acc += (double)spsc(P2, pa, pb, r);
                 }
             }
@@ -795,10 +795,9 @@ __global__ __launch_bounds__(WAVE) void k_energy_query(int nq, const Query* __re
             DPart<float> pb = load_part<float>(O[k], Q[k], pb4.w, tp);
             // the pair in its canonical order, the higher ID as part1 (allToAll's)
             if (idk > q.id)
-                acc += (double)pair_energy<float>(pp[pb4.w * sc.nt + q.t], pb, pa, mk<float>(-r.x, -r.y, -r.z),
-                                                  sc.sqmaxcut);
+                acc += pair_energy_d<float>(pp[pb4.w * sc.nt + q.t], pb, pa, mk<float>(-r.x, -r.y, -r.z), sc.sqmaxcut);
             else
-                acc += (double)pair_energy<float>(pp[q.t * sc.nt + pb4.w], pa, pb, r, sc.sqmaxcut);
+                acc += pair_energy_d<float>(pp[q.t * sc.nt + pb4.w], pa, pb, r, sc.sqmaxcut);
         }
     }
     // chain partners, wherever they are
@@ -830,7 +829,10 @@ __global__ void k_pairs(int m, const int* __restrict__ pi, const int* __restrict
         V3<float> rf = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
         r = mk<R>((R)rf.x, (R)rf.y, (R)rf.z);
     }
-    double e = (double)pair_energy<R>(pp[ua.w * sc.nt + ub.w], pa, pb, r, sqmaxcut);
+    // FP32: the terms added in FP64, as every FP32 kernel adds them; FP64: the
+    // reference's own order of the sum
+    double e = sizeof(R) == 8 ? (double)pair_energy<R>(pp[ua.w * sc.nt + ub.w], pa, pb, r, sqmaxcut)
+                              : pair_energy_d<R>(pp[ua.w * sc.nt + ub.w], pa, pb, r, sqmaxcut);
     if (ct.cw) {
         int bit = cw_bit(pj[k] - pi[k]), cw = ct.cw[pi[k]];
         if (bit >= 0 && ((cw >> bit) & 1))
@@ -2233,12 +2235,14 @@ __global__ __launch_bounds__(256) void k_chain_anchor(int nchain, const int* __r
 
 // the pair functor of a queued pair as a call of its own (its registers are
 // not added to those of the surrounding scan)
-__device__ __attribute__((noinline)) float chain_pair(const PP32* __restrict__ pp, const TP32* __restrict__ tp, int nt,
-                                                      float sqmaxcut, uint4 ua, float4 oa, float2 qa, uint4 ub,
-                                                      float4 ob, float2 qb, V3<float> r) {
+__device__ __attribute__((noinline)) double chain_pair(const PP32* __restrict__ pp, const TP32* __restrict__ tp, int nt,
+                                                       float sqmaxcut, uint4 ua, float4 oa, float2 qa, uint4 ub,
+                                                       float4 ob, float2 qb, V3<float> r, bool sw) {
     DPart<float> pa = load_part<float>(oa, qa, ua.w, tp);
     DPart<float> pb = load_part<float>(ob, qb, ub.w, tp);
-    return pair_energy<float>(pp[ua.w * nt + ub.w], pa, pb, r, sqmaxcut);
+    // the pair in its canonical order, the higher ID as part1 (allToAll's)
+    if (sw) return pair_energy_d<float>(pp[ub.w * nt + ua.w], pb, pa, mk<float>(-r.x, -r.y, -r.z), sqmaxcut);
+    return pair_energy_d<float>(pp[ua.w * nt + ub.w], pa, pb, r, sqmaxcut);
 }
 
 // Energy of one particle of molecule [f, f + m) at the state (ua, pa) with
@@ -2251,7 +2255,7 @@ __device__ __attribute__((noinline)) float chain_pair(const PP32* __restrict__ p
 // candidates inside sqmaxcut.  Candidates that pass are queued and evaluated
 // 64 at a time.
 #define CHAIN_QCAP (WAVE + 4 * WAVE)
-__device__ double chain_item_energy(uint4 ua, float4 oa, float2 qa, int f, int m, int cell, const Grid& g,
+__device__ double chain_item_energy(uint4 ua, float4 oa, float2 qa, int idk, int f, int m, int cell, const Grid& g,
                                     const Grid& gc, const SysConst& sc, const ChainMoveParams& cp,
                                     const int* __restrict__ starts, const uint4* P, const float4* O, const float2* Q,
                                     const int* __restrict__ ID, const int* __restrict__ slot,
@@ -2342,7 +2346,7 @@ __device__ double chain_item_energy(uint4 ua, float4 oa, float2 qa, int f, int m
                     int sb = queue[e];
                     uint4 ubb = P[sb];
                     V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ubb.x, ubb.y, ubb.z, sc.Lh, sc.Ll);
-                    acc += (double)chain_pair(pp, tp, sc.nt, sc.sqmaxcut, ua, oa, qa, ubb, O[sb], Q[sb], r);
+                    acc += chain_pair(pp, tp, sc.nt, sc.sqmaxcut, ua, oa, qa, ubb, O[sb], Q[sb], r, ID[sb] > idk);
                 }
             }
             wave_lds_sync();
@@ -2447,12 +2451,42 @@ __host__ __device__ constexpr size_t chain_pass_lds(int nw) {
 // molecules, one after another.  Every thread draws the same numbers; wave 0
 // builds the trial state; the energy items (particle, old/new state) are
 // spread over the waves.
+// The molecule's own energy at the state (U, O, Q) of its m particles (IDs
+// f .. f+m-1): its non-bonded pairs and its bonds, pair (a, b < a) with a as
+// part1 (allToAll's order), this thread's share (pairs tid, tid + stride, ...).
+// A rigid move leaves it unchanged in exact arithmetic, but the stored state
+// is rounded (fixed-point positions, FP32 axes); the chain trial adds its
+// change to mol2others' so that the accepted changes add up to allToAll's.
+__device__ __attribute__((noinline)) double chain_intra(int f, int m, const uint4* U, const float4* O, const float2* Q, int tid, int stride,
+                              const SysConst& sc, const PP32* __restrict__ pp, const TP32* __restrict__ tp,
+                              const ChainTab& ct) {
+    double e = 0.0;
+    int np = m * (m - 1) / 2;
+    for (int p = tid; p < np; p += stride) {
+        int a = 1;
+        while ((a + 1) * a / 2 <= p) a++;   // pair p = (a, b) with a (a - 1) / 2 <= p < (a + 1) a / 2
+        int b = p - a * (a - 1) / 2;
+        uint4 ua = U[a], ub = U[b];
+        V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
+        if (dot(r, r) <= sc.sqmaxcut) e += chain_pair(pp, tp, sc.nt, sc.sqmaxcut, ua, O[a], Q[a], ub, O[b], Q[b], r, false);
+        if (ct.cw && a - b <= 2) {
+            int cw = ct.cw[f + a], bit = cw_bit(b - a);
+            if (bit >= 0 && ((cw >> bit) & 1))
+                e += bond_energy(ct.mol[cw >> 8], ct.pp64[ua.w * sc.nt + ub.w], b - a, fx_r12(ua, ub, sc.Ld),
+                                 unit64(O[a].x, O[a].y, O[a].z), unit64(O[a].w, Q[a].x, Q[a].y),
+                                 unit64(O[b].x, O[b].y, O[b].z), unit64(O[b].w, Q[b].x, Q[b].y));
+        }
+    }
+    return e;
+}
+
 template <int NW>
 __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysConst sc, ChainMoveParams cp,
                                                           const int* __restrict__ starts, uint4* P, float4* O,
                                                           float2* Q, const int* __restrict__ ID,
                                                           const int* __restrict__ slot, const PP32* __restrict__ pp,
-                                                          const TP32* __restrict__ tp, const double* __restrict__ vol,
+                                                          const TP32* __restrict__ tp, ChainTab ct,
+                                                          const double* __restrict__ vol,
                                                           const int* __restrict__ cfirst, const int* __restrict__ csize,
                                                           const int* __restrict__ cstart,
                                                           const int* __restrict__ clist_g,
@@ -2530,15 +2564,25 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass(Grid g, Grid gc, SysCo
         ctr.w = (cp.pass_hi << 2) | 2u;
         U4 rc = philox(ctr, cp.key0, cp.key1);   // the acceptance uniform, drawn ahead of the energies
         double acc_o = 0.0, acc_n = 0.0;
-        for (int it = w; it < 2 * m; it += NW) {
+        // items: (particle, state) for mol2others, then the molecule's own
+        // pairs and bonds at each state (waves without a particle item take them)
+        for (int it = w; it < 2 * m + (m > 1 ? 2 : 0); it += NW) {
+            if (it >= 2 * m) {
+                bool st = it > 2 * m;
+                double e = chain_intra(f, m, st ? Un : Uo, st ? On : Oo, st ? Qn : Qo, lane, WAVE, sc, pp, tp, ct);
+                if (st) acc_n += e; else acc_o += e;
+                continue;
+            }
             int a = it >> 1;
             bool st = (it & 1) != 0;
             uint4 ua = st ? Un[a] : Uo[a];
-            double e = chain_item_energy(ua, st ? On[a] : Oo[a], st ? Qn[a] : Qo[a], f, m, cell, g, gc, sc, cp, starts,
+            double e = chain_item_energy(ua, st ? On[a] : Oo[a], st ? Qn[a] : Qo[a], f + a, f, m, cell, g, gc, sc, cp,
+                                         starts,
                                          P, O, Q, ID, slot, pp, tp, cfirst, ccell_id, stamp, mlist, mpre, nm, pre, cbeg,
                                          queue);
             if (st) acc_n += e; else acc_o += e;
         }
+
         double eo, en;
         int rpar = 0;   // one paired sum per trial, after the previous trial's barrier
         block_sum2<NW>(acc_o, acc_n, red, rpar, eo, en);
@@ -2619,7 +2663,8 @@ __host__ __device__ inline ChainStLayout chain_st_layout(int nw, int cap) {
 // in a region cell; stencil cells outside the region hold none), except the
 // particles that may move in this pass, which may have left the cell they
 // are staged under: those are taken from `own`, wherever they are now.
-__device__ double chain_item_energy_st(uint4 ua, float4 oa, float2 qa, int f, int m, const Grid& g, const int* f0,
+__device__ double chain_item_energy_st(uint4 ua, float4 oa, float2 qa, int idk, int f, int m, const Grid& g,
+                                       const int* f0,
                                        const int* fn, const int* spre, const int* own, int nown, const SysConst& sc,
                                        const PP32* __restrict__ pp, const TP32* __restrict__ tp, const uint4* su,
                                        const float4* so, const float2* sq, const int* sid, int* queue, int* spr) {
@@ -2697,7 +2742,8 @@ __device__ double chain_item_energy_st(uint4 ua, float4 oa, float2 qa, int f, in
                     int sb = queue[e];
                     uint4 ub = su[sb];
                     V3<float> r = fx_image_c(ua.x, ua.y, ua.z, ub.x, ub.y, ub.z, sc.Lh, sc.Ll);
-                    acc += (double)chain_pair(pp, tp, sc.nt, sc.sqmaxcut, ua, oa, qa, ub, so[sb], sq[sb], r);
+                    acc += chain_pair(pp, tp, sc.nt, sc.sqmaxcut, ua, oa, qa, ub, so[sb], sq[sb], r,
+                                      (sid[sb] & CHAIN_IDMASK) > idk);
                 }
             }
             wave_lds_sync();
@@ -2718,7 +2764,7 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass_st(Grid g, Grid gc, Sy
                                                              float4* O, float2* Q, const int* __restrict__ ID,
                                                              const int* __restrict__ slot,
                                                              const PP32* __restrict__ pp, const TP32* __restrict__ tp,
-                                                             const double* __restrict__ vol,
+                                                             ChainTab ct, const double* __restrict__ vol,
                                                              const int* __restrict__ cfirst,
                                                              const int* __restrict__ csize,
                                                              const int* __restrict__ cstart,
@@ -2965,14 +3011,24 @@ __global__ __launch_bounds__(WAVE * NW) void k_chain_pass_st(Grid g, Grid gc, Sy
         ctr.w = (cp.pass_hi << 2) | 2u;
         U4 rc = philox(ctr, cp.key0, cp.key1);   // the acceptance uniform, drawn ahead of the energies
         double acc_o = 0.0, acc_n = 0.0;
-        for (int it = w; it < 2 * m; it += NW) {
+        // items: (particle, state) for mol2others, then the molecule's own
+        // pairs and bonds at each state (waves without a particle item take them)
+        for (int it = w; it < 2 * m + (m > 1 ? 2 : 0); it += NW) {
+            if (it >= 2 * m) {
+                bool st = it > 2 * m;
+                double e = chain_intra(f, m, st ? Un : Uo, st ? On : Oo, st ? Qn : Qo, lane, WAVE, sc, pp, tp, ct);
+                if (st) acc_n += e; else acc_o += e;
+                continue;
+            }
             int a = it >> 1;
             bool st = (it & 1) != 0;
             uint4 ua = st ? Un[a] : Uo[a];
-            double e = chain_item_energy_st(ua, st ? On[a] : Oo[a], st ? Qn[a] : Qo[a], f, m, g, f0, fn, spre, own,
+            double e = chain_item_energy_st(ua, st ? On[a] : Oo[a], st ? Qn[a] : Qo[a], f + a, f, m, g, f0, fn, spre,
+                                            own,
                                             nown_s, sc, pp, tp, su, so, sq, sid, queue, spr);
             if (st) acc_n += e; else acc_o += e;
         }
+
         double eo, en;
         int rpar = 0;   // one paired sum per trial, after the previous trial's barrier
         block_sum2<NW>(acc_o, acc_n, red, rpar, eo, en);
@@ -4875,13 +4931,14 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
         if (staged)
             hipLaunchKernelGGL((k_chain_pass_st<CHAIN_NW>), dim3(nblk), dim3(WAVE * CHAIN_NW), lds_st, ctx->stream,
                                ctx->g, gc, sc, cp, cap, ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c],
-                               ctx->slot, ctx->d_pp32, ctx->d_tp32, ctx->d_vol, ctx->d_cfirst, ctx->d_csize,
-                               ctx->d_cstart, ctx->d_clist, ctx->d_ccell_id, ctx->d_stats, ctx->d_dsum);
+                               ctx->slot, ctx->d_pp32, ctx->d_tp32, chain_tab(ctx), ctx->d_vol, ctx->d_cfirst,
+                               ctx->d_csize, ctx->d_cstart, ctx->d_clist, ctx->d_ccell_id, ctx->d_stats, ctx->d_dsum);
         else
             hipLaunchKernelGGL((k_chain_pass<CHAIN_NW>), dim3(nblk), dim3(WAVE * CHAIN_NW), lds, ctx->stream, ctx->g,
                                gc, sc, cp, ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->slot,
-                               ctx->d_pp32, ctx->d_tp32, ctx->d_vol, ctx->d_cfirst, ctx->d_csize, ctx->d_cstart,
-                               ctx->d_clist, ctx->d_ccell_id, ctx->d_stamp, ctx->d_stats, ctx->d_dsum, ctx->d_flag);
+                               ctx->d_pp32, ctx->d_tp32, chain_tab(ctx), ctx->d_vol, ctx->d_cfirst, ctx->d_csize,
+                               ctx->d_cstart, ctx->d_clist, ctx->d_ccell_id, ctx->d_stamp, ctx->d_stats, ctx->d_dsum,
+                               ctx->d_flag);
         HIPCK(hipGetLastError());
         if ((rc = build_grid(ctx, r, shift, ctx->box, ctx->max_active))) return rc;
     }

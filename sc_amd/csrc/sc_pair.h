@@ -495,6 +495,20 @@ SC_HD R scsc_attr(const PairParamsT<R>& P, const DPart<R>& a, const DPart<R>& b,
     return e;
 }
 
+// The same sum in FP64 over the FP32 patch-pair terms.  The FP32 kernels add
+// a pair's terms (the cheap stage's repulsion, each patch pair) in FP64: the
+// sweep evaluates them on different lanes and can only add them in FP64, so
+// every other kernel adds the same FP32 values the same way, and a pair has
+// one value to ~1e-16 wherever it is priced.
+template <typename R>
+SC_HD double scsc_attr_d(const PairParamsT<R>& P, const DPart<R>& a, const DPart<R>& b, const V3<R>& r) {
+    double e = (double)patch_pair<R, 0, 0>(P, a, b, r);
+    if (P.fT) e += (double)patch_pair<R, 1, 0>(P, a, b, r);
+    if (P.sT) e += (double)patch_pair<R, 0, 1>(P, a, b, r);
+    if (P.fT && P.sT) e += (double)patch_pair<R, 1, 1>(P, a, b, r);
+    return e;
+}
+
 // MixSpSc::closestDist, paire.h:1077-1093
 template <typename R>
 SC_HD R sp_dist(const V3<R>& r, const V3<R>& d1, R hl, R& contt, V3<R>& dv) {
@@ -593,6 +607,28 @@ SC_HD R pair_energy(const PairParamsT<R>& P, const DPart<R>& a, const DPart<R>& 
         return spsc(P, a, b, r);
     default:
         return R(0);
+    }
+}
+
+// pair_energy with the FP32 terms added in FP64 (scsc_attr_d): the form every
+// FP32 kernel uses.
+template <typename R>
+SC_HD double pair_energy_d(const PairParamsT<R>& P, const DPart<R>& a, const DPart<R>& b, const V3<R>& r,
+                           R sqmaxcut) {
+    R r2 = dot(r, r);
+    if (r2 > sqmaxcut) return 0.0;
+    switch (P.kind) {
+    case K_PSC: case K_CPSC: case K_MIXED: case K_SCN: case K_SCA: {
+        R erep, d2;
+        bool att = scsc_cheap(P, a, b, r, erep, d2);
+        return att ? (double)erep + scsc_attr_d(P, a, b, r) : (double)erep;
+    }
+    case K_SPH_WCA: case K_SPH_TAYLOR:
+        return (double)sphere_pair(P, (R)sqrt(r2));
+    case K_SCASPA: case K_PSCSPA: case K_CPSCSPA:
+        return (double)spsc(P, a, b, r);
+    default:
+        return 0.0;
     }
 }
 

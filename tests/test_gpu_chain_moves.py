@@ -157,7 +157,9 @@ def test_chain_move_passes_are_rigid_reproducible_and_keep_the_energy(extent):
     assert abs(tries - 0.25 * n * 3) <= 0.02 * 0.25 * n * 3
     assert 0 < s["chain_disp_acc"] < s["chain_disp_try"] and 0 < s["chain_rot_acc"] < s["chain_rot_try"]
     assert s["chain_rejected"] < 0.2 * tries
-    assert abs((e1 - e0) - s["energy_change"]) <= 1e-6 * abs(e0) + 1e-3
+    drift = (e1 - e0) - s["energy_change"]
+    say("drift %.3e (%.2e of |E|)" % (drift, abs(drift) / abs(e0)))
+    assert abs(drift) <= 1e-8 * abs(e0)
     b1, p1, d1, q1 = g.get_config()
     vol = np.array([tp.ia(x, x).volume for x in t[:3]])
     moved, worst, worst_r = 0, 0.0, 0.0
