@@ -198,14 +198,25 @@ int sc_sweep(sc_ctx* ctx, int nsweeps, double temper, double transmx, double rot
  * pass waits for its fullest cell.  Both are Metropolis chains with the same
  * stationary distribution (the count never depends on the moves of the pass). */
 int sc_set_trial_mode(sc_ctx* ctx, int mode);
-/* Waves that work together on one active cell of a sweep pass: 1, 2, 4, or
- * 0 (default: 4, and 2 for the network screen, whose MFMA operands leave room
- * for one wave per SIMD, so two 2-wave blocks fill a CU).  The stencil scan,
- * the pair lists and the patch terms of every trial are spread over the
- * block's threads; the chain of trials stays sequential.  Results are
- * deterministic for each setting; they differ between settings only by the
- * order of FP64 sums. */
+/* Waves that work together on one trial of an active cell: 1, 2, 4, or 0
+ * (default: four waves, each evaluating a trial of its own; see
+ * sc_set_speculation).  With 1, 2 or 4 the stencil scan, the pair lists and
+ * the patch terms of every trial are spread over the block's threads and the
+ * chain of trials runs one trial at a time.  Results are deterministic for
+ * each setting; they differ between settings only by the order of FP64 sums. */
 int sc_set_block_waves(sc_ctx* ctx, int nw);
+/* Speculative trials in sc_sweep (on = 1, the default, with block waves 0):
+ * each of the four waves of an active cell's block evaluates one of the next
+ * four trials against the cell's current state; the trials are then kept in
+ * order until one depends on an earlier accepted trial of the round (the same
+ * particle, the two within maxcut in any of their old/new states, or chain
+ * partners), and evaluation resumes from there.  The chain of states and
+ * decisions is the one-trial-at-a-time sweep's (Updater::simulate's sequence,
+ * updater.cpp:206-230, within each cell).  A trial whose moving particle has
+ * more than 384 particles within maxcut fails the sweep with SC_ECAPACITY; on = 0
+ * evaluates such systems.  No counterpart in the reference (its loop is
+ * sequential). */
+int sc_set_speculation(sc_ctx* ctx, int on);
 /* The sweep keeps the derived vectors (patch sides, second patch, chiral
  * axes: Particle::init, particle.cpp:3-79) of a pass's home-cell particles in
  * LDS, computed once per pass and refreshed on acceptance, for cells of up to

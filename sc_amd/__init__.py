@@ -51,6 +51,7 @@ SIGNATURES = {
     "sc_load_config": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "sc_write_config": (ctypes.c_long, [_P, ctypes.c_char_p, ctypes.c_long]),
     "sc_set_block_waves": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_set_speculation": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_set_home_cache": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_set_visit_factor": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_set_max_active_cells": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -587,8 +588,12 @@ class GpuSystem:
         return s2.value, d
 
     def set_block_waves(self, nw):
-        """waves per active cell in the sweep kernels (1, 2, 4; 0 = by surrogate mode)"""
+        """waves per trial of an active cell in the sweep (1, 2, 4; 0 = four waves with a trial each)"""
         self._ck(self._L.sc_set_block_waves(self._h, int(nw)), "set_block_waves")
+
+    def set_speculation(self, on=True):
+        """speculative trials, one per wave, in sweep() (default on; block waves 0)"""
+        self._ck(self._L.sc_set_speculation(self._h, int(bool(on))), "set_speculation")
 
     def set_home_cache(self, cap):
         """cells of up to `cap` particles (<= 64) cache their derived vectors in the sweep; 0 disables it"""

@@ -625,6 +625,13 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// The synchronisation of one trial's threads: the block, or (WS) only the
+// calling wave, when the waves of a block evaluate trials of their own.
+template <bool WS>
+__device__ __forceinline__ void tsync() {
+    if constexpr (WS) wave_lds_sync();
+    else __syncthreads();
+}
 
 template <int NW>
 __global__ __launch_bounds__(WAVE * NW) void k_energy_all(Grid g, SysConst sc, const int* __restrict__ starts,
@@ -1006,12 +1013,12 @@ struct MlpConst {
 // state flags (bit 28: old, 29: new), in stencil order.  The first SB threads
 // of the block scan, PA entries each per round; the others only take part in
 // the compaction barriers.  With chain bonds it records the staged index of
-// each bonded partner.
+// each bonded partner.  tid is the thread's index among the trial's threads;
+// entries beyond ncap are counted in nn but not stored.
 template <int NW, int PA>
 __device__ __forceinline__ void scan_stencil(const Stage& S, int k, const uint4 uo, const uint4 un, bool moved,
                                              const SysConst& sc, int bmask, int idk, const TrialScratch& T, int SB,
-                                             bool scanner, int& nn, int& cpar) {
-    const int tid = (int)threadIdx.x;
+                                             bool scanner, int& nn, int& cpar, int ncap, int tid) {
     for (int base = 0; base < S.n; base += PA * SB) {
         int fl[PA], jv[PA];
 #pragma unroll
@@ -1041,7 +1048,7 @@ __device__ __forceinline__ void scan_stencil(const Stage& S, int k, const uint4 
         block_compactK<NW, PA>(fb, nn, T.cnt, cpar, pv);
 #pragma unroll
         for (int h = 0; h < PA; h++)
-            if (fl[h]) T.nl[pv[h]] = jv[h] | (fl[h] << 28);
+            if (fl[h] && pv[h] < ncap) T.nl[pv[h]] = jv[h] | (fl[h] << 28);
     }
 }
 
@@ -1085,7 +1092,7 @@ __device__ __forceinline__ bool pair_features(const Stage& S, const TrialScratch
 // bonded partner lies outside the staged stencil is rejected (stage -1): the
 // partner might be moving in another cell of the pass, and the rejection does
 // not depend on the move (the particle stays in its cell).
-template <int DA, int NW>
+template <int DA, int NW, bool WS = false>
 __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint4 uo, const uint4 un,
                                                const float4 oo, const float2 qo, const float4 on, const float2 qn,
                                                const SysConst& sc,
@@ -1094,8 +1101,9 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                                                const unsigned char* __restrict__ mlp_img,
                                                const MlpConst& mc, const ChainTab& ct, double thr1, double thr2,
                                                const float* hvk, bool translate, double& eo, double& en,
-                                               double& des, int& stage) {
-    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
+                                               double& des, int& stage, int ncap) {
+    static_assert(!WS || NW == 1, "a wave's own trial is a one-wave trial");
+    const int tid = WS ? lane_id() : (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
     int t = uo.w;
     bool moved = (uo.x != un.x) || (uo.y != un.y) || (uo.z != un.z);
     const int idk = S.id[k];
@@ -1128,17 +1136,22 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         if (lane == 0) mv_store(T.mv + MV_STRIDE, pn);
     }
     if (bmask && tid < 4) T.bp[tid] = -1;
-    if (bmask) __syncthreads();
+    if (bmask) tsync<WS>();
     // phase A: the staged particles inside sqmaxcut at either state -> near
     // list, in stencil order (one barrier per round); all waves scan a
     // translation, the waves but the last one a rotation
     int nn = 0;
     if (NW > 1 && !translate)
         scan_stencil<NW, NW >= 4 ? 3 : 8>(S, k, uo, un, moved, sc, bmask, idk, T, (NW - 1) * WAVE, wv < NW - 1, nn,
-                                         cpar);
+                                         cpar, ncap, tid);
     else
-        scan_stencil<NW, NW >= 4 ? 2 : 4>(S, k, uo, un, moved, sc, bmask, idk, T, NW * WAVE, true, nn, cpar);
-    __syncthreads();
+        scan_stencil<NW, NW >= 4 ? 2 : 4>(S, k, uo, un, moved, sc, bmask, idk, T, NW * WAVE, true, nn, cpar, ncap,
+                                         tid);
+    tsync<WS>();
+    if (nn > ncap) {   // a wave's near list is full: the trial cannot be evaluated here
+        stage = -2;
+        return false;
+    }
     const V3<float> dnew = translate ? dold : mv_vec(mvn, 0);
     double ao = 0.0, an = 0.0;
     if (bmask) {
@@ -1204,7 +1217,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             if constexpr (DA == DA_NET) T.xd[pos] = sqrtf(dseg2);
         }
     }
-    __syncthreads();
+    tsync<WS>();
     // screens of the delayed acceptance: (1) the exact cheap terms already
     // summed (repulsion, sphere pairs), (2) with DA_NET the network's patch
     // attraction, tiles of 16 pairs on MFMA.  With four waves the network runs
@@ -1234,7 +1247,11 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                 int which = 0;
                 bool net = pair_features(S, T, nx, t0 + (lane & (MLP_TILE - 1)), t, uo, un, mvo, mvn, sc, mc, f,
                                          which);
-                float out = mlp_tile_w<NW>(mr, ml, T.hx, mlp_slots(f, lane >> 4), lane, wv);
+                float out;
+                if constexpr (WS)   // a wave of its own: operands from global memory, as the last wave's screen
+                    out = mlp_tile_g(mlp_img, ml, mlp_slots(f, lane >> 4), lane);
+                else
+                    out = mlp_tile_w<NW>(mr, ml, T.hx, mlp_slots(f, lane >> 4), lane, wv);
                 if (lane < MLP_TILE && net) {
                     if (which) sn += (double)out; else so += (double)out;
                 }
@@ -1323,6 +1340,78 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
 
 // One active cell of a checkerboard pass (logical index b of the colour's
 // active cells): stage the stencil, run the cell's trial moves, write back.
+// The proposal and thresholds of trial tt of a cell: counter (trial, cell,
+// pass, stream), streams 0 (particle, move kind, translation), 1 (rotation),
+// 2 (final and first-screen uniforms); the second screen's uniform uses
+// stream 2 with the trial's top bit set.
+template <int DA>
+__device__ __forceinline__ TrialRec draw_trial(U4 ctr, int tt, const MoveParams& mp, const Stage& S, int nh,
+                                               const TP32* __restrict__ tp, float Lx, float Ly, float Lz) {
+    U4 c = ctr;
+    c.x = (uint32_t)tt;
+    c.w = mp.pass_hi << 2;
+    U4 ra = philox(c, mp.key0, mp.key1);
+    TrialRec R;
+    R.k = S.h0 + (int)(((uint64_t)ra.x * (uint64_t)nh) >> 32);
+    int g0 = tp[S.u[R.k].w].geotype;
+    R.translate = (u01f(ra.y) < 0.5f) || (g0 >= 30);
+    if (R.translate) {
+        V3<float> d = unit_from(u01f(ra.z), u01f(ra.w));
+        R.w[0] = (int32_t)rintf(d.x * mp.transmx * (4294967296.0f / Lx));
+        R.w[1] = (int32_t)rintf(d.y * mp.transmx * (4294967296.0f / Ly));
+        R.w[2] = (int32_t)rintf(d.z * mp.transmx * (4294967296.0f / Lz));
+        R.w[3] = R.w[4] = 0;
+    } else {
+        c.w = (mp.pass_hi << 2) | 1u;
+        U4 rb = philox(c, mp.key0, mp.key1);
+        V3<float> axv = unit_from(u01f(rb.x), u01f(rb.y));
+        float ang = mp.rot_half * u01f(rb.z);
+        float vc = cosf(ang), vs = sqrtf(fmaxf(0.0f, 1.0f - vc * vc));
+        if (u01f(rb.w) >= 0.5f) vs = -vs;   // Particle::pscRotate direction choice (particle.h:194-196)
+        R.w[0] = __float_as_int(axv.x);
+        R.w[1] = __float_as_int(axv.y);
+        R.w[2] = __float_as_int(axv.z);
+        R.w[3] = __float_as_int(vc);
+        R.w[4] = __float_as_int(vs);
+    }
+    R.pad = 0;
+    c.w = (mp.pass_hi << 2) | 2u;
+    U4 rc = philox(c, mp.key0, mp.key1);
+    R.thr = -mp.temper * log(u01d(rc.x, rc.y));
+    R.thr1 = R.thr2 = R.pad2 = 0.0;
+    if constexpr (DA != DA_OFF) {   // the screens' uniforms
+        R.thr1 = -mp.temper * log(u01d(rc.z, rc.w));
+        c.x |= 0x80000000u;
+        U4 rt = philox(c, mp.key0, mp.key1);
+        R.thr2 = -mp.temper * log(u01d(rt.x, rt.y));
+    }
+    return R;
+}
+
+// The trial state of record R from the particle's state (uo, oo, qo): a
+// translation by R's fixed-point step (partDisplace), or the quaternion
+// rotation of axis and patch direction (pscRotate), renormalised in FP32.
+__device__ __forceinline__ void propose(const TrialRec& R, uint4 uo, float4 oo, float2 qo, uint4& un, float4& on,
+                                        float2& qn) {
+    un = uo;
+    on = oo;
+    qn = qo;
+    if (R.translate) {
+        un.x = uo.x + (uint32_t)R.w[0];
+        un.y = uo.y + (uint32_t)R.w[1];
+        un.z = uo.z + (uint32_t)R.w[2];
+    } else {
+        V3<float> axv = mk<float>(__int_as_float(R.w[0]), __int_as_float(R.w[1]), __int_as_float(R.w[2]));
+        float vc = __int_as_float(R.w[3]), vs = __int_as_float(R.w[4]);
+        V3<float> d = qrot(mk<float>(oo.x, oo.y, oo.z), axv, vc, vs);
+        V3<float> pq = qrot(mk<float>(oo.w, qo.x, qo.y), axv, vc, vs);
+        d = normalised(d);
+        pq = normalised(perp(pq, d));
+        on = make_float4(d.x, d.y, d.z, pq.x);
+        qn = make_float2(pq.y, pq.z);
+    }
+}
+
 template <int DA, int NW>
 __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst& sc, const MoveParams& mp,
                                            const int* __restrict__ starts, uint4* __restrict__ P,
@@ -1421,45 +1510,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
             __syncthreads();
             int tt = t + tid;
             if (tid < TRIAL_CHUNK && tt < ntrial) {
-                U4 c = ctr;
-                c.x = (uint32_t)tt;
-                c.w = mp.pass_hi << 2;
-                U4 ra = philox(c, mp.key0, mp.key1);
-                TrialRec R;
-                R.k = S.h0 + (int)(((uint64_t)ra.x * (uint64_t)nh) >> 32);
-                int g0 = tp[S.u[R.k].w].geotype;
-                R.translate = (u01f(ra.y) < 0.5f) || (g0 >= 30);
-                if (R.translate) {
-                    V3<float> d = unit_from(u01f(ra.z), u01f(ra.w));
-                    R.w[0] = (int32_t)rintf(d.x * mp.transmx * (4294967296.0f / Lx));
-                    R.w[1] = (int32_t)rintf(d.y * mp.transmx * (4294967296.0f / Ly));
-                    R.w[2] = (int32_t)rintf(d.z * mp.transmx * (4294967296.0f / Lz));
-                    R.w[3] = R.w[4] = 0;
-                } else {
-                    c.w = (mp.pass_hi << 2) | 1u;
-                    U4 rb = philox(c, mp.key0, mp.key1);
-                    V3<float> axv = unit_from(u01f(rb.x), u01f(rb.y));
-                    float ang = mp.rot_half * u01f(rb.z);
-                    float vc = cosf(ang), vs = sqrtf(fmaxf(0.0f, 1.0f - vc * vc));
-                    if (u01f(rb.w) >= 0.5f) vs = -vs;   // Particle::pscRotate direction choice (particle.h:194-196)
-                    R.w[0] = __float_as_int(axv.x);
-                    R.w[1] = __float_as_int(axv.y);
-                    R.w[2] = __float_as_int(axv.z);
-                    R.w[3] = __float_as_int(vc);
-                    R.w[4] = __float_as_int(vs);
-                }
-                R.pad = 0;
-                c.w = (mp.pass_hi << 2) | 2u;
-                U4 rc = philox(c, mp.key0, mp.key1);
-                R.thr = -mp.temper * log(u01d(rc.x, rc.y));
-                R.thr1 = R.thr2 = R.pad2 = 0.0;
-                if constexpr (DA != DA_OFF) {   // the screens' uniforms
-                    R.thr1 = -mp.temper * log(u01d(rc.z, rc.w));
-                    c.x |= 0x80000000u;
-                    U4 rt = philox(c, mp.key0, mp.key1);
-                    R.thr2 = -mp.temper * log(u01d(rt.x, rt.y));
-                }
-                recs[tid] = R;
+                recs[tid] = draw_trial<DA>(ctr, tt, mp, S, nh, tp, Lx, Ly, Lz);
             }
             __syncthreads();
         }
@@ -1469,28 +1520,18 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         float4 oo = S.o[k];
         float2 qo = S.q[k];
         bool translate = R.translate != 0;
-        uint4 un = uo;
-        float4 on = oo;
-        float2 qn = qo;
+        uint4 un;
+        float4 on;
+        float2 qn;
+        propose(R, uo, oo, qo, un, on, qn);
         if (translate) {
             n_ttry++;
-            un.x = uo.x + (uint32_t)R.w[0];
-            un.y = uo.y + (uint32_t)R.w[1];
-            un.z = uo.z + (uint32_t)R.w[2];
             if (cell_of_u(g, un.x, un.y, un.z) != cell) {   // leaving the active cell: rejected
                 n_left++;
                 continue;
             }
         } else {
             n_rtry++;
-            V3<float> axv = mk<float>(__int_as_float(R.w[0]), __int_as_float(R.w[1]), __int_as_float(R.w[2]));
-            float vc = __int_as_float(R.w[3]), vs = __int_as_float(R.w[4]);
-            V3<float> d = qrot(mk<float>(oo.x, oo.y, oo.z), axv, vc, vs);
-            V3<float> pq = qrot(mk<float>(oo.w, qo.x, qo.y), axv, vc, vs);
-            d = normalised(d);
-            pq = normalised(perp(pq, d));
-            on = make_float4(d.x, d.y, d.z, pq.x);
-            qn = make_float2(pq.y, pq.z);
         }
         double eo, en, des;
         bool pass;
@@ -1498,7 +1539,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         const float* hvk = hcache ? hv + (k - S.h0) * MV_STRIDE : nullptr;
         pass = trial_energies<DA, NW>(S, k, uo, un, oo, qo, on, qn, sc, pp, tp, ppmax, T, ml, mr, mlp_img, mc, ct,
                                       R.thr1,
-                                      R.thr2, hvk, translate, eo, en, des, stage);
+                                      R.thr2, hvk, translate, eo, en, des, stage, cap);
         if (stage < 0) {   // a chain partner outside the stencil
             n_bond++;
             __syncthreads();
@@ -1564,6 +1605,360 @@ __global__ __launch_bounds__(WAVE * NW, 2) void k_sweep_pass(Grid g, SysConst sc
                                                      double* __restrict__ de_sum, int* __restrict__ flag) {
     sweep_cell<DA, NW>(xcd_slab_index((int)blockIdx.x, (int)gridDim.x), g, sc, mp, starts, P, O, Q, ID, pp, tp, cap,
                        ppmax, nonempty, mlp_img, mc, ct, stats, de_sum, flag);
+}
+
+// ------------------------------------------------- speculative trials
+// The chain of trials of a cell is sequential, and one trial leaves most of
+// a block's lanes idle (its phases hold a few dozen items).  In this variant
+// each of the block's SPEC_W waves evaluates one trial of the next SPEC_W
+// against the cell's current state, all at once; thread 0 then takes the
+// results in trial order and keeps each one unless it depends on an earlier
+// trial of the round that was accepted (the same particle, chain partners, or
+// two particles whose pair term can be non-zero in any of their old/new
+// states, by the pair code's own cutoff tests), where the round stops: that
+// trial and the later ones are evaluated again in the next round, against the
+// updated state.  A kept
+// trial saw exactly the state it would have seen in sequence, so the chain
+// of states and decisions is the sequential sweep's (only the FP64 sums run
+// in another order, as between the 1- and 4-wave sweeps).
+#define SPEC_W 4
+#define SPEC_LCAP 384   // near-list entries of one wave's trial (a stencil of ~450 has ~60 within maxcut)
+
+struct __align__(16) SpecRes {
+    uint4 uo, un;    // position at the round's start and proposed
+    float4 on;       // proposed orientation record
+    float2 qn;
+    int k, id;       // staged index, particle ID
+    double de;       // E_new - E_old (kept trials that reached the final stage)
+    int status;      // 0 evaluated, 1 left the cell, 2 chain partner outside the stencil, 3 near list full
+    int translate, acc, stage;
+    int dep;         // bit i: the trial depends on the round's trial i, should that one be accepted
+};
+
+// LDS of the speculative sweep: the stage, then SPEC_W per-wave trial
+// scratch blocks, then what the waves share (byte offsets; per-wave fields
+// relative to the wave's block).
+struct SpecLayout {
+    size_t nl, xl, xd, mv, bp, cnt, red, netd, wstride;
+    size_t wbase, sten, misc, res, recs, hv, ml, tab, end;
+};
+__host__ __device__ inline size_t spec_align(size_t x, size_t a) { return (x + a - 1) / a * a; }
+__host__ __device__ inline SpecLayout spec_layout(int cap, int lcap, bool net, size_t tab_bytes) {
+    SpecLayout L;
+    L.nl = 0;
+    L.xl = L.nl + 4 * (size_t)lcap;
+    L.xd = L.xl + 8 * (size_t)lcap;
+    L.mv = spec_align(L.xd + (net ? 8 * (size_t)lcap : 0), 16);
+    L.bp = L.mv + 4 * 2 * MV_STRIDE;
+    L.cnt = L.bp + 16;
+    L.red = spec_align(L.cnt + 4 * 2 * CNT_HALF, 16);
+    L.netd = L.red + 8 * 16;
+    L.wstride = spec_align(L.netd + 16, 16);
+    L.wbase = spec_align((size_t)cap * (16 + 16 + 8 + 4), 16);   // the stage: u, o, q, id
+    size_t o = L.wbase + SPEC_W * L.wstride;
+    L.sten = o;
+    o += 4 * (2 * 125 + 2);
+    L.misc = spec_align(o, 16);
+    o = L.misc + 4 * 16;
+    L.res = spec_align(o, 16);
+    o = L.res + SPEC_W * sizeof(SpecRes);
+    L.recs = spec_align(o, 16);
+    o = L.recs + TRIAL_CHUNK * sizeof(TrialRec);
+    L.hv = spec_align(o, 16);
+    o = L.hv + 4 * (size_t)HCAP * MV_STRIDE;
+    L.ml = spec_align(o, 16);
+    o = L.ml + (net ? MLP_C_BYTES : 0);
+    L.tab = spec_align(o, 16);
+    L.end = L.tab + tab_bytes;
+    return L;
+}
+
+// Can the pair of particles 1 (position u1, orientation record o1, ID id1)
+// and 2 have a non-zero pair term?  The canonical evaluation's own FP32 tests
+// (higher ID as part1): no beyond sqmaxcut (phase A), none for a type pair
+// without a functor, and for spherocylinder pairs none when the segment
+// distance is beyond both the attraction's and the repulsion's range (phase
+// B's scsc_cheap; a relative margin of 1e-5 covers the K_SCA term's
+// square-root test).  Sphere pairs: any pair inside sqmaxcut.
+__device__ __forceinline__ bool pair_active(uint4 u1, float4 o1, int id1, uint4 u2, float4 o2, int id2,
+                                            const SysConst& sc, const PP32* __restrict__ pp) {
+    if (id1 < id2) {
+        uint4 tu = u1; u1 = u2; u2 = tu;
+        float4 to = o1; o1 = o2; o2 = to;
+    }
+    V3<float> r = fx_image_c(u1.x, u1.y, u1.z, u2.x, u2.y, u2.z, sc.Lh, sc.Ll);
+    if (dot(r, r) > sc.sqmaxcut) return false;
+    const PP32& P = pp[u1.w * sc.nt + u2.w];
+    if (P.kind == K_NONE) return false;
+    if (P.kind > K_SCA || P.kind < 0) return true;
+    V3<float> dv = min_dist(normalised(mk<float>(o1.x, o1.y, o1.z)), normalised(mk<float>(o2.x, o2.y, o2.z)), P.hl0,
+                            P.hl1, r);
+    float d2 = dot(dv, dv);
+    return d2 <= fmaxf(P.rcutSq, P.rcutwcaSq) * 1.00001f;
+}
+
+// Bit i of the result: trial `me` (particle k at uo/oo, proposed un/on) would
+// see another state had trial i < w of the round (its record Ra) been
+// accepted: the same particle, chain partners (bond and angle terms reach
+// beyond maxcut), or a pair that can interact in any of their old/new states.
+// Lanes 4i .. 4i+3 test the four state combinations of trial i.
+__device__ __forceinline__ int spec_deps(const TrialRec* recs, int t, int w, const Stage& S, int k, uint4 uo,
+                                         float4 oo, uint4 un, float4 on, const SysConst& sc,
+                                         const PP32* __restrict__ pp, bool bonded) {
+    int lane = lane_id(), i = lane >> 2, c = lane & 3;
+    bool d = false;
+    if (i < w) {
+        const TrialRec& Ra = recs[(t + i) & (TRIAL_CHUNK - 1)];
+        int ka = Ra.k;
+        if (ka == k || (bonded && abs(S.id[ka] - S.id[k]) <= 2)) {
+            d = true;
+        } else {
+            uint4 uoa = S.u[ka], una;
+            float4 ooa = S.o[ka], ona;
+            float2 qoa = S.q[ka], qna;
+            propose(Ra, uoa, ooa, qoa, una, ona, qna);
+            d = pair_active((c & 1) ? una : uoa, (c & 1) ? ona : ooa, S.id[ka], (c & 2) ? un : uo, (c & 2) ? on : oo,
+                            S.id[k], sc, pp);
+        }
+    }
+    unsigned long long m = __ballot(d);
+    int dep = 0;
+    for (int q = 0; q < SPEC_W - 1; q++)
+        if ((m >> (4 * q)) & 15ull) dep |= 1 << q;
+    return dep;
+}
+
+template <int DA>
+__device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysConst& sc, const MoveParams& mp,
+                                                const int* __restrict__ starts, uint4* __restrict__ P,
+                                                float4* __restrict__ O, float2* __restrict__ Q,
+                                                const int* __restrict__ ID, const PP32* __restrict__ pp,
+                                                const TP32* __restrict__ tp, int cap, int lcap, int ppmax,
+                                                const int* __restrict__ nonempty,
+                                                const unsigned char* __restrict__ mlp_img, const MlpConst& mc,
+                                                const ChainTab& ct, unsigned long long* __restrict__ stats,
+                                                double* __restrict__ de_sum, int* __restrict__ flag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr bool NET = DA == DA_NET;
+    int ax = b % mp.nact[0], ay = (b / mp.nact[0]) % mp.nact[1], az = b / (mp.nact[0] * mp.nact[1]);
+    int cx = mp.ox + mp.p * ax, cy = mp.oy + mp.p * ay, cz = mp.oz + mp.p * az;
+    int cell = cx + g.nc[0] * (cy + g.nc[1] * cz);
+    int nh = starts[cell + 1] - starts[cell];
+    if (nh == 0) return;
+    const SpecLayout L = spec_layout(cap, lcap, NET, 0);
+    Stage S;
+    S.u = (uint4*)smem;
+    S.o = (float4*)(S.u + cap);
+    S.q = (float2*)(S.o + cap);
+    S.id = (int*)(S.q + cap);
+    const int tid = (int)threadIdx.x, lane = tid & (WAVE - 1), w = tid >> 6;
+    unsigned char* wb = smem + L.wbase + (size_t)w * L.wstride;
+    TrialScratch T;
+    T.nl = (int*)(wb + L.nl);
+    T.xl = (int*)(wb + L.xl);
+    T.xd = (float*)(wb + L.xd);
+    T.mv = (float*)(wb + L.mv);
+    T.bp = (int*)(wb + L.bp);
+    T.cnt = (int*)(wb + L.cnt);
+    T.red = (double*)(wb + L.red);
+    T.hx = nullptr;
+    T.netd = (double*)(wb + L.netd);
+    int* sten = (int*)(smem + L.sten);
+    int* misc = (int*)(smem + L.misc);
+    SpecRes* res = (SpecRes*)(smem + L.res);
+    TrialRec* recs = (TrialRec*)(smem + L.recs);
+    float* hv = (float*)(smem + L.hv);
+    float* ml = nullptr;
+    MlpRegsW<4> mr;   // not read: a wave's network screen takes its operands from global memory
+    if constexpr (NET) {
+        ml = (float*)(smem + L.ml);
+        for (int i = tid; i < MLP_C_BYTES / 16; i += WAVE * SPEC_W) ((uint4*)ml)[i] = ((const uint4*)(mlp_img + MLP_C_OFF))[i];
+    }
+#if SWEEP_TABLES_IN_LDS
+    if (sc.nt <= SWEEP_TABLE_MAXT) {
+        PP32* ppl = (PP32*)(smem + L.tab);
+        TP32* tpl = (TP32*)(ppl + sc.nt * sc.nt);
+        const int w4 = (int)(sizeof(PP32) / 4), t4 = (int)(sizeof(TP32) / 4);
+        for (int e = tid; e < sc.nt * sc.nt * w4; e += WAVE * SPEC_W) ((int*)ppl)[e] = ((const int*)pp)[e];
+        for (int e = tid; e < sc.nt * t4; e += WAVE * SPEC_W) ((int*)tpl)[e] = ((const int*)tp)[e];
+        pp = ppl;
+        tp = tpl;
+    }
+#endif
+    if (!stage_load<SPEC_W>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, sten)) {
+        if (tid == 0) atomicOr(flag, 1);
+        return;
+    }
+    const float Lx = sc.L[0], Ly = sc.L[1], Lz = sc.L[2];
+    U4 ctr;
+    ctr.y = (uint32_t)cell;
+    ctr.z = mp.pass_lo;
+    int ntrial = nh * mp.visit;
+    if (mp.trial_mode == 1) {
+        ctr.x = 0xffffffffu;
+        ctr.w = (mp.pass_hi << 2) | 3u;
+        U4 rd = philox(ctr, mp.key0, mp.key1);
+        float mean = (float)mp.n * (float)mp.visit / (float)max(1, *nonempty);
+        int fl = (int)floorf(mean);
+        ntrial = fl + ((u01f(rd.x) < mean - (float)fl) ? 1 : 0);
+    }
+    const bool hcache = nh <= min(HCAP, mp.hcap);
+    if (hcache)
+        for (int i = tid; i < nh; i += WAVE * SPEC_W)
+            mv_store(hv + i * MV_STRIDE, load_part<float>(S.o[S.h0 + i], S.q[S.h0 + i], S.u[S.h0 + i].w, tp));
+    // thread 0 keeps the counters of the kept trials
+    unsigned n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0, n_s2try = 0,
+             n_s2acc = 0, n_bond = 0;
+    double dsum = 0.0;
+    const bool bonded = ct.cw != nullptr;
+    int t = 0, drawn = 0;
+    while (t < ntrial) {   // t is block-uniform (from LDS)
+        if (t >= drawn) {
+            __syncthreads();   // the previous records are no longer read
+            int tt = drawn + tid;
+            if (tid < TRIAL_CHUNK && tt < ntrial) recs[tid] = draw_trial<DA>(ctr, tt, mp, S, nh, tp, Lx, Ly, Lz);
+            drawn += TRIAL_CHUNK;
+            __syncthreads();
+        }
+        const int W = min(SPEC_W, min(ntrial - t, drawn - t));
+        if (w < W) {
+            const TrialRec& R = recs[(t + w) & (TRIAL_CHUNK - 1)];
+            int k = R.k;
+            uint4 uo = S.u[k];
+            float4 oo = S.o[k];
+            float2 qo = S.q[k];
+            bool translate = R.translate != 0;
+            uint4 un;
+            float4 on;
+            float2 qn;
+            propose(R, uo, oo, qo, un, on, qn);
+            int status = 0, stage = 0;
+            bool acc = false;
+            double de = 0.0;
+            const int dep = spec_deps(recs, t, w, S, k, uo, oo, un, on, sc, pp, bonded);
+            if (translate && cell_of_u(g, un.x, un.y, un.z) != cell) {   // leaving the active cell: rejected
+                status = 1;
+            } else {
+                double eo, en, des;
+                const float* hvk = hcache ? hv + (k - S.h0) * MV_STRIDE : nullptr;
+                bool pass = trial_energies<DA, 1, true>(S, k, uo, un, oo, qo, on, qn, sc, pp, tp, ppmax, T, ml, mr,
+                                                        mlp_img, mc, ct, R.thr1, R.thr2, hvk, translate, eo, en, des,
+                                                        stage, lcap);
+                if (stage == -1) {
+                    status = 2;
+                } else if (stage == -2) {
+                    status = 3;
+                } else if (pass) {
+                    de = en - eo;
+                    // MoveCreator::moveTry (movecreator.h:175-187), with the screens' changes divided out
+                    double de2 = de - des;
+                    acc = de2 <= 0.0 || de2 < R.thr;
+                }
+            }
+            if (lane == 0) {
+                SpecRes& r = res[w];
+                r.uo = uo;
+                r.un = un;
+                r.on = on;
+                r.qn = qn;
+                r.k = k;
+                r.id = S.id[k];
+                r.de = de;
+                r.status = status;
+                r.translate = translate ? 1 : 0;
+                r.acc = acc ? 1 : 0;
+                r.stage = stage;
+                r.dep = dep;
+            }
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int nfin = 0, abort = 0;
+            for (int i = 0; i < W; i++) {
+                const SpecRes& a = res[i];
+                if (a.status == 3) {
+                    abort = 1;
+                    break;
+                }
+                bool dep = false;
+                for (int j = 0; j < i && !dep; j++) dep = res[j].acc && ((a.dep >> j) & 1);
+                if (dep) break;
+                nfin++;
+                if (a.translate) {
+                    n_ttry++;
+                    if (a.status == 1) {
+                        n_left++;
+                        continue;
+                    }
+                } else {
+                    n_rtry++;
+                }
+                if (a.status == 2) {
+                    n_bond++;
+                    continue;
+                }
+                if constexpr (DA != DA_OFF) {
+                    n_s1try++;
+                    if (a.stage >= 1) n_s1acc++;
+                    if constexpr (DA == DA_NET) {
+                        if (a.stage >= 1) n_s2try++;
+                        if (a.stage >= 2) n_s2acc++;
+                    }
+                }
+                if (a.acc) {
+                    if (a.translate) n_tacc++; else n_racc++;
+                    dsum += a.de;
+                    S.u[a.k] = a.un;
+                    S.o[a.k] = a.on;
+                    S.q[a.k] = a.qn;
+                }
+            }
+            misc[0] = nfin;
+            misc[1] = abort;
+        }
+        __syncthreads();
+        const int nfin = misc[0];
+        if (hcache && w < nfin && lane < 27) {   // the derived vectors of a kept, accepted rotation
+            const SpecRes& a = res[w];
+            if (a.acc && !a.translate) hv[(a.k - S.h0) * MV_STRIDE + lane] = T.mv[MV_STRIDE + lane];
+        }
+        if (misc[1]) {   // a trial's near list overflowed: stop the cell, report
+            if (tid == 0) atomicOr(flag, 16);
+            break;
+        }
+        t += nfin;
+        __syncthreads();
+    }
+    __syncthreads();
+    // write the home cell back (same slots it was staged from)
+    int gs = starts[cell];
+    for (int e = tid; e < nh; e += WAVE * SPEC_W) {
+        P[gs + e] = S.u[S.h0 + e];
+        O[gs + e] = S.o[S.h0 + e];
+        Q[gs + e] = S.q[S.h0 + e];
+    }
+    if (tid == 0) {
+        int rep = (int)(blockIdx.x & (STAT_REP - 1));
+        const unsigned v[10] = {n_tacc, n_ttry, n_racc, n_rtry, n_left, n_s1acc, n_s1try, n_s2acc, n_s2try, n_bond};
+        for (int q = 0; q < 10; q++)
+            if (v[q]) atomicAdd(&stats[rep * STAT_STRIDE + q], (unsigned long long)v[q]);
+        atomicAdd(&de_sum[rep * STAT_STRIDE], dsum);
+    }
+}
+
+template <int DA>
+__global__ __launch_bounds__(WAVE * SPEC_W, 2) void k_sweep_spec(Grid g, SysConst sc, MoveParams mp,
+                                                             const int* __restrict__ starts, uint4* __restrict__ P,
+                                                             float4* __restrict__ O, float2* __restrict__ Q,
+                                                             const int* __restrict__ ID,
+                                                             const PP32* __restrict__ pp,
+                                                             const TP32* __restrict__ tp, int cap, int lcap,
+                                                             int ppmax, const int* __restrict__ nonempty,
+                                                             const unsigned char* __restrict__ mlp_img, MlpConst mc,
+                                                             ChainTab ct, unsigned long long* __restrict__ stats,
+                                                             double* __restrict__ de_sum, int* __restrict__ flag) {
+    sweep_cell_spec<DA>(xcd_slab_index((int)blockIdx.x, (int)gridDim.x), g, sc, mp, starts, P, O, Q, ID, pp, tp, cap,
+                        lcap, ppmax, nonempty, mlp_img, mc, ct, stats, de_sum, flag);
 }
 
 // Per-replica parameters of one checkerboard pass of a replica batch.
@@ -3187,6 +3582,7 @@ struct sc_ctx {
     double e_track = 0.0;
     int e_age = 0;                      // volume moves since the last full evaluation
     int sweep_nw = 0;                   // waves per active cell in the sweep kernels (1, 2, 4; 0 = by mode)
+    int spec = 1;                       // single-system sweep: speculative trials, one per wave (sweep_nw 0)
     int home_cache = HCAP;              // cell size limit of the sweep's derived-vector cache (0: off)
     int visit = 1;                      // colour passes of a sweep = colours / visit, visit x the trials each
     unsigned char* noalloc = nullptr;   // this device's address of `scratch` (sc_pair_energies_noalloc)
@@ -3562,6 +3958,10 @@ static int check_flag(sc_ctx* ctx, const char* what) {
     if (f) {
         HIPCK(hipMemset(ctx->d_flag, 0, sizeof(int)));
         if (f & 4) return fail(ctx, SC_ECAPACITY, std::string(what) + ": a coarse cell held more than 4096 molecules");
+        if (f & 16)
+            return fail(ctx, SC_ECAPACITY, std::string(what) + ": more than 384 particles within maxcut of a moving "
+                                                               "particle (the speculative sweep's near list); "
+                                                               "sc_set_speculation(ctx, 0) evaluates such systems");
         return fail(ctx, SC_ECAPACITY, std::string(what) + ": a stencil held more particles than the stage capacity");
     }
     return SC_OK;
@@ -4970,6 +5370,18 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         int cap = stage_cap(ctx, g, ctx->box);
         int mode = ctx->da_mode;
         bool da = mode == DA_NET;
+        // the speculative variant: the stage that fits beside SPEC_W trials' lists
+        bool spec = ctx->spec && ctx->sweep_nw == 0;
+        int cap_spec = cap;
+        size_t lds_spec = 0;
+        if (spec) {
+            size_t tabb = sweep_table_bytes(ctx->nt);
+            long room = (long)SWEEP_LDS_BUDGET - (long)spec_layout(0, SPEC_LCAP, da, tabb).end - 16;
+            int cmax = room > 0 ? (int)(room / (long)stage_bytes(1)) : 0;
+            if (cap_spec > cmax) cap_spec = (cmax / 64) * 64;
+            spec = cap_spec >= 64;
+            if (spec) lds_spec = spec_layout(cap_spec, SPEC_LCAP, da, tabb).end;
+        }
         if (da && (ctx->mlp_type < 0 || ctx->ref2dense[ctx->mlp_type] < 0))
             return fail(ctx, SC_EINVAL, "the network screen needs the network's particle type in the system");
         MlpConst mlpc = ctx->mlp_c;
@@ -5030,10 +5442,19 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
     hipLaunchKernelGGL((k_sweep_pass<DAM, NWV>), dim3(nblk), dim3(WAVE * NWV), lds, ctx->stream, g, sc, mp,         \
                        ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,     \
                        ctx->ppmax, ctx->d_nonempty, IMG, MC, chain_tab(ctx), ctx->d_stats, ctx->d_dsum, ctx->d_flag)
+#define SPEC_PASS(DAM, IMG, MC)                                                                                \
+    hipLaunchKernelGGL((k_sweep_spec<DAM>), dim3(nblk), dim3(WAVE * SPEC_W), lds_spec, ctx->stream, g, sc, mp,     \
+                       ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,          \
+                       cap_spec, SPEC_LCAP, ctx->ppmax, ctx->d_nonempty, IMG, MC, chain_tab(ctx), ctx->d_stats,    \
+                       ctx->d_dsum, ctx->d_flag)
             // four waves per cell in every mode (the network screen splits its
             // hidden units, and so its MFMA operands, between the four waves)
             int nw = ctx->sweep_nw ? ctx->sweep_nw : 4;
-            if (mode == DA_NET) {
+            if (spec) {
+                if (mode == DA_NET) SPEC_PASS(DA_NET, ctx->mlp_img, mlpc);
+                else if (mode == DA_REP) SPEC_PASS(DA_REP, nullptr, ctx->mlp_c);
+                else SPEC_PASS(DA_OFF, nullptr, ctx->mlp_c);
+            } else if (mode == DA_NET) {
                 if (nw == 4) SWEEP_PASS(DA_NET, 4, ctx->mlp_img, mlpc);
                 else if (nw == 2) SWEEP_PASS(DA_NET, 2, ctx->mlp_img, mlpc);
                 else SWEEP_PASS(DA_NET, 1, ctx->mlp_img, mlpc);
@@ -5047,6 +5468,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
                 else SWEEP_PASS(DA_OFF, 1, nullptr, ctx->mlp_c);
             }
 #undef SWEEP_PASS
+#undef SPEC_PASS
             if (ctx->profile) HIPCK(hipEventRecord(ctx->kev[2 * launches + 1], ctx->stream));
             launches++;
         }
@@ -6007,6 +6429,14 @@ static int sc_set_block_waves_impl(sc_ctx* ctx, int nw) {
 int sc_set_block_waves(sc_ctx* ctx, int nw) {
     SC_GUARD(ctx, sc_set_block_waves_impl(ctx, nw))
 }
+
+static int sc_set_speculation_impl(sc_ctx* ctx, int on) {
+    if (!ctx || (on != 0 && on != 1)) return SC_EINVAL;
+    ctx->spec = on;
+    return SC_OK;
+}
+
+int sc_set_speculation(sc_ctx* ctx, int on) { SC_GUARD(ctx, sc_set_speculation_impl(ctx, on)) }
 
 static int sc_set_home_cache_impl(sc_ctx* ctx, int cap) {
     if (!ctx || cap < 0) return SC_EINVAL;

@@ -167,6 +167,32 @@ def test_chain_sweep_drift_and_volume_moves():
     g.close()
 
 
+@pytest.mark.parametrize("r", [1, 2])
+def test_speculative_chain_sweep_reproduces_sequential(r):
+    """The speculative sweep (a trial per wave, kept in order up to the first
+    dependence on an accepted trial, chain partners always dependent) gives
+    the one-trial-at-a-time trajectory of a bonded system bit for bit."""
+    say("case: chain sweep, speculative vs one trial at a time, r =", r)
+    out = []
+    for nw in (0, 1):
+        g = sc_amd.GpuSystem(0)
+        chain_system(g, 400, 0.2, 6)
+        g.set_trial_mode(1)
+        g.set_visit_factor(8 if r == 1 else 9)
+        g.set_block_waves(nw)
+        s = g.sweep(10, 1.1, 0.3, ROT_HALF, r=r, seed=17, counter=0)
+        out.append((s, g.get_config()))
+        g.close()
+    (s1, c1), (s2, c2) = out
+    say("accepted %d / %d, bond-rejected %d" % (s1["trans_acc"] + s1["rot_acc"], s1["trans_try"] + s1["rot_try"],
+                                                s1["bond_rejected"]))
+    for k in ("trans_acc", "trans_try", "rot_acc", "rot_try", "left_cell", "bond_rejected"):
+        assert s1[k] == s2[k], k
+    assert abs(s1["energy_change"] - s2["energy_change"]) <= 1e-9 * max(1.0, abs(s1["energy_change"]))
+    for x, y in zip(c1, c2):
+        np.testing.assert_array_equal(x, y)
+
+
 def _blocked(x, nb=8):
     b = x[: len(x) // nb * nb].reshape(nb, -1).mean(1)
     return b.mean(), b.std(ddof=1) / np.sqrt(nb)

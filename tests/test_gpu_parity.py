@@ -389,14 +389,16 @@ def test_cluster_labels_and_nematic_order_match_oracle(dev):
 
 @pytest.mark.parametrize("mode", [0, 2])
 def test_block_waves_reproduce_wave_sweep(dev, mode):
-    """The sweep with 2 or 4 waves per active cell follows the 1-wave sweep
-    move for move: the same trial moves, pair lists and decisions; only the order of
-    the FP64 sums differs (~1e-16 relative), which could flip a Metropolis
-    decision only for a uniform within that of its threshold."""
-    say("case: 2- and 4-wave vs 1-wave sweep, surrogate mode", mode)
+    """The sweep with 2 or 4 waves per trial, and the speculative sweep (a
+    trial per wave, kept in order up to the first dependence on an accepted
+    one), follow the 1-wave sweep move for move: the same trial moves, pair
+    lists and decisions; only the order of the FP64 sums differs (~1e-16
+    relative), which could flip a Metropolis decision only for a uniform
+    within that of its threshold."""
+    say("case: 2- and 4-wave and speculative vs 1-wave sweep, surrogate mode", mode)
     w = W.workload("c1_psc_1000")
     out = []
-    for nw in (1, 2, 4):
+    for nw in (1, 2, 4, 0):
         g = sc_amd.GpuSystem(dev)
         g.load_topology(w["top"])
         pos, d, q = g.generate_rsa(w["types"], w["box"], seed=23)

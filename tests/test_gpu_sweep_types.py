@@ -137,6 +137,34 @@ def test_home_cache_reproduces_uncached_derivation(prepared, name, mode):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("name", list(SYSTEMS))
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_speculative_sweep_reproduces_sequential(prepared, name, mode):
+    """A trial per wave, kept in order up to the first dependence on an
+    accepted trial of the round, gives the one-trial-at-a-time trajectory:
+    the same counters and the same configuration bit for bit (at visit factor
+    8, 128 trials per cell visit)."""
+    say("case: %s sweep, mode %d, speculative vs one trial at a time" % (name, mode))
+    out = []
+    for nw in (0, 1):
+        g = gpu_system(prepared, name, mode)
+        g.set_visit_factor(8)
+        g.set_block_waves(nw)
+        s = g.sweep(20, 1.0, 0.3, ROT_HALF, r=1, seed=7, counter=0)
+        out.append((s, g.get_config()))
+        g.close()
+    (s1, c1), (s2, c2) = out
+    for k in ("trans_acc", "trans_try", "rot_acc", "rot_try", "left_cell", "stage1_acc", "stage1_try",
+              "stage2_acc", "stage2_try"):
+        assert s1[k] == s2[k], k
+    say("accepted %d / %d, energy change %.9f vs %.9f" % (s1["trans_acc"] + s1["rot_acc"],
+                                                          s1["trans_try"] + s1["rot_try"], s1["energy_change"],
+                                                          s2["energy_change"]))
+    assert abs(s1["energy_change"] - s2["energy_change"]) <= 1e-9 * max(1.0, abs(s1["energy_change"]))
+    for a, b in zip(c1, c2):
+        np.testing.assert_array_equal(a, b)
+
+
 def _blocked(x, nb=8):
     b = np.asarray(x)[: len(x) // nb * nb].reshape(nb, -1).mean(1)
     return b.mean(), b.std(ddof=1) / np.sqrt(nb)
