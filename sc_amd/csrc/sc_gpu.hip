@@ -1997,6 +1997,33 @@ __global__ __launch_bounds__(WAVE * NW, 2) void k_sweep_batch(const RepPass* __r
                        de_sum + (size_t)R.rep * STAT_REP * STAT_STRIDE, flag);
 }
 
+// The speculative sweep of every replica's active cells of one colour pass.
+template <int DA>
+__global__ __launch_bounds__(WAVE * SPEC_W, 2) void k_sweep_batch_spec(const RepPass* __restrict__ rp, int nrep,
+                                                                   const int* __restrict__ starts,
+                                                                   uint4* __restrict__ P, float4* __restrict__ O,
+                                                                   float2* __restrict__ Q,
+                                                                   const int* __restrict__ ID,
+                                                                   const PP32* __restrict__ pp,
+                                                                   const TP32* __restrict__ tp, int cap, int lcap,
+                                                                   int ppmax, const int* __restrict__ nonempty,
+                                                                   const unsigned char* __restrict__ mlp_img,
+                                                                   MlpConst mc, ChainTab ct,
+                                                                   unsigned long long* __restrict__ stats,
+                                                                   double* __restrict__ de_sum,
+                                                                   int* __restrict__ flag) {
+    int lb = xcd_slab_index((int)blockIdx.x, (int)gridDim.x);
+    int lo = 0, hi = nrep - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (rp[mid].blk_off <= lb) lo = mid; else hi = mid - 1;
+    }
+    const RepPass R = rp[lo];
+    sweep_cell_spec<DA>(lb - R.blk_off, R.g, R.sc, R.mp, starts + R.cell_off, P, O, Q, ID, pp, tp, cap, lcap, ppmax,
+                        nonempty + R.rep, mlp_img, mc, ct, stats + (size_t)R.rep * STAT_REP * STAT_STRIDE,
+                        de_sum + (size_t)R.rep * STAT_REP * STAT_STRIDE, flag);
+}
+
 // cell list of a replica batch: the cell of every particle in its replica's
 // grid, offset into the batch's cell arrays.  The replica of a particle is
 // found from its index (replicas are index ranges part_off..), by binary
@@ -5686,6 +5713,18 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             double b[3] = {ctx->rep_box[q][0], ctx->rep_box[q][1], ctx->rep_box[q][2]};
             cap = std::max(cap, stage_cap_n(ctx->rep_n[q], ctx->minvol, ctx->rep_grid[q].g, b));
         }
+        // the speculative variant (block waves 0): the stage beside SPEC_W trials' lists
+        bool spec = ctx->spec && ctx->sweep_nw == 0;
+        int cap_spec = cap;
+        size_t lds_spec = 0;
+        if (spec) {
+            size_t tabb = sweep_table_bytes(ctx->nt);
+            long room = (long)SWEEP_LDS_BUDGET - (long)spec_layout(0, SPEC_LCAP, false, tabb).end - 16;
+            int cmax = room > 0 ? (int)(room / (long)stage_bytes(1)) : 0;
+            if (cap_spec > cmax) cap_spec = (cmax / 64) * 64;
+            spec = cap_spec >= 64;
+            if (spec) lds_spec = spec_layout(cap_spec, SPEC_LCAP, false, tabb).end;
+        }
         size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + SWEEP_SMALL_LDS +
                        sweep_table_bytes(ctx->nt);
         size_t per = stage_bytes(1) + sizeof(int) * 3;
@@ -5737,8 +5776,16 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
               ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->ppmax,                \
               ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ChainTab{nullptr, nullptr, ctx->d_pp64}, ctx->d_rep_stats,  \
               ctx->d_rep_dsum, ctx->d_flag)
+#define SPEC_BATCH(DAM)                                                                                        \
+    SC_LAUNCH((k_sweep_batch_spec<DAM>), dim3(nblk[ci]), dim3(WAVE * SPEC_W), lds_spec, ctx->stream, rp, R,           \
+              ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap_spec, SPEC_LCAP,  \
+              ctx->ppmax, ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ChainTab{nullptr, nullptr, ctx->d_pp64},           \
+              ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag)
             int nw = ctx->sweep_nw ? ctx->sweep_nw : 4;
-            if (mode == DA_REP) {
+            if (spec) {
+                if (mode == DA_REP) SPEC_BATCH(DA_REP);
+                else SPEC_BATCH(DA_OFF);
+            } else if (mode == DA_REP) {
                 if (nw == 4) SWEEP_BATCH(DA_REP, 4);
                 else if (nw == 2) SWEEP_BATCH(DA_REP, 2);
                 else SWEEP_BATCH(DA_REP, 1);
@@ -5748,6 +5795,7 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
                 else SWEEP_BATCH(DA_OFF, 1);
             }
 #undef SWEEP_BATCH
+#undef SPEC_BATCH
             launches++;
         }
         HIPCK(hipGetLastError());

@@ -1,8 +1,9 @@
 """Replica batches (parallel tempering) and the autotuner on the GPU.
 
-  * a batch sweep is bit-identical, replica by replica, to sweeping each
-    replica alone with its replica seed (positions, axes, patch directions,
-    acceptance counts and the FP64 energy change);
+  * a batch sweep, speculative or one trial at a time, is bit-identical,
+    replica by replica, to sweeping each replica alone with its replica seed
+    (positions, axes, patch directions, acceptance counts and the FP64 energy
+    change);
   * batch energies equal the single-system whole-system energies of the same
     configurations (1e-6 relative: loading a configuration renormalises the
     axes in FP64 and rounds them to FP32 again, which moves their last bits);
@@ -42,12 +43,16 @@ def small_batch(g, sizes=(700, 1300, 400), etas=(0.18, 0.2, 0.15), seed=5):
     return (pos, d, q, ty), np.array(boxes), np.array(sizes)
 
 
-def test_batch_sweep_bit_identical_to_single_replicas():
-    say("case: batch sweep vs single-replica sweeps, 3 replicas")
+@pytest.mark.parametrize("nw", [0, 1])
+def test_batch_sweep_bit_identical_to_single_replicas(nw):
+    """nw 0: the batch's speculative sweep; 1: one trial at a time (the single
+    systems sweep speculatively)."""
+    say("case: batch sweep (block waves %d) vs single-replica sweeps, 3 replicas" % nw)
     g = sc_amd.GpuSystem(0)
     (pos, d, q, ty), boxes, sizes = small_batch(g)
     temps = np.array([0.9, 1.0, 1.2])
     g.set_trial_mode(1)
+    g.set_block_waves(nw)
     st = g.batch_sweep(6, temps, 0.3, ROT_HALF, r=2, seed=77, counter=11)
     top = pt.c3_topology(int(sizes.sum()))
     for k in range(3):
