@@ -213,7 +213,8 @@ def main():
     ap.add_argument("--workload", default="c4_tchpsc_65536")
     ap.add_argument("--r", type=int, default=1, help="checkerboard stencil radius (1: 2^3 colours, 2: 3^3)")
     ap.add_argument("--waves", type=int, default=0,
-                    help="waves per active cell in the sweep kernels (1, 2, 4; 0: 4, and 2 with the network screen)")
+                    help="waves per trial in the sweep kernels (1, 2, 4: one trial at a time; 0: the speculative "
+                         "sweep, four waves with a trial each)")
     ap.add_argument("--trial-mode", type=int, default=1,
                     help="trials per active cell: 0 = its occupancy, 1 = dithered mean occupancy")
     ap.add_argument("--visit", type=int, default=8,
@@ -283,7 +284,8 @@ def main():
     _, start, _, nc = g.build_cells(r=a.r)
     ncell = int(np.prod(nc))
     ncol = (a.r + 1) ** 3
-    # algorithmic bytes per k_sweep_pass launch: every record of the stencils
+    # algorithmic bytes per sweep-pass launch (k_sweep_spec, the speculative
+    # sweep; k_sweep_pass with --waves 1/2/4): every record of the stencils
     # read once (N * 44 B: 40-byte record + 4-byte index), the active cells
     # written back (N/ncol * 40 B) and the cell CSR (4 B per cell)
     bytes_launch = n * 44.0 + n / ncol * 40.0 + 4.0 * (ncell + 1)
@@ -307,7 +309,8 @@ def main():
         achieved = bytes_launch / avg_launch_s / 1e9
         tries = max(1, st["trans_try"] + st["rot_try"])
         acc = st["trans_acc"] + st["rot_acc"]
-        kname = {0: "k_sweep_pass<DA_OFF>", 1: "k_sweep_pass<DA_NET>", 2: "k_sweep_pass<DA_REP>"}[mode]
+        kern = "k_sweep_spec" if a.waves == 0 else "k_sweep_pass"
+        kname = kern + {0: "<DA_OFF>", 1: "<DA_NET>", 2: "<DA_REP>"}[mode]
         r = {"sweeps_per_s": sps, "ms_per_step": 1e3 * wall / a.steps, "wall_s": wall, "device_ms": dev_ms,
              "trial_moves_per_particle": done / a.steps,
              "acceptance": {"translation": st["trans_acc"] / max(1, st["trans_try"]),
@@ -316,8 +319,8 @@ def main():
              "energy_per_particle": g.allToAll() / n,
              "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": achieved / HBM_PEAK_GBS,
-                          "traffic": read_traffic({0: "k_sweep_pass_exact", 1: "k_sweep_pass_da_net",
-                                                   2: "k_sweep_pass_da_rep"}[mode]),
+                          "traffic": read_traffic(kern + {0: "_exact", 1: "_da_net", 2: "_da_rep"}[mode])
+                          if a.waves == 0 else None,
                           "kernel": kname, "bytes_per_launch": bytes_launch, "avg_launch_ms": 1e3 * avg_launch_s,
                           "launches": launches}}
         # ensemble average of E/N over the following untimed sweeps (every 25)

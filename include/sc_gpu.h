@@ -44,7 +44,7 @@ typedef struct {
     uint64_t trans_acc, trans_try, rot_acc, rot_try;   /* Statistics trans/rot acc/rej */
     uint64_t left_cell;       /* translations rejected for leaving their cell */
     uint64_t stage1_acc, stage1_try;                    /* delayed acceptance: screen on the exact cheap terms */
-    uint64_t stage2_acc, stage2_try;                    /* delayed acceptance: network screen */
+    uint64_t stage2_acc, stage2_try;                    /* delayed acceptance: network screen (trials it ran for) */
     double energy_change;     /* sum of accepted dE (for the drift check, updater.cpp:122-150) */
     uint64_t bond_rejected;   /* moves rejected because a bonded chain partner lay outside the stencil */
     /* whole-molecule moves (sc_set_chain_moves): Statistics chainm / chainr acc and tries */
@@ -213,8 +213,9 @@ int sc_set_block_waves(sc_ctx* ctx, int nw);
  * partners), and evaluation resumes from there.  The chain of states and
  * decisions is the one-trial-at-a-time sweep's (Updater::simulate's sequence,
  * updater.cpp:206-230, within each cell).  A trial whose moving particle has
- * more than 384 particles within maxcut fails the sweep with SC_ECAPACITY; on = 0
- * evaluates such systems.  No counterpart in the reference (its loop is
+ * more than 320 particles within maxcut, or more than 192 (pair, state)
+ * entries within the attraction's range, fails the sweep with SC_ECAPACITY;
+ * on = 0 evaluates such systems.  No counterpart in the reference (its loop is
  * sequential). */
 int sc_set_speculation(sc_ctx* ctx, int on);
 /* The sweep keeps the derived vectors (patch sides, second patch, chiral
@@ -340,6 +341,15 @@ int sc_mlp_pair_features(sc_ctx* ctx, int m, const int32_t* i, const int32_t* j,
  * Boltzmann distribution; sc_sweep_stats.stage1_* / stage2_* count the
  * screens. */
 int sc_set_surrogate(sc_ctx* ctx, int mode);
+/* The network screen of mode 1 runs for trials with at least m (pair, state)
+ * entries within the attraction's range (both states counted, so a move and
+ * its reverse are screened alike and the scheme stays exact); the others go
+ * from the first screen to the exact energy.  m < 0 (default): the entries
+ * one wave's round of patch-pair items holds, plus one (64 / (2 x patch pairs
+ * per pair) + 1: 9 for two-patch particles), so the network stands in for
+ * phase C where that takes more than one round.  m = 0: every trial.  No
+ * counterpart in the reference. */
+int sc_set_network_min_entries(sc_ctx* ctx, int m);
 
 /* --- replica batches (parallel tempering) ------------------------------------ */
 /* The replicas of MoveCreator::replicaExchangeMove (movecreator.cpp:555-760),

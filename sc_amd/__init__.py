@@ -52,6 +52,7 @@ SIGNATURES = {
     "sc_write_config": (ctypes.c_long, [_P, ctypes.c_char_p, ctypes.c_long]),
     "sc_set_block_waves": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_set_speculation": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_set_network_min_entries": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_set_home_cache": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_set_visit_factor": (ctypes.c_int, [_P, ctypes.c_int]),
     "sc_set_max_active_cells": (ctypes.c_int, [_P, ctypes.c_int]),
@@ -590,6 +591,10 @@ class GpuSystem:
     def set_block_waves(self, nw):
         """waves per trial of an active cell in the sweep (1, 2, 4; 0 = four waves with a trial each)"""
         self._ck(self._L.sc_set_block_waves(self._h, int(nw)), "set_block_waves")
+
+    def set_network_min_entries(self, m=-1):
+        """the network screen runs for trials with >= m (pair, state) entries in range (< 0: automatic; 0: all)"""
+        self._ck(self._L.sc_set_network_min_entries(self._h, int(m)), "set_network_min_entries")
 
     def set_speculation(self, on=True):
         """speculative trials, one per wave, in sweep() (default on; block waves 0)"""

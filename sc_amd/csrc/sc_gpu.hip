@@ -1006,6 +1006,7 @@ static_assert(sizeof(TrialRec) == 64, "trial record size");
 struct MlpConst {
     float lsc, rcut, pdis, pswitch;
     int tnet;   // dense type of the network's pair (the screen's other pairs count 0)
+    int minx;   // the network screens trials with at least minx (pair, state) entries in range
 };
 
 // Phase A of a trial: every staged particle j != k inside sqmaxcut of the
@@ -1101,7 +1102,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                                                const unsigned char* __restrict__ mlp_img,
                                                const MlpConst& mc, const ChainTab& ct, double thr1, double thr2,
                                                const float* hvk, bool translate, double& eo, double& en,
-                                               double& des, int& stage, int ncap) {
+                                               double& des, int& stage, int ncap, int xcap) {
     static_assert(!WS || NW == 1, "a wave's own trial is a one-wave trial");
     const int tid = WS ? lane_id() : (int)threadIdx.x, lane = tid & (WAVE - 1), BS = WAVE * NW;
     int t = uo.w;
@@ -1212,12 +1213,16 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             }
         }
         int pos = block_compact1<NW>(need, nx, T.cnt, cpar);
-        if (need) {
+        if (need && pos < xcap) {
             T.xl[pos] = jj | (which << 30);
             if constexpr (DA == DA_NET) T.xd[pos] = sqrtf(dseg2);
         }
     }
     tsync<WS>();
+    if (nx > xcap) {   // the expensive list is full (only with a wave's own bounded lists)
+        stage = -2;
+        return false;
+    }
     // screens of the delayed acceptance: (1) the exact cheap terms already
     // summed (repulsion, sphere pairs), (2) with DA_NET the network's patch
     // attraction, tiles of 16 pairs on MFMA.  With four waves the network runs
@@ -1240,6 +1245,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         des = rep;
         if constexpr (DA == DA_NET && !NET_PAR) {
             stage = 1;
+            if (nx >= mc.minx) {   // block-uniform: the entries of both states, the same for the reverse move
             double so = 0.0, sn = 0.0;
             const int wv = tid >> 6;
             for (int t0 = 0; t0 < nx; t0 += MLP_TILE) {   // block-uniform: every wave in every tile
@@ -1261,8 +1267,10 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             double ds = dn - d0;
             if (!(ds <= 0.0 || ds < thr2)) return false;
             des = rep + ds;
+            }
         }
-        stage = 2;
+        // 2: through every screen; 3: past the first, the network not run
+        stage = (DA == DA_NET && nx < mc.minx) ? 3 : 2;
     }
     // phase C: (expensive pair, patch pair, side) items.  The two wedge
     // intersections of a patch pair run on neighbouring lanes (side = lane
@@ -1272,7 +1280,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         // the network screen on the last wave: the whole network per tile,
         // its operands read from global memory (L1/L2) layer by layer
         double so = 0.0, sn = 0.0;
-        for (int t0 = 0; t0 < nx; t0 += MLP_TILE) {
+        for (int t0 = 0; t0 < (nx >= mc.minx ? nx : 0); t0 += MLP_TILE) {
             float f[MLP_NF];
             int which = 0;
             bool net = pair_features(S, T, nx, t0 + (lane & (MLP_TILE - 1)), t, uo, un, mvo, mvn, sc, mc, f, which);
@@ -1330,10 +1338,10 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     block_sum2<NW>(ao, an, T.red, rpar, eo, en);
     if constexpr (NET_PAR) {
         stage = 1;
-        double ds = *T.netd;   // written before the sum's barrier
+        double ds = *T.netd;   // written before the sum's barrier (0 when the network did not run)
         if (!(ds <= 0.0 || ds < thr2)) return false;
         des = rep + ds;
-        stage = 2;
+        stage = nx < mc.minx ? 3 : 2;
     }
     return true;
 }
@@ -1539,7 +1547,7 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
         const float* hvk = hcache ? hv + (k - S.h0) * MV_STRIDE : nullptr;
         pass = trial_energies<DA, NW>(S, k, uo, un, oo, qo, on, qn, sc, pp, tp, ppmax, T, ml, mr, mlp_img, mc, ct,
                                       R.thr1,
-                                      R.thr2, hvk, translate, eo, en, des, stage, cap);
+                                      R.thr2, hvk, translate, eo, en, des, stage, cap, 2 * cap);
         if (stage < 0) {   // a chain partner outside the stencil
             n_bond++;
             __syncthreads();
@@ -1549,8 +1557,8 @@ __device__ __forceinline__ void sweep_cell(int b, const Grid& g, const SysConst&
             n_s1try++;
             if (stage >= 1) n_s1acc++;
             if constexpr (DA == DA_NET) {
-                if (stage >= 1) n_s2try++;
-                if (stage >= 2) n_s2acc++;
+                if (stage == 1 || stage == 2) n_s2try++;   // 3: the network did not run
+                if (stage == 2) n_s2acc++;
             }
             if (!pass) { __syncthreads(); continue; }
         }
@@ -1622,7 +1630,8 @@ __global__ __launch_bounds__(WAVE * NW, 2) void k_sweep_pass(Grid g, SysConst sc
 // of states and decisions is the sequential sweep's (only the FP64 sums run
 // in another order, as between the 1- and 4-wave sweeps).
 #define SPEC_W 4
-#define SPEC_LCAP 384   // near-list entries of one wave's trial (a stencil of ~450 has ~60 within maxcut)
+#define SPEC_LCAP 320   // near-list entries of one wave's trial (a stencil of ~450 has ~60 within maxcut)
+#define SPEC_XCAP 192   // its (pair, state) entries within the attraction's range (typically 10-20)
 
 struct __align__(16) SpecRes {
     uint4 uo, un;    // position at the round's start and proposed
@@ -1630,7 +1639,7 @@ struct __align__(16) SpecRes {
     float2 qn;
     int k, id;       // staged index, particle ID
     double de;       // E_new - E_old (kept trials that reached the final stage)
-    int status;      // 0 evaluated, 1 left the cell, 2 chain partner outside the stencil, 3 near list full
+    int status;      // 0 evaluated, 1 left the cell, 2 chain partner outside the stencil, 3 a list full
     int translate, acc, stage;
     int dep;         // bit i: the trial depends on the round's trial i, should that one be accepted
 };
@@ -1643,12 +1652,12 @@ struct SpecLayout {
     size_t wbase, sten, misc, res, recs, hv, ml, tab, end;
 };
 __host__ __device__ inline size_t spec_align(size_t x, size_t a) { return (x + a - 1) / a * a; }
-__host__ __device__ inline SpecLayout spec_layout(int cap, int lcap, bool net, size_t tab_bytes) {
+__host__ __device__ inline SpecLayout spec_layout(int cap, int lcap, int xcap, bool net, size_t tab_bytes) {
     SpecLayout L;
     L.nl = 0;
     L.xl = L.nl + 4 * (size_t)lcap;
-    L.xd = L.xl + 8 * (size_t)lcap;
-    L.mv = spec_align(L.xd + (net ? 8 * (size_t)lcap : 0), 16);
+    L.xd = L.xl + 4 * (size_t)xcap;
+    L.mv = spec_align(L.xd + (net ? 4 * (size_t)xcap : 0), 16);
     L.bp = L.mv + 4 * 2 * MV_STRIDE;
     L.cnt = L.bp + 16;
     L.red = spec_align(L.cnt + 4 * 2 * CNT_HALF, 16);
@@ -1745,7 +1754,7 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
     int cell = cx + g.nc[0] * (cy + g.nc[1] * cz);
     int nh = starts[cell + 1] - starts[cell];
     if (nh == 0) return;
-    const SpecLayout L = spec_layout(cap, lcap, NET, 0);
+    const SpecLayout L = spec_layout(cap, lcap, SPEC_XCAP, NET, 0);
     Stage S;
     S.u = (uint4*)smem;
     S.o = (float4*)(S.u + cap);
@@ -1843,7 +1852,7 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
                 const float* hvk = hcache ? hv + (k - S.h0) * MV_STRIDE : nullptr;
                 bool pass = trial_energies<DA, 1, true>(S, k, uo, un, oo, qo, on, qn, sc, pp, tp, ppmax, T, ml, mr,
                                                         mlp_img, mc, ct, R.thr1, R.thr2, hvk, translate, eo, en, des,
-                                                        stage, lcap);
+                                                        stage, lcap, SPEC_XCAP);
                 if (stage == -1) {
                     status = 2;
                 } else if (stage == -2) {
@@ -1901,8 +1910,8 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
                     n_s1try++;
                     if (a.stage >= 1) n_s1acc++;
                     if constexpr (DA == DA_NET) {
-                        if (a.stage >= 1) n_s2try++;
-                        if (a.stage >= 2) n_s2acc++;
+                        if (a.stage == 1 || a.stage == 2) n_s2try++;   // 3: the network did not run
+                        if (a.stage == 2) n_s2acc++;
                     }
                 }
                 if (a.acc) {
@@ -3610,6 +3619,7 @@ struct sc_ctx {
     int e_age = 0;                      // volume moves since the last full evaluation
     int sweep_nw = 0;                   // waves per active cell in the sweep kernels (1, 2, 4; 0 = by mode)
     int spec = 1;                       // single-system sweep: speculative trials, one per wave (sweep_nw 0)
+    int net_minx = -1;                  // network screen: least (pair, state) entries (< 0: one phase-C round)
     int home_cache = HCAP;              // cell size limit of the sweep's derived-vector cache (0: off)
     int visit = 1;                      // colour passes of a sweep = colours / visit, visit x the trials each
     unsigned char* noalloc = nullptr;   // this device's address of `scratch` (sc_pair_energies_noalloc)
@@ -3986,8 +3996,9 @@ static int check_flag(sc_ctx* ctx, const char* what) {
         HIPCK(hipMemset(ctx->d_flag, 0, sizeof(int)));
         if (f & 4) return fail(ctx, SC_ECAPACITY, std::string(what) + ": a coarse cell held more than 4096 molecules");
         if (f & 16)
-            return fail(ctx, SC_ECAPACITY, std::string(what) + ": more than 384 particles within maxcut of a moving "
-                                                               "particle (the speculative sweep's near list); "
+            return fail(ctx, SC_ECAPACITY, std::string(what) + ": more than 320 particles within maxcut, or 192 "
+                                                               "(pair, state) entries within the attraction's range, "
+                                                               "of a moving particle (the speculative sweep's lists); "
                                                                "sc_set_speculation(ctx, 0) evaluates such systems");
         return fail(ctx, SC_ECAPACITY, std::string(what) + ": a stencil held more particles than the stage capacity");
     }
@@ -5403,15 +5414,18 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         size_t lds_spec = 0;
         if (spec) {
             size_t tabb = sweep_table_bytes(ctx->nt);
-            long room = (long)SWEEP_LDS_BUDGET - (long)spec_layout(0, SPEC_LCAP, da, tabb).end - 16;
+            long room = (long)SWEEP_LDS_BUDGET - (long)spec_layout(0, SPEC_LCAP, SPEC_XCAP, da, tabb).end - 16;
             int cmax = room > 0 ? (int)(room / (long)stage_bytes(1)) : 0;
             if (cap_spec > cmax) cap_spec = (cmax / 64) * 64;
             spec = cap_spec >= 64;
-            if (spec) lds_spec = spec_layout(cap_spec, SPEC_LCAP, da, tabb).end;
+            if (spec) lds_spec = spec_layout(cap_spec, SPEC_LCAP, SPEC_XCAP, da, tabb).end;
         }
         if (da && (ctx->mlp_type < 0 || ctx->ref2dense[ctx->mlp_type] < 0))
             return fail(ctx, SC_EINVAL, "the network screen needs the network's particle type in the system");
         MlpConst mlpc = ctx->mlp_c;
+        // the network screens trials whose phase C would take more than one
+        // wave's round (2 sides x ppmax patch pairs per entry), unless set
+        mlpc.minx = ctx->net_minx >= 0 ? ctx->net_minx : WAVE / (2 * std::max(1, ctx->ppmax)) + 1;
         mlpc.tnet = ctx->mlp_type >= 0 ? ctx->ref2dense[ctx->mlp_type] : -1;
         // LDS: stage + near/expensive lists + segment distances + moving particle
         // + stencil scratch (+ the network and its activation tile)
@@ -5719,11 +5733,11 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
         size_t lds_spec = 0;
         if (spec) {
             size_t tabb = sweep_table_bytes(ctx->nt);
-            long room = (long)SWEEP_LDS_BUDGET - (long)spec_layout(0, SPEC_LCAP, false, tabb).end - 16;
+            long room = (long)SWEEP_LDS_BUDGET - (long)spec_layout(0, SPEC_LCAP, SPEC_XCAP, false, tabb).end - 16;
             int cmax = room > 0 ? (int)(room / (long)stage_bytes(1)) : 0;
             if (cap_spec > cmax) cap_spec = (cmax / 64) * 64;
             spec = cap_spec >= 64;
-            if (spec) lds_spec = spec_layout(cap_spec, SPEC_LCAP, false, tabb).end;
+            if (spec) lds_spec = spec_layout(cap_spec, SPEC_LCAP, SPEC_XCAP, false, tabb).end;
         }
         size_t fixed = sizeof(float) * 2 * MV_STRIDE + sizeof(int) * (2 * 125 + 2) + SWEEP_SMALL_LDS +
                        sweep_table_bytes(ctx->nt);
@@ -6096,6 +6110,7 @@ static MlpConst mlp_const_for(const schost::Ia& p) {
     c.pdis = (float)p.pdis;
     c.pswitch = (float)p.pswitch;
     c.tnet = -1;
+    c.minx = 0;
     return c;
 }
 
@@ -6485,6 +6500,14 @@ static int sc_set_speculation_impl(sc_ctx* ctx, int on) {
 }
 
 int sc_set_speculation(sc_ctx* ctx, int on) { SC_GUARD(ctx, sc_set_speculation_impl(ctx, on)) }
+
+static int sc_set_network_min_entries_impl(sc_ctx* ctx, int m) {
+    if (!ctx) return SC_EINVAL;
+    ctx->net_minx = m < 0 ? -1 : m;
+    return SC_OK;
+}
+
+int sc_set_network_min_entries(sc_ctx* ctx, int m) { SC_GUARD(ctx, sc_set_network_min_entries_impl(ctx, m)) }
 
 static int sc_set_home_cache_impl(sc_ctx* ctx, int cap) {
     if (!ctx || cap < 0) return SC_EINVAL;

@@ -241,6 +241,7 @@ def test_chain_ensemble_matches_cpu_oracle(chain_nvt_oracle, mode):
         say("network: held-out rmse %.4f (target std %.4f)" % (res["val_rmse"], res["target_std"]))
     g.set_config(box, pos, d, q, t)
     g.set_surrogate(mode)
+    g.set_network_min_entries(0)   # the network screens every trial that passes the first screen
     n = len(t)
     g.sweep(400, 1.1, 0.3, ROT_HALF, r=2, seed=5, counter=0)
     obs = []

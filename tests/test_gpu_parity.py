@@ -405,6 +405,7 @@ def test_block_waves_reproduce_wave_sweep(dev, mode):
         g.set_config(w["box"], pos, d, q)
         g.set_block_waves(nw)
         g.set_surrogate(mode)
+        g.set_network_min_entries(0)
         s = g.sweep(10, 1.0, 0.3, 20.0 / 180 * np.pi / 4, r=1, seed=5, counter=0)
         _, p, dd, qq = g.get_config()
         out.append((s, p, dd, qq))

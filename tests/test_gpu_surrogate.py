@@ -198,6 +198,7 @@ def test_da_sweep_ensemble_matches_cpu_oracle(ensemble_system, mode):
         surrogate.train(g, PSC, nsamples=1 << 20, epochs=4, seed=5, log=say)
     g.set_config(w["box"], pos, d, q, w["types"])
     g.set_surrogate(mode)
+    g.set_network_min_entries(0)   # the network screens every trial that passes the first screen
     n = len(w["types"])
     tot = ensemble_check(g, n, ec, rot_half, "DA mode %d" % mode)
     g.close()
@@ -223,6 +224,7 @@ def test_da_gain_over_exact_sweep():
     prepare.initial_state(g, w, seed=7)
     box, pos, d, q = g.get_config()
     g.set_trial_mode(1)
+    g.set_network_min_entries(0)   # the network screens every trial that passes the first screen
 
     def run(label, mode, k=20):
         g.set_config(box, pos, d, q, w["types"])

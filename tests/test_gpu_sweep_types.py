@@ -73,6 +73,7 @@ def gpu_system(prepared, name, mode, home_cache=None):
     if mode == 1:
         g.mlp_init(SYSTEMS[name][3], seed=1)
         g.mlp_set_weights(wts)
+        g.set_network_min_entries(0)   # the network screens every trial that passes the first screen
     g.set_config(box, pos, d, q, w["types"])
     g.set_trial_mode(1)
     g.set_surrogate(mode)
@@ -163,6 +164,36 @@ def test_speculative_sweep_reproduces_sequential(prepared, name, mode):
     assert abs(s1["energy_change"] - s2["energy_change"]) <= 1e-9 * max(1.0, abs(s1["energy_change"]))
     for a, b in zip(c1, c2):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_network_threshold_is_the_same_in_every_variant(prepared, name):
+    """With the network screening only trials of at least the default number
+    of (pair, state) entries, the speculative sweep, the 4-wave sweep (the
+    network on the last wave beside phase C) and the 1-wave sweep give the
+    same trajectory, and the network runs for fewer trials than the first
+    screen passes."""
+    say("case: %s, network screen at the default entry threshold, speculative / 4 / 1 waves" % name)
+    out = []
+    for nw in (0, 4, 1):
+        g = gpu_system(prepared, name, 1)
+        g.set_network_min_entries(-1)
+        g.set_visit_factor(8)
+        g.set_block_waves(nw)
+        s = g.sweep(10, 1.0, 0.3, ROT_HALF, r=1, seed=9, counter=0)
+        out.append((s, g.get_config()))
+        g.close()
+    s0 = out[0][0]
+    say("first screen passed %d of %d, network ran for %d (passed %d)" % (s0["stage1_acc"], s0["stage1_try"],
+                                                                        s0["stage2_try"], s0["stage2_acc"]))
+    assert s0["stage2_try"] < s0["stage1_acc"]
+    for s1, c1 in out[1:]:
+        for k in ("trans_acc", "trans_try", "rot_acc", "rot_try", "left_cell", "stage1_acc", "stage1_try",
+                  "stage2_acc", "stage2_try"):
+            assert s1[k] == s0[k], k
+        assert abs(s1["energy_change"] - s0["energy_change"]) <= 1e-9 * max(1.0, abs(s0["energy_change"]))
+        for a, b in zip(out[0][1], c1):
+            np.testing.assert_array_equal(a, b)
 
 
 def _blocked(x, nb=8):
