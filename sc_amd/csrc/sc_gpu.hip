@@ -1815,7 +1815,7 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
     if (hcache)
         for (int i = tid; i < nh; i += WAVE * SPEC_W)
             mv_store(hv + i * MV_STRIDE, load_part<float>(S.o[S.h0 + i], S.q[S.h0 + i], S.u[S.h0 + i].w, tp));
-    // thread 0 keeps the counters of the kept trials
+    // wave 0 keeps the counters of the kept trials (the same on all its lanes)
     unsigned n_tacc = 0, n_ttry = 0, n_racc = 0, n_rtry = 0, n_left = 0, n_s1try = 0, n_s1acc = 0, n_s2try = 0,
              n_s2acc = 0, n_bond = 0;
     double dsum = 0.0;
@@ -1881,49 +1881,52 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
             }
         }
         __syncthreads();
-        if (tid == 0) {
-            int nfin = 0, abort = 0;
-            for (int i = 0; i < W; i++) {
-                const SpecRes& a = res[i];
-                if (a.status == 3) {
-                    abort = 1;
-                    break;
-                }
-                bool dep = false;
-                for (int j = 0; j < i && !dep; j++) dep = res[j].acc && ((a.dep >> j) & 1);
-                if (dep) break;
-                nfin++;
-                if (a.translate) {
-                    n_ttry++;
-                    if (a.status == 1) {
-                        n_left++;
-                        continue;
-                    }
-                } else {
-                    n_rtry++;
-                }
-                if (a.status == 2) {
-                    n_bond++;
-                    continue;
-                }
-                if constexpr (DA != DA_OFF) {
-                    n_s1try++;
-                    if (a.stage >= 1) n_s1acc++;
-                    if constexpr (DA == DA_NET) {
-                        if (a.stage == 1 || a.stage == 2) n_s2try++;   // 3: the network did not run
-                        if (a.stage == 2) n_s2acc++;
-                    }
-                }
-                if (a.acc) {
-                    if (a.translate) n_tacc++; else n_racc++;
-                    dsum += a.de;
-                    S.u[a.k] = a.un;
-                    S.o[a.k] = a.on;
-                    S.q[a.k] = a.qn;
+        if (w == 0) {
+            // wave 0 takes the round's results, a lane each, in trial order:
+            // kept are the trials before the first that depends on an earlier
+            // accepted one (or overflowed); the counters of the kept trials are
+            // ballot counts; each kept accepted trial commits its own particle
+            // (distinct particles: the same particle is a dependence)
+            const bool valid = lane < W;
+            const SpecRes& a = res[valid ? lane : 0];
+            const int st = valid ? a.status : 0, dep = valid ? a.dep : 0, stg = valid ? a.stage : 0;
+            const bool acc = valid && a.acc, tr = valid && a.translate;
+            const unsigned long long accm = __ballot(acc);
+            const bool depends = valid && (dep & (int)accm & ((1 << lane) - 1)) != 0;
+            const bool full = valid && st == 3;
+            const unsigned long long fullm = __ballot(full);
+            const unsigned long long stopm = __ballot(depends || full);
+            const int first = stopm ? __ffsll((long long)stopm) - 1 : W;
+            const bool kept = valid && lane < first;
+            const bool left = kept && tr && st == 1, bond = kept && !left && st == 2;
+            const bool ev = kept && !left && !bond;   // reached the acceptance decision
+            n_ttry += __popcll(__ballot(kept && tr));
+            n_rtry += __popcll(__ballot(kept && !tr));
+            n_left += __popcll(__ballot(left));
+            n_bond += __popcll(__ballot(bond));
+            if constexpr (DA != DA_OFF) {
+                n_s1try += __popcll(__ballot(ev));
+                n_s1acc += __popcll(__ballot(ev && stg >= 1));
+                if constexpr (DA == DA_NET) {
+                    n_s2try += __popcll(__ballot(ev && (stg == 1 || stg == 2)));   // 3: the network did not run
+                    n_s2acc += __popcll(__ballot(ev && stg == 2));
                 }
             }
-            misc[0] = nfin;
-            misc[1] = abort;
+            const bool com = ev && acc;
+            n_tacc += __popcll(__ballot(com && tr));
+            n_racc += __popcll(__ballot(com && !tr));
+            if (com) {
+                S.u[a.k] = a.un;
+                S.o[a.k] = a.on;
+                S.q[a.k] = a.qn;
+            }
+            const double dei = com ? a.de : 0.0;
+#pragma unroll
+            for (int q = 0; q < SPEC_W; q++) dsum += __shfl(dei, q, WAVE);   // in trial order
+            if (lane == 0) {
+                misc[0] = first;
+                misc[1] = (first < W && ((fullm >> first) & 1ull)) ? 1 : 0;
+            }
         }
         __syncthreads();
         const int nfin = misc[0];
