@@ -337,9 +337,14 @@ def main():
                                                  "sweeps": len(es) * 25}
         if mode:
             r["screen1_pass"] = st["stage1_acc"] / max(1, st["stage1_try"])
+            passed = st["stage1_acc"]
             if mode == 1:
+                # the network runs for the trials past screen 1 with enough entries in range
+                # (stage2_try of them); the others reach the exact stage unscreened
+                r["network_screened"] = st["stage2_try"] / max(1, st["stage1_try"])
                 r["screen2_pass"] = st["stage2_acc"] / max(1, st["stage2_try"])
-            r["exact_stage_accept"] = acc / max(1, st["stage2_acc"] if mode == 1 else st["stage1_acc"])
+                passed = st["stage1_acc"] - st["stage2_try"] + st["stage2_acc"]
+            r["exact_stage_accept"] = acc / max(1, passed)
         log("%s sweep: %.2f sweeps/s (%.3f ms/sweep wall, %.3f ms device), %d pass launches, kernel %.3f ms, "
             "acceptance %.4f" % (label, sps, 1e3 * wall / a.steps, dev_ms / a.steps, launches, kern_ms, acc / tries))
         return r
