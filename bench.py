@@ -403,6 +403,7 @@ def main():
         from sc_amd import pt
         gp = sc_amd.GpuSystem(0)
         gp.set_block_waves(a.waves)
+        gp.set_visit_factor(a.visit)   # for every replica whose colour count it divides
         t0 = time.time()
         nrep_n, temps, _ = pt.c3_batch(gp)
         tb = time.time() - t0
@@ -419,7 +420,8 @@ def main():
                  "sweeps": a.pt_sweeps, "exchange_every": 10, "wall_s": dt,
                  "batch_sweeps_per_s": moves / float(nrep_n.sum()) / dt, "particle_moves_per_s": moves / dt,
                  "exchange_acceptance": float(acc.sum()) / max(1, int(tried.sum())),
-                 "launches_per_sweep": 8 if r_t == 1 else 27}
+                 "launches_per_sweep": (8 // a.visit if 8 % a.visit == 0 else 8) if r_t == 1
+                 else (27 // a.visit if 27 % a.visit == 0 else 27)}
         log("batched tempering: %d replicas / %d particles, %.2f batch sweeps/s, %.3e moves/s, exchanges %d/%d" % (
             len(nrep_n), nrep_n.sum(), ptres["batch_sweeps_per_s"], ptres["particle_moves_per_s"], acc.sum(),
             tried.sum()))

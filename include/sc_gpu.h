@@ -232,9 +232,10 @@ int sc_set_home_cache(sc_ctx* ctx, int cap);
  * Updater::simulate, updater.cpp:206-230, picks its particles at random; a
  * sweep need not touch every particle).  Longer chains per pass average the
  * cells' uneven trial costs, so the slowest cell of a pass lags the mean
- * less, and a sweep launches fewer passes.  Default 1; replica batches
- * always use 1.  Every pass is a product of independent Metropolis chains,
- * whatever v. */
+ * less, and a sweep launches fewer passes.  Default 1.  Replica batches
+ * (sc_batch_sweep) use v for every replica whose number of colours it
+ * divides and 1 for the others.  Every pass is a product of independent
+ * Metropolis chains, whatever v. */
 int sc_set_visit_factor(sc_ctx* ctx, int v);
 /* Largest number of active cells of one colour in sc_sweep's grid: when the
  * grid of cells >= maxcut/r would give more, it takes coarser cells (fewer

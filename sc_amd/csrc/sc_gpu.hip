@@ -342,6 +342,7 @@ __global__ void k_cell_scatter(int n, const int* __restrict__ cell_tmp, const in
 // indices are distinct), a larger one (grids of one or two cells per axis) is
 // insertion-sorted by its first lane
 #define CELL_SORT_CELLS 4   // cells (waves) per 256-thread block
+#define CELL_SORT_CHUNKS 8  // cells of up to 512 particles sort in registers; larger ones by one lane
 __global__ __launch_bounds__(WAVE * CELL_SORT_CELLS) void k_cell_sort(int ncell, const int* __restrict__ starts,
                                                                      int* __restrict__ src_of_pos,
                                                                      const int* __restrict__ ID) {
@@ -357,6 +358,31 @@ __global__ __launch_bounds__(WAVE * CELL_SORT_CELLS) void k_cell_sort(int ncell,
         for (int k = 0; k < n; k++) rank += __shfl(key, k, WAVE) < key ? 1 : 0;
         // every lane's read above precedes its write: the write needs the ranks
         if (lane < n) src_of_pos[b + rank] = v;
+        return;
+    }
+    if (n <= CELL_SORT_CHUNKS * WAVE) {
+        // the same rank sort over up to CELL_SORT_CHUNKS chunks of 64 held in
+        // registers (cells of small replicas, whose axes may hold one cell)
+        const int nch = (n + WAVE - 1) / WAVE;
+        int vv[CELL_SORT_CHUNKS], kk[CELL_SORT_CHUNKS], rk[CELL_SORT_CHUNKS];
+#pragma unroll
+        for (int m = 0; m < CELL_SORT_CHUNKS; m++) {
+            int i = m * WAVE + lane;
+            vv[m] = (m < nch && i < n) ? src_of_pos[b + i] : 0;
+            kk[m] = (m < nch && i < n) ? ID[vv[m]] : 0x7fffffff;   // IDs are distinct; padding ranks last
+            rk[m] = 0;
+        }
+        for (int m2 = 0; m2 < nch; m2++)
+            for (int k = 0; k < WAVE; k++) {
+                int key = __shfl(kk[m2], k, WAVE);
+#pragma unroll
+                for (int m = 0; m < CELL_SORT_CHUNKS; m++) rk[m] += key < kk[m] ? 1 : 0;
+            }
+#pragma unroll
+        for (int m = 0; m < CELL_SORT_CHUNKS; m++) {
+            int i = m * WAVE + lane;
+            if (m < nch && i < n) src_of_pos[b + rk[m]] = vv[m];
+        }
         return;
     }
     if (lane != 0) return;
@@ -3501,6 +3527,21 @@ __global__ void k_copy(long n16, const uint4* __restrict__ a, uint4* __restrict_
 }
 
 // ------------------------------------------------------------------ context
+// A ring of pinned host buffers for the per-sweep parameter uploads of
+// replica batches: a copy from pinned memory is truly asynchronous, so the
+// host prepares the next sweep while the device runs this one (a pageable
+// copy waits for the stream and leaves the device idle while the host
+// launches); each slot's event keeps the host from rewriting a buffer whose
+// copy has not run yet.
+#define PIN_SLOTS 4
+struct PinRing {
+    unsigned char* host[PIN_SLOTS] = {};
+    size_t cap[PIN_SLOTS] = {};
+    hipEvent_t ev[PIN_SLOTS] = {};
+    bool used[PIN_SLOTS] = {};
+    int next = 0;
+};
+
 struct sc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -3559,6 +3600,7 @@ struct sc_ctx {
     long last_launches = 0;
     bool profile = false;               // time every sweep-pass launch with events
     std::vector<hipEvent_t> kev;        // event pairs for the launches of one call
+    PinRing pin_rg, pin_rp;             // uploads of the batch grids and pass parameters
     // surrogate
     float* mlp_w = nullptr;             // FP32 master weights (MLP_NPARAM)
     float* mlp_g = nullptr;
@@ -3672,6 +3714,27 @@ static ChainTab chain_tab(const sc_ctx* ctx) {
         ctx->last_call = #kernel;                                                      \
         hipLaunchKernelGGL(kernel, __VA_ARGS__);                                       \
     } while (0)
+
+// copy `bytes` from host memory `src` to device `dst` on the context's
+// stream through the next slot of ring r (see PinRing)
+static int pin_upload(sc_ctx* ctx, PinRing& r, void* dst, const void* src, size_t bytes) {
+    int s = r.next;
+    r.next = (s + 1) % PIN_SLOTS;
+    if (r.used[s]) HIPCK(hipEventSynchronize(r.ev[s]));
+    if (bytes > r.cap[s]) {
+        if (r.host[s]) HIPCK(hipHostFree(r.host[s]));
+        r.host[s] = nullptr;
+        r.cap[s] = 0;
+        HIPCK(hipHostMalloc((void**)&r.host[s], bytes));
+        r.cap[s] = bytes;
+    }
+    if (!r.ev[s]) HIPCK(hipEventCreateWithFlags(&r.ev[s], hipEventDisableTiming));
+    std::memcpy(r.host[s], src, bytes);
+    HIPCK(hipMemcpyAsync(dst, r.host[s], bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCK(hipEventRecord(r.ev[s], ctx->stream));
+    r.used[s] = true;
+    return SC_OK;
+}
 
 static int fail(sc_ctx* ctx, int code, const std::string& m) {
     ctx->err = m;
@@ -4192,6 +4255,11 @@ static void sc_destroy_impl(sc_ctx* ctx) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->kev) hipEventDestroy(e);
+    for (PinRing* r : {&ctx->pin_rg, &ctx->pin_rp})
+        for (int k = 0; k < PIN_SLOTS; k++) {
+            if (r->host[k]) hipHostFree(r->host[k]);
+            if (r->ev[k]) hipEventDestroy(r->ev[k]);
+        }
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
@@ -5637,7 +5705,7 @@ static int batch_build_grid(sc_ctx* ctx, int r, const std::vector<std::array<uin
     }
     if ((rc = batch_alloc(ctx))) return rc;
     int n = ctx->n, c = ctx->cur, nb = (n + 255) / 256;
-    HIPCK(hipMemcpyAsync(ctx->d_rg, ctx->rep_grid.data(), R * sizeof(RepGrid), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = pin_upload(ctx, ctx->pin_rg, ctx->d_rg, ctx->rep_grid.data(), R * sizeof(RepGrid)))) return rc;
     HIPCK(hipMemsetAsync(ctx->counts, 0, sizeof(int) * total, ctx->stream));
     HIPCK(hipMemsetAsync(ctx->fill, 0, sizeof(int) * total, ctx->stream));
     SC_LAUNCH(k_cell_count_batch, dim3(nb), dim3(256), 0, ctx->stream, n, R, ctx->d_rg, ctx->P[c], ctx->ID[c],
@@ -5716,6 +5784,8 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
     std::vector<int> nblk(ncol);
     std::vector<SweepPlan> plans(R);
     std::vector<std::array<uint32_t, 3>> shifts(R);
+    std::vector<std::array<int, 27>> vcols(R);
+    std::vector<int> nvis(R), vfac(R);
     long launches = 0;
     HIPCK(hipEventRecord(ctx->ev0, ctx->stream));
     for (int sidx = 0; sidx < nsweeps; sidx++) {
@@ -5748,7 +5818,24 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
         int capmax = (int)((SWEEP_LDS_BUDGET - fixed) / per);
         if (cap > capmax) cap = (capmax / 64) * 64;
         size_t lds = per * cap + fixed;
-        for (int ci = 0; ci < ncol; ci++) {
+        // each replica's colours in its random order (an axis with a single
+        // cell has period 1), and its visit factor: the context's where it
+        // divides the replica's number of colours, else 1 (as sc_sweep on the
+        // replica alone, which refuses the other factors)
+        int nlaunch = 0;
+        for (int q = 0; q < R; q++) {
+            const Grid& g = ctx->rep_grid[q].g;
+            int pa[3], nv = 0;
+            for (int k = 0; k < 3; k++) pa[k] = (g.nc[k] >= p) ? p : 1;
+            for (int ci = 0; ci < ncol; ci++) {
+                int col = plans[q].order[ci];
+                if (col % p < pa[0] && (col / p) % p < pa[1] && col / (p * p) < pa[2]) vcols[q][nv++] = col;
+            }
+            vfac[q] = nv % ctx->visit == 0 ? ctx->visit : 1;
+            nvis[q] = nv / vfac[q];
+            nlaunch = std::max(nlaunch, nvis[q]);
+        }
+        for (int ci = 0; ci < nlaunch; ci++) {
             int boff = 0;
             for (int q = 0; q < R; q++) {
                 RepPass& P_ = hp[(size_t)ci * R + q];
@@ -5759,16 +5846,16 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
                 P_.rep = q;
                 P_.blk_off = boff;
                 MoveParams& mp = P_.mp;
-                int col = plans[q].order[ci];
+                bool skip = ci >= nvis[q];   // this replica's sweep is done
+                int col = skip ? 0 : vcols[q][ci];
                 int pa[3];
                 for (int k = 0; k < 3; k++) pa[k] = (g.nc[k] >= p) ? p : 1;
                 mp.ox = col % p; mp.oy = (col / p) % p; mp.oz = col / (p * p);
                 mp.p = p;
-                bool skip = mp.ox >= pa[0] || mp.oy >= pa[1] || mp.oz >= pa[2];   // axis with a single cell
                 for (int k = 0; k < 3; k++) mp.nact[k] = g.nc[k] / pa[k];
                 mp.trial_mode = ctx->trial_mode;
                 mp.hcap = ctx->home_cache;
-                mp.visit = 1;   // a batch visits every colour once per sweep
+                mp.visit = vfac[q];
                 mp.n = ctx->rep_n[q];
                 mp.transmx = (float)transmx;
                 mp.rot_half = (float)rot_half;
@@ -5783,11 +5870,19 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             }
             nblk[ci] = boff;
         }
-        HIPCK(hipMemcpyAsync(ctx->d_rp, hp.data(), hp.size() * sizeof(RepPass), hipMemcpyHostToDevice, ctx->stream));
+        if ((rc = pin_upload(ctx, ctx->pin_rp, ctx->d_rp, hp.data(), (size_t)nlaunch * R * sizeof(RepPass)))) return rc;
         int c = ctx->cur;
-        for (int ci = 0; ci < ncol; ci++) {
+        for (int ci = 0; ci < nlaunch; ci++) {
             if (nblk[ci] == 0) continue;
             const RepPass* rp = ctx->d_rp + (size_t)ci * R;
+            if (ctx->profile) {
+                while ((long)ctx->kev.size() < 2 * (launches + 1)) {
+                    hipEvent_t e;
+                    HIPCK(hipEventCreate(&e));
+                    ctx->kev.push_back(e);
+                }
+                HIPCK(hipEventRecord(ctx->kev[2 * launches], ctx->stream));
+            }
 #define SWEEP_BATCH(DAM, NWV)                                                                                  \
     SC_LAUNCH((k_sweep_batch<DAM, NWV>), dim3(nblk[ci]), dim3(WAVE * NWV), lds, ctx->stream, rp, R, ctx->starts,     \
               ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap, ctx->ppmax,                \
@@ -5813,6 +5908,7 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             }
 #undef SWEEP_BATCH
 #undef SPEC_BATCH
+            if (ctx->profile) HIPCK(hipEventRecord(ctx->kev[2 * launches + 1], ctx->stream));
             launches++;
         }
         HIPCK(hipGetLastError());
@@ -5830,6 +5926,12 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
     ctx->last_ms = ms;
     ctx->last_launches = launches;
     ctx->last_kernel_ms = 0;
+    if (ctx->profile)
+        for (long q = 0; q < launches; q++) {
+            float km = 0;
+            HIPCK(hipEventElapsedTime(&km, ctx->kev[2 * q], ctx->kev[2 * q + 1]));
+            ctx->last_kernel_ms += km;
+        }
     if (stats)
         for (int q = 0; q < R; q++) {
             unsigned long long hs[STAT_STRIDE] = {0};

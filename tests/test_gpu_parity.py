@@ -175,6 +175,31 @@ def test_energy_all_config2_cpsc_32768(dev):
         np.testing.assert_array_equal(srt, o2)
 
 
+@pytest.mark.parametrize("n", [300, 700])
+def test_cell_list_of_crowded_cells_bit_exact(dev, n):
+    """Cells of more than 64 particles (a box of under two maxcut per axis
+    holds one cell per axis): the in-cell order by reference index of the
+    chunked rank sort (up to 512) and of the one-lane sort beyond, against
+    the CPU's counting sort."""
+    say("case: cell list, one cell of %d particles" % n)
+    w = W.workload("c1_psc_1000", n_override=n)
+    g = sc_amd.GpuSystem(dev)
+    g.load_topology(w["top"])
+    box = np.full(3, 1.5 * g.maxcut)
+    pos = np.random.default_rng(n).random((n, 3))
+    d = np.tile([0.0, 0.0, 1.0], (n, 1))
+    q = np.tile([1.0, 0.0, 0.0], (n, 1))
+    g.set_config(box, pos, d, q, w["types"])
+    u = g.fixed_positions()
+    cell_of, start, srt, nc = g.build_cells(r=1, shift=(777, 12345678, 99))
+    c2, s2, o2 = po.cell_list(u, nc, (777, 12345678, 99))
+    assert int(np.prod(nc)) == 1
+    np.testing.assert_array_equal(cell_of, c2)
+    np.testing.assert_array_equal(start, s2)
+    np.testing.assert_array_equal(srt, o2)
+    g.close()
+
+
 def test_neighbor_pairs_bit_exact(dev):
     say("case: neighbour pairs")
     w = W.workload("c1_psc_1000", n_override=1500)

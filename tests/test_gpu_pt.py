@@ -43,17 +43,19 @@ def small_batch(g, sizes=(700, 1300, 400), etas=(0.18, 0.2, 0.15), seed=5):
     return (pos, d, q, ty), np.array(boxes), np.array(sizes)
 
 
-@pytest.mark.parametrize("nw", [0, 1])
-def test_batch_sweep_bit_identical_to_single_replicas(nw):
+@pytest.mark.parametrize("nw,r,visit", [(0, 2, 1), (1, 2, 1), (0, 1, 8)])
+def test_batch_sweep_bit_identical_to_single_replicas(nw, r, visit):
     """nw 0: the batch's speculative sweep; 1: one trial at a time (the single
-    systems sweep speculatively)."""
-    say("case: batch sweep (block waves %d) vs single-replica sweeps, 3 replicas" % nw)
+    systems sweep speculatively); visit 8: a sweep of one colour with 8x the
+    trials per cell, in the batch and alone."""
+    say("case: batch sweep (block waves %d, r %d, visit %d) vs single-replica sweeps, 3 replicas" % (nw, r, visit))
     g = sc_amd.GpuSystem(0)
     (pos, d, q, ty), boxes, sizes = small_batch(g)
     temps = np.array([0.9, 1.0, 1.2])
     g.set_trial_mode(1)
     g.set_block_waves(nw)
-    st = g.batch_sweep(6, temps, 0.3, ROT_HALF, r=2, seed=77, counter=11)
+    g.set_visit_factor(visit)
+    st = g.batch_sweep(6, temps, 0.3, ROT_HALF, r=r, seed=77, counter=11)
     top = pt.c3_topology(int(sizes.sum()))
     for k in range(3):
         bp, bd, bq = g.batch_get_config(k)
@@ -61,7 +63,8 @@ def test_batch_sweep_bit_identical_to_single_replicas(nw):
         s.load_topology(top)
         s.set_config(boxes[k], pos[k], d[k], q[k], ty[k])
         s.set_trial_mode(1)
-        ss = s.sweep(6, temps[k], 0.3, ROT_HALF, r=2, seed=sc_amd.GpuSystem.replica_seed(77, k), counter=11)
+        s.set_visit_factor(visit)
+        ss = s.sweep(6, temps[k], 0.3, ROT_HALF, r=r, seed=sc_amd.GpuSystem.replica_seed(77, k), counter=11)
         _, sp, sd, sq = s.get_config()
         say("replica %d: batch acc %d/%d, single acc %d/%d, dE %.12f vs %.12f" % (
             k, st[k]["trans_acc"] + st[k]["rot_acc"], st[k]["trans_try"] + st[k]["rot_try"],
