@@ -1645,7 +1645,7 @@ __global__ __launch_bounds__(WAVE * NW, 2) void k_sweep_pass(Grid g, SysConst sc
 // The chain of trials of a cell is sequential, and one trial leaves most of
 // a block's lanes idle (its phases hold a few dozen items).  In this variant
 // each of the block's SPEC_W waves evaluates one trial of the next SPEC_W
-// against the cell's current state, all at once; thread 0 then takes the
+// against the cell's current state, all at once; wave 0 then takes the
 // results in trial order and keeps each one unless it depends on an earlier
 // trial of the round that was accepted (the same particle, chain partners, or
 // two particles whose pair term can be non-zero in any of their old/new

@@ -47,6 +47,34 @@ def _blocked(x, nb=8):
 
 
 # ------------------------------------------------------------------- C4
+def test_c4_speculative_sweep_reproduces_sequential_from_compression():
+    """The speculative sweep (a trial per wave, kept up to the first that
+    depends on an accepted trial of its round) against one trial at a time,
+    on the full C4 system from its RSA start: the compression that prepares
+    it (r = 2, cells of a few particles, overlapping rods: rounds often try
+    one particle twice) and 200 sweeps at the bench's r = 1, visit factor 8.
+    The same counters and the same configuration bit for bit."""
+    say("case: C4 65,536 TCHPSC, compression + 200 sweeps, speculative vs one trial at a time")
+    w = W.workload("c4_tchpsc_65536")
+    out = []
+    for nw in (0, 1):
+        g = sc_amd.GpuSystem(0)
+        g.set_block_waves(nw)
+        prepare.initial_state(g, w, seed=2025)
+        g.set_trial_mode(1)
+        g.set_visit_factor(8)
+        s = g.sweep(200, 1.0, 0.3, ROT_HALF, r=1, seed=5, counter=10 ** 6)
+        out.append((s, g.get_config(), g.allToAll()))
+        g.close()
+    (s1, c1, e1), (s2, c2, e2) = out
+    say("E/N %.6f vs %.6f, accepted %d vs %d" % (e1 / len(w["types"]), e2 / len(w["types"]),
+                                               s1["trans_acc"] + s1["rot_acc"], s2["trans_acc"] + s2["rot_acc"]))
+    for k in ("trans_acc", "trans_try", "rot_acc", "rot_try", "left_cell"):
+        assert s1[k] == s2[k], k
+    for a, b in zip(c1, c2):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.fixture(scope="module")
 def c4():
     say("prepare C4: 65,536 TCHPSC, eta 0.25, 6000 exact sweeps")

@@ -140,18 +140,20 @@ def test_home_cache_reproduces_uncached_derivation(prepared, name, mode):
 
 @pytest.mark.parametrize("name", list(SYSTEMS))
 @pytest.mark.parametrize("mode", [0, 1, 2])
-def test_speculative_sweep_reproduces_sequential(prepared, name, mode):
+@pytest.mark.parametrize("r,visit", [(1, 8), (2, 9)])
+def test_speculative_sweep_reproduces_sequential(prepared, name, mode, r, visit):
     """A trial per wave, kept in order up to the first dependence on an
     accepted trial of the round, gives the one-trial-at-a-time trajectory:
-    the same counters and the same configuration bit for bit (at visit factor
-    8, 128 trials per cell visit)."""
-    say("case: %s sweep, mode %d, speculative vs one trial at a time" % (name, mode))
+    the same counters and the same configuration bit for bit.  r = 1 at
+    visit factor 8 (~128 trials per cell visit) and r = 2 at visit factor 9
+    (cells of a few particles, so a round often tries one particle twice)."""
+    say("case: %s sweep, mode %d, r %d, visit %d, speculative vs one trial at a time" % (name, mode, r, visit))
     out = []
     for nw in (0, 1):
         g = gpu_system(prepared, name, mode)
-        g.set_visit_factor(8)
+        g.set_visit_factor(visit)
         g.set_block_waves(nw)
-        s = g.sweep(20, 1.0, 0.3, ROT_HALF, r=1, seed=7, counter=0)
+        s = g.sweep(20, 1.0, 0.3, ROT_HALF, r=r, seed=7, counter=0)
         out.append((s, g.get_config()))
         g.close()
     (s1, c1), (s2, c2) = out
