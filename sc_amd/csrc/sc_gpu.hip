@@ -1763,7 +1763,11 @@ __device__ __forceinline__ int spec_deps(const TrialRec* recs, int t, int w, con
     return dep;
 }
 
-template <int DA>
+// LT: the pair and type tables are copied into LDS (systems of up to
+// SWEEP_TABLE_MAXT types).  A compile-time choice, so their loads are LDS
+// loads: chosen at run time, the pointer could be either, and every table
+// load would be a flat load that waits on both memory counters.
+template <int DA, bool LT>
 __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysConst& sc, const MoveParams& mp,
                                                 const int* __restrict__ starts, uint4* __restrict__ P,
                                                 float4* __restrict__ O, float2* __restrict__ Q,
@@ -1809,8 +1813,7 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
         ml = (float*)(smem + L.ml);
         for (int i = tid; i < MLP_C_BYTES / 16; i += WAVE * SPEC_W) ((uint4*)ml)[i] = ((const uint4*)(mlp_img + MLP_C_OFF))[i];
     }
-#if SWEEP_TABLES_IN_LDS
-    if (sc.nt <= SWEEP_TABLE_MAXT) {
+    if constexpr (LT) {
         PP32* ppl = (PP32*)(smem + L.tab);
         TP32* tpl = (TP32*)(ppl + sc.nt * sc.nt);
         const int w4 = (int)(sizeof(PP32) / 4), t4 = (int)(sizeof(TP32) / 4);
@@ -1819,7 +1822,6 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
         pp = ppl;
         tp = tpl;
     }
-#endif
     if (!stage_load<SPEC_W>(S, cap, g, cx, cy, cz, starts, P, O, Q, ID, sten)) {
         if (tid == 0) atomicOr(flag, 1);
         return;
@@ -1984,7 +1986,7 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
     }
 }
 
-template <int DA>
+template <int DA, bool LT>
 __global__ __launch_bounds__(WAVE * SPEC_W, 2) void k_sweep_spec(Grid g, SysConst sc, MoveParams mp,
                                                              const int* __restrict__ starts, uint4* __restrict__ P,
                                                              float4* __restrict__ O, float2* __restrict__ Q,
@@ -1995,7 +1997,7 @@ __global__ __launch_bounds__(WAVE * SPEC_W, 2) void k_sweep_spec(Grid g, SysCons
                                                              const unsigned char* __restrict__ mlp_img, MlpConst mc,
                                                              ChainTab ct, unsigned long long* __restrict__ stats,
                                                              double* __restrict__ de_sum, int* __restrict__ flag) {
-    sweep_cell_spec<DA>(xcd_slab_index((int)blockIdx.x, (int)gridDim.x), g, sc, mp, starts, P, O, Q, ID, pp, tp, cap,
+    sweep_cell_spec<DA, LT>(xcd_slab_index((int)blockIdx.x, (int)gridDim.x), g, sc, mp, starts, P, O, Q, ID, pp, tp, cap,
                         lcap, ppmax, nonempty, mlp_img, mc, ct, stats, de_sum, flag);
 }
 
@@ -2036,7 +2038,7 @@ __global__ __launch_bounds__(WAVE * NW, 2) void k_sweep_batch(const RepPass* __r
 }
 
 // The speculative sweep of every replica's active cells of one colour pass.
-template <int DA>
+template <int DA, bool LT>
 __global__ __launch_bounds__(WAVE * SPEC_W, 2) void k_sweep_batch_spec(const RepPass* __restrict__ rp, int nrep,
                                                                    const int* __restrict__ starts,
                                                                    uint4* __restrict__ P, float4* __restrict__ O,
@@ -2057,7 +2059,7 @@ __global__ __launch_bounds__(WAVE * SPEC_W, 2) void k_sweep_batch_spec(const Rep
         if (rp[mid].blk_off <= lb) lo = mid; else hi = mid - 1;
     }
     const RepPass R = rp[lo];
-    sweep_cell_spec<DA>(lb - R.blk_off, R.g, R.sc, R.mp, starts + R.cell_off, P, O, Q, ID, pp, tp, cap, lcap, ppmax,
+    sweep_cell_spec<DA, LT>(lb - R.blk_off, R.g, R.sc, R.mp, starts + R.cell_off, P, O, Q, ID, pp, tp, cap, lcap, ppmax,
                         nonempty + R.rep, mlp_img, mc, ct, stats + (size_t)R.rep * STAT_REP * STAT_STRIDE,
                         de_sum + (size_t)R.rep * STAT_REP * STAT_STRIDE, flag);
 }
@@ -5481,6 +5483,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
         bool da = mode == DA_NET;
         // the speculative variant: the stage that fits beside SPEC_W trials' lists
         bool spec = ctx->spec && ctx->sweep_nw == 0;
+        const bool spec_lt = sweep_table_bytes(ctx->nt) > 0;   // the tables fit in LDS beside the stage
         int cap_spec = cap;
         size_t lds_spec = 0;
         if (spec) {
@@ -5555,7 +5558,9 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
                        ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap,     \
                        ctx->ppmax, ctx->d_nonempty, IMG, MC, chain_tab(ctx), ctx->d_stats, ctx->d_dsum, ctx->d_flag)
 #define SPEC_PASS(DAM, IMG, MC)                                                                                \
-    hipLaunchKernelGGL((k_sweep_spec<DAM>), dim3(nblk), dim3(WAVE * SPEC_W), lds_spec, ctx->stream, g, sc, mp,     \
+    if (spec_lt) SPEC_PASS_LT(DAM, true, IMG, MC); else SPEC_PASS_LT(DAM, false, IMG, MC)
+#define SPEC_PASS_LT(DAM, LT, IMG, MC)                                                                         \
+    hipLaunchKernelGGL((k_sweep_spec<DAM, LT>), dim3(nblk), dim3(WAVE * SPEC_W), lds_spec, ctx->stream, g, sc, mp,     \
                        ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32,          \
                        cap_spec, SPEC_LCAP, ctx->ppmax, ctx->d_nonempty, IMG, MC, chain_tab(ctx), ctx->d_stats,    \
                        ctx->d_dsum, ctx->d_flag)
@@ -5581,6 +5586,7 @@ static int sc_sweep_impl(sc_ctx* ctx, int nsweeps, double temper, double transmx
             }
 #undef SWEEP_PASS
 #undef SPEC_PASS
+#undef SPEC_PASS_LT
             if (ctx->profile) HIPCK(hipEventRecord(ctx->kev[2 * launches + 1], ctx->stream));
             launches++;
         }
@@ -5802,6 +5808,7 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
         }
         // the speculative variant (block waves 0): the stage beside SPEC_W trials' lists
         bool spec = ctx->spec && ctx->sweep_nw == 0;
+        const bool spec_lt = sweep_table_bytes(ctx->nt) > 0;   // the tables fit in LDS beside the stage
         int cap_spec = cap;
         size_t lds_spec = 0;
         if (spec) {
@@ -5889,7 +5896,9 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
               ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ChainTab{nullptr, nullptr, ctx->d_pp64}, ctx->d_rep_stats,  \
               ctx->d_rep_dsum, ctx->d_flag)
 #define SPEC_BATCH(DAM)                                                                                        \
-    SC_LAUNCH((k_sweep_batch_spec<DAM>), dim3(nblk[ci]), dim3(WAVE * SPEC_W), lds_spec, ctx->stream, rp, R,           \
+    if (spec_lt) SPEC_BATCH_LT(DAM, true); else SPEC_BATCH_LT(DAM, false)
+#define SPEC_BATCH_LT(DAM, LT)                                                                                 \
+    SC_LAUNCH((k_sweep_batch_spec<DAM, LT>), dim3(nblk[ci]), dim3(WAVE * SPEC_W), lds_spec, ctx->stream, rp, R,           \
               ctx->starts, ctx->P[c], ctx->O[c], ctx->Q[c], ctx->ID[c], ctx->d_pp32, ctx->d_tp32, cap_spec, SPEC_LCAP,  \
               ctx->ppmax, ctx->d_rep_nonempty, nullptr, ctx->mlp_c, ChainTab{nullptr, nullptr, ctx->d_pp64},           \
               ctx->d_rep_stats, ctx->d_rep_dsum, ctx->d_flag)
@@ -5908,6 +5917,7 @@ static int sc_batch_sweep_impl(sc_ctx* ctx, int nsweeps, const double* temper, d
             }
 #undef SWEEP_BATCH
 #undef SPEC_BATCH
+#undef SPEC_BATCH_LT
             if (ctx->profile) HIPCK(hipEventRecord(ctx->kev[2 * launches + 1], ctx->stream));
             launches++;
         }

@@ -168,6 +168,37 @@ def test_speculative_sweep_reproduces_sequential(prepared, name, mode, r, visit)
         np.testing.assert_array_equal(a, b)
 
 
+def test_speculative_sweep_with_tables_in_global_memory():
+    """Five particle types (TCHPSC copies), more than the sweep keeps in LDS
+    beside the stage: the speculative sweep reads the pair and type tables
+    from global memory (the other instantiation of the kernel) and gives the
+    one-trial-at-a-time trajectory bit for bit."""
+    say("case: 5-type TCHPSC system (tables in global memory), speculative vs one trial at a time")
+    fields = W.TYPES["tchpsc"].split()[2:]
+    top = W.top_init(["%s %d %s" % ("ABCDE"[k], k + 1, " ".join(fields)) for k in range(5)], [200] * 5)
+    types = np.repeat(np.arange(1, 6), 200).astype(np.int32)
+    box = np.full(3, (1000 * W.particle_volume(1.0, 4.0) / 0.15) ** (1.0 / 3.0))
+    out = []
+    for nw in (0, 1):
+        g = sc_amd.GpuSystem(0)
+        g.load_topology(top)
+        pos, d, q = g.generate_rsa(types, box, 21)
+        g.set_config(box, pos, d, q, types)
+        g.set_trial_mode(1)
+        g.set_visit_factor(8)
+        g.set_block_waves(nw)
+        s = g.sweep(20, 1.0, 0.3, ROT_HALF, r=1, seed=7, counter=0)
+        out.append((s, g.get_config()))
+        g.close()
+    (s1, c1), (s2, c2) = out
+    say("accepted %d / %d" % (s1["trans_acc"] + s1["rot_acc"], s1["trans_try"] + s1["rot_try"]))
+    for k in ("trans_acc", "trans_try", "rot_acc", "rot_try", "left_cell"):
+        assert s1[k] == s2[k], k
+    assert s1["trans_acc"] + s1["rot_acc"] > 0
+    for a, b in zip(c1, c2):
+        np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("name", list(SYSTEMS))
 def test_network_threshold_is_the_same_in_every_variant(prepared, name):
     """With the network screening only trials of at least the default number
