@@ -19,9 +19,14 @@ HEADERS = [os.path.join(HERE, "csrc", h) for h in ("sc_pair.h", "sc_rng.h", "sc_
 # FP32 division and square root as the hardware's reciprocal and square root
 # (about 1 ulp) instead of the correctly rounded sequences: the pair terms are
 # FP32 anyway, and the sweep's chain of trials waits on ~35 of them per patch
-# pair.
+# pair.  FP32 denormals flushed to zero: without it every division and square
+# root carries a range-scaling guard (frexp/ldexp around the reciprocal, a
+# scaled square root below FLT_MIN), 5-7 more instructions on those chains;
+# for operands and results in the normal range the values are the same.
+# FP64 keeps IEEE denormals.
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
                "-Wno-unused-result", "-Wno-unused-value", "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+               "-fgpu-flush-denormals-to-zero",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
