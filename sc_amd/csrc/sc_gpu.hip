@@ -256,6 +256,32 @@ __host__ __device__ __forceinline__ DPart<R> load_part(const float4 o, const flo
     return p;
 }
 
+// patch k of a particle from its stored record: view(load_part(...), k),
+// computing only patch k's vectors (4 quaternion rotations for a two-patch
+// chiral type instead of derive's 7) with derive's expressions, so the values
+// are the same.  Types without patches take the full derivation.
+template <typename R>
+__host__ __device__ __forceinline__ PatchView<R> load_view(const float4 o, const float2 q, int t,
+                                                           const TypeParamsT<R>* tp, int k) {
+    const TypeParamsT<R>& T = tp[t];
+    const int g = T.geotype;
+    if (g >= 30 || g == G_SCA || g == G_SCN) return view(load_part<R>(o, q, t, tp), k);
+    V3<R> dir = normalised(mk<R>(o.x, o.y, o.z));
+    V3<R> pd = normalised(perp(mk<R>(o.w, q.x, q.y), dir));
+    R hc = T.pcoshalfi0, hs = T.psinhalfi0;
+    if (k) {   // derive's pd[1] (two-patch types only: k = 1 exists for them alone)
+        pd = normalised(perp(qrot(pd, dir, T.csec, T.ssec), dir));
+        hc = T.pcoshalfi2;
+        hs = T.psinhalfi2;
+    }
+    PatchView<R> v;
+    v.ax = is_chiral(g) ? qrot(dir, pd, T.chcos, T.chsin) : dir;
+    v.pd = pd;
+    v.s0 = qrot(pd, v.ax, hc, hs);
+    v.s1 = qrot(pd, v.ax, hc, -hs);
+    return v;
+}
+
 // ---------------------------------------------------------------- cell list
 __global__ void k_cell_count(int n, const uint4* __restrict__ P, Grid g, int* __restrict__ cell_tmp,
                              int* __restrict__ counts) {
@@ -1341,8 +1367,8 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             if (P2->kind <= K_SCA) {
                 if (mpp < npatch_pairs(*P2)) {
                     patch_pair_index(*P2, mpp, k1, k2);   // patches of part1 and part2
-                    DPart<float> pb = load_part<float>(S.o[jj], S.q[jj], b.w, tp);
-                    PatchView<float> vm = mv_view(mv, sw ? k2 : k1), vn = view(pb, sw ? k1 : k2);
+                    PatchView<float> vm = mv_view(mv, sw ? k2 : k1),
+                                     vn = load_view<float>(S.o[jj], S.q[jj], b.w, tp, sw ? k1 : k2);
                     va = sw ? vn : vm;
                     vb = sw ? vm : vn;
                     hit = patch_side<float>(*P2, va, vb, r, k1, k2, side, x1, x2);
