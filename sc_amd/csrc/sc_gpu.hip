@@ -1192,14 +1192,15 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
     if (bmask) tsync<WS>();
     // phase A: the staged particles inside sqmaxcut at either state -> near
     // list, in stencil order (one barrier per round); all waves scan a
-    // translation, the waves but the last one a rotation
+    // translation, the waves but the last one a rotation; a single wave takes
+    // eight entries per lane (one round for a stencil of up to 512)
     int nn = 0;
     if (NW > 1 && !translate)
         scan_stencil<NW, NW >= 4 ? 3 : 8>(S, k, uo, un, moved, sc, bmask, idk, T, (NW - 1) * WAVE, wv < NW - 1, nn,
                                          cpar, ncap, tid);
     else
-        scan_stencil<NW, NW >= 4 ? 2 : 4>(S, k, uo, un, moved, sc, bmask, idk, T, NW * WAVE, true, nn, cpar, ncap,
-                                         tid);
+        scan_stencil<NW, NW >= 4 ? 2 : (NW == 1 ? 8 : 4)>(S, k, uo, un, moved, sc, bmask, idk, T, NW * WAVE, true,
+                                                           nn, cpar, ncap, tid);
     tsync<WS>();
     if (nn > ncap) {   // a wave's near list is full: the trial cannot be evaluated here
         stage = -2;
