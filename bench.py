@@ -174,6 +174,7 @@ def npt_leg(a, transmx, rot_half):
         gn.set_config(bx0, p0, d0_, q0_, tc_)
         mx_, ang_ = 2.0 * a.npt_chainmmx, a.npt_chainrmx / 180.0 * np.pi / 2.0   # sim.h:362, 366
         gn.set_chain_moves(a.npt_chainprob, mx_, ang_)
+        gn.set_chain_visit_factor(a.npt_chain_visit)
         npt_steps(5, 0, 10 ** 5)
         gn.set_config(bx0, p0, d0_, q0_, tc_)
         gn.set_chain_moves(a.npt_chainprob, mx_, ang_)
@@ -183,6 +184,7 @@ def npt_leg(a, transmx, rot_half):
         lab = "exact_with_chain_moves"
         npt[lab] = {"chainprob": a.npt_chainprob, "chainmmx": a.npt_chainmmx, "chainrmx_deg": a.npt_chainrmx,
                     "extent": gn.chain_extent(), "colour_period": gn.chain_pass_info()[0],
+                    "chain_visit_factor": gn.chain_visit_info(),
                     "lds_stage": gn.chain_pass_info()[1], "sweeps_per_s": a.npt_steps / dt,
                     "ms_per_step": 1e3 * dt / a.npt_steps, "ms_sweep": 1e3 * tot["sweep_s"] / a.npt_steps,
                     "ms_volume_move": 1e3 * tot["volume_s"] / a.npt_steps,
@@ -195,12 +197,13 @@ def npt_leg(a, transmx, rot_half):
         if "exact" in npt:
             npt[lab]["ms_chain_moves_per_sweep"] = npt[lab]["ms_sweep"] - npt["exact"]["ms_sweep"]
         log("npt chains %s: %.2f steps/s, %.0f chain moves per sweep (displace acc %.3f, rotate acc %.3f; "
-            "period %d, LDS stage %s, cells skipped %d)" % (
+            "period %d, visit factor %d, LDS stage %s, cells skipped %d)" % (
                 lab, npt[lab]["sweeps_per_s"], npt[lab]["chain_moves_per_sweep"],
                 npt[lab]["chain_displace_acceptance"], npt[lab]["chain_rotate_acceptance"], npt[lab]["colour_period"],
-                npt[lab]["lds_stage"], tot["chain_cells_skipped"]))
+                npt[lab]["chain_visit_factor"], npt[lab]["lds_stage"], tot["chain_cells_skipped"]))
         npt[lab]["chain_cells_skipped"] = int(tot["chain_cells_skipped"])
         gn.set_chain_moves(0.0, 0.0, 0.0)
+        gn.set_chain_visit_factor(1)
     gn.close()
     return npt
 
@@ -239,6 +242,8 @@ def main():
                     help="chain-move probability per step of the NPT leg's chain-move run (0: no such run)")
     ap.add_argument("--npt-chainmmx", type=float, default=0.1, help="options' chainmmx (displacement, doubled)")
     ap.add_argument("--npt-chainrmx", type=float, default=5.0, help="options' chainrmx in degrees")
+    ap.add_argument("--npt-chain-visit", type=int, default=27,
+                    help="chain-move visit factor (the largest divisor of the coarse colour count not above it)")
     ap.add_argument("--npt-chains", type=int, default=43690, help="3-bead chains of the NPT leg (131,070 beads)")
     ap.add_argument("--npt-steps", type=int, default=200, help="timed NPT steps (one sweep + one volume move)")
     ap.add_argument("--npt-warmup", type=int, default=50)

@@ -269,6 +269,17 @@ int sc_get_chain_extent(sc_ctx* ctx, double* extent);
  * 0 before any) and whether the trials read an LDS copy of each cell's
  * neighbourhood (1) or global memory (0) */
 int sc_chain_pass_info(const sc_ctx* ctx, int* period, int* staged);
+/* Visit factor of the chain-move passes (1..27, default 1): a sweep's chain
+ * moves visit the first 1/v' of the coarse colours in the sweep's random
+ * order, each with v' times the molecule trials, where v' is the largest
+ * divisor of the colour count (8 or 27, by the box) not above v; chainprob * N
+ * trials per sweep on average either way.  A move never leaves its anchor's
+ * coarse cell, so every pass is a product of independent Metropolis chains;
+ * fewer passes pay the neighbourhood staging and the cell-list rebuild fewer
+ * times.  No counterpart in the reference (its chain moves are sequential). */
+int sc_set_chain_visit_factor(sc_ctx* ctx, int v);
+/* the chain visit factor v' the last sweep used (0 before any chain pass) */
+int sc_chain_visit_info(const sc_ctx* ctx, int* visit);
 /* Volume move (MoveCreator::pressureMove, movecreator.cpp:330-550) for
  * ptype 0 (one random edge) and 1 (isotropic); returns accepted=1/0 and the
  * new box.  Whole-system energies are FP32 pair terms (FP64 bonds) with FP64

@@ -82,6 +82,8 @@ SIGNATURES = {
     "sc_energy_mol_trial": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, _D, _D, _D]),
     "sc_commit_mol": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D, _D, _D]),
     "sc_set_chain_moves": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "sc_set_chain_visit_factor": (ctypes.c_int, [_P, ctypes.c_int]),
+    "sc_chain_visit_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
     "sc_get_chain_extent": (ctypes.c_int, [_P, _D]),
     "sc_chain_pass_info": (ctypes.c_int, [_P, _I, _I]),
     "sc_energy_per_particle": (ctypes.c_int, [_P, _D]),
@@ -398,6 +400,17 @@ class GpuSystem:
         chain_angle (radians); extent 0 = 1.2 x the widest molecule now"""
         self._ck(self._L.sc_set_chain_moves(self._h, float(chainprob), float(chain_mx), float(chain_angle),
                                             float(extent)), "set_chain_moves")
+
+    def set_chain_visit_factor(self, v):
+        """the chain-move passes visit 1/v' of the coarse colours with v' times the trials each (v' the largest
+        divisor of their colour count not above v; default 1)"""
+        self._ck(self._L.sc_set_chain_visit_factor(self._h, int(v)), "set_chain_visit_factor")
+
+    def chain_visit_info(self):
+        """the chain visit factor v' of the last sweep's chain passes (0 before any)"""
+        v = ctypes.c_int32()
+        self._ck(self._L.sc_chain_visit_info(self._h, ctypes.byref(v)), "chain_visit_info")
+        return v.value
 
     def chain_pass_info(self):
         """(colour period, staged) of the last sweep's chain moves"""

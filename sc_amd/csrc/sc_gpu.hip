@@ -2641,7 +2641,7 @@ __global__ void k_nematic_q(int n, const float4* __restrict__ O, double* __restr
 
 struct ChainMoveParams {
     double temper, chain_mx, chain_angle, rho2;
-    double frac;             // trials per molecule and sweep
+    double frac;             // trials per molecule per visit of its cell
     uint32_t key0, key1;
     uint32_t pass_lo, pass_hi;
     uint32_t stamp;          // marks the particles moved in this pass
@@ -3696,6 +3696,9 @@ struct sc_ctx {
     int net_minx = -1;                  // network screen: least (pair, state) entries (< 0: one phase-C round)
     int home_cache = HCAP;              // cell size limit of the sweep's derived-vector cache (0: off)
     int visit = 1;                      // colour passes of a sweep = colours / visit, visit x the trials each
+    int chain_visit = 1;                // the same for the chain-move passes (the largest divisor of their
+                                        // colour count not above it)
+    int chain_visit_eff = 0;            // the factor the last sweep's chain passes used
     unsigned char* noalloc = nullptr;   // this device's address of `scratch` (sc_pair_energies_noalloc)
     int max_active = 0;                 // sweep grids: at most this many active cells per colour (0: no limit)
     int max_active_default = 0;         // two per CU of the device
@@ -5342,6 +5345,22 @@ int sc_get_chain_extent(sc_ctx* ctx, double* extent) {
     return SC_OK;
 }
 
+static int sc_set_chain_visit_factor_impl(sc_ctx* ctx, int v) {
+    if (!ctx || v < 1 || v > 27) return SC_EINVAL;
+    ctx->chain_visit = v;
+    return SC_OK;
+}
+
+int sc_set_chain_visit_factor(sc_ctx* ctx, int v) {
+    SC_GUARD(ctx, sc_set_chain_visit_factor_impl(ctx, v))
+}
+
+int sc_chain_visit_info(const sc_ctx* ctx, int* visit) {
+    if (!ctx || !visit) return SC_EINVAL;
+    *visit = ctx->chain_visit_eff;
+    return SC_OK;
+}
+
 int sc_chain_pass_info(const sc_ctx* ctx, int* period, int* staged) {
     if (!ctx || !period || !staged) return SC_EINVAL;
     *period = ctx->chain_period;
@@ -5382,6 +5401,11 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
     uint64_t cseed = mix64(seed ^ 0x6368616E6D6F7665ull);
     SweepPlan plan;
     sweep_plan(cseed, sw, p, plan);
+    // visit factor: the passes visit the first 1/v of the colours in the
+    // sweep's random order, each with v times the trials (v the largest
+    // divisor of the colour count not above the context's chain factor); a
+    // move never leaves its anchor's coarse cell, so a cell's staged
+    // neighbourhood holds for all its trials
     Grid gc;
     gc.r = 1;
     for (int k = 0; k < 3; k++) {
@@ -5417,13 +5441,20 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
     cp.rho2 = rho2;
     // MoveCreator::chainMove is tried with probability chainprob at each of
     // the N steps of a sweep (updater.cpp:206-217): chainprob N trials
-    cp.frac = ctx->chainprob * (double)ctx->n / (double)nch;
+    int pa[3], ncv = 1;
+    for (int k = 0; k < 3; k++) {
+        pa[k] = gc.nc[k] >= p ? p : 1;
+        ncv *= pa[k];
+    }
+    int vis = 1;
+    for (int v = 1; v <= std::min(ctx->chain_visit, ncv); v++)
+        if (ncv % v == 0) vis = v;
+    ctx->chain_visit_eff = vis;
+    cp.frac = ctx->chainprob * (double)ctx->n / (double)nch * (double)vis;
     cp.key0 = (uint32_t)cseed;
     cp.key1 = (uint32_t)(cseed >> 32);
     cp.nchain = nch;
-    int pa[3];
     for (int k = 0; k < 3; k++) {
-        pa[k] = gc.nc[k] >= p ? p : 1;
         cp.pa[k] = pa[k];
         cp.nact[k] = gc.nc[k] / pa[k];
     }
@@ -5450,12 +5481,14 @@ static int chain_passes(sc_ctx* ctx, uint64_t seed, uint64_t sw, double temper, 
     size_t lds_st = chain_st_layout(CHAIN_NW, cap).total;
     ctx->chain_staged = staged;
     ctx->chain_period = p;
-    for (int ci = 0; ci < p * p * p; ci++) {
+    int visited = 0;
+    for (int ci = 0; ci < p * p * p && visited < ncv / vis; ci++) {
         int col = plan.order[ci];
         cp.ox = col % p;
         cp.oy = (col / p) % p;
         cp.oz = col / (p * p);
         if (cp.ox >= pa[0] || cp.oy >= pa[1] || cp.oz >= pa[2]) continue;
+        visited++;
         uint64_t pass = sw * 32 + (uint64_t)ci;
         cp.pass_lo = (uint32_t)pass;
         cp.pass_hi = (uint32_t)(pass >> 32);

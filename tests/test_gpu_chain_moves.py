@@ -125,19 +125,23 @@ def kabsch(a, b):
     return vt.T @ np.diag([1.0, 1.0, dd]) @ u.T
 
 
-@pytest.mark.parametrize("extent", [0.0, 7.0])
-def test_chain_move_passes_are_rigid_reproducible_and_keep_the_energy(extent):
+@pytest.mark.parametrize("extent,visit", [(0.0, 1), (7.0, 1), (0.0, 27), (7.0, 27)])
+def test_chain_move_passes_are_rigid_reproducible_and_keep_the_energy(extent, visit):
     """Particle moves of length and angle 0 (they change nothing), chain moves
     at chainprob 0.25: every molecule keeps its shape, the energy changes by
     the sum of the accepted dE, a second run with the same seed gives the
-    same state bit for bit.  With the automatic extent each coarse cell's
-    neighbourhood fits the LDS stage; an extent of 7 sigma makes the regions
-    too large for it, so the passes read global memory."""
-    say("case: chain-move passes, 8000 chains, extent", extent)
+    same state bit for bit, and the sweep makes chainprob N molecule trials.
+    With the automatic extent each coarse cell's neighbourhood fits the LDS
+    stage; an extent of 7 sigma makes the regions too large for it, so the
+    passes read global memory.  Chain visit factor 1 (a pass per colour) and
+    27 (the largest divisor of the colour count: one or a few passes, many
+    trials each)."""
+    say("case: chain-move passes, 8000 chains, extent", extent, "chain visit", visit)
     g = sc_amd.GpuSystem(0)
     nch = 8000
     tp, (box, pos, d, q, t) = lattice(g, nch, 0.2, 6)
     g.set_chain_moves(0.25, 0.6, 0.25, extent)
+    g.set_chain_visit_factor(visit)
     rho = g.chain_extent()
     say("box %.3f, extent %.3f, maxcut %.3f" % (box[0], rho, tp.maxcut))
     assert box[0] >= 2 * (2 * rho + tp.maxcut)   # at least two coarse cells per axis
@@ -145,8 +149,11 @@ def test_chain_move_passes_are_rigid_reproducible_and_keep_the_energy(extent):
     b0, p0, d0, q0 = g.get_config()
     s = g.sweep(3, 1.1, 0.0, 0.0, r=1, seed=9, counter=0)
     period, staged = g.chain_pass_info()
-    say("colour period %d, LDS stage %s, cells skipped %d" % (period, staged, s["chain_cells_skipped"]))
+    vis = g.chain_visit_info()
+    say("colour period %d, LDS stage %s, cells skipped %d, visit factor %d" % (period, staged,
+                                                                               s["chain_cells_skipped"], vis))
     assert staged == (extent == 0.0)
+    assert vis == (1 if visit == 1 else period ** 3)
     assert s["chain_cells_skipped"] == 0
     e1 = g.allToAll()
     n = len(t)
@@ -154,7 +161,11 @@ def test_chain_move_passes_are_rigid_reproducible_and_keep_the_energy(extent):
     say("chain tries %d (expected %.0f): displace %d/%d, rotate %d/%d, rejected %d; E %.4f -> %.4f, sum dE %.4f" % (
         tries, 0.25 * n * 3, s["chain_disp_acc"], s["chain_disp_try"], s["chain_rot_acc"], s["chain_rot_try"],
         s["chain_rejected"], e0, e1, s["energy_change"]))
-    assert abs(tries - 0.25 * n * 3) <= 0.02 * 0.25 * n * 3
+    # a pass's trials are its cells' molecules times chainprob N / nchains
+    # times the visit factor: with one colour per sweep (visit 27) a sweep's
+    # count follows that colour's share of the molecules (~300 of 8000 here),
+    # exact only on average over colours
+    assert abs(tries - 0.25 * n * 3) <= (0.02 if vis == 1 else 0.1) * 0.25 * n * 3
     assert 0 < s["chain_disp_acc"] < s["chain_disp_try"] and 0 < s["chain_rot_acc"] < s["chain_rot_try"]
     assert s["chain_rejected"] < 0.2 * tries
     drift = (e1 - e0) - s["energy_change"]
@@ -178,10 +189,12 @@ def test_chain_move_passes_are_rigid_reproducible_and_keep_the_energy(extent):
     say("molecules checked %d, moved %d; worst distance change %.2e, worst rotation residual %.2e" % (
         len(range(0, nch, 7)), moved, worst, worst_r))
     assert worst < 1e-4 and worst_r < 2e-5
-    assert moved > 0.5 * len(range(0, nch, 7))
+    # every sweep visits at least 1/vis of the colours (all of them at vis 1)
+    assert moved > 0.5 / vis * len(range(0, nch, 7))
     # the same run again
     g.set_config(box, pos, d, q, t)
     g.set_chain_moves(0.25, 0.6, 0.25, rho)
+    g.set_chain_visit_factor(visit)
     g.sweep(3, 1.1, 0.0, 0.0, r=1, seed=9, counter=0)
     b2, p2, d2, q2 = g.get_config()
     assert np.array_equal(p1, p2) and np.array_equal(d1, d2) and np.array_equal(q1, q2)
@@ -216,12 +229,14 @@ def _end_to_end(st):
     return np.linalg.norm(x * st.box, axis=1).mean()
 
 
-def test_chain_move_ensemble_matches_cpu_oracle():
+@pytest.mark.parametrize("visit", [1, 27])
+def test_chain_move_ensemble_matches_cpu_oracle(visit):
     """120 chains at eta 0.15, T* 1.1, particle moves plus chain moves
     (chainprob 0.1, displacement 0.5, rotations up to 2 x 0.2 rad): E/N, the
     mean bond energy and the head-to-tail distance of the GPU sweep and of
-    the oracle's sequential Metropolis with the same moves."""
-    say("case: chain-move ensemble GPU vs CPU oracle")
+    the oracle's sequential Metropolis with the same moves; chain visit
+    factor 1 and 27 (the largest divisor of the colour count)."""
+    say("case: chain-move ensemble GPU vs CPU oracle, chain visit", visit)
     nch = 120
     top = W.chain_topology(nch)
     tp = po.Topo(top)
@@ -241,9 +256,11 @@ def test_chain_move_ensemble_matches_cpu_oracle():
     g.load_topology(top)
     g.set_config(box, pos, d, q, t)
     g.set_chain_moves(0.1, 0.5, 0.2)
+    g.set_chain_visit_factor(visit)
     n = len(t)
     g.sweep(300, 1.1, 0.3, ROT_HALF, r=1, seed=5, counter=0)
     assert g.chain_pass_info()[1]   # the LDS-staged passes
+    say("chain visit factor %d, colour period %d" % (g.chain_visit_info(), g.chain_pass_info()[0]))
     obs = []
     tot = {}
     for blk in range(32):
