@@ -1346,10 +1346,16 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
         if (lane == 0) *T.netd = ds;
     } else {
     int items = nx * ppmax * 2;
-    for (int base = 0; base < items; base += CB) {
-        int it = base + tid, which = 0, k1 = 0, k2 = 0, side = it & 1;
-        bool hit = false, pair_item = false;
-        float x1 = 0.0f, x2 = 0.0f, ev = 0.0f;
+    // a wave of its own whose items take more than one pass: each lane takes
+    // both sides of its patch pair, one after the other (half the passes, and
+    // the view, set-up and attraction once per patch pair; wave-uniform, and
+    // the same values: each side is the same patch_side call)
+    const bool dual = WS && items > CB;
+    const int nit = dual ? items / 2 : items;
+    for (int base = 0; base < nit; base += CB) {
+        int it = dual ? 2 * (base + tid) : base + tid, which = 0, k1 = 0, k2 = 0, side = it & 1;
+        bool hit = false, hit2 = false, pair_item = false;
+        float x1 = 0.0f, x2 = 0.0f, ev = 0.0f, z1 = 0.0f, z2 = 0.0f;
         const PP32* P2 = pp;
         V3<float> r = mk<float>(0.0f, 0.0f, 0.0f);
         PatchView<float> va, vb;
@@ -1373,6 +1379,7 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
                     va = sw ? vn : vm;
                     vb = sw ? vm : vn;
                     hit = patch_side<float>(*P2, va, vb, r, k1, k2, side, x1, x2);
+                    if (dual) hit2 = patch_side<float>(*P2, va, vb, r, k1, k2, 1, z1, z2);
                     pair_item = true;
                 }
             } else if (mpp == 0 && side == 0) {
@@ -1382,8 +1389,13 @@ __device__ __forceinline__ bool trial_energies(const Stage& S, int k, const uint
             }
         }
         // the other side's result (lane ^ 1 holds item it ^ 1: CB is even)
-        float y1 = __shfl_xor(x1, 1), y2 = __shfl_xor(x2, 1);
-        int oh = __shfl_xor(hit ? 1 : 0, 1);
+        float y1 = z1, y2 = z2;
+        int oh = hit2 ? 1 : 0;
+        if (!dual) {
+            y1 = __shfl_xor(x1, 1);
+            y2 = __shfl_xor(x2, 1);
+            oh = __shfl_xor(hit ? 1 : 0, 1);
+        }
         if (pair_item && side == 0 && hit && oh) ev = patch_atr<float>(*P2, va, vb, r, k1, k2, x1, x2, y1, y2);
         if (which) an += (double)ev; else ao += (double)ev;
     }
