@@ -1771,35 +1771,45 @@ __device__ __forceinline__ bool pair_active(uint4 u1, float4 o1, int id1, uint4 
     return d2 <= fmaxf(P.rcutSq, P.rcutwcaSq) * 1.00001f;
 }
 
-// Bit i of the result: trial `me` (particle k at uo/oo, proposed un/on) would
-// see another state had trial i < w of the round (its record Ra) been
-// accepted: the same particle, chain partners (bond and angle terms reach
-// beyond maxcut), or a pair that can interact in any of their old/new states.
-// Lanes 4i .. 4i+3 test the four state combinations of trial i.
-__device__ __forceinline__ int spec_deps(const TrialRec* recs, int t, int w, const Stage& S, int k, uint4 uo,
-                                         float4 oo, uint4 un, float4 on, const SysConst& sc,
-                                         const PP32* __restrict__ pp, bool bonded) {
-    int lane = lane_id(), i = lane >> 2, c = lane & 3;
+// Every trial's dependences of a round, on one wave (wave 0, whose own trial
+// has none, so the other waves start their trials at once).  Bit i of
+// deps[q]: trial q and the round's earlier trial i would see each other's
+// move: the same particle, chain
+// partners (bond and angle terms reach beyond maxcut), or a pair that can
+// interact in any of their old/new states; the relation is symmetric.  Lanes
+// 4p .. 4p+3 take the p-th pair (q, i) of (1,0) (2,0) (2,1) (3,0) (3,1)
+// (3,2) and its four state combinations.
+__device__ __forceinline__ void spec_deps_all(const TrialRec* recs, int t, int W, const Stage& S, const SysConst& sc,
+                                              const PP32* __restrict__ pp, bool bonded, int* deps) {
+    static_assert(SPEC_W <= 4, "six trial pairs on 24 lanes");
+    const int lane = lane_id(), p = lane >> 2, c = lane & 3;
+    const int q = p < 1 ? 1 : (p < 3 ? 2 : 3), i = p - (q * (q - 1)) / 2;
     bool d = false;
-    if (i < w) {
+    if (p < 6 && q < W) {
+        const TrialRec& Rq = recs[(t + q) & (TRIAL_CHUNK - 1)];
         const TrialRec& Ra = recs[(t + i) & (TRIAL_CHUNK - 1)];
-        int ka = Ra.k;
+        const int k = Rq.k, ka = Ra.k;
         if (ka == k || (bonded && abs(S.id[ka] - S.id[k]) <= 2)) {
             d = true;
         } else {
-            uint4 uoa = S.u[ka], una;
-            float4 ooa = S.o[ka], ona;
-            float2 qoa = S.q[ka], qna;
+            uint4 uo = S.u[k], un, uoa = S.u[ka], una;
+            float4 oo = S.o[k], on, ooa = S.o[ka], ona;
+            float2 qo = S.q[k], qn, qoa = S.q[ka], qna;
+            propose(Rq, uo, oo, qo, un, on, qn);
             propose(Ra, uoa, ooa, qoa, una, ona, qna);
             d = pair_active((c & 1) ? una : uoa, (c & 1) ? ona : ooa, S.id[ka], (c & 2) ? un : uo, (c & 2) ? on : oo,
                             S.id[k], sc, pp);
         }
     }
-    unsigned long long m = __ballot(d);
-    int dep = 0;
-    for (int q = 0; q < SPEC_W - 1; q++)
-        if ((m >> (4 * q)) & 15ull) dep |= 1 << q;
-    return dep;
+    const unsigned long long m = __ballot(d);
+#pragma unroll
+    for (int qq = 1; qq < SPEC_W; qq++) {
+        int dep = 0;
+#pragma unroll
+        for (int ii = 0; ii < qq; ii++)
+            if ((m >> (4 * ((qq * (qq - 1)) / 2 + ii))) & 15ull) dep |= 1 << ii;
+        deps[qq] = dep;
+    }
 }
 
 // LT: the pair and type tables are copied into LDS (systems of up to
@@ -1898,6 +1908,16 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
         }
         const int W = min(SPEC_W, min(ntrial - t, drawn - t));
         if (w < W) {
+            if (w == 0) {   // every trial's dependences, then its own trial
+                int deps[SPEC_W];
+                deps[0] = 0;
+                spec_deps_all(recs, t, W, S, sc, pp, bonded, deps);
+                if (lane == 0) {
+#pragma unroll
+                    for (int q = 0; q < SPEC_W; q++)
+                        if (q < W) res[q].dep = deps[q];
+                }
+            }
             const TrialRec& R = recs[(t + w) & (TRIAL_CHUNK - 1)];
             int k = R.k;
             uint4 uo = S.u[k];
@@ -1911,7 +1931,6 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
             int status = 0, stage = 0;
             bool acc = false;
             double de = 0.0;
-            const int dep = spec_deps(recs, t, w, S, k, uo, oo, un, on, sc, pp, bonded);
             if (translate && cell_of_u(g, un.x, un.y, un.z) != cell) {   // leaving the active cell: rejected
                 status = 1;
             } else {
@@ -1944,7 +1963,6 @@ __device__ __forceinline__ void sweep_cell_spec(int b, const Grid& g, const SysC
                 r.translate = translate ? 1 : 0;
                 r.acc = acc ? 1 : 0;
                 r.stage = stage;
-                r.dep = dep;
             }
         }
         __syncthreads();
