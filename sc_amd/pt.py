@@ -70,11 +70,15 @@ def c3_batch(g, nrep=64, seed=3, eta_rsa=0.2, steps=8, sweeps_per_step=3, relax=
 
 
 def run(g, temps, nsweeps, exchange_every=10, transmx=0.3, rot_half=20.0 / 180 * np.pi / 4, seed=1, counter=0,
-        r=0):
+        r=0, refresh_every=10):
     """nsweeps sweeps of the batch in g with a replica-exchange step every
     `exchange_every` sweeps (even/odd ladder pairs alternating as the
-    reference does).  Returns (temperatures, sweep stats summed per replica,
-    accepted and attempted exchanges per ladder pair)."""
+    reference does).  The exchange step reads each replica's whole-system
+    energy carried through the sweeps (the FP64 sum of its accepted changes:
+    temperatures move between replicas, configurations stay), evaluated in
+    full at the first exchange and at every `refresh_every`-th one after it
+    (1: at every exchange).  Returns (temperatures, sweep stats summed per
+    replica, accepted and attempted exchanges per ladder pair)."""
     temps = np.array(temps, np.float64)
     nrep = len(temps)
     acc = np.zeros(max(nrep - 1, 1), np.int64)
@@ -82,15 +86,19 @@ def run(g, temps, nsweeps, exchange_every=10, transmx=0.3, rot_half=20.0 / 180 *
     tot = [dict() for _ in range(nrep)]
     done = 0
     block = 0
+    e = None
     while done < nsweeps:
         k = min(exchange_every, nsweeps - done)
         st = g.batch_sweep(k, temps, transmx, rot_half, r=r, seed=seed, counter=counter + done)
         for t, s in zip(tot, st):
             for key, v in s.items():
                 t[key] = t.get(key, 0) + v
+        if e is not None:
+            e = e + np.array([s["energy_change"] for s in st], np.float64)
         done += k
         if k == exchange_every:
-            e = g.batch_energies()
+            if e is None or block % max(1, int(refresh_every)) == 0:
+                e = g.batch_energies()
             parity = block % 2
             temps, a = g.batch_exchange(temps, e, parity, seed=seed, counter=counter + done)
             acc[: nrep - 1] += a

@@ -146,3 +146,31 @@ def test_autotune_restores_state_and_pt_run():
     for t in tot:
         assert t["trans_try"] + t["rot_try"] > 0
     g.close()
+
+
+def test_pt_run_with_carried_energies_matches_full_evaluation():
+    """The exchange step on each replica's energy carried through the sweeps
+    (evaluated in full at the first exchange only) against a full evaluation
+    at every exchange, from the same state and seed: the same temperatures,
+    exchange counts and configurations, and the carried energies meet the
+    full ones to 1e-8 of |E|."""
+    say("case: tempering run, carried replica energies vs full evaluation at every exchange")
+    out = []
+    for refresh in (1, 1000):
+        g = sc_amd.GpuSystem(0)
+        small_batch(g, sizes=(900, 1600, 600, 1200), etas=(0.18, 0.2, 0.15, 0.17), seed=12)
+        temps = np.array([0.8, 0.95, 1.1, 1.3])
+        e0 = g.batch_energies()
+        temps2, tot, acc, tried = pt.run(g, temps, 80, exchange_every=10, seed=8, r=1, refresh_every=refresh)
+        e1 = g.batch_energies()
+        carried = e0 + np.array([t["energy_change"] for t in tot])
+        say("refresh every %d: temperatures %s, exchanges %s / %s, carried - full %s" % (
+            refresh, np.round(temps2, 3).tolist(), acc.tolist(), tried.tolist(), (carried - e1).tolist()))
+        np.testing.assert_allclose(carried, e1, rtol=1e-8, atol=1e-8)
+        out.append((temps2, acc, tried, [g.batch_get_config(k) for k in range(4)]))
+        g.close()
+    (ta, aa, ra, ca), (tb, ab, rb, cb) = out
+    assert ta.tolist() == tb.tolist() and aa.tolist() == ab.tolist() and ra.tolist() == rb.tolist()
+    for x, y in zip(ca, cb):
+        for u, v in zip(x, y):
+            np.testing.assert_array_equal(u, v)
